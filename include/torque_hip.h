@@ -1,0 +1,222 @@
+/*
+ * torque_hip.h -- C-ABI of the MI355X (gfx950) block-sparse tensor
+ * contraction library.
+ *
+ * The reference library (Walter-Feng/Torque) is a header-only C++ template
+ * library with no extern "C" surface; its GPU hot path is
+ *   torque::gpu::BlockSparseTensor<T>::contract(cublasHandle_t, B, idx)
+ *       src/gpu/block_sparse.cuh:558-1026
+ *   torque::gpu::BlockSparseTensor<T>::hard_transpose(perm)
+ *       src/gpu/block_sparse.cuh:1031-1122
+ *   torque::gpu::DenseTensor<T>::hard_transpose(perm) / contract(...)
+ *       src/gpu/dense.cuh:561-599 / :439-555
+ *   torque::gpu::SparseTensor<T>::contract(...)
+ *       src/gpu/sparse.cuh:270-375
+ * Each entry point below replaces one of those calls (or one of the
+ * library calls they make: cuTT, cuBLAS, thrust) and is what the C++
+ * drop-in classes in torque_amd/include/torque/gpu/ bind to.
+ *
+ * Conventions
+ *  - All device pointers are caller-owned unless an entry point says it
+ *    allocates (tq_malloc, plans). No entry point calls exit().
+ *  - Tensors are column-major: leg 0 is the fastest leg
+ *    (src/util/space.h:10-24).
+ *  - Block-sparse metadata uses the reference's data model
+ *    (src/gpu/block_sparse.cuh:147-178): per block, inclusive begin/end
+ *    points, an index table (element strides per leg) and an element offset
+ *    into the concatenated data buffer. Metadata arrays are block-major:
+ *    entry [b * rank + l] is leg l of block b (one reference umat column per
+ *    block).
+ *  - A handle serialises its work on one HIP stream; distinct handles may be
+ *    used from distinct threads.
+ */
+#ifndef TORQUE_HIP_H
+#define TORQUE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TQ_MAX_RANK 12
+
+typedef enum tq_status {
+  TQ_SUCCESS = 0,
+  TQ_ERROR_INVALID_VALUE = 1,      /* bad argument (null pointer, bad size)   */
+  TQ_ERROR_DIMENSION_MISMATCH = 2, /* contracted legs have different dims     */
+  TQ_ERROR_RANK_MISMATCH = 3,      /* permutation / index length != rank      */
+  TQ_ERROR_OUT_OF_RANGE = 4,       /* index beyond the tensor dimension       */
+  TQ_ERROR_ALLOCATION = 5,         /* device or host allocation failed        */
+  TQ_ERROR_HIP = 6,                /* a HIP runtime call failed               */
+  TQ_ERROR_UNSUPPORTED = 7,        /* rank above TQ_MAX_RANK, dtype/precision */
+  TQ_ERROR_NOT_INITIALIZED = 8     /* null handle or plan                     */
+} tq_status;
+
+typedef enum tq_dtype {
+  TQ_F32 = 0,
+  TQ_F64 = 1
+} tq_dtype;
+
+/* Arithmetic used by the grouped GEMM of a contraction plan.
+ * TQ_PREC_NATIVE: the data type's own MFMA (FP64 for TQ_F64, FP32 for TQ_F32).
+ * TQ_PREC_F32:    FP64 data, products and sums in FP32 MFMA.
+ * TQ_PREC_BF16X3: FP64 data split into three BF16 pieces per operand; the six
+ *                 leading piece products run on BF16 MFMA with FP32
+ *                 accumulation (the dropped terms are below 2^-24 relative).
+ * TQ_PREC_BF16X3 and TQ_PREC_F32 write FP64 results. */
+typedef enum tq_precision {
+  TQ_PREC_NATIVE = 0,
+  TQ_PREC_F32 = 1,
+  TQ_PREC_BF16X3 = 2
+} tq_precision;
+
+typedef struct tq_handle_s *tq_handle_t;
+typedef struct tq_plan_s *tq_plan_t;
+typedef struct tq_batch_s *tq_batch_t;
+
+/* Host-side view of a block-sparse tensor's metadata
+ * (src/gpu/block_sparse.cuh:147-178; src/tensor/block_sparse.h:114-145). */
+typedef struct tq_bs_desc {
+  int32_t rank;
+  int64_t n_blocks;
+  const int64_t *dims;    /* [rank]            total dimension               */
+  const int64_t *begin;   /* [n_blocks * rank] inclusive begin points        */
+  const int64_t *end;     /* [n_blocks * rank] inclusive end points          */
+  const int64_t *strides; /* [n_blocks * rank] index tables (element strides) */
+  const int64_t *offsets; /* [n_blocks]        element offset of each block  */
+} tq_bs_desc;
+
+/* Summary of a contraction plan (sizes of the host arrays to request). */
+typedef struct tq_plan_info {
+  int32_t out_rank;
+  int32_t n_legs;
+  int64_t n_pairs;      /* matched (A block, B block) pairs == output blocks  */
+  int64_t out_n_elem;   /* elements of the output data buffer (>= 1)         */
+  double flops;         /* sum over pairs of 2*M*N*K                         */
+  double bytes;         /* compulsory bytes: sizeof(T)*(sum nA + nB + nC)    */
+  int32_t a_layout;     /* 0 = read in place m-fast, 1 = in place k-fast,
+                           2 = packed by the permute kernel                 */
+  int32_t b_layout;     /* same for B (n-fast / k-fast / packed)             */
+} tq_plan_info;
+
+/* ---- runtime --------------------------------------------------------- */
+
+/* Create a handle on a HIP device. Replaces cublasCreate in the reference's
+ * callers (test/gpu/block_sparse_test.cu:7-8). */
+tq_status tq_create(tq_handle_t *handle, int device);
+/* Replaces cublasDestroy (test/gpu/block_sparse_test.cu:234). */
+tq_status tq_destroy(tq_handle_t handle);
+/* Work is issued on `stream` (a hipStream_t; NULL = the default stream). */
+tq_status tq_set_stream(tq_handle_t handle, void *stream);
+tq_status tq_synchronize(tq_handle_t handle);
+const char *tq_status_string(tq_status status);
+/* Text of the last error raised on this handle ("" if none). */
+const char *tq_last_error(tq_handle_t handle);
+
+/* Device memory helpers used by the C++ classes (reference: cudaMalloc /
+ * cudaMemcpy / cudaMemset calls at src/gpu/block_sparse.cuh:180-446).
+ * kind: 0 = host->device, 1 = device->host, 2 = device->device. */
+tq_status tq_malloc(tq_handle_t handle, void **ptr, size_t bytes);
+tq_status tq_free(tq_handle_t handle, void *ptr);
+tq_status tq_memcpy(tq_handle_t handle, void *dst, const void *src,
+                    size_t bytes, int kind);
+tq_status tq_memset_zero(tq_handle_t handle, void *ptr, size_t bytes);
+
+/* ---- permutation (replaces cuTT) ----------------------------------- */
+
+/* Dense permutation: out has dims[perm[k]] on leg k and is compact
+ * column-major; element out[n] = in[sum_k n_k * in_strides[perm[k]]].
+ * in_strides == NULL means the compact column-major table of dims.
+ * Replaces cuttPlan/cuttExecute in DenseTensor::hard_transpose
+ * (src/gpu/dense.cuh:561-599; CPU semantics src/tensor/dense.h:325-366). */
+tq_status tq_permute(tq_handle_t handle, tq_dtype dtype, int32_t rank,
+                     const int64_t *dims, const int64_t *in_strides,
+                     const int32_t *perm, const void *in, void *out);
+
+/* Block-sparse permutation: every block is permuted by perm into a compact
+ * column-major block; block order is kept and output block b starts at
+ * out_offsets[b] (the exclusive prefix sum of block sizes, written by the
+ * call into a host array of n_blocks entries, may be NULL).
+ * Replaces reshape_kernel + cuTT + reshape_kernel in
+ * BlockSparseTensor::hard_transpose (src/gpu/block_sparse.cuh:1031-1122;
+ * CPU semantics src/tensor/block_sparse.h:608-651). */
+tq_status tq_bs_permute(tq_handle_t handle, tq_dtype dtype,
+                        const tq_bs_desc *in_desc, const int32_t *perm,
+                        const void *in, void *out, int64_t *out_offsets);
+
+/* ---- block-sparse contraction -------------------------------------- */
+
+/* Plan C = contract(A, B, legs): legs is n_legs x 2, row k = (leg of A,
+ * leg of B), the reference's `contracting_indices` umat
+ * (src/gpu/block_sparse.cuh:558-561). Block matching and all output index
+ * maps are built on the device (src/tensor/block_sparse.cpp:6-175 and
+ * src/gpu/block_sparse.cuh:620-674,700-732 restated). The output keeps the
+ * reference's layout: one block per matched pair, ordered by A block then
+ * ascending B block, each block column-major over [A free legs, B free
+ * legs]. The plan owns its device memory and may be executed many times on
+ * any data with the same block structure. */
+tq_status tq_bs_contract_plan(tq_handle_t handle, const tq_bs_desc *a,
+                              const tq_bs_desc *b, int32_t n_legs,
+                              const int32_t *legs, tq_dtype dtype,
+                              tq_precision precision, tq_plan_t *plan);
+tq_status tq_plan_get_info(tq_plan_t plan, tq_plan_info *info);
+/* Output metadata, host arrays: dims[out_rank], begin/end/strides
+ * [n_pairs*out_rank] (block-major), offsets[n_pairs]. Any may be NULL. */
+tq_status tq_plan_get_output(tq_plan_t plan, int64_t *dims, int64_t *begin,
+                             int64_t *end, int64_t *strides,
+                             int64_t *offsets);
+/* Integer maps per matched pair, host arrays of n_pairs entries (any may be
+ * NULL): A block, B block, A sub-block element offset, B sub-block element
+ * offset, GEMM shape M, N, K. These are the quantities the reference computes
+ * in src/gpu/block_sparse.cuh:700-732 and src/tensor/block_sparse.h:516-525. */
+tq_status tq_plan_get_pairs(tq_plan_t plan, int64_t *a_block, int64_t *b_block,
+                            int64_t *a_offset, int64_t *b_offset, int64_t *m,
+                            int64_t *n, int64_t *k);
+/* Order in which the grouped GEMM visits the pairs (the size-sorted work
+ * list): a permutation of 0..n_pairs-1, largest estimated cost first. */
+tq_status tq_plan_get_worklist(tq_plan_t plan, int64_t *order);
+/* Run the plan: C receives out_n_elem elements. A rank-0 output (full
+ * contraction) is the network scalar, reduced on the GPU in FP64. */
+tq_status tq_bs_contract_execute(tq_plan_t plan, const void *a, const void *b,
+                                 void *c);
+/* Select the grouped-GEMM tile configuration by index (see
+ * tq_gemm_config_count); -1 restores the plan's automatic choice. */
+tq_status tq_plan_set_config(tq_plan_t plan, int32_t config);
+tq_status tq_plan_destroy(tq_plan_t plan);
+
+/* Number of grouped-GEMM tile configurations and a description of one. */
+int32_t tq_gemm_config_count(tq_precision precision);
+tq_status tq_gemm_config_describe(tq_precision precision, int32_t config,
+                                  int32_t *bm, int32_t *bn, int32_t *bk);
+
+/* ---- dense contraction --------------------------------------------- */
+
+/* Dense C = contract(A, B, legs) with compact column-major operands;
+ * C has dims [A free legs, B free legs]. Replaces the permute + cublasXgemm
+ * path of DenseTensor::contract (src/gpu/dense.cuh:439-555). */
+tq_status tq_dense_contract(tq_handle_t handle, tq_dtype dtype, int32_t rank_a,
+                            const int64_t *dims_a, int32_t rank_b,
+                            const int64_t *dims_b, int32_t n_legs,
+                            const int32_t *legs, const void *a, const void *b,
+                            void *c);
+
+/* ---- FP64 reductions ------------------------------------------------ */
+
+/* result = sum_i x[i] * y[i] in FP64 on the GPU (reference: cublasDdot +
+ * add<<<1,1>>>, src/gpu/block_sparse.cuh:979-1010). Writes *result on the
+ * host after synchronising the handle's stream. */
+tq_status tq_dot_f64(tq_handle_t handle, int64_t n, const double *x,
+                     const double *y, double *result);
+/* result = sqrt(sum x[i]^2), FP64 on the GPU. */
+tq_status tq_nrm2_f64(tq_handle_t handle, int64_t n, const double *x,
+                      double *result);
+/* x <- x * alpha on the GPU (normalisation of environment tensors). */
+tq_status tq_scale_f64(tq_handle_t handle, int64_t n, double *x, double alpha);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TORQUE_HIP_H */

@@ -1,0 +1,237 @@
+"""Block-sparse contraction on the GPU against the oracle.
+
+Integer maps (matched pairs, output begin/end, sub-block offsets, GEMM
+shapes, output offsets) must be bit-exact. FP64 data must match the oracle
+to a relative Frobenius error of 1e-12 (summation order differs); the FP32
+tier to 1e-5; FP32 data to 1e-5.
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import O, W, dense_of, from_workload, permuted_strides, random_blocks, rel_fro, \
+    tensor, torch_tensor
+from torque_amd import BlockSparseTensor, DenseTensor, TorqueError, dot_f64, gemm_configs, \
+    nrm2_f64, scale_f64
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def gpu_contract(a, da, b, db, legs, cuda, precision="native", cfg=None, dtype=torch.float64):
+    ga = torch_tensor(a, da, cuda, dtype)
+    gb = torch_tensor(b, db, cuda, dtype)
+    plan = ga.plan_contract(gb, legs, precision)
+    if cfg is not None:
+        plan.set_config(cfg)
+    out = torch.empty(max(plan.out_n_elem, 1), dtype=dtype, device=cuda)
+    plan.execute(ga.data, gb.data, out)
+    torch.cuda.synchronize()
+    return plan, out.cpu().double().numpy()[: max(plan.out_n_elem, 1)]
+
+
+def check_maps(plan, ref):
+    p = plan.pairs()
+    for k in ("a_block", "b_block", "a_offset", "b_offset", "m", "n", "k"):
+        assert np.array_equal(p[k], ref[k]), k
+    assert np.array_equal(plan.out_begin, ref["out_begin"])
+    assert np.array_equal(plan.out_end, ref["out_end"])
+    if plan.out_rank > 0:
+        assert np.array_equal(plan.out_offsets, ref["out_offsets"])
+
+
+# ---- the reference's GPU known answers (test/gpu/block_sparse_test.cu) ---
+
+def bst(dims, blocks, cuda, dtype=torch.float32):
+    t = BlockSparseTensor(dims, dtype=dtype, device=cuda)
+    for b, e, v, s in blocks:
+        t.append_block(v, b, e, s)
+    return t
+
+
+def test_known_answers_gpu(cuda):
+    t = bst([5], [([0], [1], [0, 1], [1]), ([2], [4], [2, 3, 4], [1])], cuda)
+    assert t.contract(t, [[0, 0]]).to_number() == 30
+    vec = np.array([0, 1, 0, 0, 2, 3, 0, 0, 0, 0, 4, 5, 0, 0, 6, 7], np.float32)
+    dense = DenseTensor([4, 4], vec, dtype=torch.float32, device=cuda)
+    sq = dense.contract(dense, [[1, 0]])
+    s = bst([4, 4], [([0, 0], [1, 1], [0, 1, 2, 3], [1, 2]),
+                     ([2, 2], [3, 3], [4, 5, 6, 7], [1, 2])], cuda)
+    ss = s.contract(s, [[1, 0]])
+    for i in range(4):
+        for j in range(4):
+            assert sq.query([i, j]) == ss.query([i, j])
+    t = bst([3, 3], [([0, 0], [1, 1], [0, 1, 2, 3], [1, 2]),
+                     ([1, 1], [2, 2], [0, 7, 8, 9], [1, 2])], cuda)
+    ref_a = np.array([[0, 1, 0], [2, 3, 7], [0, 8, 9]], np.float64).T
+    assert t.contract(t, [[1, 0], [0, 1]]).to_number() == np.sum(ref_a * ref_a.T)
+    t = bst([2, 2, 2], [([0, 0, 0], [1, 1, 0], [0, 1, 2, 3], [1, 2, 4]),
+                        ([0, 0, 1], [1, 1, 1], [4, 5, 6, 7], [1, 2, 4])], cuda)
+    v = bst([2], [([0], [1], [1, 2], [1])], cuda)
+    c = t.contract(v, [[0, 0]])
+    assert [c.query(i) for i in ([0, 0], [1, 0], [0, 1], [1, 1])] == [2, 8, 14, 20]
+    c = t.contract(v, [[1, 0]])
+    assert [c.query(i) for i in ([0, 0], [1, 0], [0, 1], [1, 1])] == [4, 7, 16, 19]
+    assert t.contract(v, [[2, 0]]).query([1, 0]) == 11
+    t = bst([2, 2, 3], [([0, 0, 2], [1, 1, 2], [8, 9, 10, 11], [1, 2, 4]),
+                        ([0, 0, 1], [1, 1, 1], [4, 5, 6, 7], [1, 2, 4]),
+                        ([0, 0, 0], [1, 1, 0], [0, 1, 2, 3], [1, 2, 4])], cuda)
+    m = bst([2, 2], [([0, 0], [1, 1], [1, 2, 3, 4], [1, 2])], cuda)
+    c = t.contract(m, [[0, 1], [1, 0]])
+    assert [c.query([i]) for i in range(3)] == [19, 59, 99]
+
+
+def test_dense_contract_known_and_random(cuda):
+    rng = np.random.default_rng(2)
+    a = rng.standard_normal(3 * 4 * 5)
+    b = rng.standard_normal(5 * 3 * 2)
+    A = DenseTensor([3, 4, 5], a, device=cuda)
+    B = DenseTensor([5, 3, 2], b, device=cuda)
+    C = A.contract(B, [[2, 0], [0, 1]])
+    ref, od = O.dense_contract([3, 4, 5], a, [5, 3, 2], b, [[2, 0], [0, 1]])
+    assert list(C.dimension) == od
+    assert rel_fro(C.data.cpu().numpy(), ref) < FP64_TOL
+    v = DenseTensor([5], np.arange(5), device=cuda)
+    assert v.contract(v, [[0, 0]]).to_number() == 30
+
+
+# ---- integer maps and FP64 data against the oracle -----------------------
+
+def test_c1_config(cuda):
+    a, b, legs = W.c1(0)
+    A, da = from_workload(a)
+    B, db = from_workload(b)
+    ref_c, ref = O.contract(A, da, B, db, legs)
+    plan, got = gpu_contract(A, da, B, db, legs, cuda)
+    assert plan.n_pairs == 45 and plan.out_n_elem == 44544
+    assert plan.flops == 2834432
+    check_maps(plan, ref)
+    assert rel_fro(got, ref_c) < FP64_TOL
+    # dense cross-check (config 1)
+    out = O.BS(plan.out_dims, plan.out_begin, plan.out_end)
+    dense_ref, _ = O.dense_contract(a.dims, W.densify(a), b.dims, W.densify(b), legs)
+    assert rel_fro(O.densify(out, got), dense_ref) < FP64_TOL
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_overlapping_blocks(cuda, seed):
+    rng = np.random.default_rng(500 + seed)
+    dims_a = [4, 6, 3, 5]
+    dims_b = [3, 5, 7]
+    ba, ea = random_blocks(rng, dims_a, 7)
+    bb, eb = random_blocks(rng, dims_b, 6)
+    sa = permuted_strides(ba, ea, rng) if seed % 2 else None
+    a, da = tensor(dims_a, ba, ea, rng, sa)
+    b, db = tensor(dims_b, bb, eb, rng)
+    legs = [[2, 0], [3, 1]] if seed < 3 else [[3, 1], [1, 2]]
+    ref_c, ref = O.contract(a, da, b, db, legs)
+    plan, got = gpu_contract(a, da, b, db, legs, cuda)
+    check_maps(plan, ref)
+    if plan.n_pairs:
+        assert rel_fro(got, ref_c) < FP64_TOL
+
+
+def test_every_gemm_config_agrees(cuda):
+    a, b, legs = W.c1(3)
+    A, da = from_workload(a)
+    B, db = from_workload(b)
+    ref_c, _ = O.contract(A, da, B, db, legs)
+    for cfg in range(len(gemm_configs())):
+        _, got = gpu_contract(A, da, B, db, legs, cuda, cfg=cfg)
+        assert rel_fro(got, ref_c) < FP64_TOL, cfg
+
+
+def test_peps_absorption_d6_chi64(cuda):
+    E, A, Ac, l1, l2 = W.peps(6, 64, seed=1)
+    TA, dA = from_workload(A)
+    ref_aa, ref1 = O.contract(TA, dA, TA, dA, l1)
+    plan1, aa = gpu_contract(TA, dA, TA, dA, l1, cuda)
+    check_maps(plan1, ref1)
+    assert rel_fro(aa, ref_aa) < FP64_TOL
+    ta = O.BS(plan1.out_dims, plan1.out_begin, plan1.out_end)
+    TE, dE = from_workload(E)
+    ref_c, ref2 = O.contract(TE, dE, ta, ref_aa, l2)
+    plan2, got = gpu_contract(TE, dE, ta, ref_aa, l2, cuda)
+    check_maps(plan2, ref2)
+    assert rel_fro(got, ref_c) < FP64_TOL
+    assert plan2.a_layout == 2 and plan2.b_layout == 2  # both permuted to GEMM layout
+
+
+def test_batched_pairs_large_k(cuda):
+    pairs = W.batched(3, seed=9, dim_lo=40, dim_hi=96)
+    for a, b, legs in pairs:
+        A, da = from_workload(a)
+        B, db = from_workload(b)
+        ref_c, ref = O.contract(A, da, B, db, legs)
+        plan, got = gpu_contract(A, da, B, db, legs, cuda)
+        check_maps(plan, ref)
+        assert rel_fro(got, ref_c) < FP64_TOL
+
+
+def test_rank0_outer_and_empty(cuda):
+    rng = np.random.default_rng(8)
+    ba, ea = random_blocks(rng, [6, 5], 5)
+    a, da = tensor([6, 5], ba, ea, rng)
+    ref_c, _ = O.contract(a, da, a, da, [[0, 0], [1, 1]])
+    plan, got = gpu_contract(a, da, a, da, [[0, 0], [1, 1]], cuda)
+    assert plan.out_rank == 0 and abs(got[0] - ref_c[0]) <= 1e-12 * abs(ref_c[0])
+    ref_c, ref = O.contract(a, da, a, da, np.zeros((0, 2)))
+    plan, got = gpu_contract(a, da, a, da, np.zeros((0, 2)), cuda)
+    check_maps(plan, ref)
+    assert rel_fro(got, ref_c) < FP64_TOL
+    # no matching block: empty output (the reference returns an empty tensor)
+    b, db = tensor([6, 5], np.array([[0, 0]]), np.array([[0, 0]]), rng)
+    c, dc = tensor([6, 5], np.array([[5, 4]]), np.array([[5, 4]]), rng)
+    plan, _ = gpu_contract(b, db, c, dc, [[0, 0]], cuda)
+    assert plan.n_pairs == 0
+
+
+def test_dimension_mismatch_error(cuda):
+    t = BlockSparseTensor([5], [0, 1], [[0]], [[1]], device=cuda)
+    u = BlockSparseTensor([4], [0, 1], [[0]], [[1]], device=cuda)
+    with pytest.raises(TorqueError, match="do not match"):
+        t.contract(u, [[0, 0]])
+
+
+def test_worklist_is_cost_sorted_permutation(cuda):
+    pairs = W.batched(2, seed=4)
+    a, b, legs = pairs[0]
+    A, da = from_workload(a)
+    B, db = from_workload(b)
+    plan, _ = gpu_contract(A, da, B, db, legs, cuda)
+    order = plan.worklist()
+    assert sorted(order.tolist()) == list(range(plan.n_pairs))
+    p = plan.pairs()
+    cost = p["m"] * p["n"] * (p["k"] + 4)
+    lg = np.floor(np.log2(np.maximum(cost[order], 1)))
+    assert np.all(np.diff(lg) <= 0)
+    # stable inside a bucket
+    for v in np.unique(lg):
+        idx = order[lg == v]
+        assert np.all(np.diff(idx) > 0)
+
+
+def test_fp32_tier_and_fp32_data(cuda):
+    a, b, legs = W.c1(5)
+    A, da = from_workload(a)
+    B, db = from_workload(b)
+    ref_c, _ = O.contract(A, da, B, db, legs)
+    _, got = gpu_contract(A, da, B, db, legs, cuda, precision="fp32")
+    assert rel_fro(got, ref_c) < 1e-5
+    _, got = gpu_contract(A, da, B, db, legs, cuda, dtype=torch.float32)
+    assert rel_fro(got, ref_c) < 1e-5
+
+
+def test_reductions(cuda):
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(1_000_003)
+    y = rng.standard_normal(1_000_003)
+    gx = torch.tensor(x, device=cuda)
+    gy = torch.tensor(y, device=cuda)
+    assert abs(dot_f64(gx, gy) - np.dot(x, y)) < 1e-10 * np.sqrt(len(x))
+    assert abs(nrm2_f64(gx) - np.linalg.norm(x)) < 1e-12 * np.linalg.norm(x)
+    scale_f64(gx, 1.0 / np.linalg.norm(x))
+    assert abs(nrm2_f64(gx) - 1.0) < 1e-14
+    # repeated reductions agree bitwise
+    assert dot_f64(gx, gy) == dot_f64(gx, gy)

@@ -1,0 +1,109 @@
+"""Permutation kernels against the oracle: bit-exact (pure copies)."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import O, permuted_strides, random_blocks, tensor, torch_tensor
+from torque_amd import BlockSparseTensor, DenseTensor
+
+pytestmark = pytest.mark.gpu
+
+
+def test_known_block_sparse_transpositions(cuda):
+    # test/gpu/block_sparse_test.cu:35-60
+    t = BlockSparseTensor([4, 3], [1, 2, 3, 4], [[2, 0]], [[3, 1]], dtype=torch.float32, device=cuda)
+    t = t.hard_transpose([1, 0])
+    assert t.query([0, 3]) == 2 and t.query([1, 2]) == 3
+    back = t.hard_transpose([1, 0])
+    assert back.query([3, 0]) == 2 and back.query([2, 1]) == 3
+    # test/gpu/block_sparse_test.cu:76-91: overlapping blocks, different shapes
+    t = BlockSparseTensor([2, 2, 3], dtype=torch.float32, device=cuda)
+    t.append_block([1, 2, 3, 4, 5, 6, 0, 0, 0, 0, 0, 0], [0, 0, 0], [1, 1, 2], [1, 2, 4])
+    t.append_block([7, 8, 9, 10, 11, 12], [1, 0, 0], [1, 1, 2], [1, 1, 2])
+    tt = t.hard_transpose([1, 0, 2])
+    assert [tt.query(i) for i in ([1, 0, 0], [1, 0, 1], [1, 1, 2])] == [3, 0, 12]
+
+
+def test_known_dense_transpositions(cuda):
+    # test/gpu/dense_test.cu:27-94
+    d = DenseTensor([4, 3], np.arange(1, 13), dtype=torch.float32, device=cuda)
+    assert d.query([1, 2]) == 10
+    d = d.hard_transpose([1, 0])
+    assert d.query([1, 2]) == 7
+    back = d.hard_transpose([1, 0])
+    for i in range(3):
+        for j in range(4):
+            assert d.query([i, j]) == back.query([j, i])
+    t = DenseTensor([2, 2, 3], np.arange(1, 13), device=cuda)
+    t = t.hard_transpose([0, 2, 1])
+    assert t.query([0, 2, 0]) == 9 and t.query([0, 2, 1]) == 11
+    t2 = t.hard_transpose([1, 0, 2])
+    assert t2.query([2, 0, 0]) == 9 and t2.query([2, 0, 1]) == 11
+
+
+CASES = [
+    ([7], [0]),
+    ([5, 9], [1, 0]),
+    ([3, 4, 5], [2, 0, 1]),
+    ([64, 80], [1, 0]),                 # tiled, square-ish
+    ([10, 20, 25, 8], [3, 2, 1, 0]),    # tiled, small fastest leg
+    ([33, 2, 65], [2, 1, 0]),
+    ([6, 5, 4, 3, 2, 7], [5, 4, 3, 2, 1, 0]),
+    ([6, 5, 4, 3, 2, 7], [0, 2, 1, 4, 3, 5]),  # fastest leg kept
+    ([16, 1, 24, 1, 9], [2, 1, 4, 0, 3]),      # unit legs
+    ([128, 3, 130], [2, 1, 0]),
+]
+
+
+@pytest.mark.parametrize("dims,perm", CASES)
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_dense_permute_bit_exact(cuda, dims, perm, dtype):
+    rng = np.random.default_rng(len(dims) * 31 + sum(dims))
+    x = rng.standard_normal(int(np.prod(dims)))
+    d = DenseTensor(dims, x, dtype=dtype, device=cuda)
+    got = d.hard_transpose(perm).data.cpu().numpy()
+    ref = O.dense_permute(dims, d.data.cpu().double().numpy(), perm)
+    assert np.array_equal(got.astype(np.float64), ref)
+
+
+def test_dense_permute_after_soft_transpose(cuda):
+    rng = np.random.default_rng(4)
+    dims = [6, 7, 8]
+    x = rng.standard_normal(336)
+    d = DenseTensor(dims, x, device=cuda)
+    d.soft_transpose([2, 0, 1])
+    got = d.hard_transpose([1, 2, 0]).data.cpu().numpy()
+    # element (i0,i1,i2) of the soft view is x at sum i_l * table[l]
+    table = np.array([42, 1, 6])
+    sview = np.zeros((8, 6, 7))
+    for a in range(8):
+        for b in range(6):
+            for c in range(7):
+                sview[a, b, c] = x[a * table[0] + b * table[1] + c * table[2]]
+    ref = np.transpose(sview, [1, 2, 0])
+    assert np.array_equal(got.reshape(ref.shape, order="F"), ref)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_block_sparse_permute_bit_exact(cuda, seed):
+    rng = np.random.default_rng(seed)
+    dims = [5, 7, 4, 6][: 2 + seed % 3]
+    ba, ea = random_blocks(rng, dims, 9)
+    t, d = tensor(dims, ba, ea, rng, permuted_strides(ba, ea, rng))
+    perm = list(rng.permutation(len(dims)))
+    gt = torch_tensor(t, d, cuda)
+    out = gt.hard_transpose(perm)
+    ref = O.bs_permute(t, d, perm)
+    assert np.array_equal(out.data.cpu().numpy()[: ref.size], ref)
+    assert np.array_equal(out.begin_points, t.begin[:, perm])
+    assert np.array_equal(out.block_offsets, O.nest_sum(t.n_elem))
+
+
+def test_block_sparse_permute_large_blocks(cuda):
+    from torque_amd import workloads as W
+    E, A, Ac, _, _ = W.peps(8, 128)
+    t = O.BS(E.dims, E.begin, E.end)
+    gt = torch_tensor(t, E.data, cuda)
+    out = gt.hard_transpose([0, 3, 1, 2])
+    ref = O.bs_permute(t, E.data, [0, 3, 1, 2])
+    assert np.array_equal(out.data.cpu().numpy()[: ref.size], ref)

@@ -1,0 +1,121 @@
+"""ctypes binding of libtorque_hip.so (the C-ABI declared in
+include/torque_hip.h). Loading fails loudly when the library has not been
+built: there is no CPU fallback for any entry point."""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtorque_hip.so")
+
+TQ_SUCCESS = 0
+TQ_ERROR_INVALID_VALUE = 1
+TQ_ERROR_DIMENSION_MISMATCH = 2
+TQ_ERROR_RANK_MISMATCH = 3
+TQ_ERROR_OUT_OF_RANGE = 4
+TQ_ERROR_ALLOCATION = 5
+TQ_ERROR_HIP = 6
+TQ_ERROR_UNSUPPORTED = 7
+TQ_ERROR_NOT_INITIALIZED = 8
+
+TQ_F32 = 0
+TQ_F64 = 1
+
+TQ_PREC_NATIVE = 0
+TQ_PREC_F32 = 1
+TQ_PREC_BF16X3 = 2
+
+PRECISIONS = {"native": TQ_PREC_NATIVE, "fp64": TQ_PREC_NATIVE,
+              "fp32": TQ_PREC_F32, "bf16x3": TQ_PREC_BF16X3}
+
+c_i64_p = ctypes.POINTER(ctypes.c_int64)
+c_i32_p = ctypes.POINTER(ctypes.c_int32)
+
+
+class BsDesc(ctypes.Structure):
+    _fields_ = [("rank", ctypes.c_int32), ("n_blocks", ctypes.c_int64),
+                ("dims", c_i64_p), ("begin", c_i64_p), ("end", c_i64_p),
+                ("strides", c_i64_p), ("offsets", c_i64_p)]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("out_rank", ctypes.c_int32), ("n_legs", ctypes.c_int32),
+                ("n_pairs", ctypes.c_int64), ("out_n_elem", ctypes.c_int64),
+                ("flops", ctypes.c_double), ("bytes", ctypes.c_double),
+                ("a_layout", ctypes.c_int32), ("b_layout", ctypes.c_int32)]
+
+
+class TorqueError(RuntimeError):
+    """Raised for a non-zero tq_status; the message keeps the reference's
+    "Error: " prefix (src/error.h:8-14)."""
+
+    def __init__(self, status, message):
+        super().__init__("Error: " + message)
+        self.status = status
+
+
+_lib = None
+
+
+def _proto(lib):
+    S = ctypes.c_int
+    V = ctypes.c_void_p
+    H = ctypes.c_void_p
+    sig = {
+        "tq_create": (S, [ctypes.POINTER(V), ctypes.c_int]),
+        "tq_destroy": (S, [H]),
+        "tq_set_stream": (S, [H, V]),
+        "tq_synchronize": (S, [H]),
+        "tq_status_string": (ctypes.c_char_p, [S]),
+        "tq_last_error": (ctypes.c_char_p, [H]),
+        "tq_malloc": (S, [H, ctypes.POINTER(V), ctypes.c_size_t]),
+        "tq_free": (S, [H, V]),
+        "tq_memcpy": (S, [H, V, V, ctypes.c_size_t, ctypes.c_int]),
+        "tq_memset_zero": (S, [H, V, ctypes.c_size_t]),
+        "tq_permute": (S, [H, S, ctypes.c_int32, c_i64_p, c_i64_p, c_i32_p, V, V]),
+        "tq_bs_permute": (S, [H, S, ctypes.POINTER(BsDesc), c_i32_p, V, V, c_i64_p]),
+        "tq_bs_contract_plan": (S, [H, ctypes.POINTER(BsDesc), ctypes.POINTER(BsDesc),
+                                    ctypes.c_int32, c_i32_p, S, S, ctypes.POINTER(V)]),
+        "tq_plan_get_info": (S, [V, ctypes.POINTER(PlanInfo)]),
+        "tq_plan_get_output": (S, [V, c_i64_p, c_i64_p, c_i64_p, c_i64_p, c_i64_p]),
+        "tq_plan_get_pairs": (S, [V, c_i64_p, c_i64_p, c_i64_p, c_i64_p, c_i64_p,
+                                  c_i64_p, c_i64_p]),
+        "tq_plan_get_worklist": (S, [V, c_i64_p]),
+        "tq_bs_contract_execute": (S, [V, V, V, V]),
+        "tq_plan_set_config": (S, [V, ctypes.c_int32]),
+        "tq_plan_destroy": (S, [V]),
+        "tq_gemm_config_count": (ctypes.c_int32, [S]),
+        "tq_gemm_config_describe": (S, [S, ctypes.c_int32, c_i32_p, c_i32_p, c_i32_p]),
+        "tq_dense_contract": (S, [H, S, ctypes.c_int32, c_i64_p, ctypes.c_int32, c_i64_p,
+                                  ctypes.c_int32, c_i32_p, V, V, V]),
+        "tq_dot_f64": (S, [H, ctypes.c_int64, V, V, ctypes.POINTER(ctypes.c_double)]),
+        "tq_nrm2_f64": (S, [H, ctypes.c_int64, V, ctypes.POINTER(ctypes.c_double)]),
+        "tq_scale_f64": (S, [H, ctypes.c_int64, V, ctypes.c_double]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def lib():
+    """The loaded library; raises if libtorque_hip.so is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                "libtorque_hip.so is not built (run torque_amd/build.py); "
+                "the HIP path has no fallback")
+        _lib = _proto(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def check(status, handle=None):
+    if status != TQ_SUCCESS:
+        L = lib()
+        msg = L.tq_status_string(status).decode()
+        if handle is not None:
+            extra = L.tq_last_error(handle).decode()
+            if extra and extra != msg:
+                msg = extra if extra.startswith(msg) else msg + " (" + extra + ")"
+        raise TorqueError(status, msg)

@@ -1,0 +1,152 @@
+// tq_common.h -- internal declarations shared by the HIP sources of
+// libtorque_hip.so (handle, error plumbing, device helpers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "torque_hip.h"
+
+using i64 = int64_t;
+using u64 = uint64_t;
+
+struct tq_handle_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n_cu = 256;
+  std::string last_error;
+  // pinned staging buffer for small host<->device transfers of metadata
+  void *pinned = nullptr;
+  size_t pinned_bytes = 0;
+};
+
+// Records an error message on the handle and returns the status.
+tq_status tq_fail(tq_handle_t h, tq_status s, const std::string &msg);
+
+// Checks a HIP call; on failure records the HIP error text and returns
+// TQ_ERROR_HIP from the enclosing function.
+#define TQ_HIP(h, expr)                                                     \
+  do {                                                                      \
+    hipError_t tq_e_ = (expr);                                              \
+    if (tq_e_ != hipSuccess)                                                \
+      return tq_fail((h), TQ_ERROR_HIP,                                     \
+                     std::string(#expr) + ": " + hipGetErrorString(tq_e_)); \
+  } while (0)
+
+#define TQ_CHECK(expr)                  \
+  do {                                  \
+    tq_status tq_s_ = (expr);           \
+    if (tq_s_ != TQ_SUCCESS) return tq_s_; \
+  } while (0)
+
+// Device buffer owned by a plan or a call; freed on destruction.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t b) {
+    if (p && bytes >= b) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (b == 0) b = 8;
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  template <class T>
+  T *as() const {
+    return reinterpret_cast<T *>(p);
+  }
+};
+
+// Pinned host staging for metadata uploads (grown on demand).
+tq_status tq_stage(tq_handle_t h, size_t bytes, void **host_ptr);
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) {
+  return (a + b - 1) / b;
+}
+
+// ---- permutation / gather jobs (tq_permute.hip) -------------------------
+
+// One strided-gather job: dst[dst_off + e] = src[src_off + sum_l d_l * s_l]
+// where e is the column-major linear index over dims (leg 0 fastest).
+struct CopyJob {
+  i64 src_off;
+  i64 dst_off;
+  i64 n_elem;
+  int32_t rank;
+  int32_t pad;
+  int32_t dims[TQ_MAX_RANK];
+  i64 strides[TQ_MAX_RANK];
+};
+
+// Runs a list of gather jobs that already sit in device memory. job_wg_start
+// is the exclusive prefix sum of workgroups per job (n_jobs + 1 entries).
+tq_status tq_launch_gather_jobs(tq_handle_t h, tq_dtype dt, const CopyJob *jobs,
+                                const int32_t *job_wg_start, int64_t n_jobs,
+                                int64_t total_wg, const void *src, void *dst);
+// Elements each workgroup of the gather kernel covers.
+constexpr int kGatherPerWG = 2048;
+
+// ---- grouped GEMM (tq_gemm.hip) ----------------------------------------
+
+// One matched pair as the grouped GEMM sees it: C(MxN) = A(MxK) B(NxK)^T.
+// flags: bit0 A is k-fast (element (m,k) at a_off + k + lda*m, else
+// a_off + m + lda*k); bit1 A comes from the packed buffer; bit2/bit3 the
+// same for B (n instead of m).
+struct GemmDesc {
+  i64 a_off;
+  i64 b_off;
+  i64 c_off;
+  int32_t m, n, k;
+  int32_t lda, ldb;
+  int32_t flags;
+};
+
+enum : int32_t {
+  kAKFast = 1,
+  kAPacked = 2,
+  kBKFast = 4,
+  kBPacked = 8,
+};
+
+struct GemmConfig {
+  int bm, bn, bk;
+};
+
+int tq_gemm_n_configs(tq_precision prec);
+GemmConfig tq_gemm_config(tq_precision prec, int cfg);
+
+// Launch the grouped GEMM over `tile_pair` (pair index per tile, in work-list
+// order) with tile_start[p] = first tile of pair p.
+tq_status tq_launch_grouped_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec,
+                                 int cfg, const GemmDesc *descs,
+                                 const int32_t *tile_pair,
+                                 const int32_t *tile_start, int64_t n_tiles,
+                                 const void *a, const void *a_packed,
+                                 const void *b, const void *b_packed, void *c);
+
+// ---- scans (tq_scan.hip) ----------------------------------------------
+
+// Exclusive prefix sum of n int64 values (in may equal out); writes the total
+// to *total_dev (device pointer, may be null). Uses `tmp` of
+// tq_scan_tmp_bytes(n) bytes.
+size_t tq_scan_tmp_bytes(int64_t n);
+tq_status tq_scan_i64(tq_handle_t h, const i64 *in, i64 *out, int64_t n,
+                      i64 *total_dev, void *tmp);
+
+// ---- reductions (tq_reduce.hip) ----------------------------------------
+
+// Deterministic FP64 sum of n values into *out (device pointer).
+tq_status tq_sum_f64_dev(tq_handle_t h, const double *x, int64_t n,
+                         double *out, void *tmp);
+size_t tq_sum_tmp_bytes(int64_t n);

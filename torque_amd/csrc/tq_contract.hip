@@ -1,0 +1,188 @@
+// tq_contract.hip -- execution of a contraction plan, the rank-0 (network
+// scalar) branch, tile-configuration choice, and the dense contraction.
+//
+// Execute = [pack A] -> [pack B] -> one grouped GEMM launch (or, for a
+// rank-0 output, one pair-dot launch and a deterministic FP64 sum). This
+// replaces the per-A-block sequence of src/gpu/block_sparse.cuh:693-1011
+// (reshape, cuTT, strided-batched GEMM, reshape, D2D copy, dot + add<<<1,1>>>).
+#include <algorithm>
+#include <cmath>
+#include <memory>
+
+#include "tq_common.h"
+#include "tq_plan.h"
+
+namespace {
+
+// One wavefront per pair: partial[p] = sum_k A_p[k] * B_p[k] in FP64, where
+// the operands are the pair's (M = N = 1) views read with the same layout
+// flags as the GEMM.
+template <class T>
+__global__ __launch_bounds__(256) void pair_dot_kernel(
+    const GemmDesc *__restrict__ descs, int64_t n_pairs,
+    const T *__restrict__ a, const T *__restrict__ a_packed,
+    const T *__restrict__ b, const T *__restrict__ b_packed,
+    double *__restrict__ partial) {
+  const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= n_pairs) return;
+  const int lane = threadIdx.x & 63;
+  const GemmDesc d = descs[p];
+  const T *pa = ((d.flags & kAPacked) ? a_packed : a) + d.a_off;
+  const T *pb = ((d.flags & kBPacked) ? b_packed : b) + d.b_off;
+  const i64 sa = (d.flags & kAKFast) ? 1 : d.lda;
+  const i64 sb = (d.flags & kBKFast) ? 1 : d.ldb;
+  double s = 0.0;
+  for (int k = lane; k < d.k; k += 64) s += (double)pa[sa * k] * (double)pb[sb * k];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) partial[p] = s;
+}
+
+template <class T>
+__global__ void store_scalar_kernel(const double *__restrict__ v, T *__restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = (T)v[0];
+}
+
+}  // namespace
+
+// Analytic choice of the tile configuration: models each tile as MFMA work
+// on the padded tile plus its operand and output traffic, and picks the
+// configuration with the least modelled time. The autotuner measures the
+// candidates instead and overrides this through tq_plan_set_config.
+int tq_plan_pick_config(tq_plan_t pl) {
+  const int nc = tq_gemm_n_configs(pl->prec);
+  const double es = pl->dtype == TQ_F64 ? 8.0 : 4.0;
+  const double peak = pl->dtype == TQ_F32 || pl->prec == TQ_PREC_F32 ? 157e12
+                      : pl->prec == TQ_PREC_BF16X3               ? 2.5e15 / 6
+                                                                   : 78.6e12;
+  const double bw = 8e12;
+  const double n_cu = pl->h ? pl->h->n_cu : 256;
+  int best = 0;
+  double best_t = 1e300;
+  for (int c = 0; c < nc; ++c) {
+    const GemmConfig g = tq_gemm_config(pl->prec, c);
+    double flop = 0, bytes = 0, tiles = 0;
+    for (int64_t p = 0; p < pl->n_pairs; ++p) {
+      const double tm = std::ceil((double)pl->h_m[p] / g.bm);
+      const double tn = std::ceil((double)pl->h_n[p] / g.bn);
+      const double kp = std::ceil((double)pl->h_k[p] / 4.0) * 4.0;
+      tiles += tm * tn;
+      flop += tm * tn * 2.0 * g.bm * g.bn * kp;
+      bytes += tm * tn * (g.bm + g.bn) * kp * es * 0.5 +
+               (double)pl->h_m[p] * pl->h_n[p] * es;
+    }
+    // tail effect: the last wave of tiles on 4 workgroups per CU
+    const double waves = std::ceil(tiles / (n_cu * 4));
+    const double t = std::max(flop / peak, bytes / bw) + waves * 2e-6;
+    if (t < best_t) {
+      best_t = t;
+      best = c;
+    }
+  }
+  return best;
+}
+
+extern "C" {
+
+tq_status tq_plan_set_config(tq_plan_t pl, int32_t config) {
+  if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  if (config < -1 || config >= tq_gemm_n_configs(pl->prec))
+    return TQ_ERROR_INVALID_VALUE;
+  pl->cfg = config;
+  return TQ_SUCCESS;
+}
+
+tq_status tq_bs_contract_execute(tq_plan_t pl, const void *a, const void *b,
+                                 void *c) {
+  if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  tq_handle_t h = pl->h;
+  if (!c) return TQ_ERROR_INVALID_VALUE;
+  const size_t es = pl->dtype == TQ_F64 ? 8 : 4;
+  if (pl->n_pairs == 0) {
+    if (pl->ro == 0) TQ_HIP(h, hipMemsetAsync(c, 0, es, h->stream));
+    return TQ_SUCCESS;
+  }
+  if (!a || !b) return TQ_ERROR_INVALID_VALUE;
+  if (pl->pack_a.mode != kModeDirect)
+    TQ_CHECK(tq_launch_gather_jobs(h, pl->dtype, pl->pack_a.jobs.as<CopyJob>(),
+                                   pl->pack_a.wg32.as<int32_t>(), pl->pack_a.n_jobs,
+                                   pl->pack_a.total_wg, a, pl->pack_a.panel.p));
+  if (pl->pack_b.mode != kModeDirect)
+    TQ_CHECK(tq_launch_gather_jobs(h, pl->dtype, pl->pack_b.jobs.as<CopyJob>(),
+                                   pl->pack_b.wg32.as<int32_t>(), pl->pack_b.n_jobs,
+                                   pl->pack_b.total_wg, b, pl->pack_b.panel.p));
+  if (pl->ro > 0) {
+    const int cfg = pl->cfg >= 0 ? pl->cfg : pl->auto_cfg;
+    TQ_CHECK(plan_build_tiles(pl, cfg));
+    return tq_launch_grouped_gemm(h, pl->dtype, pl->prec, cfg,
+                                  pl->desc.as<GemmDesc>(),
+                                  pl->tile_pair.as<int32_t>(),
+                                  pl->tile_first.as<int32_t>(), pl->n_tiles, a,
+                                  pl->pack_a.panel.p, b, pl->pack_b.panel.p, c);
+  }
+  // rank-0 output: the network scalar, reduced in FP64 on the GPU
+  const int64_t np = pl->n_pairs;
+  TQ_HIP(h, pl->partial.alloc(sizeof(double) * (np + 1)));
+  TQ_HIP(h, pl->red_tmp.alloc(tq_sum_tmp_bytes(np)));
+  const unsigned g = (unsigned)ceil_div(np, 4);
+  if (pl->dtype == TQ_F64) {
+    hipLaunchKernelGGL(pair_dot_kernel<double>, dim3(g), dim3(256), 0, h->stream,
+                       pl->desc.as<GemmDesc>(), np, (const double *)a,
+                       (const double *)pl->pack_a.panel.p, (const double *)b,
+                       (const double *)pl->pack_b.panel.p, pl->partial.as<double>());
+  } else {
+    hipLaunchKernelGGL(pair_dot_kernel<float>, dim3(g), dim3(256), 0, h->stream,
+                       pl->desc.as<GemmDesc>(), np, (const float *)a,
+                       (const float *)pl->pack_a.panel.p, (const float *)b,
+                       (const float *)pl->pack_b.panel.p, pl->partial.as<double>());
+  }
+  TQ_HIP(h, hipGetLastError());
+  double *sum = pl->partial.as<double>() + np;
+  TQ_CHECK(tq_sum_f64_dev(h, pl->partial.as<double>(), np, sum, pl->red_tmp.p));
+  if (pl->dtype == TQ_F64)
+    hipLaunchKernelGGL(store_scalar_kernel<double>, dim3(1), dim3(64), 0, h->stream,
+                       sum, (double *)c);
+  else
+    hipLaunchKernelGGL(store_scalar_kernel<float>, dim3(1), dim3(64), 0, h->stream,
+                       sum, (float *)c);
+  TQ_HIP(h, hipGetLastError());
+  return TQ_SUCCESS;
+}
+
+tq_status tq_dense_contract(tq_handle_t h, tq_dtype dtype, int32_t rank_a,
+                            const int64_t *dims_a, int32_t rank_b,
+                            const int64_t *dims_b, int32_t n_legs,
+                            const int32_t *legs, const void *a, const void *b,
+                            void *c) {
+  // A dense tensor is a block-sparse tensor with one block covering it all;
+  // the single matched pair is the whole GEMM (src/gpu/dense.cuh:439-555).
+  if (!h) return TQ_ERROR_NOT_INITIALIZED;
+  if (rank_a < 0 || rank_b < 0 || rank_a > TQ_MAX_RANK || rank_b > TQ_MAX_RANK)
+    return TQ_ERROR_UNSUPPORTED;
+  i64 ba[TQ_MAX_RANK] = {0}, ea[TQ_MAX_RANK], ta[TQ_MAX_RANK];
+  i64 bb[TQ_MAX_RANK] = {0}, eb[TQ_MAX_RANK], tb[TQ_MAX_RANK];
+  i64 off0 = 0;
+  i64 sa = 1, sb = 1;
+  for (int l = 0; l < rank_a; ++l) {
+    if (dims_a[l] <= 0) return TQ_ERROR_INVALID_VALUE;
+    ea[l] = dims_a[l] - 1;
+    ta[l] = sa;
+    sa *= dims_a[l];
+  }
+  for (int l = 0; l < rank_b; ++l) {
+    if (dims_b[l] <= 0) return TQ_ERROR_INVALID_VALUE;
+    eb[l] = dims_b[l] - 1;
+    tb[l] = sb;
+    sb *= dims_b[l];
+  }
+  tq_bs_desc da{rank_a, 1, dims_a, ba, ea, ta, &off0};
+  tq_bs_desc db{rank_b, 1, dims_b, bb, eb, tb, &off0};
+  tq_plan_t pl = nullptr;
+  TQ_CHECK(tq_bs_contract_plan(h, &da, &db, n_legs, legs, dtype, TQ_PREC_NATIVE, &pl));
+  std::unique_ptr<tq_plan_s> guard(pl);
+  TQ_CHECK(tq_bs_contract_execute(pl, a, b, c));
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  return TQ_SUCCESS;
+}
+
+}  // extern "C"

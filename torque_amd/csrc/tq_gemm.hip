@@ -1,0 +1,303 @@
+// tq_gemm.hip -- persistent grouped GEMM over all matched block pairs
+// (replaces the per-A-block cublas{S,D}gemmStridedBatched loop of
+// src/gpu/block_sparse.cuh:870-930 and the dense cublasXgemm of
+// src/gpu/dense.cuh:531-553).
+//
+// Every pair p computes C_p(MxN) = A_p(MxK) * B_p(NxK)^T, C_p column-major
+// at c + c_off (ldc = M) -- the reference's (N,T) product whose output is
+// already the column-major [A free..., B free...] block, so the reference's
+// reverse reshape disappears. Operands are read in place with either index
+// fastest, or from packed panels.
+//
+// One launch covers every tile of every pair: tile t belongs to pair
+// tile_pair[t]; tiles were laid out largest pair first. Workgroups are
+// persistent and walk the tile list with an XCD-aware stride so that
+// neighbouring tiles (sharing operand panels) land on the same L2.
+//
+// Arithmetic: FP64 on v_mfma_f64_16x16x4_f64; FP32 on v_mfma_f32_16x16x4_f32
+// (for FP32 data, or FP64 data at the FP32 tier); split-BF16 (three BF16
+// pieces per operand, six piece products on v_mfma_f32_16x16x32_bf16 with
+// FP32 accumulation) in tq_gemm_bf16.hip.
+#include "tq_common.h"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ inline d4 mfma(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline f4 mfma(float a, float b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <class TC>
+struct Acc;
+template <>
+struct Acc<double> {
+  typedef d4 type;
+};
+template <>
+struct Acc<float> {
+  typedef f4 type;
+};
+
+constexpr int kThreads = 256;
+constexpr int kBK = 16;
+
+// Loads one BM x kBK operand tile (rows = m or n) into registers.
+// m-fast: element (r, k) at p[r + ld*k]; thread t takes r = t % BM and
+//         k = t / BM + q * (256 / BM).
+// k-fast: element (r, k) at p[k + ld*r]; thread t takes k = t % 16 and
+//         r = t / 16 + 16 * q.
+template <class TIn, class TC, int BM>
+struct TileLoader {
+  static constexpr int kN = BM * kBK / kThreads;
+  TC v[kN];
+
+  __device__ inline void load(const TIn *__restrict__ base, bool kfast, int ld,
+                              int rows, int row0, int kcount, int k0) {
+    const int t = threadIdx.x;
+    if (!kfast) {
+      const int r = t % BM;
+      const int kb = t / BM;
+      constexpr int step = kThreads / BM;
+      const bool rok = row0 + r < rows;
+      const TIn *p = base + (row0 + r);
+#pragma unroll
+      for (int q = 0; q < kN; ++q) {
+        const int k = kb + q * step;
+        v[q] = (rok && k0 + k < kcount) ? (TC)p[(i64)ld * (k0 + k)] : (TC)0;
+      }
+    } else {
+      const int k = t % kBK;
+      const int rb = t / kBK;
+      const bool kok = k0 + k < kcount;
+      const TIn *p = base + (k0 + k);
+#pragma unroll
+      for (int q = 0; q < kN; ++q) {
+        const int r = rb + q * (kThreads / kBK);
+        v[q] = (kok && row0 + r < rows) ? (TC)p[(i64)ld * (row0 + r)] : (TC)0;
+      }
+    }
+  }
+
+  // store into LDS laid out [k][r] with row length BM + 1
+  __device__ inline void store(TC *lds, bool kfast) const {
+    const int t = threadIdx.x;
+    if (!kfast) {
+      const int r = t % BM;
+      const int kb = t / BM;
+      constexpr int step = kThreads / BM;
+#pragma unroll
+      for (int q = 0; q < kN; ++q) lds[(kb + q * step) * (BM + 1) + r] = v[q];
+    } else {
+      const int k = t % kBK;
+      const int rb = t / kBK;
+#pragma unroll
+      for (int q = 0; q < kN; ++q)
+        lds[k * (BM + 1) + rb + q * (kThreads / kBK)] = v[q];
+    }
+  }
+};
+
+// Maps a workgroup's launch index to a virtual index so that each XCD (the
+// hardware deals workgroups to the 8 XCDs round-robin) walks a contiguous
+// stretch of the tile list.
+__device__ inline int64_t xcd_virtual_id(int64_t bid, int64_t grid) {
+  const int64_t per = grid / 8;
+  if (per == 0 || grid % 8 != 0) return bid;
+  return (bid % 8) * per + bid / 8;
+}
+
+template <class TIn, class TC, class TOut, int BM, int BN>
+__global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
+    const GemmDesc *__restrict__ descs, const int32_t *__restrict__ tile_pair,
+    const int32_t *__restrict__ tile_first, int64_t n_tiles,
+    const TIn *__restrict__ a, const TIn *__restrict__ a_packed,
+    const TIn *__restrict__ b, const TIn *__restrict__ b_packed,
+    TOut *__restrict__ c) {
+  typedef typename Acc<TC>::type AccT;
+  constexpr int WM = BM / 2, WN = BN / 2;  // 2 x 2 wavefronts
+  constexpr int TM = WM / 16, TN = WN / 16;
+  __shared__ TC As[kBK * (BM + 1)];
+  __shared__ TC Bs[kBK * (BN + 1)];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
+  const int64_t vid = xcd_virtual_id(blockIdx.x, gridDim.x);
+
+  for (int64_t t = vid; t < n_tiles; t += gridDim.x) {
+    const int p = tile_pair[t];
+    const GemmDesc d = descs[p];
+    const int lt = (int)(t - tile_first[p]);
+    const int tiles_m = (d.m + BM - 1) / BM;
+    const int m0 = (lt % tiles_m) * BM;
+    const int n0 = (lt / tiles_m) * BN;
+    const bool a_kfast = d.flags & kAKFast;
+    const bool b_kfast = d.flags & kBKFast;
+    const TIn *pa = ((d.flags & kAPacked) ? a_packed : a) + d.a_off;
+    const TIn *pb = ((d.flags & kBPacked) ? b_packed : b) + d.b_off;
+
+    AccT acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = AccT{0, 0, 0, 0};
+
+    TileLoader<TIn, TC, BM> la;
+    TileLoader<TIn, TC, BN> lb;
+    la.load(pa, a_kfast, d.lda, d.m, m0, d.k, 0);
+    lb.load(pb, b_kfast, d.ldb, d.n, n0, d.k, 0);
+    for (int k0 = 0; k0 < d.k; k0 += kBK) {
+      __syncthreads();  // previous tile's LDS reads are done
+      la.store(As, a_kfast);
+      lb.store(Bs, b_kfast);
+      __syncthreads();
+      if (k0 + kBK < d.k) {  // prefetch the next k tile while computing
+        la.load(pa, a_kfast, d.lda, d.m, m0, d.k, k0 + kBK);
+        lb.load(pb, b_kfast, d.ldb, d.n, n0, d.k, k0 + kBK);
+      }
+      const int ksteps = min(kBK / 4, (d.k - k0 + 3) / 4);
+      for (int ks = 0; ks < ksteps; ++ks) {
+        const int kr = ks * 4 + (lane >> 4);
+        TC af[TM], bf[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+          af[i] = As[kr * (BM + 1) + wm0 + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          bf[j] = Bs[kr * (BN + 1) + wn0 + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) acc[i][j] = mfma(af[i], bf[j], acc[i][j]);
+      }
+    }
+    // epilogue: lane holds D[4*(lane/16) + r][lane % 16] of each 16x16 block
+    TOut *pc = c + d.c_off;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int n = n0 + wn0 + j * 16 + (lane & 15);
+      if (n >= d.n) continue;
+      TOut *col = pc + (i64)d.m * n;
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        const int mb = m0 + wm0 + i * 16 + 4 * (lane >> 4);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (mb + r < d.m) col[mb + r] = (TOut)acc[i][j][r];
+      }
+    }
+  }
+}
+
+struct Cfg {
+  int bm, bn;
+};
+constexpr Cfg kCfgs[] = {{64, 64}, {128, 64}, {64, 128}, {128, 128},
+                         {256, 32}, {32, 256}, {32, 32}};
+constexpr int kNCfg = sizeof(kCfgs) / sizeof(kCfgs[0]);
+
+template <class TIn, class TC, class TOut, int BM, int BN>
+tq_status launch_one(tq_handle_t h, const GemmDesc *descs,
+                     const int32_t *tile_pair, const int32_t *tile_first,
+                     int64_t n_tiles, const void *a, const void *ap,
+                     const void *b, const void *bp, void *c) {
+  // persistent grid: a few workgroups per CU, a multiple of the 8 XCDs
+  int64_t grid = std::min<int64_t>(n_tiles, (int64_t)h->n_cu * 4);
+  if (grid >= 8) grid -= grid % 8;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL((grouped_gemm_kernel<TIn, TC, TOut, BM, BN>),
+                     dim3((unsigned)grid), dim3(kThreads), 0, h->stream, descs,
+                     tile_pair, tile_first, n_tiles,
+                     reinterpret_cast<const TIn *>(a),
+                     reinterpret_cast<const TIn *>(ap),
+                     reinterpret_cast<const TIn *>(b),
+                     reinterpret_cast<const TIn *>(bp),
+                     reinterpret_cast<TOut *>(c));
+  TQ_HIP(h, hipGetLastError());
+  return TQ_SUCCESS;
+}
+
+template <class TIn, class TC, class TOut>
+tq_status dispatch(tq_handle_t h, int cfg, const GemmDesc *descs,
+                   const int32_t *tile_pair, const int32_t *tile_first,
+                   int64_t n_tiles, const void *a, const void *ap, const void *b,
+                   const void *bp, void *c) {
+#define TQ_CFG(I, BM, BN)                                                   \
+  case I:                                                                   \
+    return launch_one<TIn, TC, TOut, BM, BN>(h, descs, tile_pair, tile_first, \
+                                             n_tiles, a, ap, b, bp, c);
+  switch (cfg) {
+    TQ_CFG(0, 64, 64)
+    TQ_CFG(1, 128, 64)
+    TQ_CFG(2, 64, 128)
+    TQ_CFG(3, 128, 128)
+    TQ_CFG(4, 256, 32)
+    TQ_CFG(5, 32, 256)
+    TQ_CFG(6, 32, 32)
+  }
+#undef TQ_CFG
+  return tq_fail(h, TQ_ERROR_INVALID_VALUE, "gemm: unknown tile configuration");
+}
+
+}  // namespace
+
+tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, const GemmDesc *descs,
+                                const int32_t *tile_pair,
+                                const int32_t *tile_first, int64_t n_tiles,
+                                const void *a_split, const void *b_split,
+                                int64_t a_plane, int64_t b_plane, void *c);
+int tq_bf16x3_n_configs();
+GemmConfig tq_bf16x3_config(int cfg);
+
+int tq_gemm_n_configs(tq_precision prec) {
+  if (prec == TQ_PREC_BF16X3) return tq_bf16x3_n_configs();
+  return kNCfg;
+}
+
+GemmConfig tq_gemm_config(tq_precision prec, int cfg) {
+  if (prec == TQ_PREC_BF16X3) return tq_bf16x3_config(cfg);
+  if (cfg < 0 || cfg >= kNCfg) cfg = 0;
+  return GemmConfig{kCfgs[cfg].bm, kCfgs[cfg].bn, kBK};
+}
+
+tq_status tq_launch_grouped_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec,
+                                 int cfg, const GemmDesc *descs,
+                                 const int32_t *tile_pair,
+                                 const int32_t *tile_first, int64_t n_tiles,
+                                 const void *a, const void *a_packed,
+                                 const void *b, const void *b_packed, void *c) {
+  if (n_tiles <= 0) return TQ_SUCCESS;
+  if (prec == TQ_PREC_BF16X3)
+    return tq_fail(h, TQ_ERROR_UNSUPPORTED, "split-BF16 tier needs split panels");
+  if (dt == TQ_F32)
+    return dispatch<float, float, float>(h, cfg, descs, tile_pair, tile_first,
+                                         n_tiles, a, a_packed, b, b_packed, c);
+  if (prec == TQ_PREC_F32)
+    return dispatch<double, float, double>(h, cfg, descs, tile_pair, tile_first,
+                                           n_tiles, a, a_packed, b, b_packed, c);
+  return dispatch<double, double, double>(h, cfg, descs, tile_pair, tile_first,
+                                          n_tiles, a, a_packed, b, b_packed, c);
+}
+
+extern "C" {
+
+int32_t tq_gemm_config_count(tq_precision precision) {
+  return tq_gemm_n_configs(precision);
+}
+
+tq_status tq_gemm_config_describe(tq_precision precision, int32_t config,
+                                  int32_t *bm, int32_t *bn, int32_t *bk) {
+  if (config < 0 || config >= tq_gemm_n_configs(precision))
+    return TQ_ERROR_INVALID_VALUE;
+  GemmConfig c = tq_gemm_config(precision, config);
+  if (bm) *bm = c.bm;
+  if (bn) *bn = c.bn;
+  if (bk) *bk = c.bk;
+  return TQ_SUCCESS;
+}
+
+}  // extern "C"

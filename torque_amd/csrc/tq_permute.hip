@@ -1,0 +1,487 @@
+// tq_permute.hip -- tensor permutation kernels (replacing cuTT and the
+// reference's reshape_kernel gather/scatter, src/gpu/block_sparse.cu:9-64).
+//
+// Two kernels, both pure copies of the raw bits (bit-exact):
+//  * gather: each workgroup covers kGatherPerWG output elements of one job;
+//    output writes are coalesced, each source address comes from a
+//    column-major decode of the output index. Used for blocks and for
+//    permutations that keep the fastest leg in place.
+//  * tiled: 64x64 tiles through LDS for permutations that move the input's
+//    fastest leg. The tile's "output group" is the leading output legs and
+//    its "input group" the legs with the smallest input strides; offset
+//    tables for both groups are built per tile in LDS, so reads are
+//    coalesced along the input and writes along the output.
+// Semantics: out dims[k] = in dims[perm[k]]; out[n] = in[sum_k n_k *
+// in_stride[perm[k]]] (src/tensor/dense.h:325-366,
+// src/tensor/block_sparse.h:608-651).
+#include <algorithm>
+#include <numeric>
+
+#include "tq_common.h"
+
+namespace {
+
+constexpr int kGatherThreads = 256;
+constexpr int kPerThread = kGatherPerWG / kGatherThreads;  // 8
+
+template <class W>
+__device__ inline void gather_range(const CopyJob &job, int64_t e0,
+                                    const W *__restrict__ src,
+                                    W *__restrict__ dst) {
+  // elements e0 + q*256 + tid, q < 8
+  const int r = job.rank;
+#pragma unroll
+  for (int q = 0; q < kPerThread; ++q) {
+    const int64_t e = e0 + (int64_t)q * kGatherThreads + threadIdx.x;
+    if (e >= job.n_elem) break;
+    uint32_t rem = (uint32_t)e;
+    i64 off = job.src_off;
+    for (int l = 0; l < r; ++l) {
+      const uint32_t d = (uint32_t)job.dims[l];
+      const uint32_t qd = rem / d;
+      off += (i64)(rem - qd * d) * job.strides[l];
+      rem = qd;
+    }
+    dst[job.dst_off + e] = src[off];
+  }
+}
+
+// Persistent gather over a job list: chunk c of the whole list belongs to job
+// j with job_wg_start[j] <= c < job_wg_start[j+1]; the chunk count is read
+// from job_wg_start[n_jobs] so no host round trip is needed.
+template <class W>
+__global__ __launch_bounds__(kGatherThreads) void gather_jobs_kernel(
+    const CopyJob *__restrict__ jobs, const int32_t *__restrict__ job_wg_start,
+    int64_t n_jobs, const W *__restrict__ src, W *__restrict__ dst) {
+  const int32_t total = job_wg_start[n_jobs];
+  for (int32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    // binary search: last j with job_wg_start[j] <= c
+    int64_t lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+      int64_t mid = (lo + hi + 1) >> 1;
+      if (job_wg_start[mid] <= c)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const CopyJob &job = jobs[lo];
+    const int64_t e0 = (int64_t)(c - job_wg_start[lo]) * kGatherPerWG;
+    gather_range(job, e0, src, dst);
+  }
+}
+
+template <class W>
+__global__ __launch_bounds__(kGatherThreads) void gather_single_kernel(
+    CopyJob job, const W *__restrict__ src, W *__restrict__ dst) {
+  const int64_t e0 = (int64_t)blockIdx.x * kGatherPerWG;
+  gather_range(job, e0, src, dst);
+}
+
+// ---- tiled transpose -----------------------------------------------------
+
+constexpr int kT = 64;  // tile edge
+
+struct TileDesc {
+  i64 src_off, dst_off;
+  int32_t n_og, n_ig, n_b;       // legs in each group
+  int32_t og_dims[TQ_MAX_RANK];  // output legs 0..n_og-1 (contiguous output)
+  i64 og_in_str[TQ_MAX_RANK];
+  int32_t ig_dims[TQ_MAX_RANK];
+  i64 ig_in_str[TQ_MAX_RANK];
+  i64 ig_out_str[TQ_MAX_RANK];
+  int32_t b_dims[TQ_MAX_RANK];
+  i64 b_in_str[TQ_MAX_RANK];
+  i64 b_out_str[TQ_MAX_RANK];
+  i64 p_og, p_ig;  // extents of the two groups
+  i64 n_og_tiles, n_ig_tiles;
+  i64 n_tiles;
+};
+
+template <class W>
+__global__ __launch_bounds__(256) void tiled_permute_kernel(
+    TileDesc td, const W *__restrict__ src, W *__restrict__ dst) {
+  __shared__ W tile[kT][kT + 1];
+  __shared__ i64 in_og[kT];
+  __shared__ i64 in_ig[kT];
+  __shared__ i64 out_ig[kT];
+  for (i64 t = blockIdx.x; t < td.n_tiles; t += gridDim.x) {
+    i64 rest = t;
+    const i64 c_og = rest % td.n_og_tiles;
+    rest /= td.n_og_tiles;
+    const i64 c_ig = rest % td.n_ig_tiles;
+    i64 bidx = rest / td.n_ig_tiles;
+    i64 in_base = td.src_off, out_base = td.dst_off;
+    for (int l = 0; l < td.n_b; ++l) {
+      const i64 d = td.b_dims[l];
+      const i64 q = bidx / d;
+      const i64 dg = bidx - q * d;
+      in_base += dg * td.b_in_str[l];
+      out_base += dg * td.b_out_str[l];
+      bidx = q;
+    }
+    const i64 og0 = c_og * kT, ig0 = c_ig * kT;
+    const int tid = threadIdx.x;
+    if (tid < kT) {
+      i64 og = og0 + tid, off = 0;
+      if (og < td.p_og) {
+        for (int l = 0; l < td.n_og; ++l) {
+          const i64 d = td.og_dims[l];
+          const i64 q = og / d;
+          off += (og - q * d) * td.og_in_str[l];
+          og = q;
+        }
+      }
+      in_og[tid] = off;
+    } else if (tid < 2 * kT) {
+      const int u = tid - kT;
+      i64 ig = ig0 + u, oi = 0, oo = 0;
+      if (ig < td.p_ig) {
+        for (int l = 0; l < td.n_ig; ++l) {
+          const i64 d = td.ig_dims[l];
+          const i64 q = ig / d;
+          const i64 dg = ig - q * d;
+          oi += dg * td.ig_in_str[l];
+          oo += dg * td.ig_out_str[l];
+          ig = q;
+        }
+      }
+      in_ig[u] = oi;
+      out_ig[u] = oo;
+    }
+    __syncthreads();
+    const int n_og = (int)min<i64>(kT, td.p_og - og0);
+    const int n_ig = (int)min<i64>(kT, td.p_ig - ig0);
+    // read: input-group index fastest across lanes
+    {
+      const int u = tid & (kT - 1);
+      if (u < n_ig) {
+        const i64 iu = in_ig[u];
+        for (int r = tid >> 6; r < n_og; r += 4)
+          tile[r][u] = src[in_base + in_og[r] + iu];
+      }
+    }
+    __syncthreads();
+    // write: output-group index fastest across lanes (contiguous output)
+    {
+      const int r = tid & (kT - 1);
+      if (r < n_og) {
+        const i64 ob = out_base + og0 + r;
+        for (int u = tid >> 6; u < n_ig; u += 4)
+          dst[ob + out_ig[u]] = tile[r][u];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <class W>
+tq_status launch_gather_single(tq_handle_t h, const CopyJob &job,
+                               const void *src, void *dst) {
+  const int64_t nwg = ceil_div(job.n_elem, kGatherPerWG);
+  if (nwg == 0) return TQ_SUCCESS;
+  hipLaunchKernelGGL(gather_single_kernel<W>, dim3((unsigned)nwg),
+                     dim3(kGatherThreads), 0, h->stream, job,
+                     reinterpret_cast<const W *>(src),
+                     reinterpret_cast<W *>(dst));
+  TQ_HIP(h, hipGetLastError());
+  return TQ_SUCCESS;
+}
+
+size_t elem_size(tq_dtype dt) { return dt == TQ_F64 ? 8 : 4; }
+
+// Effective legs after dropping unit dims and fusing output-adjacent legs
+// that are also adjacent (same order, contiguous) in the input.
+struct Legs {
+  int r = 0;
+  i64 d[TQ_MAX_RANK];
+  i64 s[TQ_MAX_RANK];  // input stride of each output leg
+};
+
+Legs simplify(int rank, const i64 *dims, const i64 *in_str,
+              const int32_t *perm) {
+  Legs L;
+  for (int k = 0; k < rank; ++k) {
+    const i64 d = dims[perm[k]];
+    if (d == 1) continue;
+    const i64 s = in_str[perm[k]];
+    if (L.r > 0 && L.s[L.r - 1] * L.d[L.r - 1] == s) {
+      L.d[L.r - 1] *= d;
+    } else {
+      L.d[L.r] = d;
+      L.s[L.r] = s;
+      ++L.r;
+    }
+  }
+  return L;
+}
+
+template <class W>
+tq_status dense_permute_t(tq_handle_t h, const Legs &L, i64 n_total,
+                          const void *in, void *out) {
+  if (n_total == 0) return TQ_SUCCESS;
+  if (L.r == 0 || (L.r == 1 && L.s[0] == 1)) {
+    TQ_HIP(h, hipMemcpyAsync(out, in, (size_t)n_total * sizeof(W),
+                             hipMemcpyDeviceToDevice, h->stream));
+    return TQ_SUCCESS;
+  }
+  // input-fastest output leg
+  int f = 0;
+  for (int k = 1; k < L.r; ++k)
+    if (L.s[k] < L.s[f]) f = k;
+  if (f == 0 || n_total < 4096) {
+    if (n_total >= (i64(1) << 32))
+      return tq_fail(h, TQ_ERROR_UNSUPPORTED, "permute: tensor above 2^32 elements");
+    CopyJob job{};
+    job.src_off = 0;
+    job.dst_off = 0;
+    job.n_elem = n_total;
+    job.rank = L.r;
+    for (int k = 0; k < L.r; ++k) {
+      job.dims[k] = (int32_t)L.d[k];
+      job.strides[k] = L.s[k];
+    }
+    return launch_gather_single<W>(h, job, in, out);
+  }
+  TileDesc td{};
+  i64 ostr[TQ_MAX_RANK];
+  ostr[0] = 1;
+  for (int k = 1; k < L.r; ++k) ostr[k] = ostr[k - 1] * L.d[k - 1];
+  bool used[TQ_MAX_RANK] = {false};
+  // output group: leading output legs (stop before the input-fastest leg)
+  i64 p_og = 1;
+  int a = 0;
+  while (a < f && p_og < kT) {
+    td.og_dims[td.n_og] = (int32_t)L.d[a];
+    td.og_in_str[td.n_og] = L.s[a];
+    ++td.n_og;
+    p_og *= L.d[a];
+    used[a] = true;
+    ++a;
+  }
+  // input group: legs by increasing input stride, starting at f
+  i64 p_ig = 1;
+  while (p_ig < kT) {
+    int best = -1;
+    for (int k = 0; k < L.r; ++k)
+      if (!used[k] && (best < 0 || L.s[k] < L.s[best])) best = k;
+    if (best < 0) break;
+    used[best] = true;
+    td.ig_dims[td.n_ig] = (int32_t)L.d[best];
+    td.ig_in_str[td.n_ig] = L.s[best];
+    td.ig_out_str[td.n_ig] = ostr[best];
+    ++td.n_ig;
+    p_ig *= L.d[best];
+  }
+  i64 n_b = 1;
+  for (int k = 0; k < L.r; ++k) {
+    if (used[k]) continue;
+    td.b_dims[td.n_b] = (int32_t)L.d[k];
+    td.b_in_str[td.n_b] = L.s[k];
+    td.b_out_str[td.n_b] = ostr[k];
+    ++td.n_b;
+    n_b *= L.d[k];
+  }
+  td.p_og = p_og;
+  td.p_ig = p_ig;
+  td.n_og_tiles = ceil_div(p_og, kT);
+  td.n_ig_tiles = ceil_div(p_ig, kT);
+  td.n_tiles = td.n_og_tiles * td.n_ig_tiles * n_b;
+  const i64 grid = std::min<i64>(td.n_tiles, (i64)h->n_cu * 16);
+  hipLaunchKernelGGL(tiled_permute_kernel<W>, dim3((unsigned)grid), dim3(256),
+                     0, h->stream, td, reinterpret_cast<const W *>(in),
+                     reinterpret_cast<W *>(out));
+  TQ_HIP(h, hipGetLastError());
+  return TQ_SUCCESS;
+}
+
+bool valid_perm(int rank, const int32_t *perm) {
+  bool seen[TQ_MAX_RANK] = {false};
+  for (int k = 0; k < rank; ++k) {
+    if (perm[k] < 0 || perm[k] >= rank || seen[perm[k]]) return false;
+    seen[perm[k]] = true;
+  }
+  return true;
+}
+
+// Builds one gather job per block of a block-sparse tensor (device side):
+// dims/strides permuted by perm, destination offsets from the scan of sizes.
+__global__ void bs_permute_jobs_kernel(int32_t rank, int64_t n_blocks,
+                                       const i64 *__restrict__ begin,
+                                       const i64 *__restrict__ end,
+                                       const i64 *__restrict__ strides,
+                                       const i64 *__restrict__ offsets,
+                                       const i64 *__restrict__ dst_offsets,
+                                       const int32_t *__restrict__ perm,
+                                       CopyJob *__restrict__ jobs,
+                                       i64 *__restrict__ wg_count) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_blocks) return;
+  CopyJob j{};
+  j.src_off = offsets[b];
+  j.dst_off = dst_offsets[b];
+  j.rank = rank;
+  i64 n = 1;
+  for (int k = 0; k < rank; ++k) {
+    const int l = perm[k];
+    const i64 d = end[b * rank + l] - begin[b * rank + l] + 1;
+    j.dims[k] = (int32_t)d;
+    j.strides[k] = strides[b * rank + l];
+    n *= d;
+  }
+  j.n_elem = n;
+  jobs[b] = j;
+  wg_count[b] = ceil_div(n, kGatherPerWG);
+}
+
+__global__ void block_sizes_kernel(int32_t rank, int64_t n_blocks,
+                                   const i64 *__restrict__ begin,
+                                   const i64 *__restrict__ end,
+                                   i64 *__restrict__ sizes) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_blocks) return;
+  i64 n = 1;
+  for (int l = 0; l < rank; ++l)
+    n *= end[b * rank + l] - begin[b * rank + l] + 1;
+  sizes[b] = n;
+}
+
+__global__ void i64_to_i32_kernel(const i64 *__restrict__ in,
+                                  int32_t *__restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (int32_t)in[i];
+}
+
+}  // namespace
+
+tq_status tq_launch_gather_jobs(tq_handle_t h, tq_dtype dt, const CopyJob *jobs,
+                                const int32_t *job_wg_start, int64_t n_jobs,
+                                int64_t total_wg, const void *src, void *dst) {
+  if (n_jobs <= 0) return TQ_SUCCESS;
+  i64 grid = std::max<i64>(1, std::min<i64>(total_wg, (i64)h->n_cu * 8));
+  if (dt == TQ_F64)
+    hipLaunchKernelGGL(gather_jobs_kernel<uint64_t>, dim3((unsigned)grid),
+                       dim3(kGatherThreads), 0, h->stream, jobs, job_wg_start,
+                       n_jobs, reinterpret_cast<const uint64_t *>(src),
+                       reinterpret_cast<uint64_t *>(dst));
+  else
+    hipLaunchKernelGGL(gather_jobs_kernel<uint32_t>, dim3((unsigned)grid),
+                       dim3(kGatherThreads), 0, h->stream, jobs, job_wg_start,
+                       n_jobs, reinterpret_cast<const uint32_t *>(src),
+                       reinterpret_cast<uint32_t *>(dst));
+  TQ_HIP(h, hipGetLastError());
+  return TQ_SUCCESS;
+}
+
+extern "C" {
+
+tq_status tq_permute(tq_handle_t h, tq_dtype dtype, int32_t rank,
+                     const int64_t *dims, const int64_t *in_strides,
+                     const int32_t *perm, const void *in, void *out) {
+  if (!h) return TQ_ERROR_NOT_INITIALIZED;
+  if (rank < 0 || rank > TQ_MAX_RANK)
+    return tq_fail(h, TQ_ERROR_UNSUPPORTED, "permute: rank above TQ_MAX_RANK");
+  if (dtype != TQ_F32 && dtype != TQ_F64) return TQ_ERROR_UNSUPPORTED;
+  if ((rank > 0 && (!dims || !perm)) || !in || !out)
+    return TQ_ERROR_INVALID_VALUE;
+  if (!valid_perm(rank, perm))
+    return tq_fail(h, TQ_ERROR_RANK_MISMATCH,
+                   "The number of permutation does not match the rank of tensor");
+  i64 table[TQ_MAX_RANK];
+  i64 n_total = 1;
+  for (int l = 0; l < rank; ++l) {
+    if (dims[l] < 0) return TQ_ERROR_INVALID_VALUE;
+    table[l] = n_total;
+    n_total *= dims[l];
+  }
+  const i64 *s = in_strides ? in_strides : table;
+  Legs L = simplify(rank, dims, s, perm);
+  if (dtype == TQ_F64) return dense_permute_t<uint64_t>(h, L, n_total, in, out);
+  return dense_permute_t<uint32_t>(h, L, n_total, in, out);
+}
+
+tq_status tq_bs_permute(tq_handle_t h, tq_dtype dtype, const tq_bs_desc *d,
+                        const int32_t *perm, const void *in, void *out,
+                        int64_t *out_offsets) {
+  if (!h) return TQ_ERROR_NOT_INITIALIZED;
+  if (!d) return TQ_ERROR_INVALID_VALUE;
+  const int r = d->rank;
+  if (r < 0 || r > TQ_MAX_RANK) return TQ_ERROR_UNSUPPORTED;
+  if (dtype != TQ_F32 && dtype != TQ_F64) return TQ_ERROR_UNSUPPORTED;
+  if (r > 0 && !perm) return TQ_ERROR_INVALID_VALUE;
+  if (!valid_perm(r, perm))
+    return tq_fail(h, TQ_ERROR_RANK_MISMATCH,
+                   "The number of permutation does not match the rank of tensor");
+  const i64 nb = d->n_blocks;
+  // host-side offsets of the output blocks (nest_sum of the block sizes,
+  // src/gpu/block_sparse.cuh:1113-1117 via the constructor at :189-230)
+  if (out_offsets) {
+    i64 acc = 0;
+    for (i64 b = 0; b < nb; ++b) {
+      i64 n = 1;
+      for (int l = 0; l < r; ++l) n *= d->end[b * r + l] - d->begin[b * r + l] + 1;
+      out_offsets[b] = acc;
+      acc += n;
+    }
+  }
+  if (nb == 0) return TQ_SUCCESS;
+  if (!in || !out || !d->begin || !d->end || !d->strides || !d->offsets)
+    return TQ_ERROR_INVALID_VALUE;
+  if (r == 0) {
+    // rank-0 blocks hold one element each
+    TQ_HIP(h, hipMemcpyAsync(out, in, (size_t)nb * (dtype == TQ_F64 ? 8 : 4),
+                             hipMemcpyDeviceToDevice, h->stream));
+    return TQ_SUCCESS;
+  }
+  // one upload: begin, end, strides (nb*r each), offsets (nb), perm (r)
+  const size_t meta = (size_t)nb * r * 3 + nb;
+  const size_t bytes = meta * 8 + 64;
+  void *host = nullptr;
+  TQ_CHECK(tq_stage(h, bytes, &host));
+  i64 *hp = reinterpret_cast<i64 *>(host);
+  std::copy(d->begin, d->begin + nb * r, hp);
+  std::copy(d->end, d->end + nb * r, hp + nb * r);
+  std::copy(d->strides, d->strides + nb * r, hp + 2 * nb * r);
+  std::copy(d->offsets, d->offsets + nb, hp + 3 * nb * r);
+  int32_t *hperm = reinterpret_cast<int32_t *>(hp + meta);
+  std::copy(perm, perm + r, hperm);
+  DevBuf dmeta, djobs, dsz, dwg, dwg32, dtmp;
+  TQ_HIP(h, dmeta.alloc(bytes));
+  TQ_HIP(h, hipMemcpyAsync(dmeta.p, host, bytes, hipMemcpyHostToDevice, h->stream));
+  i64 *dbeg = dmeta.as<i64>(), *dend = dbeg + nb * r, *dstr = dend + nb * r,
+      *doff = dstr + nb * r;
+  const int32_t *dperm = reinterpret_cast<const int32_t *>(doff + nb);
+  TQ_HIP(h, djobs.alloc(sizeof(CopyJob) * nb));
+  TQ_HIP(h, dsz.alloc(sizeof(i64) * (nb + 1)));
+  TQ_HIP(h, dwg.alloc(sizeof(i64) * (nb + 1)));
+  TQ_HIP(h, dwg32.alloc(sizeof(int32_t) * (nb + 1)));
+  TQ_HIP(h, dtmp.alloc(tq_scan_tmp_bytes(nb + 1)));
+  const unsigned g = (unsigned)ceil_div(nb, 256);
+  hipLaunchKernelGGL(block_sizes_kernel, dim3(g), dim3(256), 0, h->stream, r, nb,
+                     dbeg, dend, dsz.as<i64>());
+  TQ_CHECK(tq_scan_i64(h, dsz.as<i64>(), dsz.as<i64>(), nb, nullptr, dtmp.p));
+  hipLaunchKernelGGL(bs_permute_jobs_kernel, dim3(g), dim3(256), 0, h->stream, r,
+                     nb, dbeg, dend, dstr, doff, dsz.as<i64>(), dperm,
+                     djobs.as<CopyJob>(), dwg.as<i64>());
+  TQ_CHECK(tq_scan_i64(h, dwg.as<i64>(), dwg.as<i64>(), nb,
+                       dwg.as<i64>() + nb, dtmp.p));
+  hipLaunchKernelGGL(i64_to_i32_kernel, dim3((unsigned)ceil_div(nb + 1, 256)),
+                     dim3(256), 0, h->stream, dwg.as<i64>(), dwg32.as<int32_t>(),
+                     nb + 1);
+  TQ_HIP(h, hipGetLastError());
+  // total chunk count on the host only bounds the grid size
+  i64 total_elem = 0;
+  for (i64 b = 0; b < nb; ++b) {
+    i64 n = 1;
+    for (int l = 0; l < r; ++l) n *= d->end[b * r + l] - d->begin[b * r + l] + 1;
+    if (n >= (i64(1) << 32))
+      return tq_fail(h, TQ_ERROR_UNSUPPORTED, "permute: block above 2^32 elements");
+    total_elem += ceil_div(n, kGatherPerWG);
+  }
+  TQ_CHECK(tq_launch_gather_jobs(h, dtype, djobs.as<CopyJob>(),
+                                 dwg32.as<int32_t>(), nb, total_elem, in, out));
+  // the temporaries are freed on return: wait for the kernels using them
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  return TQ_SUCCESS;
+}
+
+}  // extern "C"

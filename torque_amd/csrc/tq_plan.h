@@ -1,0 +1,109 @@
+// tq_plan.h -- the contraction plan object and the argument blocks of the
+// planning kernels (internal).
+#pragma once
+
+#include "tq_common.h"
+
+struct LegList {
+  int32_t a[TQ_MAX_RANK];
+  int32_t b[TQ_MAX_RANK];
+};
+
+struct GeomArgs {
+  int64_t n_pairs;
+  int32_t a_rank, b_rank, nl;
+  const i64 *a_begin, *a_end, *a_str, *a_off;
+  const i64 *b_begin, *b_end, *b_str, *b_off;
+  LegList legs;
+  int32_t a_free[TQ_MAX_RANK];
+  int32_t b_free[TQ_MAX_RANK];
+};
+
+struct PanelArgs {
+  int64_t n_blocks;
+  int32_t rank;
+  const i64 *begin, *end, *strides, *offsets;
+  int32_t order[TQ_MAX_RANK];  // [free..., contracted in pair order...]
+};
+
+struct BoxArgs {
+  int64_t n_pairs;
+  int32_t is_b;
+  int32_t self_rank, other_rank, nl, n_free;
+  const i64 *self_begin, *self_end, *other_begin, *other_end;
+  int32_t free_legs[TQ_MAX_RANK];
+  int32_t self_legs[TQ_MAX_RANK];
+  int32_t other_legs[TQ_MAX_RANK];
+};
+
+// Flags OR-ed over all pairs by the geometry kernel.
+enum : int32_t {
+  kNeedPackA = 1,       // some pair's A view is not a plain matrix
+  kNeedPackB = 2,
+  kNeedPairPanelA = 4,  // some pair's A sub-box is not plain inside a block panel
+  kNeedPairPanelB = 8,
+  kTooLarge = 16,
+};
+
+enum { kModeDirect = 0, kModeBlockPanel = 1, kModePairPanel = 2 };
+
+struct PlanUpload {
+  const i64 *a_begin = nullptr, *a_end = nullptr, *a_str = nullptr, *a_off = nullptr;
+  const i64 *b_begin = nullptr, *b_end = nullptr, *b_str = nullptr, *b_off = nullptr;
+};
+
+struct PackSide {
+  int mode = kModeDirect;
+  DevBuf panel;      // packed operand
+  DevBuf panel_off;  // per block or per pair
+  DevBuf jobs;       // CopyJob list
+  DevBuf wg;         // int64 workgroup prefix (scratch)
+  DevBuf wg32;       // int32 workgroup prefix used by the gather kernel
+  int64_t n_jobs = 0;
+  int64_t total_wg = 0;
+  int64_t panel_elems = 0;
+};
+
+struct tq_plan_s {
+  tq_handle_t h = nullptr;
+  tq_dtype dtype = TQ_F64;
+  tq_precision prec = TQ_PREC_NATIVE;
+  int ra = 0, rb = 0, ro = 0, nl = 0;
+  int64_t na = 0, nb = 0;
+  int32_t legs_a[TQ_MAX_RANK] = {0}, legs_b[TQ_MAX_RANK] = {0};
+  int32_t a_free[TQ_MAX_RANK] = {0}, b_free[TQ_MAX_RANK] = {0};
+  std::vector<i64> out_dims;
+  int64_t n_pairs = 0;
+  int64_t out_n_elem = 1;
+  double flops = 0.0, bytes = 0.0;
+  int a_layout = 0, b_layout = 0;
+
+  DevBuf meta;       // uploaded A/B metadata
+  PlanUpload up;     // device pointers into meta
+  DevBuf pair_a, pair_b;  // int32 per pair
+  DevBuf sub_off;    // int64 [2*n_pairs]: A and B sub-block offsets
+  DevBuf desc;       // GemmDesc per pair
+  DevBuf out_be;     // out begin then out end (device)
+  DevBuf c_off;      // int64 [n_pairs+1]
+  DevBuf a_used, b_used;  // uint8 per block
+  DevBuf order;      // int64 work list (pair ids, largest cost first)
+  PackSide pack_a, pack_b;
+
+  // grouped-GEMM tile list for the current configuration
+  DevBuf tile_pair, tile_first;
+  int64_t n_tiles = 0;
+  int tiles_cfg = -1;
+  bool tiles_ready = false;
+  int cfg = -1;       // user-selected configuration (-1 = automatic)
+  int auto_cfg = 0;
+
+  // rank-0 reduction scratch
+  DevBuf partial, red_tmp;
+
+  // host copies
+  std::vector<i64> h_m, h_n, h_k;
+  std::vector<i64> h_out_begin, h_out_end, h_c_off;
+};
+
+tq_status plan_build_tiles(tq_plan_t pl, int cfg);
+int tq_plan_pick_config(tq_plan_t pl);

@@ -1,0 +1,1059 @@
+// tq_plan.hip -- block-sparse contraction planning on the device.
+//
+// Restates, as device kernels, the host metadata work of the reference's GPU
+// contract (src/gpu/block_sparse.cuh:570-732) and of
+// block_sparse::block_in_range (src/tensor/block_sparse.cpp:6-175):
+//   1. match:   one wavefront per (A block, slice of B blocks); lane j tests
+//               B block j for a non-empty intersection on every contracted
+//               leg pair (max(begin) <= min(end), block_sparse.cpp:42-60) and
+//               a ballot packs the matches. Counting pass, exclusive scan,
+//               then a fill pass that writes pairs at scanned positions with
+//               the ballot's lane prefix, so pairs come out ordered by A block
+//               then ascending B block (block_sparse.cuh:626-652).
+//   2. geometry: per pair, output begin/end (A free ++ B free,
+//               block_sparse.cpp:66-119), GEMM shape M, N, K, the A and B
+//               sub-block offsets (block_sparse.cuh:700-732), whether each
+//               operand can be read in place, and the output block offsets
+//               (exclusive scan of M*N, block_sparse.cuh:667-671).
+//   3. pack jobs for operands that must be permuted to GEMM layout.
+//   4. work list: a stable device counting sort of the pairs by estimated
+//               cost (largest first) using wavefront ballots to rank equal
+//               keys, then the tile list of the grouped GEMM.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+#include "tq_common.h"
+#include "tq_plan.h"
+
+namespace {
+
+constexpr int kWave = 64;
+
+struct MatchArgs {
+  int32_t nl;
+  int32_t a_rank;
+  int64_t n_a, n_b;
+  int32_t chunks_per_split;  // 64-block chunks each wavefront scans
+  int64_t n_split;
+  const i64 *a_begin, *a_end;      // [n_a * a_rank]
+  const int32_t *b_lo, *b_hi;      // [nl * n_b]  contracted legs, SoA
+  int32_t a_legs[TQ_MAX_RANK];
+};
+
+__device__ inline uint64_t lanemask_lt() {
+  const int lane = threadIdx.x & 63;
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Predicate of block_in_range for (A block i, B block j).
+__device__ inline bool block_match(const MatchArgs &g, const i64 *alo,
+                                   const i64 *ahi, int64_t j) {
+  bool ok = true;
+  for (int k = 0; k < g.nl; ++k) {
+    const i64 lo = max(alo[k], (i64)g.b_lo[(int64_t)k * g.n_b + j]);
+    const i64 hi = min(ahi[k], (i64)g.b_hi[(int64_t)k * g.n_b + j]);
+    ok = ok && (lo <= hi);
+  }
+  return ok;
+}
+
+// kFill = false: counts[item] = number of matches of the item's B slice.
+// kFill = true:  writes pair_a/pair_b at starts[item] + rank.
+template <bool kFill>
+__global__ __launch_bounds__(256) void match_kernel(
+    MatchArgs g, i64 *__restrict__ counts, const i64 *__restrict__ starts,
+    int32_t *__restrict__ pair_a, int32_t *__restrict__ pair_b,
+    uint8_t *__restrict__ a_used, uint8_t *__restrict__ b_used) {
+  const int64_t item = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (item >= g.n_a * g.n_split) return;  // whole wavefront exits together
+  const int64_t i = item / g.n_split;
+  const int64_t s = item - i * g.n_split;
+  const int lane = threadIdx.x & 63;
+  i64 alo[TQ_MAX_RANK], ahi[TQ_MAX_RANK];
+  for (int k = 0; k < g.nl; ++k) {
+    alo[k] = g.a_begin[i * g.a_rank + g.a_legs[k]];
+    ahi[k] = g.a_end[i * g.a_rank + g.a_legs[k]];
+  }
+  const int64_t j0 = s * (int64_t)g.chunks_per_split * kWave;
+  int64_t pos = kFill ? starts[item] : 0;
+  i64 cnt = 0;
+  for (int c = 0; c < g.chunks_per_split; ++c) {
+    const int64_t j = j0 + (int64_t)c * kWave + lane;
+    const bool m = (j < g.n_b) && block_match(g, alo, ahi, j);
+    const uint64_t mask = __ballot(m);
+    if (kFill) {
+      if (m) {
+        const int64_t p = pos + __popcll(mask & lanemask_lt());
+        pair_a[p] = (int32_t)i;
+        pair_b[p] = (int32_t)j;
+        b_used[j] = 1;
+      }
+      if (mask && lane == 0) a_used[i] = 1;
+      pos += __popcll(mask);
+    } else {
+      cnt += __popcll(mask);
+    }
+  }
+  if (!kFill && lane == 0) counts[item] = cnt;
+}
+
+__global__ void soa_legs_kernel(int64_t n_b, int32_t b_rank, int32_t nl,
+                                const i64 *__restrict__ b_begin,
+                                const i64 *__restrict__ b_end, LegList legs,
+                                int32_t *__restrict__ lo,
+                                int32_t *__restrict__ hi) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_b) return;
+  for (int k = 0; k < nl; ++k) {
+    lo[(int64_t)k * n_b + j] = (int32_t)b_begin[j * b_rank + legs.b[k]];
+    hi[(int64_t)k * n_b + j] = (int32_t)b_end[j * b_rank + legs.b[k]];
+  }
+}
+
+// Layout test of one operand view: legs listed [free..., contracted...] with
+// view dims vd and strides vs; returns the leading dimension if the view is
+// a plain matrix with the free index fastest (kfast = false) or the
+// contracted index fastest (kfast = true), or -1.
+__device__ inline i64 plain_ld(int nf, int nl, const i64 *vd, const i64 *vs,
+                               bool kfast, i64 rows, i64 cols) {
+  // fast group: must be a unit-stride chain; slow group: chain from ld
+  const int f0 = kfast ? nf : 0, f1 = kfast ? nf + nl : nf;
+  const int s0 = kfast ? 0 : nf, s1 = kfast ? nf : nf + nl;
+  i64 expect = 1;
+  for (int q = f0; q < f1; ++q) {
+    if (vd[q] == 1) continue;
+    if (vs[q] != expect) return -1;
+    expect *= vd[q];
+  }
+  i64 ld = -1;
+  for (int q = s0; q < s1; ++q) {
+    if (vd[q] == 1) continue;
+    if (ld < 0) {
+      ld = vs[q];
+      expect = vs[q] * vd[q];
+    } else {
+      if (vs[q] != expect) return -1;
+      expect *= vd[q];
+    }
+  }
+  const i64 fast_extent = kfast ? cols : rows;
+  if (ld < 0) ld = fast_extent > 0 ? fast_extent : 1;
+  if (ld < fast_extent && (kfast ? rows : cols) > 1) return -1;
+  return ld;
+}
+
+__global__ void geometry_kernel(GeomArgs g, const int32_t *__restrict__ pair_a,
+                                const int32_t *__restrict__ pair_b,
+                                GemmDesc *__restrict__ desc,
+                                i64 *__restrict__ sub_off,
+                                i64 *__restrict__ out_begin,
+                                i64 *__restrict__ out_end,
+                                i64 *__restrict__ mn, i64 *__restrict__ cost,
+                                int32_t *__restrict__ flags_any) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n_pairs) return;
+  const int64_t i = pair_a[p], j = pair_b[p];
+  const int ra = g.a_rank, rb = g.b_rank, nl = g.nl;
+  const i64 *ab = g.a_begin + i * ra, *ae = g.a_end + i * ra,
+            *as = g.a_str + i * ra;
+  const i64 *bb = g.b_begin + j * rb, *be = g.b_end + j * rb,
+            *bs = g.b_str + j * rb;
+  i64 avd[TQ_MAX_RANK], avs[TQ_MAX_RANK], bvd[TQ_MAX_RANK], bvs[TQ_MAX_RANK];
+  i64 a_off = g.a_off[i], b_off = g.b_off[j];
+  i64 M = 1, N = 1, K = 1;
+  const int naf = ra - nl, nbf = rb - nl;
+  const int ro = naf + nbf;
+  for (int q = 0; q < naf; ++q) {
+    const int l = g.a_free[q];
+    avd[q] = ae[l] - ab[l] + 1;
+    avs[q] = as[l];
+    M *= avd[q];
+    out_begin[p * ro + q] = ab[l];
+    out_end[p * ro + q] = ae[l];
+  }
+  for (int q = 0; q < nbf; ++q) {
+    const int l = g.b_free[q];
+    bvd[q] = be[l] - bb[l] + 1;
+    bvs[q] = bs[l];
+    N *= bvd[q];
+    out_begin[p * ro + naf + q] = bb[l];
+    out_end[p * ro + naf + q] = be[l];
+  }
+  bool a_full = true, b_full = true;
+  bool a_box = true, b_box = true;  // sub-box of the block panel is plain
+  int last_nonunit = -1;
+  for (int k = 0; k < nl; ++k) {
+    const int la = g.legs.a[k], lb = g.legs.b[k];
+    const i64 lo = max(ab[la], bb[lb]);
+    const i64 hi = min(ae[la], be[lb]);
+    const i64 cd = hi - lo + 1;
+    a_off += (lo - ab[la]) * as[la];
+    b_off += (lo - bb[lb]) * bs[lb];
+    avd[naf + k] = cd;
+    avs[naf + k] = as[la];
+    bvd[nbf + k] = cd;
+    bvs[nbf + k] = bs[lb];
+    K *= cd;
+    a_full = a_full && (cd == ae[la] - ab[la] + 1);
+    b_full = b_full && (cd == be[lb] - bb[lb] + 1);
+    if (cd != 1) last_nonunit = k;
+  }
+  for (int k = 0; k < last_nonunit; ++k) {
+    const int la = g.legs.a[k], lb = g.legs.b[k];
+    const i64 cd = avd[naf + k];
+    if (cd == 1) continue;
+    a_box = a_box && (cd == ae[la] - ab[la] + 1);
+    b_box = b_box && (cd == be[lb] - bb[lb] + 1);
+  }
+  sub_off[2 * p] = a_off;
+  sub_off[2 * p + 1] = b_off;
+  GemmDesc d;
+  d.a_off = a_off;
+  d.b_off = b_off;
+  d.c_off = 0;
+  d.m = (int32_t)M;
+  d.n = (int32_t)N;
+  d.k = (int32_t)K;
+  d.flags = 0;
+  int32_t need = 0;
+  i64 lda = plain_ld(naf, nl, avd, avs, false, M, K);
+  if (lda < 0) {
+    lda = plain_ld(naf, nl, avd, avs, true, M, K);
+    if (lda >= 0)
+      d.flags |= kAKFast;
+    else
+      need |= kNeedPackA;
+  }
+  i64 ldb = plain_ld(nbf, nl, bvd, bvs, false, N, K);
+  if (ldb < 0) {
+    ldb = plain_ld(nbf, nl, bvd, bvs, true, N, K);
+    if (ldb >= 0)
+      d.flags |= kBKFast;
+    else
+      need |= kNeedPackB;
+  }
+  if (lda > 0x7fffffff || ldb > 0x7fffffff) need |= kNeedPackA | kNeedPackB;
+  d.lda = (int32_t)(lda < 0 ? M : lda);
+  d.ldb = (int32_t)(ldb < 0 ? N : ldb);
+  if (!(a_full || a_box)) need |= kNeedPairPanelA;
+  if (!(b_full || b_box)) need |= kNeedPairPanelB;
+  if (M * K > 0x7fffffff || N * K > 0x7fffffff) need |= kTooLarge;
+  desc[p] = d;
+  mn[p] = M * N;
+  // work estimate: MFMA work plus output writes
+  cost[p] = M * N * (K + 4);
+  if (need) atomicOr(flags_any, need);
+}
+
+__global__ void set_c_offsets_kernel(int64_t n, const i64 *__restrict__ c_off,
+                                     GemmDesc *__restrict__ desc) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) desc[p].c_off = c_off[p];
+}
+
+// Per-block sizes for whole-block panels: the panel of block i is the block
+// permuted to [free..., contracted in pair order...]; unused blocks get 0.
+__global__ void block_panel_sizes_kernel(int64_t n_blocks, int32_t rank,
+                                         const i64 *__restrict__ begin,
+                                         const i64 *__restrict__ end,
+                                         const uint8_t *__restrict__ used,
+                                         i64 *__restrict__ sizes) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_blocks) return;
+  i64 n = 1;
+  for (int l = 0; l < rank; ++l) n *= end[b * rank + l] - begin[b * rank + l] + 1;
+  sizes[b] = used[b] ? n : 0;
+}
+
+__global__ void block_panel_jobs_kernel(PanelArgs g,
+                                        const i64 *__restrict__ panel_off,
+                                        const uint8_t *__restrict__ used,
+                                        CopyJob *__restrict__ jobs,
+                                        i64 *__restrict__ wg) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= g.n_blocks) return;
+  CopyJob j{};
+  j.src_off = g.offsets[b];
+  j.dst_off = panel_off[b];
+  j.rank = g.rank;
+  i64 n = 1;
+  for (int q = 0; q < g.rank; ++q) {
+    const int l = g.order[q];
+    const i64 d = g.end[b * g.rank + l] - g.begin[b * g.rank + l] + 1;
+    j.dims[q] = (int32_t)d;
+    j.strides[q] = g.strides[b * g.rank + l];
+    n *= d;
+  }
+  j.n_elem = used[b] ? n : 0;
+  jobs[b] = j;
+  wg[b] = ceil_div(j.n_elem, kGatherPerWG);
+}
+
+// Redirect each pair's operand into its block panel.
+__global__ void block_panel_desc_kernel(BoxArgs g,
+                                        const int32_t *__restrict__ pair_blk,
+                                        const int32_t *__restrict__ pair_other,
+                                        const i64 *__restrict__ panel_off,
+                                        GemmDesc *__restrict__ desc) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n_pairs) return;
+  const int64_t i = pair_blk[p], j = pair_other[p];
+  const i64 *sb = g.self_begin + i * g.self_rank, *se = g.self_end + i * g.self_rank;
+  const i64 *ob = g.other_begin + j * g.other_rank,
+            *oe = g.other_end + j * g.other_rank;
+  i64 rows = 1;
+  for (int q = 0; q < g.n_free; ++q) {
+    const int l = g.free_legs[q];
+    rows *= se[l] - sb[l] + 1;
+  }
+  i64 off = panel_off[i], cstr = rows;
+  for (int k = 0; k < g.nl; ++k) {
+    const int ls = g.self_legs[k], lo_ = g.other_legs[k];
+    const i64 lo = max(sb[ls], ob[lo_]);
+    off += (lo - sb[ls]) * cstr;
+    cstr *= se[ls] - sb[ls] + 1;
+  }
+  GemmDesc d = desc[p];
+  if (g.is_b) {
+    d.b_off = off;
+    d.ldb = (int32_t)(rows > 0 ? rows : 1);
+    d.flags = (d.flags & ~kBKFast) | kBPacked;
+  } else {
+    d.a_off = off;
+    d.lda = (int32_t)(rows > 0 ? rows : 1);
+    d.flags = (d.flags & ~kAKFast) | kAPacked;
+  }
+  desc[p] = d;
+}
+
+// Per-pair panels (general overlapping blocks): panel p holds the pair's
+// sub-block permuted to [free..., contracted...].
+__global__ void pair_panel_sizes_kernel(int64_t n_pairs, int32_t is_b,
+                                        const GemmDesc *__restrict__ desc,
+                                        i64 *__restrict__ sizes) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  const GemmDesc d = desc[p];
+  sizes[p] = (i64)(is_b ? d.n : d.m) * d.k;
+}
+
+__global__ void pair_panel_jobs_kernel(BoxArgs g,
+                                       const int32_t *__restrict__ pair_blk,
+                                       const int32_t *__restrict__ pair_other,
+                                       const i64 *__restrict__ self_str,
+                                       const i64 *__restrict__ self_offsets,
+                                       const i64 *__restrict__ panel_off,
+                                       GemmDesc *__restrict__ desc,
+                                       CopyJob *__restrict__ jobs,
+                                       i64 *__restrict__ wg) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.n_pairs) return;
+  const int64_t i = pair_blk[p], j = pair_other[p];
+  const int r = g.self_rank;
+  const i64 *sb = g.self_begin + i * r, *se = g.self_end + i * r,
+            *ss = self_str + i * r;
+  const i64 *ob = g.other_begin + j * g.other_rank,
+            *oe = g.other_end + j * g.other_rank;
+  CopyJob job{};
+  job.rank = g.n_free + g.nl;
+  i64 src = self_offsets[i], n = 1, rows = 1;
+  for (int q = 0; q < g.n_free; ++q) {
+    const int l = g.free_legs[q];
+    const i64 d = se[l] - sb[l] + 1;
+    job.dims[q] = (int32_t)d;
+    job.strides[q] = ss[l];
+    n *= d;
+    rows *= d;
+  }
+  for (int k = 0; k < g.nl; ++k) {
+    const int ls = g.self_legs[k], lo_ = g.other_legs[k];
+    const i64 lo = max(sb[ls], ob[lo_]);
+    const i64 hi = min(se[ls], oe[lo_]);
+    const i64 d = hi - lo + 1;
+    src += (lo - sb[ls]) * ss[ls];
+    job.dims[g.n_free + k] = (int32_t)d;
+    job.strides[g.n_free + k] = ss[ls];
+    n *= d;
+  }
+  job.src_off = src;
+  job.dst_off = panel_off[p];
+  job.n_elem = n;
+  jobs[p] = job;
+  wg[p] = ceil_div(n, kGatherPerWG);
+  GemmDesc d = desc[p];
+  if (g.is_b) {
+    d.b_off = panel_off[p];
+    d.ldb = (int32_t)(rows > 0 ? rows : 1);
+    d.flags = (d.flags & ~kBKFast) | kBPacked;
+  } else {
+    d.a_off = panel_off[p];
+    d.lda = (int32_t)(rows > 0 ? rows : 1);
+    d.flags = (d.flags & ~kAKFast) | kAPacked;
+  }
+  desc[p] = d;
+}
+
+__global__ void i64_to_i32_kernel(const i64 *__restrict__ in,
+                                  int32_t *__restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (int32_t)in[i];
+}
+
+// ---- stable counting sort of pairs by cost (descending) ------------------
+
+constexpr int kSortChunk = 1024;
+constexpr int kBuckets = 64;
+
+__device__ inline int cost_bucket(i64 c) {
+  // larger cost -> smaller key, so an ascending sort puts big pairs first
+  const int lg = c <= 0 ? 0 : 63 - __clzll((unsigned long long)c);
+  return 63 - lg;
+}
+
+__global__ __launch_bounds__(256) void sort_hist_kernel(
+    const i64 *__restrict__ cost, int64_t n, int64_t n_chunks,
+    i64 *__restrict__ hist) {
+  __shared__ int h[kBuckets];
+  if (threadIdx.x < kBuckets) h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kSortChunk;
+  for (int q = threadIdx.x; q < kSortChunk; q += 256) {
+    const int64_t idx = base + q;
+    if (idx < n) atomicAdd(&h[cost_bucket(cost[idx])], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < kBuckets)
+    hist[(int64_t)threadIdx.x * n_chunks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: items of a chunk are ranked in order; within a round of
+// 256 items each wavefront finds lanes with equal keys from six ballots (one
+// per key bit) and counts the earlier ones.
+__global__ __launch_bounds__(256) void sort_scatter_kernel(
+    const i64 *__restrict__ cost, int64_t n, int64_t n_chunks,
+    const i64 *__restrict__ base_off, i64 *__restrict__ order) {
+  __shared__ i64 running[kBuckets];
+  __shared__ int wave_cnt[4][kBuckets];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x < kBuckets)
+    running[threadIdx.x] = base_off[(int64_t)threadIdx.x * n_chunks + blockIdx.x];
+  const int64_t base = (int64_t)blockIdx.x * kSortChunk;
+  for (int round = 0; round < kSortChunk / 256; ++round) {
+    for (int q = threadIdx.x; q < 4 * kBuckets; q += 256)
+      (&wave_cnt[0][0])[q] = 0;
+    __syncthreads();
+    const int64_t idx = base + round * 256 + threadIdx.x;
+    const bool valid = idx < n;
+    const int key = valid ? cost_bucket(cost[idx]) : kBuckets;  // invalid: 64
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 7; ++bit) {
+      const uint64_t bm = __ballot((key >> bit) & 1);
+      peers &= ((key >> bit) & 1) ? bm : ~bm;
+    }
+    const int rank_in_wave = __popcll(peers & lanemask_lt());
+    const int wave_count = __popcll(peers);
+    if (valid && rank_in_wave == wave_count - 1) wave_cnt[wave][key] = wave_count;
+    __syncthreads();
+    if (valid) {
+      i64 off = running[key] + rank_in_wave;
+      for (int w = 0; w < wave; ++w) off += wave_cnt[w][key];
+      order[off] = idx;
+    }
+    __syncthreads();
+    if (threadIdx.x < kBuckets) {
+      int t = 0;
+      for (int w = 0; w < 4; ++w) t += wave_cnt[w][threadIdx.x];
+      running[threadIdx.x] += t;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void tiles_per_position_kernel(int64_t n, const i64 *__restrict__ order,
+                                          const GemmDesc *__restrict__ desc,
+                                          int32_t bm, int32_t bn,
+                                          i64 *__restrict__ tiles) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const GemmDesc d = desc[order[q]];
+  tiles[q] = ceil_div(d.m, bm) * ceil_div(d.n, bn);
+}
+
+__global__ void expand_tiles_kernel(int64_t n, const i64 *__restrict__ order,
+                                    const i64 *__restrict__ tile_start,
+                                    int32_t *__restrict__ tile_pair,
+                                    int32_t *__restrict__ tile_first) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int64_t p = order[q];
+  const i64 t0 = tile_start[q], t1 = tile_start[q + 1];
+  tile_first[p] = (int32_t)t0;
+  for (i64 t = t0; t < t1; ++t) tile_pair[t] = (int32_t)p;
+}
+
+unsigned grid_for(int64_t n, int threads = 256) {
+  return (unsigned)std::max<int64_t>(1, ceil_div(n, threads));
+}
+
+bool valid_legs(int ra, int rb, int nl, const int32_t *legs) {
+  bool sa[TQ_MAX_RANK] = {false}, sb[TQ_MAX_RANK] = {false};
+  for (int k = 0; k < nl; ++k) {
+    const int a = legs[2 * k], b = legs[2 * k + 1];
+    if (a < 0 || a >= ra || b < 0 || b >= rb || sa[a] || sb[b]) return false;
+    sa[a] = sb[b] = true;
+  }
+  return true;
+}
+
+bool valid_desc(const tq_bs_desc *d) {
+  if (!d || d->rank < 0 || d->rank > TQ_MAX_RANK || d->n_blocks < 0) return false;
+  if (d->rank > 0 && !d->dims) return false;
+  if (d->n_blocks > 0 && d->rank > 0 &&
+      (!d->begin || !d->end || !d->strides || !d->offsets))
+    return false;
+  if (d->n_blocks > 0x7fffffff) return false;
+  for (int l = 0; l < d->rank; ++l)
+    if (d->dims[l] < 0 || d->dims[l] > 0x7fffffff) return false;
+  for (int64_t b = 0; b < d->n_blocks; ++b)
+    for (int l = 0; l < d->rank; ++l) {
+      const i64 lo = d->begin[b * d->rank + l], hi = d->end[b * d->rank + l];
+      if (lo < 0 || hi < lo || hi >= d->dims[l]) return false;
+    }
+  return true;
+}
+
+}  // namespace
+
+// ---- plan object ----------------------------------------------------------
+
+tq_status plan_build_tiles(tq_plan_t pl, int cfg) {
+  tq_handle_t h = pl->h;
+  if (pl->tiles_cfg == cfg && pl->tiles_ready) return TQ_SUCCESS;
+  const GemmConfig c = tq_gemm_config(pl->prec, cfg);
+  const int64_t np = pl->n_pairs;
+  int64_t total = 0;
+  for (int64_t p = 0; p < np; ++p)
+    total += ceil_div(pl->h_m[p], c.bm) * ceil_div(pl->h_n[p], c.bn);
+  if (total > 0x7fffffff)
+    return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many GEMM tiles");
+  DevBuf tiles, tmp;
+  TQ_HIP(h, tiles.alloc(sizeof(i64) * (np + 1)));
+  TQ_HIP(h, tmp.alloc(tq_scan_tmp_bytes(np + 1)));
+  TQ_HIP(h, pl->tile_pair.alloc(sizeof(int32_t) * std::max<int64_t>(total, 1)));
+  TQ_HIP(h, pl->tile_first.alloc(sizeof(int32_t) * std::max<int64_t>(np, 1)));
+  hipLaunchKernelGGL(tiles_per_position_kernel, dim3(grid_for(np)), dim3(256), 0,
+                     h->stream, np, pl->order.as<i64>(), pl->desc.as<GemmDesc>(),
+                     c.bm, c.bn, tiles.as<i64>());
+  TQ_CHECK(tq_scan_i64(h, tiles.as<i64>(), tiles.as<i64>(), np,
+                       tiles.as<i64>() + np, tmp.p));
+  hipLaunchKernelGGL(expand_tiles_kernel, dim3(grid_for(np)), dim3(256), 0,
+                     h->stream, np, pl->order.as<i64>(), tiles.as<i64>(),
+                     pl->tile_pair.as<int32_t>(), pl->tile_first.as<int32_t>());
+  TQ_HIP(h, hipGetLastError());
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  pl->n_tiles = total;
+  pl->tiles_cfg = cfg;
+  pl->tiles_ready = true;
+  return TQ_SUCCESS;
+}
+
+static tq_status build_pack(tq_plan_t pl, bool is_b, int mode,
+                            const PlanUpload &up) {
+  tq_handle_t h = pl->h;
+  const int64_t np = pl->n_pairs;
+  PackSide &side = is_b ? pl->pack_b : pl->pack_a;
+  side.mode = mode;
+  if (mode == kModeDirect) return TQ_SUCCESS;
+  const int self_rank = is_b ? pl->rb : pl->ra;
+  const int other_rank = is_b ? pl->ra : pl->rb;
+  const int nl = pl->nl;
+  BoxArgs box{};
+  box.n_pairs = np;
+  box.is_b = is_b;
+  box.self_rank = self_rank;
+  box.other_rank = other_rank;
+  box.nl = nl;
+  box.self_begin = is_b ? up.b_begin : up.a_begin;
+  box.self_end = is_b ? up.b_end : up.a_end;
+  box.other_begin = is_b ? up.a_begin : up.b_begin;
+  box.other_end = is_b ? up.a_end : up.b_end;
+  box.n_free = self_rank - nl;
+  for (int q = 0; q < box.n_free; ++q)
+    box.free_legs[q] = is_b ? pl->b_free[q] : pl->a_free[q];
+  for (int k = 0; k < nl; ++k) {
+    box.self_legs[k] = is_b ? pl->legs_b[k] : pl->legs_a[k];
+    box.other_legs[k] = is_b ? pl->legs_a[k] : pl->legs_b[k];
+  }
+  const int32_t *pair_self = is_b ? pl->pair_b.as<int32_t>() : pl->pair_a.as<int32_t>();
+  const int32_t *pair_other = is_b ? pl->pair_a.as<int32_t>() : pl->pair_b.as<int32_t>();
+  const i64 *self_str = is_b ? up.b_str : up.a_str;
+  const i64 *self_off = is_b ? up.b_off : up.a_off;
+  DevBuf sizes, tmp;
+  int64_t n_jobs;
+  if (mode == kModeBlockPanel) {
+    const int64_t nb = is_b ? pl->nb : pl->na;
+    n_jobs = nb;
+    TQ_HIP(h, sizes.alloc(sizeof(i64) * (nb + 1)));
+    TQ_HIP(h, tmp.alloc(tq_scan_tmp_bytes(nb + 1)));
+    TQ_HIP(h, side.panel_off.alloc(sizeof(i64) * (nb + 1)));
+    TQ_HIP(h, side.jobs.alloc(sizeof(CopyJob) * std::max<int64_t>(nb, 1)));
+    TQ_HIP(h, side.wg.alloc(sizeof(i64) * (nb + 1)));
+    TQ_HIP(h, side.wg32.alloc(sizeof(int32_t) * (nb + 1)));
+    const uint8_t *used = is_b ? pl->b_used.as<uint8_t>() : pl->a_used.as<uint8_t>();
+    hipLaunchKernelGGL(block_panel_sizes_kernel, dim3(grid_for(nb)), dim3(256), 0,
+                       h->stream, nb, self_rank, box.self_begin, box.self_end, used,
+                       sizes.as<i64>());
+    TQ_CHECK(tq_scan_i64(h, sizes.as<i64>(), side.panel_off.as<i64>(), nb,
+                         side.panel_off.as<i64>() + nb, tmp.p));
+    PanelArgs pa{};
+    pa.n_blocks = nb;
+    pa.rank = self_rank;
+    pa.begin = box.self_begin;
+    pa.end = box.self_end;
+    pa.strides = self_str;
+    pa.offsets = self_off;
+    for (int q = 0; q < box.n_free; ++q) pa.order[q] = box.free_legs[q];
+    for (int k = 0; k < nl; ++k) pa.order[box.n_free + k] = box.self_legs[k];
+    hipLaunchKernelGGL(block_panel_jobs_kernel, dim3(grid_for(nb)), dim3(256), 0,
+                       h->stream, pa, side.panel_off.as<i64>(), used,
+                       side.jobs.as<CopyJob>(), side.wg.as<i64>());
+    hipLaunchKernelGGL(block_panel_desc_kernel, dim3(grid_for(np)), dim3(256), 0,
+                       h->stream, box, pair_self, pair_other,
+                       side.panel_off.as<i64>(), pl->desc.as<GemmDesc>());
+  } else {
+    n_jobs = np;
+    TQ_HIP(h, sizes.alloc(sizeof(i64) * (np + 1)));
+    TQ_HIP(h, tmp.alloc(tq_scan_tmp_bytes(np + 1)));
+    TQ_HIP(h, side.panel_off.alloc(sizeof(i64) * (np + 1)));
+    TQ_HIP(h, side.jobs.alloc(sizeof(CopyJob) * std::max<int64_t>(np, 1)));
+    TQ_HIP(h, side.wg.alloc(sizeof(i64) * (np + 1)));
+    TQ_HIP(h, side.wg32.alloc(sizeof(int32_t) * (np + 1)));
+    hipLaunchKernelGGL(pair_panel_sizes_kernel, dim3(grid_for(np)), dim3(256), 0,
+                       h->stream, np, (int32_t)is_b, pl->desc.as<GemmDesc>(),
+                       sizes.as<i64>());
+    TQ_CHECK(tq_scan_i64(h, sizes.as<i64>(), side.panel_off.as<i64>(), np,
+                         side.panel_off.as<i64>() + np, tmp.p));
+    hipLaunchKernelGGL(pair_panel_jobs_kernel, dim3(grid_for(np)), dim3(256), 0,
+                       h->stream, box, pair_self, pair_other, self_str, self_off,
+                       side.panel_off.as<i64>(), pl->desc.as<GemmDesc>(),
+                       side.jobs.as<CopyJob>(), side.wg.as<i64>());
+  }
+  TQ_HIP(h, hipGetLastError());
+  TQ_CHECK(tq_scan_i64(h, side.wg.as<i64>(), side.wg.as<i64>(), n_jobs,
+                       side.wg.as<i64>() + n_jobs, tmp.p));
+  hipLaunchKernelGGL(i64_to_i32_kernel, dim3(grid_for(n_jobs + 1)), dim3(256), 0,
+                     h->stream, side.wg.as<i64>(), side.wg32.as<int32_t>(),
+                     n_jobs + 1);
+  TQ_HIP(h, hipGetLastError());
+  i64 sizes_h[2];
+  TQ_HIP(h, hipMemcpyAsync(&sizes_h[0], side.panel_off.as<i64>() + n_jobs,
+                           sizeof(i64), hipMemcpyDeviceToHost, h->stream));
+  TQ_HIP(h, hipMemcpyAsync(&sizes_h[1], side.wg.as<i64>() + n_jobs, sizeof(i64),
+                           hipMemcpyDeviceToHost, h->stream));
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  if (sizes_h[1] > 0x7fffffff)
+    return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: pack too large");
+  side.n_jobs = n_jobs;
+  side.total_wg = sizes_h[1];
+  side.panel_elems = sizes_h[0];
+  const size_t es = pl->dtype == TQ_F64 ? 8 : 4;
+  TQ_HIP(h, side.panel.alloc(es * std::max<int64_t>(sizes_h[0], 1)));
+  return TQ_SUCCESS;
+}
+
+extern "C" {
+
+tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
+                              const tq_bs_desc *b, int32_t n_legs,
+                              const int32_t *legs, tq_dtype dtype,
+                              tq_precision precision, tq_plan_t *plan_out) {
+  if (!h) return TQ_ERROR_NOT_INITIALIZED;
+  if (!plan_out) return TQ_ERROR_INVALID_VALUE;
+  *plan_out = nullptr;
+  if (!valid_desc(a) || !valid_desc(b))
+    return tq_fail(h, TQ_ERROR_INVALID_VALUE, "contract: invalid tensor metadata");
+  if (dtype != TQ_F32 && dtype != TQ_F64) return TQ_ERROR_UNSUPPORTED;
+  if (precision != TQ_PREC_NATIVE && dtype != TQ_F64)
+    return tq_fail(h, TQ_ERROR_UNSUPPORTED,
+                   "contract: reduced-precision tiers take FP64 data");
+  if (precision < TQ_PREC_NATIVE || precision > TQ_PREC_BF16X3)
+    return TQ_ERROR_UNSUPPORTED;
+  if (n_legs < 0 || n_legs > std::min(a->rank, b->rank) || (n_legs > 0 && !legs) ||
+      !valid_legs(a->rank, b->rank, n_legs, legs))
+    return tq_fail(h, TQ_ERROR_INVALID_VALUE, "contract: invalid contracting indices");
+  for (int k = 0; k < n_legs; ++k)
+    if (a->dims[legs[2 * k]] != b->dims[legs[2 * k + 1]])
+      return tq_fail(h, TQ_ERROR_DIMENSION_MISMATCH,
+                     "The dimensions from two tensors to be contracted do not match");
+
+  std::unique_ptr<tq_plan_s> pl(new tq_plan_s());
+  pl->h = h;
+  pl->dtype = dtype;
+  pl->prec = precision;
+  pl->ra = a->rank;
+  pl->rb = b->rank;
+  pl->nl = n_legs;
+  pl->na = a->n_blocks;
+  pl->nb = b->n_blocks;
+  for (int k = 0; k < n_legs; ++k) {
+    pl->legs_a[k] = legs[2 * k];
+    pl->legs_b[k] = legs[2 * k + 1];
+  }
+  // free legs in ascending order (arma shed_rows, block_sparse.cuh:611-616)
+  {
+    int q = 0;
+    for (int l = 0; l < a->rank; ++l) {
+      bool c = false;
+      for (int k = 0; k < n_legs; ++k) c |= legs[2 * k] == l;
+      if (!c) {
+        pl->a_free[q++] = l;
+        pl->out_dims.push_back(a->dims[l]);
+      }
+    }
+    q = 0;
+    for (int l = 0; l < b->rank; ++l) {
+      bool c = false;
+      for (int k = 0; k < n_legs; ++k) c |= legs[2 * k + 1] == l;
+      if (!c) {
+        pl->b_free[q++] = l;
+        pl->out_dims.push_back(b->dims[l]);
+      }
+    }
+  }
+  pl->ro = (int)pl->out_dims.size();
+  const int ra = a->rank, rb = b->rank, ro = pl->ro, nl = n_legs;
+  const int64_t na = a->n_blocks, nb = b->n_blocks;
+
+  // ---- upload A and B metadata in one transfer ----
+  const size_t n_a_meta = (size_t)na * ra * 3 + na;
+  const size_t n_b_meta = (size_t)nb * rb * 3 + nb;
+  const size_t up_bytes = (n_a_meta + n_b_meta) * 8 + 16;
+  void *host = nullptr;
+  TQ_CHECK(tq_stage(h, up_bytes, &host));
+  i64 *hp = reinterpret_cast<i64 *>(host);
+  auto put = [&](const int64_t *src, size_t n) {
+    if (n) std::memcpy(hp, src, n * 8);
+    hp += n;
+  };
+  put(a->begin, (size_t)na * ra);
+  put(a->end, (size_t)na * ra);
+  put(a->strides, (size_t)na * ra);
+  put(a->offsets, (size_t)na);
+  put(b->begin, (size_t)nb * rb);
+  put(b->end, (size_t)nb * rb);
+  put(b->strides, (size_t)nb * rb);
+  put(b->offsets, (size_t)nb);
+  TQ_HIP(h, pl->meta.alloc(up_bytes));
+  TQ_HIP(h, hipMemcpyAsync(pl->meta.p, host, up_bytes, hipMemcpyHostToDevice,
+                           h->stream));
+  PlanUpload up;
+  {
+    i64 *dp = pl->meta.as<i64>();
+    up.a_begin = dp; dp += (size_t)na * ra;
+    up.a_end = dp; dp += (size_t)na * ra;
+    up.a_str = dp; dp += (size_t)na * ra;
+    up.a_off = dp; dp += na;
+    up.b_begin = dp; dp += (size_t)nb * rb;
+    up.b_end = dp; dp += (size_t)nb * rb;
+    up.b_str = dp; dp += (size_t)nb * rb;
+    up.b_off = dp; dp += nb;
+  }
+  pl->up = up;
+
+  // ---- 1. match: count, scan, fill ----
+  int64_t n_pairs = 0;
+  DevBuf b_soa, counts, scan_tmp;
+  const int64_t b_chunks = std::max<int64_t>(1, ceil_div(nb, kWave));
+  int64_t n_split = std::min<int64_t>(b_chunks, std::max<int64_t>(1, ceil_div(2048, std::max<int64_t>(na, 1))));
+  const int32_t chunks_per_split = (int32_t)ceil_div(b_chunks, n_split);
+  n_split = ceil_div(b_chunks, chunks_per_split);
+  const int64_t n_items = na * n_split;
+  TQ_HIP(h, pl->a_used.alloc(std::max<int64_t>(na, 1)));
+  TQ_HIP(h, pl->b_used.alloc(std::max<int64_t>(nb, 1)));
+  TQ_HIP(h, hipMemsetAsync(pl->b_used.p, 0, std::max<int64_t>(nb, 1), h->stream));
+  TQ_HIP(h, hipMemsetAsync(pl->a_used.p, 0, std::max<int64_t>(na, 1), h->stream));
+  if (na > 0 && nb > 0) {
+    TQ_HIP(h, b_soa.alloc(sizeof(int32_t) * 2 * std::max<int64_t>(1, nl * nb)));
+    int32_t *b_lo = b_soa.as<int32_t>();
+    int32_t *b_hi = b_lo + std::max<int64_t>(1, nl * nb);
+    LegList ll{};
+    for (int k = 0; k < nl; ++k) ll.b[k] = pl->legs_b[k];
+    if (nl > 0)
+      hipLaunchKernelGGL(soa_legs_kernel, dim3(grid_for(nb)), dim3(256), 0,
+                         h->stream, nb, rb, nl, up.b_begin, up.b_end, ll, b_lo,
+                         b_hi);
+    MatchArgs mg{};
+    mg.nl = nl;
+    mg.a_rank = ra;
+    mg.n_a = na;
+    mg.n_b = nb;
+    mg.chunks_per_split = chunks_per_split;
+    mg.n_split = n_split;
+    mg.a_begin = up.a_begin;
+    mg.a_end = up.a_end;
+    mg.b_lo = b_lo;
+    mg.b_hi = b_hi;
+    for (int k = 0; k < nl; ++k) mg.a_legs[k] = pl->legs_a[k];
+    TQ_HIP(h, counts.alloc(sizeof(i64) * (n_items + 1)));
+    TQ_HIP(h, scan_tmp.alloc(tq_scan_tmp_bytes(n_items + 1)));
+    const unsigned mgrid = (unsigned)ceil_div(n_items, 4);
+    hipLaunchKernelGGL(match_kernel<false>, dim3(mgrid), dim3(256), 0, h->stream,
+                       mg, counts.as<i64>(), nullptr, nullptr, nullptr, nullptr, nullptr);
+    TQ_HIP(h, hipGetLastError());
+    TQ_CHECK(tq_scan_i64(h, counts.as<i64>(), counts.as<i64>(), n_items,
+                         counts.as<i64>() + n_items, scan_tmp.p));
+    i64 np_h = 0;
+    TQ_HIP(h, hipMemcpyAsync(&np_h, counts.as<i64>() + n_items, sizeof(i64),
+                             hipMemcpyDeviceToHost, h->stream));
+    TQ_HIP(h, hipStreamSynchronize(h->stream));
+    n_pairs = np_h;
+    if (n_pairs > 0x7fffffff)
+      return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many block pairs");
+    TQ_HIP(h, pl->pair_a.alloc(sizeof(int32_t) * std::max<int64_t>(n_pairs, 1)));
+    TQ_HIP(h, pl->pair_b.alloc(sizeof(int32_t) * std::max<int64_t>(n_pairs, 1)));
+    if (n_pairs > 0) {
+      hipLaunchKernelGGL(match_kernel<true>, dim3(mgrid), dim3(256), 0, h->stream,
+                         mg, nullptr, counts.as<i64>(), pl->pair_a.as<int32_t>(),
+                         pl->pair_b.as<int32_t>(), pl->a_used.as<uint8_t>(),
+                         pl->b_used.as<uint8_t>());
+      TQ_HIP(h, hipGetLastError());
+    }
+  }
+  pl->n_pairs = n_pairs;
+
+  // ---- 2. geometry ----
+  const int64_t np = n_pairs;
+  DevBuf mn, cost, flags_any, ctmp;
+  TQ_HIP(h, pl->desc.alloc(sizeof(GemmDesc) * std::max<int64_t>(np, 1)));
+  TQ_HIP(h, pl->sub_off.alloc(sizeof(i64) * 2 * std::max<int64_t>(np, 1)));
+  TQ_HIP(h, pl->out_be.alloc(sizeof(i64) * 2 * std::max<int64_t>(np * ro, 1)));
+  TQ_HIP(h, mn.alloc(sizeof(i64) * (np + 1)));
+  TQ_HIP(h, cost.alloc(sizeof(i64) * (np + 1)));
+  TQ_HIP(h, flags_any.alloc(sizeof(int32_t)));
+  TQ_HIP(h, ctmp.alloc(tq_scan_tmp_bytes(np + 1)));
+  TQ_HIP(h, hipMemsetAsync(flags_any.p, 0, sizeof(int32_t), h->stream));
+  i64 *d_out_begin = pl->out_be.as<i64>();
+  i64 *d_out_end = d_out_begin + std::max<int64_t>(np * ro, 1);
+  if (np > 0) {
+    GeomArgs gg{};
+    gg.n_pairs = np;
+    gg.a_rank = ra;
+    gg.b_rank = rb;
+    gg.nl = nl;
+    gg.a_begin = up.a_begin;
+    gg.a_end = up.a_end;
+    gg.a_str = up.a_str;
+    gg.a_off = up.a_off;
+    gg.b_begin = up.b_begin;
+    gg.b_end = up.b_end;
+    gg.b_str = up.b_str;
+    gg.b_off = up.b_off;
+    for (int k = 0; k < nl; ++k) {
+      gg.legs.a[k] = pl->legs_a[k];
+      gg.legs.b[k] = pl->legs_b[k];
+    }
+    for (int q = 0; q < ra - nl; ++q) gg.a_free[q] = pl->a_free[q];
+    for (int q = 0; q < rb - nl; ++q) gg.b_free[q] = pl->b_free[q];
+    hipLaunchKernelGGL(geometry_kernel, dim3(grid_for(np)), dim3(256), 0,
+                       h->stream, gg, pl->pair_a.as<int32_t>(),
+                       pl->pair_b.as<int32_t>(), pl->desc.as<GemmDesc>(),
+                       pl->sub_off.as<i64>(), d_out_begin, d_out_end, mn.as<i64>(), cost.as<i64>(),
+                       flags_any.as<int32_t>());
+    TQ_HIP(h, hipGetLastError());
+  }
+  TQ_CHECK(tq_scan_i64(h, mn.as<i64>(), mn.as<i64>(), np, mn.as<i64>() + np,
+                       ctmp.p));
+  TQ_HIP(h, pl->c_off.alloc(sizeof(i64) * (np + 1)));
+  TQ_HIP(h, hipMemcpyAsync(pl->c_off.p, mn.p, sizeof(i64) * (np + 1),
+                           hipMemcpyDeviceToDevice, h->stream));
+  if (np > 0)
+    hipLaunchKernelGGL(set_c_offsets_kernel, dim3(grid_for(np)), dim3(256), 0,
+                       h->stream, np, pl->c_off.as<i64>(), pl->desc.as<GemmDesc>());
+
+  // ---- 3. results the host object needs (one batch of transfers) ----
+  i64 flags_h = 0;
+  int32_t fl32 = 0;
+  i64 out_total = 0;
+  pl->h_m.resize(np);
+  pl->h_n.resize(np);
+  pl->h_k.resize(np);
+  std::vector<GemmDesc> hd(np);
+  if (np > 0)
+    TQ_HIP(h, hipMemcpyAsync(hd.data(), pl->desc.p, sizeof(GemmDesc) * np,
+                             hipMemcpyDeviceToHost, h->stream));
+  TQ_HIP(h, hipMemcpyAsync(&fl32, flags_any.p, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, h->stream));
+  TQ_HIP(h, hipMemcpyAsync(&out_total, pl->c_off.as<i64>() + np, sizeof(i64),
+                           hipMemcpyDeviceToHost, h->stream));
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  flags_h = fl32;
+  if (flags_h & kTooLarge)
+    return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: block GEMM above 2^31 elements");
+  double flops = 0.0;
+  for (int64_t p = 0; p < np; ++p) {
+    pl->h_m[p] = hd[p].m;
+    pl->h_n[p] = hd[p].n;
+    pl->h_k[p] = hd[p].k;
+    flops += 2.0 * hd[p].m * (double)hd[p].n * hd[p].k;
+  }
+  pl->flops = flops;
+  pl->out_n_elem = ro == 0 ? 1 : out_total;
+  {
+    double na_elem = 0, nb_elem = 0;
+    for (int64_t i = 0; i < na; ++i) {
+      double n = 1;
+      for (int l = 0; l < ra; ++l) n *= a->end[i * ra + l] - a->begin[i * ra + l] + 1;
+      na_elem += n;
+    }
+    for (int64_t j = 0; j < nb; ++j) {
+      double n = 1;
+      for (int l = 0; l < rb; ++l) n *= b->end[j * rb + l] - b->begin[j * rb + l] + 1;
+      nb_elem += n;
+    }
+    pl->bytes = (dtype == TQ_F64 ? 8.0 : 4.0) * (na_elem + nb_elem + (double)pl->out_n_elem);
+  }
+
+  // ---- 4. operand layouts and pack jobs ----
+  if (np > 0) {
+    const int mode_a = !(flags_h & kNeedPackA) ? kModeDirect
+                       : (flags_h & kNeedPairPanelA) ? kModePairPanel
+                                                     : kModeBlockPanel;
+    const int mode_b = !(flags_h & kNeedPackB) ? kModeDirect
+                       : (flags_h & kNeedPairPanelB) ? kModePairPanel
+                                                     : kModeBlockPanel;
+    TQ_CHECK(build_pack(pl.get(), false, mode_a, up));
+    TQ_CHECK(build_pack(pl.get(), true, mode_b, up));
+
+    // ---- 5. work list: stable device sort by cost, largest first ----
+    const int64_t n_chunks = ceil_div(np, kSortChunk);
+    DevBuf hist, stmp;
+    TQ_HIP(h, hist.alloc(sizeof(i64) * (kBuckets * n_chunks + 1)));
+    TQ_HIP(h, stmp.alloc(tq_scan_tmp_bytes(kBuckets * n_chunks + 1)));
+    TQ_HIP(h, pl->order.alloc(sizeof(i64) * np));
+    hipLaunchKernelGGL(sort_hist_kernel, dim3((unsigned)n_chunks), dim3(256), 0,
+                       h->stream, cost.as<i64>(), np, n_chunks, hist.as<i64>());
+    TQ_CHECK(tq_scan_i64(h, hist.as<i64>(), hist.as<i64>(), kBuckets * n_chunks,
+                         nullptr, stmp.p));
+    hipLaunchKernelGGL(sort_scatter_kernel, dim3((unsigned)n_chunks), dim3(256), 0,
+                       h->stream, cost.as<i64>(), np, n_chunks, hist.as<i64>(),
+                       pl->order.as<i64>());
+    TQ_HIP(h, hipGetLastError());
+  }
+  pl->a_layout = pl->pack_a.mode == kModeDirect ? 0 : 2;
+  pl->b_layout = pl->pack_b.mode == kModeDirect ? 0 : 2;
+  if (np > 0 && pl->pack_a.mode == kModeDirect && (hd[0].flags & kAKFast)) pl->a_layout = 1;
+  if (np > 0 && pl->pack_b.mode == kModeDirect && (hd[0].flags & kBKFast)) pl->b_layout = 1;
+
+  // ---- output metadata on the host (the drop-in tensor keeps it there) ----
+  pl->h_out_begin.resize((size_t)np * ro);
+  pl->h_out_end.resize((size_t)np * ro);
+  pl->h_c_off.resize(np + 1);
+  if (np > 0 && ro > 0) {
+    TQ_HIP(h, hipMemcpyAsync(pl->h_out_begin.data(), d_out_begin,
+                             sizeof(i64) * np * ro, hipMemcpyDeviceToHost, h->stream));
+    TQ_HIP(h, hipMemcpyAsync(pl->h_out_end.data(), d_out_end, sizeof(i64) * np * ro,
+                             hipMemcpyDeviceToHost, h->stream));
+  }
+  TQ_HIP(h, hipMemcpyAsync(pl->h_c_off.data(), pl->c_off.p, sizeof(i64) * (np + 1),
+                           hipMemcpyDeviceToHost, h->stream));
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+
+  pl->auto_cfg = tq_plan_pick_config(pl.get());
+  pl->cfg = -1;
+  if (np > 0 && ro > 0) TQ_CHECK(plan_build_tiles(pl.get(), pl->auto_cfg));
+  *plan_out = pl.release();
+  return TQ_SUCCESS;
+}
+
+tq_status tq_plan_get_info(tq_plan_t pl, tq_plan_info *info) {
+  if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  if (!info) return TQ_ERROR_INVALID_VALUE;
+  info->out_rank = pl->ro;
+  info->n_legs = pl->nl;
+  info->n_pairs = pl->n_pairs;
+  info->out_n_elem = pl->out_n_elem;
+  info->flops = pl->flops;
+  info->bytes = pl->bytes;
+  info->a_layout = pl->a_layout;
+  info->b_layout = pl->b_layout;
+  return TQ_SUCCESS;
+}
+
+tq_status tq_plan_get_output(tq_plan_t pl, int64_t *dims, int64_t *begin,
+                             int64_t *end, int64_t *strides, int64_t *offsets) {
+  if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  const int ro = pl->ro;
+  const int64_t np = pl->n_pairs;
+  if (dims) std::copy(pl->out_dims.begin(), pl->out_dims.end(), dims);
+  if (begin) std::copy(pl->h_out_begin.begin(), pl->h_out_begin.end(), begin);
+  if (end) std::copy(pl->h_out_end.begin(), pl->h_out_end.end(), end);
+  if (strides) {
+    // compact column-major index tables (generate_index_table,
+    // src/gpu/block_sparse.cuh:658-664)
+    for (int64_t p = 0; p < np; ++p) {
+      i64 t = 1;
+      for (int l = 0; l < ro; ++l) {
+        strides[p * ro + l] = t;
+        t *= pl->h_out_end[p * ro + l] - pl->h_out_begin[p * ro + l] + 1;
+      }
+    }
+  }
+  if (offsets && ro > 0) std::copy(pl->h_c_off.begin(), pl->h_c_off.begin() + np, offsets);
+  return TQ_SUCCESS;
+}
+
+tq_status tq_plan_get_pairs(tq_plan_t pl, int64_t *a_block, int64_t *b_block,
+                            int64_t *a_offset, int64_t *b_offset, int64_t *m,
+                            int64_t *n, int64_t *k) {
+  if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  tq_handle_t h = pl->h;
+  const int64_t np = pl->n_pairs;
+  if (np == 0) return TQ_SUCCESS;
+  std::vector<int32_t> pa(np), pb(np);
+  TQ_HIP(h, hipMemcpyAsync(pa.data(), pl->pair_a.p, sizeof(int32_t) * np,
+                           hipMemcpyDeviceToHost, h->stream));
+  TQ_HIP(h, hipMemcpyAsync(pb.data(), pl->pair_b.p, sizeof(int32_t) * np,
+                           hipMemcpyDeviceToHost, h->stream));
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  for (int64_t p = 0; p < np; ++p) {
+    if (a_block) a_block[p] = pa[p];
+    if (b_block) b_block[p] = pb[p];
+    if (m) m[p] = pl->h_m[p];
+    if (n) n[p] = pl->h_n[p];
+    if (k) k[p] = pl->h_k[p];
+  }
+  if (a_offset || b_offset) {
+    // sub-block offsets as the reference computes them
+    // (src/gpu/block_sparse.cuh:700-732)
+    std::vector<i64> sub(np * 2);
+    TQ_HIP(h, hipMemcpyAsync(sub.data(), pl->sub_off.p, sizeof(i64) * 2 * np,
+                             hipMemcpyDeviceToHost, h->stream));
+    TQ_HIP(h, hipStreamSynchronize(h->stream));
+    for (int64_t p = 0; p < np; ++p) {
+      if (a_offset) a_offset[p] = sub[2 * p];
+      if (b_offset) b_offset[p] = sub[2 * p + 1];
+    }
+  }
+  return TQ_SUCCESS;
+}
+
+tq_status tq_plan_get_worklist(tq_plan_t pl, int64_t *order) {
+  if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  if (!order) return TQ_ERROR_INVALID_VALUE;
+  if (pl->n_pairs == 0) return TQ_SUCCESS;
+  TQ_HIP(pl->h, hipMemcpyAsync(order, pl->order.p, sizeof(i64) * pl->n_pairs,
+                               hipMemcpyDeviceToHost, pl->h->stream));
+  TQ_HIP(pl->h, hipStreamSynchronize(pl->h->stream));
+  return TQ_SUCCESS;
+}
+
+tq_status tq_plan_destroy(tq_plan_t pl) {
+  if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  (void)hipStreamSynchronize(pl->h->stream);
+  delete pl;
+  return TQ_SUCCESS;
+}
+
+}  // extern "C"

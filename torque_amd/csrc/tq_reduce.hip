@@ -1,0 +1,126 @@
+// tq_reduce.hip -- FP64 reductions on the GPU: deterministic sums, dot
+// products, norms and scaling. They replace cublasDdot + add<<<1,1>>> of the
+// reference's rank-0 branch (src/gpu/block_sparse.cuh:979-1010) and give the
+// network-scalar and norm reductions of environment tensors a fixed
+// summation order (two stages, fixed grid), so repeated runs agree bitwise.
+#include <algorithm>
+#include <cmath>
+
+#include "tq_common.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxBlocks = 1024;
+
+__device__ inline double block_sum(double v) {
+  __shared__ double ws[kThreads / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) ws[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kThreads / 64; ++w) t += ws[w];
+  __syncthreads();
+  return t;
+}
+
+// Each block sums a contiguous slice [lo, hi) with a fixed thread order.
+template <bool kDot>
+__global__ __launch_bounds__(kThreads) void partial_kernel(
+    const double *__restrict__ x, const double *__restrict__ y, int64_t n,
+    int64_t per_block, double *__restrict__ out) {
+  const int64_t lo = (int64_t)blockIdx.x * per_block;
+  const int64_t hi = min(n, lo + per_block);
+  double s = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kThreads)
+    s += kDot ? x[i] * y[i] : x[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void final_kernel(
+    const double *__restrict__ part, int64_t n, double *__restrict__ out,
+    int take_sqrt) {
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kThreads) s += part[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) out[0] = take_sqrt ? sqrt(s) : s;
+}
+
+__global__ void scale_kernel(double *__restrict__ x, int64_t n, double alpha) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    x[i] *= alpha;
+}
+
+int64_t n_blocks_for(int64_t n) {
+  return std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, ceil_div(n, 8 * kThreads)));
+}
+
+tq_status reduce(tq_handle_t h, const double *x, const double *y, int64_t n,
+                 double *out_dev, void *tmp, bool take_sqrt) {
+  const int64_t nb = n_blocks_for(n);
+  const int64_t per = ceil_div(std::max<int64_t>(n, 1), nb);
+  double *part = reinterpret_cast<double *>(tmp);
+  if (y)
+    hipLaunchKernelGGL(partial_kernel<true>, dim3((unsigned)nb), dim3(kThreads), 0,
+                       h->stream, x, y, n, per, part);
+  else
+    hipLaunchKernelGGL(partial_kernel<false>, dim3((unsigned)nb), dim3(kThreads), 0,
+                       h->stream, x, nullptr, n, per, part);
+  hipLaunchKernelGGL(final_kernel, dim3(1), dim3(kThreads), 0, h->stream, part, nb,
+                     out_dev, take_sqrt ? 1 : 0);
+  TQ_HIP(h, hipGetLastError());
+  return TQ_SUCCESS;
+}
+
+}  // namespace
+
+size_t tq_sum_tmp_bytes(int64_t n) { return sizeof(double) * (n_blocks_for(n) + 1); }
+
+tq_status tq_sum_f64_dev(tq_handle_t h, const double *x, int64_t n, double *out,
+                         void *tmp) {
+  return reduce(h, x, nullptr, n, out, tmp, false);
+}
+
+extern "C" {
+
+tq_status tq_dot_f64(tq_handle_t h, int64_t n, const double *x, const double *y,
+                     double *result) {
+  if (!h) return TQ_ERROR_NOT_INITIALIZED;
+  if (n < 0 || !result || (n > 0 && (!x || !y))) return TQ_ERROR_INVALID_VALUE;
+  DevBuf tmp;
+  TQ_HIP(h, tmp.alloc(tq_sum_tmp_bytes(n) + sizeof(double)));
+  double *out = tmp.as<double>() + n_blocks_for(n) + 1;
+  TQ_CHECK(reduce(h, x, y, n, out, tmp.p, false));
+  TQ_HIP(h, hipMemcpyAsync(result, out, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  return TQ_SUCCESS;
+}
+
+tq_status tq_nrm2_f64(tq_handle_t h, int64_t n, const double *x, double *result) {
+  if (!h) return TQ_ERROR_NOT_INITIALIZED;
+  if (n < 0 || !result || (n > 0 && !x)) return TQ_ERROR_INVALID_VALUE;
+  DevBuf tmp;
+  TQ_HIP(h, tmp.alloc(tq_sum_tmp_bytes(n) + sizeof(double)));
+  double *out = tmp.as<double>() + n_blocks_for(n) + 1;
+  TQ_CHECK(reduce(h, x, x, n, out, tmp.p, true));
+  TQ_HIP(h, hipMemcpyAsync(result, out, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  return TQ_SUCCESS;
+}
+
+tq_status tq_scale_f64(tq_handle_t h, int64_t n, double *x, double alpha) {
+  if (!h) return TQ_ERROR_NOT_INITIALIZED;
+  if (n < 0 || (n > 0 && !x)) return TQ_ERROR_INVALID_VALUE;
+  if (n == 0) return TQ_SUCCESS;
+  const int64_t g = std::min<int64_t>(ceil_div(n, 256), (int64_t)h->n_cu * 16);
+  hipLaunchKernelGGL(scale_kernel, dim3((unsigned)g), dim3(256), 0, h->stream, x, n, alpha);
+  TQ_HIP(h, hipGetLastError());
+  return TQ_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,523 @@
+"""Python mirror of the reference's GPU tensor classes over the C-ABI.
+
+`BlockSparseTensor` follows torque::gpu::BlockSparseTensor<T>
+(src/gpu/block_sparse.cuh:142-1181) and `DenseTensor` follows
+torque::gpu::DenseTensor<T> (src/gpu/dense.cuh:52-604): same member names,
+same method names and argument meaning, same "Error: " messages. Device data
+lives in a flat torch tensor (torch provides the allocation and the stream);
+every contraction and permutation runs in libtorque_hip.so and raises when
+the tensor is not on a GPU -- there is no host fallback.
+
+Metadata arrays are numpy int64 and block-major: `begin_points[b, l]` is leg
+l of block b (the reference stores one umat column per block).
+"""
+import ctypes
+import hashlib
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import BsDesc, PlanInfo, TorqueError, check, lib
+
+_DT = {torch.float64: _lib.TQ_F64, torch.float32: _lib.TQ_F32}
+
+
+def _i64p(a):
+    return a.ctypes.data_as(_lib.c_i64_p) if a is not None and a.size else None
+
+
+def _i32p(a):
+    return a.ctypes.data_as(_lib.c_i32_p) if a is not None and a.size else None
+
+
+def index_table(dims):
+    """Column-major strides, util::generate_index_table (src/util/space.h:10-24)."""
+    t = np.ones(len(dims), dtype=np.int64)
+    acc = 1
+    for i, d in enumerate(dims):
+        t[i] = acc
+        acc *= int(d)
+    return t
+
+
+def nest_sum(v, initial=0):
+    """Exclusive prefix sum, util::nest_sum (src/util/space.h:76-88)."""
+    v = np.asarray(v, dtype=np.int64)
+    out = np.empty_like(v)
+    acc = initial
+    for i, x in enumerate(v):
+        out[i] = acc
+        acc += int(x)
+    return out
+
+
+class Handle:
+    """A library handle on one device; work is issued on torch's current
+    stream of that device (replaces the reference's cublasHandle_t)."""
+
+    def __init__(self, device=0):
+        self.device = int(device)
+        h = ctypes.c_void_p()
+        check(lib().tq_create(ctypes.byref(h), self.device))
+        self.ptr = h
+        self.plans = {}
+
+    def bind_stream(self):
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        check(lib().tq_set_stream(self.ptr, ctypes.c_void_p(s)), self.ptr)
+
+    def check(self, status):
+        check(status, self.ptr)
+
+    def synchronize(self):
+        self.check(lib().tq_synchronize(self.ptr))
+
+    def __del__(self):
+        try:
+            for p in self.plans.values():
+                p.destroy()
+            if self.ptr:
+                lib().tq_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+_handles = {}
+
+
+def default_handle(device=None):
+    if device is None:
+        device = torch.cuda.current_device()
+    if device not in _handles:
+        _handles[device] = Handle(device)
+    h = _handles[device]
+    h.bind_stream()
+    return h
+
+
+def _require_gpu(t):
+    if not t.is_cuda:
+        raise TorqueError(_lib.TQ_ERROR_UNSUPPORTED,
+                          "tensor data must be on a GPU (no host fallback)")
+
+
+class BlockSparseTensor:
+    """Block-sparse tensor; blocks may overlap and query() sums them."""
+
+    def __init__(self, dimension, source_data=None, begin_points=None,
+                 end_points=None, index_tables=None, dtype=torch.float64,
+                 device=None):
+        dim = np.asarray(dimension, dtype=np.int64).reshape(-1)
+        self.rank = int(dim.size)
+        self.dimension = dim
+        self.dtype = dtype
+        self.device = torch.device(device) if device is not None else (
+            torch.device("cuda") if torch.cuda.is_available() else torch.device("cpu"))
+        if begin_points is None:
+            # explicit BlockSparseTensor(dimension): one zeroed element
+            # (src/gpu/block_sparse.cuh:180-186)
+            self.begin_points = np.zeros((0, self.rank), dtype=np.int64)
+            self.end_points = np.zeros((0, self.rank), dtype=np.int64)
+            self.index_tables = np.zeros((0, self.rank), dtype=np.int64)
+            self.data = torch.zeros(1, dtype=dtype, device=self.device)
+            self._meta_changed()
+            return
+        b = np.asarray(begin_points, dtype=np.int64).reshape(-1, self.rank)
+        e = np.asarray(end_points, dtype=np.int64).reshape(-1, self.rank)
+        self.begin_points, self.end_points = b.copy(), e.copy()
+        bd = e - b + 1
+        if index_tables is None:
+            self.index_tables = np.stack([index_table(r) for r in bd]) if len(bd) else \
+                np.zeros((0, self.rank), dtype=np.int64)
+        else:
+            self.index_tables = np.asarray(index_tables, dtype=np.int64).reshape(-1, self.rank).copy()
+        n = int(np.prod(bd, axis=1).sum()) if len(bd) else 0
+        if source_data is None:
+            raise TorqueError(_lib.TQ_ERROR_INVALID_VALUE, "Source data not allocated!")
+        src = torch.as_tensor(source_data).reshape(-1)
+        self.data = src[:max(n, 1)].to(device=self.device, dtype=dtype).clone()
+        self._meta_changed()
+
+    # -- derived metadata (src/gpu/block_sparse.cuh:199-212) --
+    def _meta_changed(self):
+        self.blocks_dimension = self.end_points - self.begin_points + 1
+        self.block_n_elem = (np.prod(self.blocks_dimension, axis=1).astype(np.int64)
+                             if len(self.blocks_dimension) else np.zeros(0, np.int64))
+        self.block_offsets = nest_sum(self.block_n_elem)
+        self._desc = None
+        self._key = None
+
+    @property
+    def n_blocks(self):
+        return int(self.begin_points.shape[0])
+
+    def desc(self):
+        if self._desc is None:
+            self._arrs = [np.ascontiguousarray(a, dtype=np.int64) for a in (
+                self.dimension, self.begin_points, self.end_points,
+                self.index_tables, self.block_offsets)]
+            d = BsDesc()
+            d.rank = self.rank
+            d.n_blocks = self.n_blocks
+            d.dims, d.begin, d.end, d.strides, d.offsets = [_i64p(a) for a in self._arrs]
+            self._desc = d
+        return self._desc
+
+    def structure_key(self):
+        if self._key is None:
+            h = hashlib.sha1()
+            for a in (self.dimension, self.begin_points, self.end_points,
+                      self.index_tables, self.block_offsets):
+                h.update(np.ascontiguousarray(a).tobytes())
+                h.update(b"|")
+            self._key = h.hexdigest()
+        return self._key
+
+    # -- element access (src/gpu/block_sparse.cuh:448-552) --
+    def _check_indices(self, indices):
+        idx = np.asarray(indices, dtype=np.int64).reshape(-1)
+        if idx.size != self.rank:
+            raise TorqueError(_lib.TQ_ERROR_RANK_MISMATCH, "Rank does not match")
+        if np.any(idx >= self.dimension) or np.any(idx < 0):
+            raise TorqueError(_lib.TQ_ERROR_OUT_OF_RANGE, "Indices out of boundary")
+        return idx
+
+    def _covering(self, idx):
+        if self.n_blocks == 0:
+            return np.zeros(0, dtype=np.int64)
+        ok = np.all((idx >= self.begin_points) & (idx <= self.end_points), axis=1)
+        return np.nonzero(ok)[0]
+
+    def _address(self, blk, idx):
+        rel = idx - self.begin_points[blk]
+        return int(self.block_offsets[blk] + np.sum(rel * self.index_tables[blk]))
+
+    def query(self, indices):
+        idx = self._check_indices(indices)
+        if self.rank == 0:
+            return float(self.data[0])
+        cov = self._covering(idx)
+        total = 0.0
+        for blk in cov:
+            total += float(self.data[self._address(blk, idx)])
+        return total
+
+    def modify(self, indices, number):
+        idx = self._check_indices(indices)
+        if self.rank == 0:
+            self.data[0] = number
+            return
+        cov = self._covering(idx)
+        if len(cov):
+            self.data[self._address(cov[0], idx)] = number
+            for blk in cov[1:]:
+                self.data[self._address(blk, idx)] = 0
+        elif number != 0:
+            self.append_block([number], idx, idx, np.zeros(self.rank, dtype=np.int64))
+
+    def to_number(self):
+        assert self.rank == 0
+        return float(self.data[0])
+
+    def append_block(self, source_data, begin_point, end_point, index_table):
+        """src/gpu/block_sparse.cuh:376-410"""
+        b = np.asarray(begin_point, dtype=np.int64).reshape(1, self.rank)
+        e = np.asarray(end_point, dtype=np.int64).reshape(1, self.rank)
+        t = np.asarray(index_table, dtype=np.int64).reshape(1, self.rank)
+        n = int(np.prod(e - b + 1))
+        had_blocks = self.n_blocks > 0
+        old_n = int(self.block_n_elem.sum())
+        src = torch.as_tensor(source_data).reshape(-1)[:n].to(self.device, self.dtype)
+        self.begin_points = np.concatenate([self.begin_points, b])
+        self.end_points = np.concatenate([self.end_points, e])
+        self.index_tables = np.concatenate([self.index_tables, t])
+        keep = self.data[:old_n] if had_blocks else self.data[:0]
+        self.data = torch.cat([keep, src]).contiguous()
+        self._meta_changed()
+
+    def soft_transpose(self, permutation):
+        perm = np.asarray(permutation, dtype=np.int64)
+        if perm.size != self.rank:
+            raise TorqueError(_lib.TQ_ERROR_RANK_MISMATCH,
+                              "The number of permutation does not match the rank of tensor")
+        self.dimension = self.dimension[perm]
+        self.begin_points = self.begin_points[:, perm]
+        self.end_points = self.end_points[:, perm]
+        self.index_tables = self.index_tables[:, perm]
+        self._meta_changed()
+
+    # -- GPU operations --
+    def hard_transpose(self, permutation):
+        """BlockSparseTensor::hard_transpose (src/gpu/block_sparse.cuh:1031-1122)."""
+        perm = np.asarray(permutation, dtype=np.int32).reshape(-1)
+        if perm.size != self.rank:
+            raise TorqueError(_lib.TQ_ERROR_RANK_MISMATCH,
+                              "The number of permutation does not match the rank of tensor")
+        _require_gpu(self.data)
+        h = default_handle(self.data.device.index)
+        n = int(self.block_n_elem.sum())
+        out = torch.empty(max(n, 1), dtype=self.dtype, device=self.data.device)
+        offs = np.zeros(self.n_blocks, dtype=np.int64)
+        h.check(lib().tq_bs_permute(h.ptr, _DT[self.dtype], ctypes.byref(self.desc()),
+                                    _i32p(perm), ctypes.c_void_p(self.data.data_ptr()),
+                                    ctypes.c_void_p(out.data_ptr()), _i64p(offs)))
+        res = BlockSparseTensor.__new__(BlockSparseTensor)
+        res.rank, res.dtype, res.device = self.rank, self.dtype, self.data.device
+        res.dimension = self.dimension[perm]
+        res.begin_points = self.begin_points[:, perm].copy()
+        res.end_points = self.end_points[:, perm].copy()
+        bd = res.end_points - res.begin_points + 1
+        res.index_tables = (np.stack([index_table(r) for r in bd]) if len(bd)
+                            else np.zeros((0, self.rank), np.int64))
+        res.data = out
+        res._meta_changed()
+        return res
+
+    def plan_contract(self, tensor, contracting_indices, precision="native", handle=None):
+        """Build (or reuse) the device plan of contract(tensor, legs)."""
+        legs = np.asarray(contracting_indices, dtype=np.int32).reshape(-1, 2)
+        h = handle or default_handle(self.data.device.index if self.data.is_cuda else None)
+        key = (self.structure_key(), tensor.structure_key(), legs.tobytes(),
+               str(self.dtype), precision)
+        plan = h.plans.get(key)
+        if plan is None:
+            plan = ContractionPlan(h, self, tensor, legs, precision)
+            if len(h.plans) > 64:
+                h.plans.pop(next(iter(h.plans))).destroy()
+            h.plans[key] = plan
+        return plan
+
+    def contract(self, tensor, contracting_indices, precision="native", handle=None):
+        """BlockSparseTensor::contract (src/gpu/block_sparse.cuh:558-1026):
+        one output block per matched pair, ordered by A block then B block."""
+        if tensor.dtype != self.dtype:
+            raise TorqueError(_lib.TQ_ERROR_UNSUPPORTED, "operands must share a dtype")
+        _require_gpu(self.data)
+        _require_gpu(tensor.data)
+        plan = self.plan_contract(tensor, contracting_indices, precision, handle)
+        out = torch.empty(max(plan.out_n_elem, 1), dtype=self.dtype,
+                          device=self.data.device)
+        plan.execute(self.data, tensor.data, out)
+        return plan.output_tensor(out)
+
+
+class ContractionPlan:
+    """Owner of a tq_plan_t: device matching, index maps, pack jobs and the
+    grouped-GEMM work list of one contraction structure."""
+
+    def __init__(self, handle, a, b, legs, precision="native"):
+        self.h = handle
+        legs = np.ascontiguousarray(np.asarray(legs, dtype=np.int32).reshape(-1, 2))
+        self.legs = legs
+        self.dtype = a.dtype
+        p = ctypes.c_void_p()
+        handle.check(lib().tq_bs_contract_plan(
+            handle.ptr, ctypes.byref(a.desc()), ctypes.byref(b.desc()),
+            legs.shape[0], _i32p(legs), _DT[a.dtype], _lib.PRECISIONS[precision],
+            ctypes.byref(p)))
+        self.ptr = p
+        info = PlanInfo()
+        handle.check(lib().tq_plan_get_info(p, ctypes.byref(info)))
+        self.out_rank = info.out_rank
+        self.n_pairs = info.n_pairs
+        self.out_n_elem = info.out_n_elem
+        self.flops = info.flops
+        self.bytes = info.bytes
+        self.a_layout, self.b_layout = info.a_layout, info.b_layout
+        ro, npairs = self.out_rank, self.n_pairs
+        self.out_dims = np.zeros(ro, np.int64)
+        self.out_begin = np.zeros((npairs, ro), np.int64)
+        self.out_end = np.zeros((npairs, ro), np.int64)
+        self.out_strides = np.zeros((npairs, ro), np.int64)
+        self.out_offsets = np.zeros(npairs, np.int64)
+        handle.check(lib().tq_plan_get_output(
+            p, _i64p(self.out_dims), _i64p(self.out_begin), _i64p(self.out_end),
+            _i64p(self.out_strides), _i64p(self.out_offsets)))
+        self.device = a.data.device
+
+    def pairs(self):
+        n = self.n_pairs
+        arrs = [np.zeros(n, np.int64) for _ in range(7)]
+        self.h.check(lib().tq_plan_get_pairs(self.ptr, *[_i64p(x) for x in arrs]))
+        keys = ["a_block", "b_block", "a_offset", "b_offset", "m", "n", "k"]
+        return dict(zip(keys, arrs))
+
+    def worklist(self):
+        order = np.zeros(self.n_pairs, np.int64)
+        self.h.check(lib().tq_plan_get_worklist(self.ptr, _i64p(order)))
+        return order
+
+    def set_config(self, cfg):
+        self.h.check(lib().tq_plan_set_config(self.ptr, int(cfg)))
+
+    def execute(self, a_data, b_data, out):
+        self.h.bind_stream()
+        self.h.check(lib().tq_bs_contract_execute(
+            self.ptr, ctypes.c_void_p(a_data.data_ptr()),
+            ctypes.c_void_p(b_data.data_ptr()), ctypes.c_void_p(out.data_ptr())))
+
+    def output_tensor(self, out):
+        res = BlockSparseTensor.__new__(BlockSparseTensor)
+        res.rank, res.dtype, res.device = self.out_rank, self.dtype, out.device
+        res.dimension = self.out_dims.copy()
+        if self.out_rank == 0:
+            res.begin_points = np.zeros((0, 0), np.int64)
+            res.end_points = np.zeros((0, 0), np.int64)
+            res.index_tables = np.zeros((0, 0), np.int64)
+        else:
+            res.begin_points = self.out_begin.copy()
+            res.end_points = self.out_end.copy()
+            res.index_tables = self.out_strides.copy()
+        res.data = out
+        res._meta_changed()
+        return res
+
+    def destroy(self):
+        if getattr(self, "ptr", None):
+            lib().tq_plan_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def gemm_configs(precision="native"):
+    L = lib()
+    pr = _lib.PRECISIONS[precision]
+    out = []
+    for c in range(L.tq_gemm_config_count(pr)):
+        bm, bn, bk = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(L.tq_gemm_config_describe(pr, c, ctypes.byref(bm), ctypes.byref(bn),
+                                        ctypes.byref(bk)))
+        out.append((bm.value, bn.value, bk.value))
+    return out
+
+
+class DenseTensor:
+    """Dense tensor (src/gpu/dense.cuh:52-604); column-major, leg 0 fastest."""
+
+    def __init__(self, dimension, source_data=None, dtype=torch.float64, device=None):
+        dim = np.asarray(dimension, dtype=np.int64).reshape(-1)
+        self.rank = int(dim.size)
+        self.dimension = dim
+        self.index_table = index_table(dim)
+        self.dtype = dtype
+        dev = torch.device(device) if device is not None else (
+            torch.device("cuda") if torch.cuda.is_available() else torch.device("cpu"))
+        n = int(np.prod(dim)) if self.rank else 1
+        if source_data is None:
+            self.data = torch.zeros(n, dtype=dtype, device=dev)
+        else:
+            self.data = torch.as_tensor(source_data).reshape(-1)[:n].to(dev, dtype).clone()
+
+    def query(self, indices):
+        idx = np.asarray(indices, dtype=np.int64).reshape(-1)
+        if idx.size != self.rank:
+            raise TorqueError(_lib.TQ_ERROR_RANK_MISMATCH, "Rank does not match")
+        if np.any(idx >= self.dimension) or np.any(idx < 0):
+            raise TorqueError(_lib.TQ_ERROR_OUT_OF_RANGE, "Indices out of boundary")
+        return float(self.data[int(np.sum(idx * self.index_table))])
+
+    def modify(self, indices, number):
+        idx = np.asarray(indices, dtype=np.int64).reshape(-1)
+        if idx.size != self.rank:
+            raise TorqueError(_lib.TQ_ERROR_RANK_MISMATCH, "Rank does not match")
+        if np.any(idx >= self.dimension) or np.any(idx < 0):
+            raise TorqueError(_lib.TQ_ERROR_OUT_OF_RANGE, "Indices out of boundary")
+        self.data[int(np.sum(idx * self.index_table))] = number
+
+    def initialize(self, source_data):
+        n = self.data.numel()
+        self.data.copy_(torch.as_tensor(source_data).reshape(-1)[:n].to(self.data))
+
+    def to_number(self):
+        assert self.rank == 0
+        return float(self.data[0])
+
+    def soft_transpose(self, permutation):
+        perm = np.asarray(permutation, dtype=np.int64)
+        if perm.size != self.rank:
+            raise TorqueError(_lib.TQ_ERROR_RANK_MISMATCH,
+                              "The number of permutation does not match the rank of tensor")
+        self.index_table = self.index_table[perm]
+        self.dimension = self.dimension[perm]
+
+    def hard_transpose(self, permutation):
+        """DenseTensor::hard_transpose (src/gpu/dense.cuh:561-599)."""
+        perm = np.asarray(permutation, dtype=np.int32).reshape(-1)
+        if perm.size != self.rank:
+            raise TorqueError(_lib.TQ_ERROR_RANK_MISMATCH,
+                              "The number of permutation does not match the rank of tensor")
+        _require_gpu(self.data)
+        h = default_handle(self.data.device.index)
+        # undo a soft transpose: storage dims are the dims sorted by stride
+        order = np.argsort(self.index_table, kind="stable")
+        store_dims = np.ascontiguousarray(self.dimension[order], dtype=np.int64)
+        inv = np.empty_like(order)
+        inv[order] = np.arange(self.rank)
+        # leg l of this tensor is storage leg inv[l]; out leg k = leg perm[k]
+        sperm = np.ascontiguousarray(inv[perm], dtype=np.int32)
+        out = torch.empty_like(self.data)
+        h.check(lib().tq_permute(h.ptr, _DT[self.dtype], self.rank, _i64p(store_dims),
+                                 None, _i32p(sperm), ctypes.c_void_p(self.data.data_ptr()),
+                                 ctypes.c_void_p(out.data_ptr())))
+        res = DenseTensor.__new__(DenseTensor)
+        res.rank, res.dtype = self.rank, self.dtype
+        res.dimension = self.dimension[perm.astype(np.int64)]
+        res.index_table = index_table(res.dimension)
+        res.data = out
+        return res
+
+    def contract(self, tensor, contracting_indices):
+        """DenseTensor::contract (src/gpu/dense.cuh:439-555)."""
+        legs = np.ascontiguousarray(np.asarray(contracting_indices, dtype=np.int32).reshape(-1, 2))
+        for a_leg, b_leg in legs:
+            if self.dimension[a_leg] != tensor.dimension[b_leg]:
+                raise TorqueError(_lib.TQ_ERROR_DIMENSION_MISMATCH,
+                                  "The dimensions from two tensors to be contracted do not match")
+        _require_gpu(self.data)
+        h = default_handle(self.data.device.index)
+        a_free = [l for l in range(self.rank) if l not in legs[:, 0]]
+        b_free = [l for l in range(tensor.rank) if l not in legs[:, 1]]
+        new_dim = np.concatenate([self.dimension[a_free], tensor.dimension[b_free]]).astype(np.int64)
+        n = int(np.prod(new_dim)) if new_dim.size else 1
+        out = torch.empty(n, dtype=self.dtype, device=self.data.device)
+        da = np.ascontiguousarray(self.dimension, np.int64)
+        db = np.ascontiguousarray(tensor.dimension, np.int64)
+        h.check(lib().tq_dense_contract(h.ptr, _DT[self.dtype], self.rank, _i64p(da),
+                                        tensor.rank, _i64p(db), legs.shape[0], _i32p(legs),
+                                        ctypes.c_void_p(self.data.data_ptr()),
+                                        ctypes.c_void_p(tensor.data.data_ptr()),
+                                        ctypes.c_void_p(out.data_ptr())))
+        res = DenseTensor.__new__(DenseTensor)
+        res.rank, res.dtype = int(new_dim.size), self.dtype
+        res.dimension = new_dim
+        res.index_table = index_table(new_dim)
+        res.data = out
+        return res
+
+
+def dot_f64(x, y):
+    h = default_handle(x.device.index)
+    r = ctypes.c_double()
+    h.check(lib().tq_dot_f64(h.ptr, x.numel(), ctypes.c_void_p(x.data_ptr()),
+                             ctypes.c_void_p(y.data_ptr()), ctypes.byref(r)))
+    return r.value
+
+
+def nrm2_f64(x):
+    h = default_handle(x.device.index)
+    r = ctypes.c_double()
+    h.check(lib().tq_nrm2_f64(h.ptr, x.numel(), ctypes.c_void_p(x.data_ptr()),
+                              ctypes.byref(r)))
+    return r.value
+
+
+def scale_f64(x, alpha):
+    h = default_handle(x.device.index)
+    h.check(lib().tq_scale_f64(h.ptr, x.numel(), ctypes.c_void_p(x.data_ptr()),
+                               ctypes.c_double(alpha)))
