@@ -1,0 +1,218 @@
+"""Benchmark: block-sparse contraction GFLOP/s on one MI355X.
+
+Workload (default): the PEPS CTMRG-style absorption step at D=8, chi=128
+with U(1) charges -2..2 and entries N(0,1) * 10^U(-3,0) per block. One step
+is the double layer a = A . conj(A) over the physical leg followed by
+E(chi,D,D,chi) . a over (l, l'): two block-sparse contractions through the
+C-ABI (operand blocks permuted to GEMM layout, then one grouped-GEMM launch
+each). Plans are built once (the block structure is fixed across CTMRG
+steps); inputs are resident in HBM when the timed region starts.
+
+Prints one JSON line: FP64 GFLOP/s as the value, a live roofline of the
+dominant kernel (the E . a grouped GEMM, timed with HIP events on the stream
+it runs on), the FP32 and split-BF16 tiers with their relative Frobenius
+error against the FP64 CPU oracle on a sample of output blocks, and the CPU
+oracle timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Block-sparse contract GFLOP/s + %peak: FP64 vs 3xBF16 at ≤1e-6 rel err"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--D", type=int, default=8)
+    p.add_argument("--chi", type=int, default=128)
+    p.add_argument("--seed", type=int, default=0)
+    p.add_argument("--tiers", default="fp32,bf16x3")
+    p.add_argument("--cpu-sample-blocks", type=int, default=6,
+                   help="E blocks in the CPU-oracle sample")
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see the sharding design); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from torque_amd import BlockSparseTensor, TorqueError
+    from torque_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    E, A, Ac, l1, l2 = W.peps(args.D, args.chi, seed=args.seed, log10_scale_range=(-3.0, 0.0))
+    gE = BlockSparseTensor(E.dims, E.data, E.begin, E.end, device=dev)
+    gA = BlockSparseTensor(A.dims, A.data, A.begin, A.end, device=dev)
+    gAc = BlockSparseTensor(Ac.dims, Ac.data, Ac.begin, Ac.end, device=dev)
+
+    t0 = time.perf_counter()
+    p1 = gA.plan_contract(gAc, l1)
+    aa = torch.empty(max(p1.out_n_elem, 1), dtype=torch.float64, device=dev)
+    p1.execute(gA.data, gAc.data, aa)
+    ga = p1.output_tensor(aa)
+    p2 = gE.plan_contract(ga, l2)
+    torch.cuda.synchronize()
+    plan_s = time.perf_counter() - t0
+    out = torch.empty(max(p2.out_n_elem, 1), dtype=torch.float64, device=dev)
+    print("# workload D=%d chi=%d: A.A* %d pairs, E.a %d pairs, %.3f GFLOP, out %.1f M elem, "
+          "plans %.3f s" % (args.D, args.chi, p1.n_pairs, p2.n_pairs,
+                            (p1.flops + p2.flops) / 1e9, p2.out_n_elem / 1e6, plan_s),
+          flush=True)
+
+    def step():
+        p1.execute(gA.data, gAc.data, aa)
+        p2.execute(gE.data, aa, out)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    flops = p1.flops + p2.flops
+    value = flops / dt / 1e9
+    print("# fp64 step %.3f ms, %.1f GFLOP/s" % (dt * 1e3, value), flush=True)
+
+    # dominant kernel: the E.a grouped GEMM, events on the handle's stream
+    stream = torch.cuda.current_stream(dev)
+    p2.pack(gE.data, aa)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    n_rep = max(args.steps, 5)
+    ev0.record(stream)
+    for _ in range(n_rep):
+        p2.multiply(gE.data, aa, out)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    gemm_s = ev0.elapsed_time(ev1) / 1e3 / n_rep
+    alg_bytes = p2.bytes
+    achieved = alg_bytes / gemm_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "grouped_gemm_kernel (E.a)", "kernel_ms": round(gemm_s * 1e3, 4),
+                "kernel_gflops": round(p2.flops / gemm_s / 1e9, 1),
+                "algorithmic_bytes": alg_bytes}
+    print("# E.a grouped GEMM %.3f ms: %.0f GB/s algorithmic (%.1f%% of HBM), %.0f GFLOP/s"
+          % (gemm_s * 1e3, achieved, 100 * achieved / HBM_PEAK_GBS, p2.flops / gemm_s / 1e9),
+          flush=True)
+
+    # ---- oracle sample: E blocks [0, s) x all of a, pairs contiguous in out
+    from oracle import oracle as O
+    s_blocks = min(args.cpu_sample_blocks, E.n_blocks)
+    n_sub = int(np.prod(E.end[:s_blocks] - E.begin[:s_blocks] + 1, axis=1).sum())
+    TEs = O.BS(E.dims, E.begin[:s_blocks], E.end[:s_blocks])
+    Ta = O.BS(ga.dimension, ga.begin_points, ga.end_points)
+    aa_host = aa.cpu().numpy()
+    threads = min(16, os.cpu_count() or 1)
+    cpu_baseline = None
+    t0 = time.perf_counter()
+    ref_c, ref_p = O.contract(TEs, E.data[:n_sub], Ta, aa_host, l2, threads=threads)
+    cpu_s = time.perf_counter() - t0
+    sample_flops = float((2 * ref_p["m"] * ref_p["n"] * ref_p["k"]).sum())
+    if not args.no_cpu:
+        # time again (warm) and report
+        t0 = time.perf_counter()
+        O.contract(TEs, E.data[:n_sub], Ta, aa_host, l2, threads=threads)
+        cpu_s = time.perf_counter() - t0
+        cpu_baseline = {"value": round(sample_flops / cpu_s / 1e9, 3), "unit": "GFLOP/s",
+                        "cores": threads, "kind": "port",
+                        "sample": "E.a restricted to the first %d E blocks (%d pairs, %.3f "
+                                  "GFLOP), oracle per-pair pack + GEMM, %.2f s"
+                                  % (s_blocks, ref_p["m"].size, sample_flops / 1e9, cpu_s)}
+    n_ref = ref_c.size
+
+    def rel(x):
+        return float(np.linalg.norm(x[:n_ref] - ref_c) / np.linalg.norm(ref_c))
+
+    def per_block_max(x):
+        worst = 0.0
+        offs = ref_p["out_offsets"]
+        sizes = ref_p["m"] * ref_p["n"]
+        for o, n in zip(offs, sizes):
+            r = ref_c[o:o + n]
+            nr = np.linalg.norm(r)
+            if nr > 0:
+                worst = max(worst, float(np.linalg.norm(x[o:o + n] - r) / nr))
+        return worst
+
+    got64 = out[:n_ref].cpu().numpy()
+    err64 = rel(got64)
+    tiers = {"fp64": {"gflops": round(value, 1), "rel_err": err64,
+                      "max_block_rel_err": per_block_max(got64)}}
+
+    for tier in [t for t in args.tiers.split(",") if t]:
+        try:
+            pt1 = gA.plan_contract(gAc, l1, precision=tier)
+            pt2 = gE.plan_contract(ga, l2, precision=tier)
+        except TorqueError as e:
+            tiers[tier] = {"error": str(e)}
+            continue
+        aa_t = torch.empty_like(aa)
+        out_t = torch.empty_like(out)
+
+        def tstep():
+            pt1.execute(gA.data, gAc.data, aa_t)
+            pt2.execute(gE.data, aa_t, out_t)
+
+        try:
+            for _ in range(args.warmup):
+                tstep()
+        except TorqueError as e:
+            tiers[tier] = {"error": str(e)}
+            continue
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            tstep()
+        torch.cuda.synchronize()
+        tdt = (time.perf_counter() - t0) / args.steps
+        # error of the tier's E.a on the FP64 double layer, vs the oracle
+        pt2.execute(gE.data, aa, out_t)
+        got = out_t[:n_ref].cpu().numpy()
+        tiers[tier] = {"gflops": round(flops / tdt / 1e9, 1), "ms_per_step": round(tdt * 1e3, 3),
+                       "rel_err": rel(got), "max_block_rel_err": per_block_max(got),
+                       "speedup_vs_fp64": round(dt / tdt, 3)}
+        print("# %s step %.3f ms, %.1f GFLOP/s, rel err %.2e" %
+              (tier, tdt * 1e3, flops / tdt / 1e9, rel(got)), flush=True)
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "GFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "peps_absorption_D%d_chi%d" % (args.D, args.chi),
+                   "charges": "-2..2" if args.chi <= 128 else "-3..3",
+                   "pairs": [int(p1.n_pairs), int(p2.n_pairs)],
+                   "gflop_per_step": round(flops / 1e9, 4),
+                   "out_elements": int(p2.out_n_elem), "plan_seconds": round(plan_s, 3),
+                   "entries": "N(0,1)*10^U(-3,0) per block", "seed": args.seed},
+        "roofline": roofline,
+        "cpu_baseline": cpu_baseline,
+        "tiers": tiers,
+        "fp64_rel_err_vs_oracle": err64,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -181,6 +181,15 @@ tq_status tq_plan_get_worklist(tq_plan_t plan, int64_t *order);
  * contraction) is the network scalar, reduced on the GPU in FP64. */
 tq_status tq_bs_contract_execute(tq_plan_t plan, const void *a, const void *b,
                                  void *c);
+/* The two stages of tq_bs_contract_execute, for callers that time or
+ * schedule them separately: pack permutes the operand blocks that are not
+ * already in GEMM layout into the plan's panels (the reference's
+ * reshape_kernel + cuTT steps, src/gpu/block_sparse.cuh:755-865); multiply
+ * runs the grouped GEMM (or the rank-0 reduction) and must follow a pack of
+ * the same A and B. */
+tq_status tq_bs_contract_pack(tq_plan_t plan, const void *a, const void *b);
+tq_status tq_bs_contract_multiply(tq_plan_t plan, const void *a, const void *b,
+                                  void *c);
 /* Select the grouped-GEMM tile configuration by index (see
  * tq_gemm_config_count); -1 restores the plan's automatic choice. */
 tq_status tq_plan_set_config(tq_plan_t plan, int32_t config);

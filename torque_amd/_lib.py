@@ -81,6 +81,8 @@ def _proto(lib):
                                   c_i64_p, c_i64_p]),
         "tq_plan_get_worklist": (S, [V, c_i64_p]),
         "tq_bs_contract_execute": (S, [V, V, V, V]),
+        "tq_bs_contract_pack": (S, [V, V, V]),
+        "tq_bs_contract_multiply": (S, [V, V, V, V]),
         "tq_plan_set_config": (S, [V, ctypes.c_int32]),
         "tq_plan_destroy": (S, [V]),
         "tq_gemm_config_count": (ctypes.c_int32, [S]),

@@ -92,16 +92,10 @@ tq_status tq_plan_set_config(tq_plan_t pl, int32_t config) {
   return TQ_SUCCESS;
 }
 
-tq_status tq_bs_contract_execute(tq_plan_t pl, const void *a, const void *b,
-                                 void *c) {
+tq_status tq_bs_contract_pack(tq_plan_t pl, const void *a, const void *b) {
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
   tq_handle_t h = pl->h;
-  if (!c) return TQ_ERROR_INVALID_VALUE;
-  const size_t es = pl->dtype == TQ_F64 ? 8 : 4;
-  if (pl->n_pairs == 0) {
-    if (pl->ro == 0) TQ_HIP(h, hipMemsetAsync(c, 0, es, h->stream));
-    return TQ_SUCCESS;
-  }
+  if (pl->n_pairs == 0) return TQ_SUCCESS;
   if (!a || !b) return TQ_ERROR_INVALID_VALUE;
   if (pl->pack_a.mode != kModeDirect)
     TQ_CHECK(tq_launch_gather_jobs(h, pl->dtype, pl->pack_a.jobs.as<CopyJob>(),
@@ -111,6 +105,20 @@ tq_status tq_bs_contract_execute(tq_plan_t pl, const void *a, const void *b,
     TQ_CHECK(tq_launch_gather_jobs(h, pl->dtype, pl->pack_b.jobs.as<CopyJob>(),
                                    pl->pack_b.wg32.as<int32_t>(), pl->pack_b.n_jobs,
                                    pl->pack_b.total_wg, b, pl->pack_b.panel.p));
+  return TQ_SUCCESS;
+}
+
+tq_status tq_bs_contract_multiply(tq_plan_t pl, const void *a, const void *b,
+                                  void *c) {
+  if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  tq_handle_t h = pl->h;
+  if (!c) return TQ_ERROR_INVALID_VALUE;
+  const size_t es = pl->dtype == TQ_F64 ? 8 : 4;
+  if (pl->n_pairs == 0) {
+    if (pl->ro == 0) TQ_HIP(h, hipMemsetAsync(c, 0, es, h->stream));
+    return TQ_SUCCESS;
+  }
+  if (!a || !b) return TQ_ERROR_INVALID_VALUE;
   if (pl->ro > 0) {
     const int cfg = pl->cfg >= 0 ? pl->cfg : pl->auto_cfg;
     TQ_CHECK(plan_build_tiles(pl, cfg));
@@ -147,6 +155,12 @@ tq_status tq_bs_contract_execute(tq_plan_t pl, const void *a, const void *b,
                        sum, (float *)c);
   TQ_HIP(h, hipGetLastError());
   return TQ_SUCCESS;
+}
+
+tq_status tq_bs_contract_execute(tq_plan_t pl, const void *a, const void *b,
+                                 void *c) {
+  TQ_CHECK(tq_bs_contract_pack(pl, a, b));
+  return tq_bs_contract_multiply(pl, a, b, c);
 }
 
 tq_status tq_dense_contract(tq_handle_t h, tq_dtype dtype, int32_t rank_a,

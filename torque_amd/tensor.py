@@ -357,6 +357,17 @@ class ContractionPlan:
             self.ptr, ctypes.c_void_p(a_data.data_ptr()),
             ctypes.c_void_p(b_data.data_ptr()), ctypes.c_void_p(out.data_ptr())))
 
+    def pack(self, a_data, b_data):
+        self.h.bind_stream()
+        self.h.check(lib().tq_bs_contract_pack(
+            self.ptr, ctypes.c_void_p(a_data.data_ptr()), ctypes.c_void_p(b_data.data_ptr())))
+
+    def multiply(self, a_data, b_data, out):
+        self.h.bind_stream()
+        self.h.check(lib().tq_bs_contract_multiply(
+            self.ptr, ctypes.c_void_p(a_data.data_ptr()),
+            ctypes.c_void_p(b_data.data_ptr()), ctypes.c_void_p(out.data_ptr())))
+
     def output_tensor(self, out):
         res = BlockSparseTensor.__new__(BlockSparseTensor)
         res.rank, res.dtype, res.device = self.out_rank, self.dtype, out.device
