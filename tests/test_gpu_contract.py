@@ -235,3 +235,27 @@ def test_reductions(cuda):
     assert abs(nrm2_f64(gx) - 1.0) < 1e-14
     # repeated reductions agree bitwise
     assert dot_f64(gx, gy) == dot_f64(gx, gy)
+
+
+@pytest.mark.parametrize("maker", ["c1", "batched", "overlap"])
+def test_bf16x3_tier(cuda, maker):
+    # relative Frobenius error against the FP64 oracle stays below 1e-6
+    if maker == "c1":
+        a, b, legs = W.c1(6)
+        (A, da), (B, db) = from_workload(a), from_workload(b)
+    elif maker == "batched":
+        a, b, legs = W.batched(1, seed=13, dim_lo=40, dim_hi=96)[0]
+        (A, da), (B, db) = from_workload(a), from_workload(b)
+    else:
+        rng = np.random.default_rng(77)
+        ba, ea = random_blocks(rng, [6, 5, 7], 6)
+        bb, eb = random_blocks(rng, [7, 5, 4], 5)
+        A, da = tensor([6, 5, 7], ba, ea, rng)
+        B, db = tensor([7, 5, 4], bb, eb, rng)
+        legs = [[2, 0], [1, 1]]
+    ref_c, ref = O.contract(A, da, B, db, legs)
+    for cfg in range(len(gemm_configs("bf16x3"))):
+        plan, got = gpu_contract(A, da, B, db, legs, cuda, precision="bf16x3", cfg=cfg)
+        check_maps(plan, ref)
+        if plan.n_pairs:
+            assert rel_fro(got, ref_c) < 1e-6, cfg

@@ -79,15 +79,34 @@ __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) {
 
 // One strided-gather job: dst[dst_off + e] = src[src_off + sum_l d_l * s_l]
 // where e is the column-major linear index over dims (leg 0 fastest).
+struct GemmDesc;
+
+// For the split-BF16 pack (kdim > 0) the leading legs multiply to kdim and
+// element e lands at dst_off + (e / kdim) * kpad + e % kdim of each of the
+// three BF16 planes.
 struct CopyJob {
   i64 src_off;
   i64 dst_off;
   i64 n_elem;
   int32_t rank;
+  int32_t kdim;
+  int32_t kpad;
   int32_t pad;
   int32_t dims[TQ_MAX_RANK];
   i64 strides[TQ_MAX_RANK];
 };
+
+// Split-BF16 pack: like tq_launch_gather_jobs, reading FP64 and writing the
+// three BF16 pieces of each value to planes dst, dst + plane, dst + 2*plane.
+tq_status tq_launch_split_jobs(tq_handle_t h, const CopyJob *jobs,
+                               const int32_t *job_wg_start, int64_t n_jobs,
+                               int64_t total_wg, const double *src,
+                               uint16_t *dst, int64_t plane);
+tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, const GemmDesc *descs,
+                                const int32_t *tile_pair,
+                                const int32_t *tile_first, int64_t n_tiles,
+                                const uint16_t *a_split, const uint16_t *b_split,
+                                int64_t a_plane, int64_t b_plane, double *c);
 
 // Runs a list of gather jobs that already sit in device memory. job_wg_start
 // is the exclusive prefix sum of workgroups per job (n_jobs + 1 entries).

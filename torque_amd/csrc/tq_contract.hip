@@ -97,14 +97,21 @@ tq_status tq_bs_contract_pack(tq_plan_t pl, const void *a, const void *b) {
   tq_handle_t h = pl->h;
   if (pl->n_pairs == 0) return TQ_SUCCESS;
   if (!a || !b) return TQ_ERROR_INVALID_VALUE;
-  if (pl->pack_a.mode != kModeDirect)
-    TQ_CHECK(tq_launch_gather_jobs(h, pl->dtype, pl->pack_a.jobs.as<CopyJob>(),
-                                   pl->pack_a.wg32.as<int32_t>(), pl->pack_a.n_jobs,
-                                   pl->pack_a.total_wg, a, pl->pack_a.panel.p));
-  if (pl->pack_b.mode != kModeDirect)
-    TQ_CHECK(tq_launch_gather_jobs(h, pl->dtype, pl->pack_b.jobs.as<CopyJob>(),
-                                   pl->pack_b.wg32.as<int32_t>(), pl->pack_b.n_jobs,
-                                   pl->pack_b.total_wg, b, pl->pack_b.panel.p));
+  const void *src[2] = {a, b};
+  PackSide *sides[2] = {&pl->pack_a, &pl->pack_b};
+  for (int q = 0; q < 2; ++q) {
+    PackSide &sd = *sides[q];
+    if (sd.mode == kModeDirect) continue;
+    if (sd.split)
+      TQ_CHECK(tq_launch_split_jobs(h, sd.jobs.as<CopyJob>(), sd.wg32.as<int32_t>(),
+                                    sd.n_jobs, sd.total_wg,
+                                    reinterpret_cast<const double *>(src[q]),
+                                    sd.panel.as<uint16_t>(), sd.panel_elems));
+    else
+      TQ_CHECK(tq_launch_gather_jobs(h, pl->dtype, sd.jobs.as<CopyJob>(),
+                                     sd.wg32.as<int32_t>(), sd.n_jobs, sd.total_wg,
+                                     src[q], sd.panel.p));
+  }
   return TQ_SUCCESS;
 }
 
@@ -122,6 +129,14 @@ tq_status tq_bs_contract_multiply(tq_plan_t pl, const void *a, const void *b,
   if (pl->ro > 0) {
     const int cfg = pl->cfg >= 0 ? pl->cfg : pl->auto_cfg;
     TQ_CHECK(plan_build_tiles(pl, cfg));
+    if (pl->prec == TQ_PREC_BF16X3)
+      return tq_launch_bf16x3_gemm(h, cfg, pl->desc.as<GemmDesc>(),
+                                   pl->tile_pair.as<int32_t>(),
+                                   pl->tile_first.as<int32_t>(), pl->n_tiles,
+                                   pl->pack_a.panel.as<uint16_t>(),
+                                   pl->pack_b.panel.as<uint16_t>(),
+                                   pl->pack_a.panel_elems, pl->pack_b.panel_elems,
+                                   reinterpret_cast<double *>(c));
     return tq_launch_grouped_gemm(h, pl->dtype, pl->prec, cfg,
                                   pl->desc.as<GemmDesc>(),
                                   pl->tile_pair.as<int32_t>(),

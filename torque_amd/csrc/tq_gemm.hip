@@ -245,11 +245,6 @@ tq_status dispatch(tq_handle_t h, int cfg, const GemmDesc *descs,
 
 }  // namespace
 
-tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, const GemmDesc *descs,
-                                const int32_t *tile_pair,
-                                const int32_t *tile_first, int64_t n_tiles,
-                                const void *a_split, const void *b_split,
-                                int64_t a_plane, int64_t b_plane, void *c);
 int tq_bf16x3_n_configs();
 GemmConfig tq_bf16x3_config(int cfg);
 
@@ -272,7 +267,7 @@ tq_status tq_launch_grouped_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec,
                                  const void *b, const void *b_packed, void *c) {
   if (n_tiles <= 0) return TQ_SUCCESS;
   if (prec == TQ_PREC_BF16X3)
-    return tq_fail(h, TQ_ERROR_UNSUPPORTED, "split-BF16 tier needs split panels");
+    return tq_fail(h, TQ_ERROR_INVALID_VALUE, "split-BF16 tier runs on split panels");
   if (dt == TQ_F32)
     return dispatch<float, float, float>(h, cfg, descs, tile_pair, tile_first,
                                          n_tiles, a, a_packed, b, b_packed, c);

@@ -353,6 +353,79 @@ __global__ void i64_to_i32_kernel(const i64 *__restrict__ in,
 
 }  // namespace
 
+namespace {
+
+// Three-way BF16 split of an FP64 value: hi = bf16(x), mid = bf16(x - hi),
+// lo = bf16(x - hi - mid), each rounded to nearest even from FP32. The
+// residuals are exact in FP64 (each piece is within a factor two of what it
+// approximates), so x - (hi + mid + lo) is below 2^-24 |x|.
+__device__ inline uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline double bf16_value(uint16_t h) {
+  return (double)__uint_as_float((uint32_t)h << 16);
+}
+
+__global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
+    const CopyJob *__restrict__ jobs, const int32_t *__restrict__ job_wg_start,
+    int64_t n_jobs, const double *__restrict__ src, uint16_t *__restrict__ dst,
+    int64_t plane) {
+  const int32_t total = job_wg_start[n_jobs];
+  for (int32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    int64_t lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+      int64_t mid = (lo + hi + 1) >> 1;
+      if (job_wg_start[mid] <= c)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const CopyJob &job = jobs[lo];
+    const int64_t e0 = (int64_t)(c - job_wg_start[lo]) * kGatherPerWG;
+    const uint32_t kd = (uint32_t)job.kdim;
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+      const int64_t e = e0 + (int64_t)q * kGatherThreads + threadIdx.x;
+      if (e >= job.n_elem) break;
+      uint32_t rem = (uint32_t)e;
+      i64 off = job.src_off;
+      for (int l = 0; l < job.rank; ++l) {
+        const uint32_t d = (uint32_t)job.dims[l];
+        const uint32_t qd = rem / d;
+        off += (i64)(rem - qd * d) * job.strides[l];
+        rem = qd;
+      }
+      const uint32_t row = (uint32_t)e / kd;
+      const i64 o = job.dst_off + (i64)row * job.kpad + ((uint32_t)e - row * kd);
+      const double x = src[off];
+      const uint16_t h = bf16_rne((float)x);
+      const double r1 = x - bf16_value(h);
+      const uint16_t m = bf16_rne((float)r1);
+      const double r2 = r1 - bf16_value(m);
+      const uint16_t l = bf16_rne((float)r2);
+      dst[o] = h;
+      dst[plane + o] = m;
+      dst[2 * plane + o] = l;
+    }
+  }
+}
+
+}  // namespace
+
+tq_status tq_launch_split_jobs(tq_handle_t h, const CopyJob *jobs,
+                               const int32_t *job_wg_start, int64_t n_jobs,
+                               int64_t total_wg, const double *src,
+                               uint16_t *dst, int64_t plane) {
+  if (n_jobs <= 0) return TQ_SUCCESS;
+  i64 grid = std::max<i64>(1, std::min<i64>(total_wg, (i64)h->n_cu * 8));
+  hipLaunchKernelGGL(split_jobs_kernel, dim3((unsigned)grid), dim3(kGatherThreads), 0,
+                     h->stream, jobs, job_wg_start, n_jobs, src, dst, plane);
+  TQ_HIP(h, hipGetLastError());
+  return TQ_SUCCESS;
+}
+
 tq_status tq_launch_gather_jobs(tq_handle_t h, tq_dtype dt, const CopyJob *jobs,
                                 const int32_t *job_wg_start, int64_t n_jobs,
                                 int64_t total_wg, const void *src, void *dst) {

@@ -19,16 +19,22 @@ struct GeomArgs {
   int32_t b_free[TQ_MAX_RANK];
 };
 
+// Whole-block panels. Plain panels list legs [free..., contracted in pair
+// order...]; split-BF16 panels list [contracted..., free...] (k fastest) and
+// pad each row of kdim values to kpad = kdim rounded up to 8.
 struct PanelArgs {
   int64_t n_blocks;
   int32_t rank;
+  int32_t nk;     // contracted legs
+  int32_t split;  // 1 = split-BF16 panel
   const i64 *begin, *end, *strides, *offsets;
-  int32_t order[TQ_MAX_RANK];  // [free..., contracted in pair order...]
+  int32_t order[TQ_MAX_RANK];
 };
 
 struct BoxArgs {
   int64_t n_pairs;
   int32_t is_b;
+  int32_t split;
   int32_t self_rank, other_rank, nl, n_free;
   const i64 *self_begin, *self_end, *other_begin, *other_end;
   int32_t free_legs[TQ_MAX_RANK];
@@ -43,6 +49,8 @@ enum : int32_t {
   kNeedPairPanelA = 4,  // some pair's A sub-box is not plain inside a block panel
   kNeedPairPanelB = 8,
   kTooLarge = 16,
+  kNotFullA = 32,  // some pair uses only part of its A block's contracted extent
+  kNotFullB = 64,
 };
 
 enum { kModeDirect = 0, kModeBlockPanel = 1, kModePairPanel = 2 };
@@ -54,6 +62,7 @@ struct PlanUpload {
 
 struct PackSide {
   int mode = kModeDirect;
+  bool split = false;  // panels hold three BF16 planes
   DevBuf panel;      // packed operand
   DevBuf panel_off;  // per block or per pair
   DevBuf jobs;       // CopyJob list
@@ -61,7 +70,7 @@ struct PackSide {
   DevBuf wg32;       // int32 workgroup prefix used by the gather kernel
   int64_t n_jobs = 0;
   int64_t total_wg = 0;
-  int64_t panel_elems = 0;
+  int64_t panel_elems = 0;  // elements per plane
 };
 
 struct tq_plan_s {

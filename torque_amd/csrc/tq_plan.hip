@@ -199,10 +199,11 @@ __global__ void geometry_kernel(GeomArgs g, const int32_t *__restrict__ pair_a,
     b_full = b_full && (cd == be[lb] - bb[lb] + 1);
     if (cd != 1) last_nonunit = k;
   }
+  // inside a block panel the sub-box is plain only if every contracted leg
+  // before the last non-unit one spans its whole block extent
   for (int k = 0; k < last_nonunit; ++k) {
     const int la = g.legs.a[k], lb = g.legs.b[k];
     const i64 cd = avd[naf + k];
-    if (cd == 1) continue;
     a_box = a_box && (cd == ae[la] - ab[la] + 1);
     b_box = b_box && (cd == be[lb] - bb[lb] + 1);
   }
@@ -238,6 +239,8 @@ __global__ void geometry_kernel(GeomArgs g, const int32_t *__restrict__ pair_a,
   d.ldb = (int32_t)(ldb < 0 ? N : ldb);
   if (!(a_full || a_box)) need |= kNeedPairPanelA;
   if (!(b_full || b_box)) need |= kNeedPairPanelB;
+  if (!a_full) need |= kNotFullA;
+  if (!b_full) need |= kNotFullB;
   if (M * K > 0x7fffffff || N * K > 0x7fffffff) need |= kTooLarge;
   desc[p] = d;
   mn[p] = M * N;
@@ -252,17 +255,24 @@ __global__ void set_c_offsets_kernel(int64_t n, const i64 *__restrict__ c_off,
   if (p < n) desc[p].c_off = c_off[p];
 }
 
-// Per-block sizes for whole-block panels: the panel of block i is the block
-// permuted to [free..., contracted in pair order...]; unused blocks get 0.
-__global__ void block_panel_sizes_kernel(int64_t n_blocks, int32_t rank,
-                                         const i64 *__restrict__ begin,
-                                         const i64 *__restrict__ end,
+__device__ inline i64 round8(i64 x) { return (x + 7) & ~(i64)7; }
+
+// Per-block panel sizes (unused blocks get 0). A plain panel is the block
+// permuted to [free..., contracted in pair order...]; a split-BF16 panel is
+// [contracted..., free...] with each row of kdim values padded to kpad.
+__global__ void block_panel_sizes_kernel(PanelArgs g,
                                          const uint8_t *__restrict__ used,
                                          i64 *__restrict__ sizes) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n_blocks) return;
-  i64 n = 1;
-  for (int l = 0; l < rank; ++l) n *= end[b * rank + l] - begin[b * rank + l] + 1;
+  if (b >= g.n_blocks) return;
+  i64 n = 1, kd = 1;
+  for (int q = 0; q < g.rank; ++q) {
+    const int l = g.order[q];
+    const i64 d = g.end[b * g.rank + l] - g.begin[b * g.rank + l] + 1;
+    n *= d;
+    if (q < g.nk) kd *= d;
+  }
+  if (g.split) n = (n / kd) * round8(kd);
   sizes[b] = used[b] ? n : 0;
 }
 
@@ -277,20 +287,27 @@ __global__ void block_panel_jobs_kernel(PanelArgs g,
   j.src_off = g.offsets[b];
   j.dst_off = panel_off[b];
   j.rank = g.rank;
-  i64 n = 1;
+  i64 n = 1, kd = 1;
   for (int q = 0; q < g.rank; ++q) {
     const int l = g.order[q];
     const i64 d = g.end[b * g.rank + l] - g.begin[b * g.rank + l] + 1;
     j.dims[q] = (int32_t)d;
     j.strides[q] = g.strides[b * g.rank + l];
     n *= d;
+    if (q < g.nk) kd *= d;
+  }
+  if (g.split) {
+    j.kdim = (int32_t)kd;
+    j.kpad = (int32_t)round8(kd);
   }
   j.n_elem = used[b] ? n : 0;
   jobs[b] = j;
   wg[b] = ceil_div(j.n_elem, kGatherPerWG);
 }
 
-// Redirect each pair's operand into its block panel.
+// Redirect each pair's operand into its block panel: the pair's sub-box of
+// the contracted legs starts at (lo - begin) and, being plain, has a single
+// leading dimension.
 __global__ void block_panel_desc_kernel(BoxArgs g,
                                         const int32_t *__restrict__ pair_blk,
                                         const int32_t *__restrict__ pair_other,
@@ -300,42 +317,48 @@ __global__ void block_panel_desc_kernel(BoxArgs g,
   if (p >= g.n_pairs) return;
   const int64_t i = pair_blk[p], j = pair_other[p];
   const i64 *sb = g.self_begin + i * g.self_rank, *se = g.self_end + i * g.self_rank;
-  const i64 *ob = g.other_begin + j * g.other_rank,
-            *oe = g.other_end + j * g.other_rank;
-  i64 rows = 1;
+  const i64 *ob = g.other_begin + j * g.other_rank;
+  i64 rows = 1, kfull = 1;
   for (int q = 0; q < g.n_free; ++q) {
     const int l = g.free_legs[q];
     rows *= se[l] - sb[l] + 1;
   }
-  i64 off = panel_off[i], cstr = rows;
+  for (int k = 0; k < g.nl; ++k) kfull *= se[g.self_legs[k]] - sb[g.self_legs[k]] + 1;
+  // plain panel: column stride of contracted index starts at rows;
+  // split panel: contracted index is fastest (stride 1), rows at kpad
+  i64 off = panel_off[i], cstr = g.split ? 1 : rows;
   for (int k = 0; k < g.nl; ++k) {
     const int ls = g.self_legs[k], lo_ = g.other_legs[k];
     const i64 lo = max(sb[ls], ob[lo_]);
     off += (lo - sb[ls]) * cstr;
     cstr *= se[ls] - sb[ls] + 1;
   }
+  const i64 ld = g.split ? round8(kfull) : (rows > 0 ? rows : 1);
+  const int32_t kf = g.split ? (g.is_b ? kBKFast : kAKFast) : 0;
   GemmDesc d = desc[p];
   if (g.is_b) {
     d.b_off = off;
-    d.ldb = (int32_t)(rows > 0 ? rows : 1);
-    d.flags = (d.flags & ~kBKFast) | kBPacked;
+    d.ldb = (int32_t)ld;
+    d.flags = (d.flags & ~kBKFast) | kBPacked | kf;
   } else {
     d.a_off = off;
-    d.lda = (int32_t)(rows > 0 ? rows : 1);
-    d.flags = (d.flags & ~kAKFast) | kAPacked;
+    d.lda = (int32_t)ld;
+    d.flags = (d.flags & ~kAKFast) | kAPacked | kf;
   }
   desc[p] = d;
 }
 
 // Per-pair panels (general overlapping blocks): panel p holds the pair's
-// sub-block permuted to [free..., contracted...].
-__global__ void pair_panel_sizes_kernel(int64_t n_pairs, int32_t is_b,
+// sub-block permuted to [free..., contracted...], or for split-BF16 to
+// [contracted..., free...] with rows padded to a multiple of 8.
+__global__ void pair_panel_sizes_kernel(int64_t n_pairs, int32_t is_b, int32_t split,
                                         const GemmDesc *__restrict__ desc,
                                         i64 *__restrict__ sizes) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n_pairs) return;
   const GemmDesc d = desc[p];
-  sizes[p] = (i64)(is_b ? d.n : d.m) * d.k;
+  const i64 rows = is_b ? d.n : d.m;
+  sizes[p] = rows * (split ? round8(d.k) : (i64)d.k);
 }
 
 __global__ void pair_panel_jobs_kernel(BoxArgs g,
@@ -357,12 +380,14 @@ __global__ void pair_panel_jobs_kernel(BoxArgs g,
             *oe = g.other_end + j * g.other_rank;
   CopyJob job{};
   job.rank = g.n_free + g.nl;
-  i64 src = self_offsets[i], n = 1, rows = 1;
+  const int f0 = g.split ? g.nl : 0;   // position of the first free leg
+  const int c0 = g.split ? 0 : g.n_free;  // position of the first contracted leg
+  i64 src = self_offsets[i], n = 1, rows = 1, kk = 1;
   for (int q = 0; q < g.n_free; ++q) {
     const int l = g.free_legs[q];
     const i64 d = se[l] - sb[l] + 1;
-    job.dims[q] = (int32_t)d;
-    job.strides[q] = ss[l];
+    job.dims[f0 + q] = (int32_t)d;
+    job.strides[f0 + q] = ss[l];
     n *= d;
     rows *= d;
   }
@@ -372,9 +397,14 @@ __global__ void pair_panel_jobs_kernel(BoxArgs g,
     const i64 hi = min(se[ls], oe[lo_]);
     const i64 d = hi - lo + 1;
     src += (lo - sb[ls]) * ss[ls];
-    job.dims[g.n_free + k] = (int32_t)d;
-    job.strides[g.n_free + k] = ss[ls];
+    job.dims[c0 + k] = (int32_t)d;
+    job.strides[c0 + k] = ss[ls];
     n *= d;
+    kk *= d;
+  }
+  if (g.split) {
+    job.kdim = (int32_t)kk;
+    job.kpad = (int32_t)round8(kk);
   }
   job.src_off = src;
   job.dst_off = panel_off[p];
@@ -382,14 +412,15 @@ __global__ void pair_panel_jobs_kernel(BoxArgs g,
   jobs[p] = job;
   wg[p] = ceil_div(n, kGatherPerWG);
   GemmDesc d = desc[p];
+  const i64 ld = g.split ? round8(kk) : (rows > 0 ? rows : 1);
   if (g.is_b) {
     d.b_off = panel_off[p];
-    d.ldb = (int32_t)(rows > 0 ? rows : 1);
-    d.flags = (d.flags & ~kBKFast) | kBPacked;
+    d.ldb = (int32_t)ld;
+    d.flags = (d.flags & ~kBKFast) | kBPacked | (g.split ? kBKFast : 0);
   } else {
     d.a_off = panel_off[p];
-    d.lda = (int32_t)(rows > 0 ? rows : 1);
-    d.flags = (d.flags & ~kAKFast) | kAPacked;
+    d.lda = (int32_t)ld;
+    d.flags = (d.flags & ~kAKFast) | kAPacked | (g.split ? kAKFast : 0);
   }
   desc[p] = d;
 }
@@ -559,12 +590,13 @@ tq_status plan_build_tiles(tq_plan_t pl, int cfg) {
   return TQ_SUCCESS;
 }
 
-static tq_status build_pack(tq_plan_t pl, bool is_b, int mode,
+static tq_status build_pack(tq_plan_t pl, bool is_b, int mode, bool split,
                             const PlanUpload &up) {
   tq_handle_t h = pl->h;
   const int64_t np = pl->n_pairs;
   PackSide &side = is_b ? pl->pack_b : pl->pack_a;
   side.mode = mode;
+  side.split = split;
   if (mode == kModeDirect) return TQ_SUCCESS;
   const int self_rank = is_b ? pl->rb : pl->ra;
   const int other_rank = is_b ? pl->ra : pl->rb;
@@ -572,6 +604,7 @@ static tq_status build_pack(tq_plan_t pl, bool is_b, int mode,
   BoxArgs box{};
   box.n_pairs = np;
   box.is_b = is_b;
+  box.split = split;
   box.self_rank = self_rank;
   box.other_rank = other_rank;
   box.nl = nl;
@@ -602,20 +635,27 @@ static tq_status build_pack(tq_plan_t pl, bool is_b, int mode,
     TQ_HIP(h, side.wg.alloc(sizeof(i64) * (nb + 1)));
     TQ_HIP(h, side.wg32.alloc(sizeof(int32_t) * (nb + 1)));
     const uint8_t *used = is_b ? pl->b_used.as<uint8_t>() : pl->a_used.as<uint8_t>();
-    hipLaunchKernelGGL(block_panel_sizes_kernel, dim3(grid_for(nb)), dim3(256), 0,
-                       h->stream, nb, self_rank, box.self_begin, box.self_end, used,
-                       sizes.as<i64>());
-    TQ_CHECK(tq_scan_i64(h, sizes.as<i64>(), side.panel_off.as<i64>(), nb,
-                         side.panel_off.as<i64>() + nb, tmp.p));
     PanelArgs pa{};
     pa.n_blocks = nb;
     pa.rank = self_rank;
+    pa.nk = nl;
+    pa.split = split;
     pa.begin = box.self_begin;
     pa.end = box.self_end;
     pa.strides = self_str;
     pa.offsets = self_off;
-    for (int q = 0; q < box.n_free; ++q) pa.order[q] = box.free_legs[q];
-    for (int k = 0; k < nl; ++k) pa.order[box.n_free + k] = box.self_legs[k];
+    if (split) {
+      for (int k = 0; k < nl; ++k) pa.order[k] = box.self_legs[k];
+      for (int q = 0; q < box.n_free; ++q) pa.order[nl + q] = box.free_legs[q];
+    } else {
+      for (int q = 0; q < box.n_free; ++q) pa.order[q] = box.free_legs[q];
+      for (int k = 0; k < nl; ++k) pa.order[box.n_free + k] = box.self_legs[k];
+      pa.nk = 0;
+    }
+    hipLaunchKernelGGL(block_panel_sizes_kernel, dim3(grid_for(nb)), dim3(256), 0,
+                       h->stream, pa, used, sizes.as<i64>());
+    TQ_CHECK(tq_scan_i64(h, sizes.as<i64>(), side.panel_off.as<i64>(), nb,
+                         side.panel_off.as<i64>() + nb, tmp.p));
     hipLaunchKernelGGL(block_panel_jobs_kernel, dim3(grid_for(nb)), dim3(256), 0,
                        h->stream, pa, side.panel_off.as<i64>(), used,
                        side.jobs.as<CopyJob>(), side.wg.as<i64>());
@@ -631,8 +671,8 @@ static tq_status build_pack(tq_plan_t pl, bool is_b, int mode,
     TQ_HIP(h, side.wg.alloc(sizeof(i64) * (np + 1)));
     TQ_HIP(h, side.wg32.alloc(sizeof(int32_t) * (np + 1)));
     hipLaunchKernelGGL(pair_panel_sizes_kernel, dim3(grid_for(np)), dim3(256), 0,
-                       h->stream, np, (int32_t)is_b, pl->desc.as<GemmDesc>(),
-                       sizes.as<i64>());
+                       h->stream, np, (int32_t)is_b, (int32_t)split,
+                       pl->desc.as<GemmDesc>(), sizes.as<i64>());
     TQ_CHECK(tq_scan_i64(h, sizes.as<i64>(), side.panel_off.as<i64>(), np,
                          side.panel_off.as<i64>() + np, tmp.p));
     hipLaunchKernelGGL(pair_panel_jobs_kernel, dim3(grid_for(np)), dim3(256), 0,
@@ -658,8 +698,16 @@ static tq_status build_pack(tq_plan_t pl, bool is_b, int mode,
   side.n_jobs = n_jobs;
   side.total_wg = sizes_h[1];
   side.panel_elems = sizes_h[0];
-  const size_t es = pl->dtype == TQ_F64 ? 8 : 4;
-  TQ_HIP(h, side.panel.alloc(es * std::max<int64_t>(sizes_h[0], 1)));
+  if (split) {
+    // three BF16 planes; row padding stays zero (packs never write it)
+    const size_t bytes = 3 * sizeof(uint16_t) * std::max<int64_t>(sizes_h[0], 1);
+    TQ_HIP(h, side.panel.alloc(bytes));
+    TQ_HIP(h, hipMemsetAsync(side.panel.p, 0, bytes, h->stream));
+    TQ_HIP(h, hipStreamSynchronize(h->stream));
+  } else {
+    const size_t es = pl->dtype == TQ_F64 ? 8 : 4;
+    TQ_HIP(h, side.panel.alloc(es * std::max<int64_t>(sizes_h[0], 1)));
+  }
   return TQ_SUCCESS;
 }
 
@@ -723,6 +771,8 @@ tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
     }
   }
   pl->ro = (int)pl->out_dims.size();
+  // a rank-0 result (the network scalar) is always reduced in FP64
+  if (pl->ro == 0) pl->prec = TQ_PREC_NATIVE;
   const int ra = a->rank, rb = b->rank, ro = pl->ro, nl = n_legs;
   const int64_t na = a->n_blocks, nb = b->n_blocks;
 
@@ -917,14 +967,24 @@ tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
 
   // ---- 4. operand layouts and pack jobs ----
   if (np > 0) {
-    const int mode_a = !(flags_h & kNeedPackA) ? kModeDirect
-                       : (flags_h & kNeedPairPanelA) ? kModePairPanel
-                                                     : kModeBlockPanel;
-    const int mode_b = !(flags_h & kNeedPackB) ? kModeDirect
-                       : (flags_h & kNeedPairPanelB) ? kModePairPanel
-                                                     : kModeBlockPanel;
-    TQ_CHECK(build_pack(pl.get(), false, mode_a, up));
-    TQ_CHECK(build_pack(pl.get(), true, mode_b, up));
+    // the split-BF16 tier always reads split panels
+    const bool split = pl->prec == TQ_PREC_BF16X3;
+    // split panels are read with 16-byte loads: a pair that starts inside a
+    // block panel row would be misaligned, so such pairs get their own panels
+    if (split && (flags_h & kNotFullA)) flags_h |= kNeedPairPanelA;
+    if (split && (flags_h & kNotFullB)) flags_h |= kNeedPairPanelB;
+    const int mode_a = (flags_h & kNeedPairPanelA) ? kModePairPanel
+                       : (split || (flags_h & kNeedPackA)) ? kModeBlockPanel
+                                                           : kModeDirect;
+    const int mode_b = (flags_h & kNeedPairPanelB) ? kModePairPanel
+                       : (split || (flags_h & kNeedPackB)) ? kModeBlockPanel
+                                                           : kModeDirect;
+    TQ_CHECK(build_pack(pl.get(), false, (split || (flags_h & kNeedPackA)) ? mode_a
+                                                                          : kModeDirect,
+                        split, up));
+    TQ_CHECK(build_pack(pl.get(), true, (split || (flags_h & kNeedPackB)) ? mode_b
+                                                                         : kModeDirect,
+                        split, up));
 
     // ---- 5. work list: stable device sort by cost, largest first ----
     const int64_t n_chunks = ceil_div(np, kSortChunk);
