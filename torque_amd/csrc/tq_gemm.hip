@@ -175,7 +175,12 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
           for (int j = 0; j < TN; ++j) acc[i][j] = mfma(af[i], bf[j], acc[i][j]);
       }
     }
-    // epilogue: lane holds D[4*(lane/16) + r][lane % 16] of each 16x16 block
+    // epilogue: in each 16x16 block, lane holds column lane % 16; register r
+    // holds row 4*(lane/16) + r for the FP32 MFMA and row lane/16 + 4*r for
+    // the FP64 MFMA
+    constexpr bool kF64 = sizeof(TC) == 8;
+    const int rbase = kF64 ? (lane >> 4) : 4 * (lane >> 4);
+    const int rstep = kF64 ? 4 : 1;
     TOut *pc = c + d.c_off;
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
@@ -184,10 +189,10 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
       TOut *col = pc + (i64)d.m * n;
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
-        const int mb = m0 + wm0 + i * 16 + 4 * (lane >> 4);
+        const int mb = m0 + wm0 + i * 16 + rbase;
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (mb + r < d.m) col[mb + r] = (TOut)acc[i][j][r];
+          if (mb + r * rstep < d.m) col[mb + r * rstep] = (TOut)acc[i][j][r];
       }
     }
   }
