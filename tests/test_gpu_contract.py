@@ -118,7 +118,7 @@ def test_c1_config(cuda):
 def test_random_overlapping_blocks(cuda, seed):
     rng = np.random.default_rng(500 + seed)
     dims_a = [4, 6, 3, 5]
-    dims_b = [3, 5, 7]
+    dims_b = [3, 5, 6]
     ba, ea = random_blocks(rng, dims_a, 7)
     bb, eb = random_blocks(rng, dims_b, 6)
     sa = permuted_strides(ba, ea, rng) if seed % 2 else None
