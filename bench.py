@@ -39,7 +39,7 @@ def parse():
     p.add_argument("--chi", type=int, default=128)
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--tiers", default="fp32,bf16x3")
-    p.add_argument("--cpu-sample-blocks", type=int, default=6,
+    p.add_argument("--cpu-sample-blocks", type=int, default=40,
                    help="E blocks in the CPU-oracle sample")
     p.add_argument("--no-cpu", action="store_true")
     return p.parse_args()
@@ -72,6 +72,12 @@ def main():
     torch.cuda.synchronize()
     plan_s = time.perf_counter() - t0
     out = torch.empty(max(p2.out_n_elem, 1), dtype=torch.float64, device=dev)
+    t0 = time.perf_counter()
+    best1, _ = p1.autotune(gA.data, gAc.data, aa)
+    best2, times2 = p2.autotune(gE.data, aa, out)
+    tune_s = time.perf_counter() - t0
+    print("# autotune %.2f s: A.A* config %d, E.a config %d; E.a ms per config %s"
+          % (tune_s, best1, best2, ["%.3f" % x for x in times2]), flush=True)
     print("# workload D=%d chi=%d: A.A* %d pairs, E.a %d pairs, %.3f GFLOP, out %.1f M elem, "
           "plans %.3f s" % (args.D, args.chi, p1.n_pairs, p2.n_pairs,
                             (p1.flops + p2.flops) / 1e9, p2.out_n_elem / 1e6, plan_s),
@@ -169,6 +175,12 @@ def main():
             continue
         aa_t = torch.empty_like(aa)
         out_t = torch.empty_like(out)
+        try:
+            pt1.autotune(gA.data, gAc.data, aa_t)
+            pt2.autotune(gE.data, aa, out_t)
+        except TorqueError as e:
+            tiers[tier] = {"error": str(e)}
+            continue
 
         def tstep():
             pt1.execute(gA.data, gAc.data, aa_t)
@@ -205,6 +217,7 @@ def main():
                    "pairs": [int(p1.n_pairs), int(p2.n_pairs)],
                    "gflop_per_step": round(flops / 1e9, 4),
                    "out_elements": int(p2.out_n_elem), "plan_seconds": round(plan_s, 3),
+                   "gemm_config": {"A.A*": best1, "E.a": best2},
                    "entries": "N(0,1)*10^U(-3,0) per block", "seed": args.seed},
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,

@@ -195,6 +195,17 @@ tq_status tq_bs_contract_multiply(tq_plan_t plan, const void *a, const void *b,
 tq_status tq_plan_set_config(tq_plan_t plan, int32_t config);
 tq_status tq_plan_destroy(tq_plan_t plan);
 
+/* Autotuner (no reference counterpart: the reference measures a cuTT plan
+ * per call, src/gpu/block_sparse.cuh:836-863). Packs A and B once, then times
+ * `reps` grouped-GEMM launches of every tile configuration with HIP events on
+ * the handle's stream and keeps the fastest for this plan. times_ms (host,
+ * tq_gemm_config_count(precision) entries, may be NULL) receives the mean
+ * time of each configuration; *best (may be NULL) the chosen index. C is
+ * written by the timed launches. */
+tq_status tq_plan_autotune(tq_plan_t plan, const void *a, const void *b,
+                           void *c, int32_t reps, int32_t *best,
+                           float *times_ms);
+
 /* Number of grouped-GEMM tile configurations and a description of one. */
 int32_t tq_gemm_config_count(tq_precision precision);
 tq_status tq_gemm_config_describe(tq_precision precision, int32_t config,

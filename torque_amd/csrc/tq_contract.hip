@@ -172,6 +172,51 @@ tq_status tq_bs_contract_multiply(tq_plan_t pl, const void *a, const void *b,
   return TQ_SUCCESS;
 }
 
+tq_status tq_plan_autotune(tq_plan_t pl, const void *a, const void *b, void *c,
+                           int32_t reps, int32_t *best, float *times_ms) {
+  if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  tq_handle_t h = pl->h;
+  if (reps < 1) reps = 1;
+  const int nc = tq_gemm_n_configs(pl->prec);
+  if (pl->n_pairs == 0 || pl->ro == 0) {
+    if (best) *best = pl->cfg >= 0 ? pl->cfg : pl->auto_cfg;
+    if (times_ms)
+      for (int q = 0; q < nc; ++q) times_ms[q] = 0.f;
+    return TQ_SUCCESS;
+  }
+  TQ_CHECK(tq_bs_contract_pack(pl, a, b));
+  hipEvent_t e0, e1;
+  TQ_HIP(h, hipEventCreate(&e0));
+  TQ_HIP(h, hipEventCreate(&e1));
+  int best_cfg = 0;
+  float best_t = 3.4e38f;
+  tq_status st = TQ_SUCCESS;
+  for (int q = 0; q < nc && st == TQ_SUCCESS; ++q) {
+    pl->cfg = q;
+    st = tq_bs_contract_multiply(pl, a, b, c);  // builds the tile list, warms up
+    if (st != TQ_SUCCESS) break;
+    (void)hipEventRecord(e0, h->stream);
+    for (int r = 0; r < reps && st == TQ_SUCCESS; ++r)
+      st = tq_bs_contract_multiply(pl, a, b, c);
+    (void)hipEventRecord(e1, h->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    ms /= reps;
+    if (times_ms) times_ms[q] = ms;
+    if (ms < best_t) {
+      best_t = ms;
+      best_cfg = q;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != TQ_SUCCESS) return st;
+  pl->cfg = best_cfg;
+  if (best) *best = best_cfg;
+  return plan_build_tiles(pl, best_cfg);
+}
+
 tq_status tq_bs_contract_execute(tq_plan_t pl, const void *a, const void *b,
                                  void *c) {
   TQ_CHECK(tq_bs_contract_pack(pl, a, b));

@@ -18,9 +18,11 @@
 // (for FP32 data, or FP64 data at the FP32 tier); split-BF16 (three BF16
 // pieces per operand, six piece products on v_mfma_f32_16x16x32_bf16 with
 // FP32 accumulation) in tq_gemm_bf16.hip.
-#include "tq_common.h"
+#include "tq_gemm_common.h"
 
 namespace {
+
+using tqg::kThreads;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -43,75 +45,57 @@ struct Acc<float> {
   typedef f4 type;
 };
 
-constexpr int kThreads = 256;
-constexpr int kBK = 16;
-
-// Loads one BM x kBK operand tile (rows = m or n) into registers.
-// m-fast: element (r, k) at p[r + ld*k]; thread t takes r = t % BM and
-//         k = t / BM + q * (256 / BM).
-// k-fast: element (r, k) at p[k + ld*r]; thread t takes k = t % 16 and
-//         r = t / 16 + 16 * q.
-template <class TIn, class TC, int BM>
+// Loads one ROWS x BK operand tile (rows = m or n) into registers and then
+// into LDS laid out [k][row] with row length ROWS + 1. Element e = t + 256 q
+// of the tile is (row e % ROWS, k e / ROWS) when the row index is fastest in
+// memory (p[row + ld*k]) and (row e / BK, k e % BK) when k is fastest
+// (p[k + ld*row]), so consecutive lanes read consecutive addresses.
+template <class TIn, class TC, int ROWS, int BK>
 struct TileLoader {
-  static constexpr int kN = BM * kBK / kThreads;
+  static constexpr int kElems = ROWS * BK;
+  static constexpr int kN = kElems >= kThreads ? kElems / kThreads : 1;
   TC v[kN];
 
   __device__ inline void load(const TIn *__restrict__ base, bool kfast, int ld,
                               int rows, int row0, int kcount, int k0) {
     const int t = threadIdx.x;
-    if (!kfast) {
-      const int r = t % BM;
-      const int kb = t / BM;
-      constexpr int step = kThreads / BM;
-      const bool rok = row0 + r < rows;
-      const TIn *p = base + (row0 + r);
 #pragma unroll
-      for (int q = 0; q < kN; ++q) {
-        const int k = kb + q * step;
-        v[q] = (rok && k0 + k < kcount) ? (TC)p[(i64)ld * (k0 + k)] : (TC)0;
-      }
-    } else {
-      const int k = t % kBK;
-      const int rb = t / kBK;
-      const bool kok = k0 + k < kcount;
-      const TIn *p = base + (k0 + k);
-#pragma unroll
-      for (int q = 0; q < kN; ++q) {
-        const int r = rb + q * (kThreads / kBK);
-        v[q] = (kok && row0 + r < rows) ? (TC)p[(i64)ld * (row0 + r)] : (TC)0;
-      }
+    for (int q = 0; q < kN; ++q) {
+      const int e = t + q * kThreads;
+      const int r = kfast ? e / BK : e % ROWS;
+      const int k = kfast ? e % BK : e / ROWS;
+      const bool ok = e < kElems && row0 + r < rows && k0 + k < kcount;
+      const int64_t off = kfast ? (int64_t)(k0 + k) + (int64_t)ld * (row0 + r)
+                                : (int64_t)(row0 + r) + (int64_t)ld * (k0 + k);
+      v[q] = ok ? (TC)base[off] : (TC)0;
     }
   }
 
-  // store into LDS laid out [k][r] with row length BM + 1
   __device__ inline void store(TC *lds, bool kfast) const {
     const int t = threadIdx.x;
-    if (!kfast) {
-      const int r = t % BM;
-      const int kb = t / BM;
-      constexpr int step = kThreads / BM;
 #pragma unroll
-      for (int q = 0; q < kN; ++q) lds[(kb + q * step) * (BM + 1) + r] = v[q];
-    } else {
-      const int k = t % kBK;
-      const int rb = t / kBK;
-#pragma unroll
-      for (int q = 0; q < kN; ++q)
-        lds[k * (BM + 1) + rb + q * (kThreads / kBK)] = v[q];
+    for (int q = 0; q < kN; ++q) {
+      const int e = t + q * kThreads;
+      if (e >= kElems) continue;
+      const int r = kfast ? e / BK : e % ROWS;
+      const int k = kfast ? e % BK : e / ROWS;
+      lds[k * (ROWS + 1) + r] = v[q];
     }
   }
 };
 
-// Maps a workgroup's launch index to a virtual index so that each XCD (the
-// hardware deals workgroups to the 8 XCDs round-robin) walks a contiguous
-// stretch of the tile list.
-__device__ inline int64_t xcd_virtual_id(int64_t bid, int64_t grid) {
-  const int64_t per = grid / 8;
-  if (per == 0 || grid % 8 != 0) return bid;
-  return (bid % 8) * per + bid / 8;
-}
+template <class TC, class TOut, int BM, int BN, int BK>
+struct Smem {
+  static constexpr size_t kAB = sizeof(TC) * (size_t)BK * (BM + 1 + BN + 1);
+  static constexpr size_t kStage = sizeof(TOut) * 16 * (size_t)(BM + 1);
+  static constexpr size_t kBytes = kAB > kStage ? kAB : kStage;
+};
 
-template <class TIn, class TC, class TOut, int BM, int BN>
+// One workgroup of 4 wavefronts (WAVES_M along M x 4/WAVES_M along N) per
+// BM x BN tile; each wavefront owns a (BM/WAVES_M) x (BN*WAVES_M/4) sub-tile
+// of 16x16 MFMA blocks. The next k tile is prefetched into registers while
+// the current one is multiplied from LDS.
+template <class TIn, class TC, class TOut, int BM, int BN, int BK, int WAVES_M>
 __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
     const GemmDesc *__restrict__ descs, const int32_t *__restrict__ tile_pair,
     const int32_t *__restrict__ tile_first, int64_t n_tiles,
@@ -119,14 +103,18 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
     const TIn *__restrict__ b, const TIn *__restrict__ b_packed,
     TOut *__restrict__ c) {
   typedef typename Acc<TC>::type AccT;
-  constexpr int WM = BM / 2, WN = BN / 2;  // 2 x 2 wavefronts
+  constexpr int WAVES_N = 4 / WAVES_M;
+  constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
   constexpr int TM = WM / 16, TN = WN / 16;
-  __shared__ TC As[kBK * (BM + 1)];
-  __shared__ TC Bs[kBK * (BN + 1)];
+  static_assert(TM >= 1 && TN >= 1 && BK % 4 == 0, "tile shape");
+  __shared__ __attribute__((aligned(16))) char smem[Smem<TC, TOut, BM, BN, BK>::kBytes];
+  TC *As = reinterpret_cast<TC *>(smem);
+  TC *Bs = As + BK * (BM + 1);
+  TOut *stage = reinterpret_cast<TOut *>(smem);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
-  const int64_t vid = xcd_virtual_id(blockIdx.x, gridDim.x);
+  const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
+  const int64_t vid = tqg::xcd_virtual_id(blockIdx.x, gridDim.x);
 
   for (int64_t t = vid; t < n_tiles; t += gridDim.x) {
     const int p = tile_pair[t];
@@ -139,6 +127,9 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
     const bool b_kfast = d.flags & kBKFast;
     const TIn *pa = ((d.flags & kAPacked) ? a_packed : a) + d.a_off;
     const TIn *pb = ((d.flags & kBPacked) ? b_packed : b) + d.b_off;
+    // MFMA blocks of this wavefront that hold output (wave-uniform)
+    const int tm_live = min(TM, max(0, (d.m - m0 - wm0 + 15) / 16));
+    const int tn_live = min(TN, max(0, (d.n - n0 - wn0 + 15) / 16));
 
     AccT acc[TM][TN];
 #pragma unroll
@@ -146,20 +137,20 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
 #pragma unroll
       for (int j = 0; j < TN; ++j) acc[i][j] = AccT{0, 0, 0, 0};
 
-    TileLoader<TIn, TC, BM> la;
-    TileLoader<TIn, TC, BN> lb;
+    TileLoader<TIn, TC, BM, BK> la;
+    TileLoader<TIn, TC, BN, BK> lb;
     la.load(pa, a_kfast, d.lda, d.m, m0, d.k, 0);
     lb.load(pb, b_kfast, d.ldb, d.n, n0, d.k, 0);
-    for (int k0 = 0; k0 < d.k; k0 += kBK) {
-      __syncthreads();  // previous tile's LDS reads are done
+    for (int k0 = 0; k0 < d.k; k0 += BK) {
+      __syncthreads();  // the previous k tile is no longer read
       la.store(As, a_kfast);
       lb.store(Bs, b_kfast);
       __syncthreads();
-      if (k0 + kBK < d.k) {  // prefetch the next k tile while computing
-        la.load(pa, a_kfast, d.lda, d.m, m0, d.k, k0 + kBK);
-        lb.load(pb, b_kfast, d.ldb, d.n, n0, d.k, k0 + kBK);
+      if (k0 + BK < d.k) {
+        la.load(pa, a_kfast, d.lda, d.m, m0, d.k, k0 + BK);
+        lb.load(pb, b_kfast, d.ldb, d.n, n0, d.k, k0 + BK);
       }
-      const int ksteps = min(kBK / 4, (d.k - k0 + 3) / 4);
+      const int ksteps = min(BK / 4, (d.k - k0 + 3) / 4);
       for (int ks = 0; ks < ksteps; ++ks) {
         const int kr = ks * 4 + (lane >> 4);
         TC af[TM], bf[TN];
@@ -172,40 +163,25 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
-          for (int j = 0; j < TN; ++j) acc[i][j] = mfma(af[i], bf[j], acc[i][j]);
+          for (int j = 0; j < TN; ++j)
+            if (i < tm_live && j < tn_live) acc[i][j] = mfma(af[i], bf[j], acc[i][j]);
       }
     }
-    // epilogue: in each 16x16 block, lane holds column lane % 16; register r
-    // holds row 4*(lane/16) + r for the FP32 MFMA and row lane/16 + 4*r for
-    // the FP64 MFMA
-    constexpr bool kF64 = sizeof(TC) == 8;
-    const int rbase = kF64 ? (lane >> 4) : 4 * (lane >> 4);
-    const int rstep = kF64 ? 4 : 1;
-    TOut *pc = c + d.c_off;
-#pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const int n = n0 + wn0 + j * 16 + (lane & 15);
-      if (n >= d.n) continue;
-      TOut *col = pc + (i64)d.m * n;
-#pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        const int mb = m0 + wm0 + i * 16 + rbase;
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (mb + r * rstep < d.m) col[mb + r * rstep] = (TOut)acc[i][j][r];
-      }
-    }
+    tqg::store_tile<TOut, sizeof(TC) == 8, BM, BN, WAVES_M, TM, TN>(acc, c, d, m0, n0,
+                                                                    stage);
   }
 }
 
 struct Cfg {
-  int bm, bn;
+  int bm, bn, bk;
 };
-constexpr Cfg kCfgs[] = {{64, 64}, {128, 64}, {64, 128}, {128, 128},
-                         {256, 32}, {32, 256}, {32, 32}};
+// keep in step with the switch in dispatch()
+constexpr Cfg kCfgs[] = {{64, 64, 16},  {128, 64, 16}, {64, 128, 16}, {128, 128, 16},
+                         {256, 32, 16}, {32, 256, 16}, {32, 32, 16},  {128, 16, 16},
+                         {256, 16, 8},  {512, 16, 4}};
 constexpr int kNCfg = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
-template <class TIn, class TC, class TOut, int BM, int BN>
+template <class TIn, class TC, class TOut, int BM, int BN, int BK, int WAVES_M>
 tq_status launch_one(tq_handle_t h, const GemmDesc *descs,
                      const int32_t *tile_pair, const int32_t *tile_first,
                      int64_t n_tiles, const void *a, const void *ap,
@@ -214,7 +190,7 @@ tq_status launch_one(tq_handle_t h, const GemmDesc *descs,
   int64_t grid = std::min<int64_t>(n_tiles, (int64_t)h->n_cu * 4);
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((grouped_gemm_kernel<TIn, TC, TOut, BM, BN>),
+  hipLaunchKernelGGL((grouped_gemm_kernel<TIn, TC, TOut, BM, BN, BK, WAVES_M>),
                      dim3((unsigned)grid), dim3(kThreads), 0, h->stream, descs,
                      tile_pair, tile_first, n_tiles,
                      reinterpret_cast<const TIn *>(a),
@@ -231,18 +207,22 @@ tq_status dispatch(tq_handle_t h, int cfg, const GemmDesc *descs,
                    const int32_t *tile_pair, const int32_t *tile_first,
                    int64_t n_tiles, const void *a, const void *ap, const void *b,
                    const void *bp, void *c) {
-#define TQ_CFG(I, BM, BN)                                                   \
-  case I:                                                                   \
-    return launch_one<TIn, TC, TOut, BM, BN>(h, descs, tile_pair, tile_first, \
-                                             n_tiles, a, ap, b, bp, c);
+#define TQ_CFG(I, BM, BN, BK, WM)                                                \
+  case I:                                                                        \
+    return launch_one<TIn, TC, TOut, BM, BN, BK, WM>(h, descs, tile_pair,        \
+                                                     tile_first, n_tiles, a, ap, \
+                                                     b, bp, c);
   switch (cfg) {
-    TQ_CFG(0, 64, 64)
-    TQ_CFG(1, 128, 64)
-    TQ_CFG(2, 64, 128)
-    TQ_CFG(3, 128, 128)
-    TQ_CFG(4, 256, 32)
-    TQ_CFG(5, 32, 256)
-    TQ_CFG(6, 32, 32)
+    TQ_CFG(0, 64, 64, 16, 2)
+    TQ_CFG(1, 128, 64, 16, 2)
+    TQ_CFG(2, 64, 128, 16, 2)
+    TQ_CFG(3, 128, 128, 16, 2)
+    TQ_CFG(4, 256, 32, 16, 2)
+    TQ_CFG(5, 32, 256, 16, 2)
+    TQ_CFG(6, 32, 32, 16, 2)
+    TQ_CFG(7, 128, 16, 16, 4)
+    TQ_CFG(8, 256, 16, 8, 4)
+    TQ_CFG(9, 512, 16, 4, 4)
   }
 #undef TQ_CFG
   return tq_fail(h, TQ_ERROR_INVALID_VALUE, "gemm: unknown tile configuration");
@@ -261,7 +241,7 @@ int tq_gemm_n_configs(tq_precision prec) {
 GemmConfig tq_gemm_config(tq_precision prec, int cfg) {
   if (prec == TQ_PREC_BF16X3) return tq_bf16x3_config(cfg);
   if (cfg < 0 || cfg >= kNCfg) cfg = 0;
-  return GemmConfig{kCfgs[cfg].bm, kCfgs[cfg].bn, kBK};
+  return GemmConfig{kCfgs[cfg].bm, kCfgs[cfg].bn, kCfgs[cfg].bk};
 }
 
 tq_status tq_launch_grouped_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec,

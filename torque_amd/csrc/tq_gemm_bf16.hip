@@ -11,22 +11,16 @@
 //
 // Same work list, tile mapping and persistence as the FP64 kernel
 // (tq_gemm.hip); operands are always the packed split panels.
-#include "tq_common.h"
+#include "tq_gemm_common.h"
 
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int kThreads = 256;
+using tqg::kThreads;
 constexpr int kBK = 32;
 constexpr int kLDK = kBK + 8;  // LDS row stride in BF16 elements (80 bytes)
-
-__device__ inline int64_t xcd_virtual_id(int64_t bid, int64_t grid) {
-  const int64_t per = grid / 8;
-  if (per == 0 || grid % 8 != 0) return bid;
-  return (bid % 8) * per + bid / 8;
-}
 
 // Loads a ROWS x 32 tile of the three planes into LDS (16-byte chunks:
 // thread t covers row t / 4, k chunk (t % 4) * 8, stepping 64 rows).
@@ -50,21 +44,32 @@ __device__ inline void load_planes(const uint16_t *__restrict__ base, int64_t pl
 }
 
 template <int BM, int BN>
+struct Smem {
+  static constexpr size_t kAB = 2 * 3 * (size_t)(BM + BN) * kLDK;
+  static constexpr size_t kStage = 8 * 16 * (size_t)(BM + 1);
+  static constexpr size_t kBytes = kAB > kStage ? kAB : kStage;
+};
+
+template <int BM, int BN, int WAVES_M>
 __global__ __launch_bounds__(kThreads) void bf16x3_gemm_kernel(
     const GemmDesc *__restrict__ descs, const int32_t *__restrict__ tile_pair,
     const int32_t *__restrict__ tile_first, int64_t n_tiles,
     const uint16_t *__restrict__ a, const uint16_t *__restrict__ b,
     int64_t a_plane, int64_t b_plane, double *__restrict__ c) {
-  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int WAVES_N = 4 / WAVES_M;
+  constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
   constexpr int TM = WM / 16, TN = WN / 16;
-  __shared__ uint4 As[3 * BM * kLDK / 8];
-  __shared__ uint4 Bs[3 * BN * kLDK / 8];
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
-  const int64_t vid = xcd_virtual_id(blockIdx.x, gridDim.x);
+  static_assert(TM >= 1 && TN >= 1, "tile shape");
+  __shared__ __attribute__((aligned(16))) char smem[Smem<BM, BN>::kBytes];
+  uint4 *As = reinterpret_cast<uint4 *>(smem);
+  uint4 *Bs = As + 3 * BM * kLDK / 8;
+  double *stage = reinterpret_cast<double *>(smem);
   const __bf16 *Ab = reinterpret_cast<const __bf16 *>(As);
   const __bf16 *Bb = reinterpret_cast<const __bf16 *>(Bs);
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
+  const int64_t vid = tqg::xcd_virtual_id(blockIdx.x, gridDim.x);
 
   for (int64_t t = vid; t < n_tiles; t += gridDim.x) {
     const int p = tile_pair[t];
@@ -76,6 +81,8 @@ __global__ __launch_bounds__(kThreads) void bf16x3_gemm_kernel(
     const int kpad = (d.k + 7) & ~7;
     const uint16_t *pa = a + d.a_off;
     const uint16_t *pb = b + d.b_off;
+    const int tm_live = min(TM, max(0, (d.m - m0 - wm0 + 15) / 16));
+    const int tn_live = min(TN, max(0, (d.n - n0 - wn0 + 15) / 16));
     f4 acc[TM][TN];
 #pragma unroll
     for (int i = 0; i < TM; ++i)
@@ -98,12 +105,14 @@ __global__ __launch_bounds__(kThreads) void bf16x3_gemm_kernel(
       }
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
+        if (j >= tn_live) continue;
         const int r = (wn0 + j * 16 + (lane & 15)) * kLDK + kk;
         const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(Bb + r);
         const bf16x8 bm = *reinterpret_cast<const bf16x8 *>(Bb + BN * kLDK + r);
         const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(Bb + 2 * BN * kLDK + r);
 #pragma unroll
         for (int i = 0; i < TM; ++i) {
+          if (i >= tm_live) continue;
           f4 x = acc[i][j];
           // smallest terms first
           x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, x, 0, 0, 0);
@@ -116,38 +125,28 @@ __global__ __launch_bounds__(kThreads) void bf16x3_gemm_kernel(
         }
       }
     }
-    double *pc = c + d.c_off;
-#pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const int n = n0 + wn0 + j * 16 + (lane & 15);
-      if (n >= d.n) continue;
-      double *col = pc + (int64_t)d.m * n;
-#pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        const int mb = m0 + wm0 + i * 16 + 4 * (lane >> 4);
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (mb + r < d.m) col[mb + r] = (double)acc[i][j][r];
-      }
-    }
+    tqg::store_tile<double, false, BM, BN, WAVES_M, TM, TN>(acc, c, d, m0, n0, stage);
   }
 }
 
 struct Cfg {
   int bm, bn;
 };
-constexpr Cfg kCfgs[] = {{64, 64}, {128, 64}, {64, 128}, {128, 128}, {256, 32}, {32, 32}};
+// keep in step with the switch in tq_launch_bf16x3_gemm()
+constexpr Cfg kCfgs[] = {{64, 64}, {128, 64}, {64, 128}, {128, 128}, {256, 32},
+                         {32, 32}, {128, 16},  {256, 16}};
 constexpr int kNCfg = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
-template <int BM, int BN>
+template <int BM, int BN, int WAVES_M>
 tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *tile_pair,
                  const int32_t *tile_first, int64_t n_tiles, const uint16_t *a,
                  const uint16_t *b, int64_t ap, int64_t bp, double *c) {
   int64_t grid = std::min<int64_t>(n_tiles, (int64_t)h->n_cu * 4);
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((bf16x3_gemm_kernel<BM, BN>), dim3((unsigned)grid), dim3(kThreads),
-                     0, h->stream, descs, tile_pair, tile_first, n_tiles, a, b, ap, bp, c);
+  hipLaunchKernelGGL((bf16x3_gemm_kernel<BM, BN, WAVES_M>), dim3((unsigned)grid),
+                     dim3(kThreads), 0, h->stream, descs, tile_pair, tile_first, n_tiles,
+                     a, b, ap, bp, c);
   TQ_HIP(h, hipGetLastError());
   return TQ_SUCCESS;
 }
@@ -168,12 +167,19 @@ tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, const GemmDesc *descs,
                                 int64_t b_plane, double *c) {
   if (n_tiles <= 0) return TQ_SUCCESS;
   switch (cfg) {
-    case 0: return launch<64, 64>(h, descs, tile_pair, tile_first, n_tiles, a_split, b_split, a_plane, b_plane, c);
-    case 1: return launch<128, 64>(h, descs, tile_pair, tile_first, n_tiles, a_split, b_split, a_plane, b_plane, c);
-    case 2: return launch<64, 128>(h, descs, tile_pair, tile_first, n_tiles, a_split, b_split, a_plane, b_plane, c);
-    case 3: return launch<128, 128>(h, descs, tile_pair, tile_first, n_tiles, a_split, b_split, a_plane, b_plane, c);
-    case 4: return launch<256, 32>(h, descs, tile_pair, tile_first, n_tiles, a_split, b_split, a_plane, b_plane, c);
-    case 5: return launch<32, 32>(h, descs, tile_pair, tile_first, n_tiles, a_split, b_split, a_plane, b_plane, c);
+#define TQ_CFG(I, BM, BN, WM)                                                        \
+  case I:                                                                            \
+    return launch<BM, BN, WM>(h, descs, tile_pair, tile_first, n_tiles, a_split,     \
+                              b_split, a_plane, b_plane, c);
+    TQ_CFG(0, 64, 64, 2)
+    TQ_CFG(1, 128, 64, 2)
+    TQ_CFG(2, 64, 128, 2)
+    TQ_CFG(3, 128, 128, 2)
+    TQ_CFG(4, 256, 32, 2)
+    TQ_CFG(5, 32, 32, 2)
+    TQ_CFG(6, 128, 16, 4)
+    TQ_CFG(7, 256, 16, 4)
+#undef TQ_CFG
   }
   return tq_fail(h, TQ_ERROR_INVALID_VALUE, "gemm: unknown tile configuration");
 }

@@ -311,6 +311,7 @@ class ContractionPlan:
         legs = np.ascontiguousarray(np.asarray(legs, dtype=np.int32).reshape(-1, 2))
         self.legs = legs
         self.dtype = a.dtype
+        self.precision = precision
         p = ctypes.c_void_p()
         handle.check(lib().tq_bs_contract_plan(
             handle.ptr, ctypes.byref(a.desc()), ctypes.byref(b.desc()),
@@ -347,6 +348,18 @@ class ContractionPlan:
         order = np.zeros(self.n_pairs, np.int64)
         self.h.check(lib().tq_plan_get_worklist(self.ptr, _i64p(order)))
         return order
+
+    def autotune(self, a_data, b_data, out, reps=3):
+        """Time every tile configuration (tq_plan_autotune); returns
+        (best index, per-configuration mean ms)."""
+        self.h.bind_stream()
+        n = lib().tq_gemm_config_count(_lib.PRECISIONS[self.precision])
+        times = (ctypes.c_float * max(n, 1))()
+        best = ctypes.c_int32()
+        self.h.check(lib().tq_plan_autotune(
+            self.ptr, ctypes.c_void_p(a_data.data_ptr()), ctypes.c_void_p(b_data.data_ptr()),
+            ctypes.c_void_p(out.data_ptr()), int(reps), ctypes.byref(best), times))
+        return best.value, [times[i] for i in range(n)]
 
     def set_config(self, cfg):
         self.h.check(lib().tq_plan_set_config(self.ptr, int(cfg)))
