@@ -39,8 +39,8 @@ def parse():
     p.add_argument("--chi", type=int, default=128)
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--tiers", default="fp32,bf16x3")
-    p.add_argument("--cpu-sample-blocks", type=int, default=40,
-                   help="E blocks in the CPU-oracle sample")
+    p.add_argument("--cpu-sample-blocks", type=int, default=0,
+                   help="E blocks in the CPU-oracle sample (0 = all)")
     p.add_argument("--no-cpu", action="store_true")
     return p.parse_args()
 
@@ -124,7 +124,8 @@ def main():
 
     # ---- oracle sample: E blocks [0, s) x all of a, pairs contiguous in out
     from oracle import oracle as O
-    s_blocks = min(args.cpu_sample_blocks, E.n_blocks)
+    s_blocks = E.n_blocks if args.cpu_sample_blocks <= 0 else min(args.cpu_sample_blocks,
+                                                                 E.n_blocks)
     n_sub = int(np.prod(E.end[:s_blocks] - E.begin[:s_blocks] + 1, axis=1).sum())
     TEs = O.BS(E.dims, E.begin[:s_blocks], E.end[:s_blocks])
     Ta = O.BS(ga.dimension, ga.begin_points, ga.end_points)
