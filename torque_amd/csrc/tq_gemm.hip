@@ -187,7 +187,17 @@ tq_status launch_one(tq_handle_t h, const GemmDesc *descs,
                      int64_t n_tiles, const void *a, const void *ap,
                      const void *b, const void *bp, void *c) {
   // persistent grid: a few workgroups per CU, a multiple of the 8 XCDs
-  int64_t grid = std::min<int64_t>(n_tiles, (int64_t)h->n_cu * 4);
+  // persistent grid: exactly the workgroups that can be resident at once
+  // (occupancy from registers and LDS), a multiple of the 8 XCDs
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (grouped_gemm_kernel<TIn, TC, TOut, BM, BN, BK, WAVES_M>), kThreads, 0) !=
+            hipSuccess || occ < 1)
+      occ = 1;
+    per_cu = occ;
+  }
+  int64_t grid = std::min<int64_t>(n_tiles, (int64_t)h->n_cu * per_cu);
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL((grouped_gemm_kernel<TIn, TC, TOut, BM, BN, BK, WAVES_M>),

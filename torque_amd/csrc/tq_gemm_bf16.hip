@@ -141,7 +141,17 @@ template <int BM, int BN, int WAVES_M>
 tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *tile_pair,
                  const int32_t *tile_first, int64_t n_tiles, const uint16_t *a,
                  const uint16_t *b, int64_t ap, int64_t bp, double *c) {
-  int64_t grid = std::min<int64_t>(n_tiles, (int64_t)h->n_cu * 4);
+  // persistent grid: exactly the workgroups that can be resident at once
+  // (occupancy from registers and LDS), a multiple of the 8 XCDs
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (bf16x3_gemm_kernel<BM, BN, WAVES_M>), kThreads, 0) !=
+            hipSuccess || occ < 1)
+      occ = 1;
+    per_cu = occ;
+  }
+  int64_t grid = std::min<int64_t>(n_tiles, (int64_t)h->n_cu * per_cu);
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL((bf16x3_gemm_kernel<BM, BN, WAVES_M>), dim3((unsigned)grid),
