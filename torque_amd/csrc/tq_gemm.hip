@@ -84,6 +84,35 @@ struct TileLoader {
   }
 };
 
+// Where one tile of the work list reads and writes.
+template <class TIn>
+struct TileMeta {
+  GemmDesc d;
+  int m0, n0;
+  bool a_kfast, b_kfast;
+  const TIn *pa, *pb;
+};
+
+template <class TIn, int BM, int BN>
+__device__ inline TileMeta<TIn> fetch_tile(int64_t t, const GemmDesc *__restrict__ descs,
+                                           const int32_t *__restrict__ tile_pair,
+                                           const int32_t *__restrict__ tile_first,
+                                           const TIn *a, const TIn *a_packed,
+                                           const TIn *b, const TIn *b_packed) {
+  TileMeta<TIn> m;
+  const int p = tile_pair[t];
+  m.d = descs[p];
+  const int lt = (int)(t - tile_first[p]);
+  const int tiles_m = (m.d.m + BM - 1) / BM;
+  m.m0 = (lt % tiles_m) * BM;
+  m.n0 = (lt / tiles_m) * BN;
+  m.a_kfast = m.d.flags & kAKFast;
+  m.b_kfast = m.d.flags & kBKFast;
+  m.pa = ((m.d.flags & kAPacked) ? a_packed : a) + m.d.a_off;
+  m.pb = ((m.d.flags & kBPacked) ? b_packed : b) + m.d.b_off;
+  return m;
+}
+
 template <class TC, class TOut, int BM, int BN, int BK>
 struct Smem {
   static constexpr size_t kAB = sizeof(TC) * (size_t)BK * (BM + 1 + BN + 1);
@@ -116,39 +145,42 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
   const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
   const int64_t vid = tqg::xcd_virtual_id(blockIdx.x, gridDim.x);
 
-  for (int64_t t = vid; t < n_tiles; t += gridDim.x) {
-    const int p = tile_pair[t];
-    const GemmDesc d = descs[p];
-    const int lt = (int)(t - tile_first[p]);
-    const int tiles_m = (d.m + BM - 1) / BM;
-    const int m0 = (lt % tiles_m) * BM;
-    const int n0 = (lt / tiles_m) * BN;
-    const bool a_kfast = d.flags & kAKFast;
-    const bool b_kfast = d.flags & kBKFast;
-    const TIn *pa = ((d.flags & kAPacked) ? a_packed : a) + d.a_off;
-    const TIn *pb = ((d.flags & kBPacked) ? b_packed : b) + d.b_off;
+  int64_t t = vid;
+  if (t >= n_tiles) return;
+  TileMeta<TIn> cur = fetch_tile<TIn, BM, BN>(t, descs, tile_pair, tile_first, a, a_packed,
+                                              b, b_packed);
+  TileLoader<TIn, TC, BM, BK> la;
+  TileLoader<TIn, TC, BN, BK> lb;
+  la.load(cur.pa, cur.a_kfast, cur.d.lda, cur.d.m, cur.m0, cur.d.k, 0);
+  lb.load(cur.pb, cur.b_kfast, cur.d.ldb, cur.d.n, cur.n0, cur.d.k, 0);
+  while (true) {
+    const GemmDesc &d = cur.d;
     // MFMA blocks of this wavefront that hold output (wave-uniform)
-    const int tm_live = min(TM, max(0, (d.m - m0 - wm0 + 15) / 16));
-    const int tn_live = min(TN, max(0, (d.n - n0 - wn0 + 15) / 16));
-
+    const int tm_live = min(TM, max(0, (d.m - cur.m0 - wm0 + 15) / 16));
+    const int tn_live = min(TN, max(0, (d.n - cur.n0 - wn0 + 15) / 16));
     AccT acc[TM][TN];
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j) acc[i][j] = AccT{0, 0, 0, 0};
 
-    TileLoader<TIn, TC, BM, BK> la;
-    TileLoader<TIn, TC, BN, BK> lb;
-    la.load(pa, a_kfast, d.lda, d.m, m0, d.k, 0);
-    lb.load(pb, b_kfast, d.ldb, d.n, n0, d.k, 0);
+    const int64_t t_next = t + gridDim.x;
+    TileMeta<TIn> nxt;
     for (int k0 = 0; k0 < d.k; k0 += BK) {
-      __syncthreads();  // the previous k tile is no longer read
-      la.store(As, a_kfast);
-      lb.store(Bs, b_kfast);
+      __syncthreads();  // the previous k tile (or epilogue staging) is done
+      la.store(As, cur.a_kfast);
+      lb.store(Bs, cur.b_kfast);
       __syncthreads();
       if (k0 + BK < d.k) {
-        la.load(pa, a_kfast, d.lda, d.m, m0, d.k, k0 + BK);
-        lb.load(pb, b_kfast, d.ldb, d.n, n0, d.k, k0 + BK);
+        la.load(cur.pa, cur.a_kfast, d.lda, d.m, cur.m0, d.k, k0 + BK);
+        lb.load(cur.pb, cur.b_kfast, d.ldb, d.n, cur.n0, d.k, k0 + BK);
+      } else if (t_next < n_tiles) {
+        // last k tile: start the next tile's loads so their latency overlaps
+        // this tile's MFMAs and epilogue stores
+        nxt = fetch_tile<TIn, BM, BN>(t_next, descs, tile_pair, tile_first, a, a_packed,
+                                      b, b_packed);
+        la.load(nxt.pa, nxt.a_kfast, nxt.d.lda, nxt.d.m, nxt.m0, nxt.d.k, 0);
+        lb.load(nxt.pb, nxt.b_kfast, nxt.d.ldb, nxt.d.n, nxt.n0, nxt.d.k, 0);
       }
       const int ksteps = min(BK / 4, (d.k - k0 + 3) / 4);
       for (int ks = 0; ks < ksteps; ++ks) {
@@ -167,8 +199,11 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
             if (i < tm_live && j < tn_live) acc[i][j] = mfma(af[i], bf[j], acc[i][j]);
       }
     }
-    tqg::store_tile<TOut, sizeof(TC) == 8, BM, BN, WAVES_M, TM, TN>(acc, c, d, m0, n0,
-                                                                    stage);
+    tqg::store_tile<TOut, sizeof(TC) == 8, BM, BN, WAVES_M, TM, TN>(acc, c, d, cur.m0,
+                                                                    cur.n0, stage);
+    if (t_next >= n_tiles) break;
+    t = t_next;
+    cur = nxt;
   }
 }
 

@@ -62,8 +62,29 @@ __device__ inline void store_tile(const AccT (&acc)[TM][TN], TOut *__restrict__ 
     __syncthreads();
     const int total = rows * cols;
     TOut *base = cbase + (int64_t)d.m * col0;
-    if (rows == d.m) {
-      // the slab is one contiguous run of output
+    if (rows == d.m && sizeof(TOut) == 8) {
+      // the slab is one contiguous run of output: 16-byte stores of aligned
+      // pairs, with a scalar head and tail
+      const int head = (reinterpret_cast<uintptr_t>(base) & 8) ? 1 : 0;
+      const int npair = (total - head) >> 1;
+      if (tid == 0 && head && total > 0) base[0] = stage[0];
+      for (int q = tid; q < npair; q += kThreads) {
+        const int e = head + 2 * q;
+        const int c0 = e / rows, r0 = e - c0 * rows;
+        const int c1 = (r0 + 1 == rows) ? c0 + 1 : c0;
+        const int r1 = (r0 + 1 == rows) ? 0 : r0 + 1;
+        typedef TOut pair_t __attribute__((ext_vector_type(2)));
+        pair_t v;
+        v.x = stage[c0 * (BM + 1) + r0];
+        v.y = stage[c1 * (BM + 1) + r1];
+        *reinterpret_cast<pair_t *>(base + e) = v;
+      }
+      if (tid == 0 && ((total - head) & 1)) {
+        const int e = total - 1;
+        const int cc = e / rows, rr = e - cc * rows;
+        base[e] = stage[cc * (BM + 1) + rr];
+      }
+    } else if (rows == d.m) {
       for (int e = tid; e < total; e += kThreads) {
         const int cc = e / rows, rr = e - cc * rows;
         base[e] = stage[cc * (BM + 1) + rr];
