@@ -195,6 +195,12 @@ tq_status tq_bs_contract_multiply(tq_plan_t plan, const void *a, const void *b,
 tq_status tq_plan_set_config(tq_plan_t plan, int32_t config);
 tq_status tq_plan_destroy(tq_plan_t plan);
 
+/* Pairs whose contracted extent K is at most max_k are written by the
+ * streaming small-K kernel (2K flops per output element: bound by writing C)
+ * instead of MFMA tiles; 0 sends every pair to the MFMA GEMM. The default is
+ * 8 (0 for the split-BF16 tier). */
+tq_status tq_plan_set_small_k(tq_plan_t plan, int32_t max_k);
+
 /* Autotuner (no reference counterpart: the reference measures a cuTT plan
  * per call, src/gpu/block_sparse.cuh:836-863). Packs A and B once, then times
  * `reps` grouped-GEMM launches of every tile configuration with HIP events on

@@ -19,10 +19,13 @@ pytestmark = pytest.mark.gpu
 FP64_TOL = 1e-12
 
 
-def gpu_contract(a, da, b, db, legs, cuda, precision="native", cfg=None, dtype=torch.float64):
+def gpu_contract(a, da, b, db, legs, cuda, precision="native", cfg=None, dtype=torch.float64,
+                 small_k=None):
     ga = torch_tensor(a, da, cuda, dtype)
     gb = torch_tensor(b, db, cuda, dtype)
     plan = ga.plan_contract(gb, legs, precision)
+    if small_k is not None:
+        plan.set_small_k(small_k)
     if cfg is not None:
         plan.set_config(cfg)
     out = torch.empty(max(plan.out_n_elem, 1), dtype=dtype, device=cuda)
@@ -140,6 +143,26 @@ def test_every_gemm_config_agrees(cuda):
     for cfg in range(len(gemm_configs())):
         _, got = gpu_contract(A, da, B, db, legs, cuda, cfg=cfg)
         assert rel_fro(got, ref_c) < FP64_TOL, cfg
+
+
+@pytest.mark.parametrize("small_k", [0, 8, 64])
+def test_small_k_stream_and_mfma_agree(cuda, small_k):
+    # the same contraction through MFMA tiles only, the default split, and
+    # the streaming kernel for every pair
+    rng = np.random.default_rng(31)
+    dims_a = [7, 9, 4, 3]
+    dims_b = [4, 3, 5]
+    ba, ea = random_blocks(rng, dims_a, 8)
+    bb, eb = random_blocks(rng, dims_b, 6)
+    a, da = tensor(dims_a, ba, ea, rng)
+    b, db = tensor(dims_b, bb, eb, rng)
+    legs = [[2, 0], [3, 1]]
+    ref_c, ref = O.contract(a, da, b, db, legs)
+    plan, got = gpu_contract(a, da, b, db, legs, cuda, small_k=small_k)
+    check_maps(plan, ref)
+    assert rel_fro(got, ref_c) < FP64_TOL
+    _, got32 = gpu_contract(a, da, b, db, legs, cuda, small_k=small_k, precision="fp32")
+    assert rel_fro(got32, ref_c) < 1e-5
 
 
 def test_peps_absorption_d6_chi64(cuda):

@@ -138,6 +138,17 @@ enum : int32_t {
   kBPacked = 8,
 };
 
+// Pairs with K <= kStreamMaxK are written by the streaming small-K kernel
+// in chunks of kStreamChunk output elements instead of MFMA tiles.
+constexpr int kStreamMaxK = 8;
+constexpr int kStreamChunk = 4096;
+
+tq_status tq_launch_small_k(tq_handle_t h, tq_dtype dt, tq_precision prec,
+                            const GemmDesc *descs, const int32_t *chunk_pair,
+                            const int32_t *chunk_first, int64_t n_chunks, const void *a,
+                            const void *a_packed, const void *b, const void *b_packed,
+                            void *c);
+
 struct GemmConfig {
   int bm, bn, bk;
 };

@@ -63,6 +63,7 @@ int tq_plan_pick_config(tq_plan_t pl) {
     const GemmConfig g = tq_gemm_config(pl->prec, c);
     double flop = 0, bytes = 0, tiles = 0;
     for (int64_t p = 0; p < pl->n_pairs; ++p) {
+      if (pl->h_k[p] <= pl->stream_max_k) continue;
       const double tm = std::ceil((double)pl->h_m[p] / g.bm);
       const double tn = std::ceil((double)pl->h_n[p] / g.bn);
       const double kp = std::ceil((double)pl->h_k[p] / 4.0) * 4.0;
@@ -129,6 +130,12 @@ tq_status tq_bs_contract_multiply(tq_plan_t pl, const void *a, const void *b,
   if (pl->ro > 0) {
     const int cfg = pl->cfg >= 0 ? pl->cfg : pl->auto_cfg;
     TQ_CHECK(plan_build_tiles(pl, cfg));
+    if (pl->n_chunks > 0)
+      TQ_CHECK(tq_launch_small_k(h, pl->dtype, pl->prec, pl->desc.as<GemmDesc>(),
+                                 pl->chunk_pair.as<int32_t>(),
+                                 pl->chunk_first.as<int32_t>(), pl->n_chunks, a,
+                                 pl->pack_a.panel.p, b, pl->pack_b.panel.p, c));
+    if (pl->n_tiles == 0) return TQ_SUCCESS;
     if (pl->prec == TQ_PREC_BF16X3)
       return tq_launch_bf16x3_gemm(h, cfg, pl->desc.as<GemmDesc>(),
                                    pl->tile_pair.as<int32_t>(),
@@ -169,6 +176,18 @@ tq_status tq_bs_contract_multiply(tq_plan_t pl, const void *a, const void *b,
     hipLaunchKernelGGL(store_scalar_kernel<float>, dim3(1), dim3(64), 0, h->stream,
                        sum, (float *)c);
   TQ_HIP(h, hipGetLastError());
+  return TQ_SUCCESS;
+}
+
+tq_status tq_plan_set_small_k(tq_plan_t pl, int32_t max_k) {
+  if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  if (max_k < 0) return TQ_ERROR_INVALID_VALUE;
+  if (pl->prec == TQ_PREC_BF16X3 && max_k > 0)
+    return tq_fail(pl->h, TQ_ERROR_UNSUPPORTED, "split-BF16 tier runs every pair on MFMA");
+  pl->stream_max_k = max_k;
+  pl->chunks_ready = false;
+  pl->tiles_ready = false;
+  pl->auto_cfg = tq_plan_pick_config(pl);
   return TQ_SUCCESS;
 }
 

@@ -98,9 +98,14 @@ struct tq_plan_s {
   DevBuf order;      // int64 work list (pair ids, largest cost first)
   PackSide pack_a, pack_b;
 
-  // grouped-GEMM tile list for the current configuration
+  // grouped-GEMM tile list for the current configuration (pairs with
+  // K > stream_max_k) and the streaming chunk list (K <= stream_max_k)
   DevBuf tile_pair, tile_first;
   int64_t n_tiles = 0;
+  int stream_max_k = 0;
+  DevBuf chunk_pair, chunk_first;
+  int64_t n_chunks = 0;
+  bool chunks_ready = false;
   int tiles_cfg = -1;
   bool tiles_ready = false;
   int cfg = -1;       // user-selected configuration (-1 = automatic)

@@ -502,14 +502,20 @@ __global__ __launch_bounds__(256) void sort_scatter_kernel(
   }
 }
 
+// Work items per position of the work list: MFMA tiles of pairs with
+// K > max_k (chunk = 0), or streaming chunks of pairs with K <= max_k
+// (chunk > 0, the chunk size in elements).
 __global__ void tiles_per_position_kernel(int64_t n, const i64 *__restrict__ order,
                                           const GemmDesc *__restrict__ desc,
-                                          int32_t bm, int32_t bn,
-                                          i64 *__restrict__ tiles) {
+                                          int32_t bm, int32_t bn, int32_t max_k,
+                                          int32_t chunk, i64 *__restrict__ tiles) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const GemmDesc d = desc[order[q]];
-  tiles[q] = ceil_div(d.m, bm) * ceil_div(d.n, bn);
+  if (chunk > 0)
+    tiles[q] = d.k <= max_k ? ceil_div((i64)d.m * d.n, chunk) : 0;
+  else
+    tiles[q] = d.k > max_k ? ceil_div(d.m, bm) * ceil_div(d.n, bn) : 0;
 }
 
 __global__ void expand_tiles_kernel(int64_t n, const i64 *__restrict__ order,
@@ -559,31 +565,56 @@ bool valid_desc(const tq_bs_desc *d) {
 
 // ---- plan object ----------------------------------------------------------
 
-tq_status plan_build_tiles(tq_plan_t pl, int cfg) {
+// Expands the work list into per-item pair indices (tile_pair / chunk_pair)
+// and each pair's first item; `total` is known on the host from the shapes.
+static tq_status expand_work(tq_plan_t pl, int32_t bm, int32_t bn, int32_t chunk,
+                             int64_t total, DevBuf &item_pair, DevBuf &item_first) {
   tq_handle_t h = pl->h;
-  if (pl->tiles_cfg == cfg && pl->tiles_ready) return TQ_SUCCESS;
-  const GemmConfig c = tq_gemm_config(pl->prec, cfg);
   const int64_t np = pl->n_pairs;
-  int64_t total = 0;
-  for (int64_t p = 0; p < np; ++p)
-    total += ceil_div(pl->h_m[p], c.bm) * ceil_div(pl->h_n[p], c.bn);
-  if (total > 0x7fffffff)
-    return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many GEMM tiles");
-  DevBuf tiles, tmp;
-  TQ_HIP(h, tiles.alloc(sizeof(i64) * (np + 1)));
+  DevBuf items, tmp;
+  TQ_HIP(h, items.alloc(sizeof(i64) * (np + 1)));
   TQ_HIP(h, tmp.alloc(tq_scan_tmp_bytes(np + 1)));
-  TQ_HIP(h, pl->tile_pair.alloc(sizeof(int32_t) * std::max<int64_t>(total, 1)));
-  TQ_HIP(h, pl->tile_first.alloc(sizeof(int32_t) * std::max<int64_t>(np, 1)));
+  TQ_HIP(h, item_pair.alloc(sizeof(int32_t) * std::max<int64_t>(total, 1)));
+  TQ_HIP(h, item_first.alloc(sizeof(int32_t) * std::max<int64_t>(np, 1)));
   hipLaunchKernelGGL(tiles_per_position_kernel, dim3(grid_for(np)), dim3(256), 0,
-                     h->stream, np, pl->order.as<i64>(), pl->desc.as<GemmDesc>(),
-                     c.bm, c.bn, tiles.as<i64>());
-  TQ_CHECK(tq_scan_i64(h, tiles.as<i64>(), tiles.as<i64>(), np,
-                       tiles.as<i64>() + np, tmp.p));
-  hipLaunchKernelGGL(expand_tiles_kernel, dim3(grid_for(np)), dim3(256), 0,
-                     h->stream, np, pl->order.as<i64>(), tiles.as<i64>(),
-                     pl->tile_pair.as<int32_t>(), pl->tile_first.as<int32_t>());
+                     h->stream, np, pl->order.as<i64>(), pl->desc.as<GemmDesc>(), bm, bn,
+                     (int32_t)pl->stream_max_k, chunk, items.as<i64>());
+  TQ_CHECK(tq_scan_i64(h, items.as<i64>(), items.as<i64>(), np, items.as<i64>() + np,
+                       tmp.p));
+  hipLaunchKernelGGL(expand_tiles_kernel, dim3(grid_for(np)), dim3(256), 0, h->stream,
+                     np, pl->order.as<i64>(), items.as<i64>(), item_pair.as<int32_t>(),
+                     item_first.as<int32_t>());
   TQ_HIP(h, hipGetLastError());
   TQ_HIP(h, hipStreamSynchronize(h->stream));
+  return TQ_SUCCESS;
+}
+
+tq_status plan_build_tiles(tq_plan_t pl, int cfg) {
+  tq_handle_t h = pl->h;
+  const int64_t np = pl->n_pairs;
+  if (!pl->chunks_ready) {
+    int64_t total = 0;
+    for (int64_t p = 0; p < np; ++p)
+      if (pl->h_k[p] <= pl->stream_max_k)
+        total += ceil_div(pl->h_m[p] * pl->h_n[p], kStreamChunk);
+    if (total > 0x7fffffff)
+      return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many output chunks");
+    if (total > 0)
+      TQ_CHECK(expand_work(pl, 0, 0, kStreamChunk, total, pl->chunk_pair,
+                           pl->chunk_first));
+    pl->n_chunks = total;
+    pl->chunks_ready = true;
+  }
+  if (pl->tiles_cfg == cfg && pl->tiles_ready) return TQ_SUCCESS;
+  const GemmConfig c = tq_gemm_config(pl->prec, cfg);
+  int64_t total = 0;
+  for (int64_t p = 0; p < np; ++p)
+    if (pl->h_k[p] > pl->stream_max_k)
+      total += ceil_div(pl->h_m[p], c.bm) * ceil_div(pl->h_n[p], c.bn);
+  if (total > 0x7fffffff)
+    return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many GEMM tiles");
+  if (total > 0)
+    TQ_CHECK(expand_work(pl, c.bm, c.bn, 0, total, pl->tile_pair, pl->tile_first));
   pl->n_tiles = total;
   pl->tiles_cfg = cfg;
   pl->tiles_ready = true;
@@ -1020,6 +1051,9 @@ tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
                            hipMemcpyDeviceToHost, h->stream));
   TQ_HIP(h, hipStreamSynchronize(h->stream));
 
+  // short contractions are written as streams; the split-BF16 tier keeps
+  // every pair on the BF16 MFMA
+  pl->stream_max_k = pl->prec == TQ_PREC_BF16X3 ? 0 : kStreamMaxK;
   pl->auto_cfg = tq_plan_pick_config(pl.get());
   pl->cfg = -1;
   if (np > 0 && ro > 0) TQ_CHECK(plan_build_tiles(pl.get(), pl->auto_cfg));

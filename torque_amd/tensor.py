@@ -361,6 +361,9 @@ class ContractionPlan:
             ctypes.c_void_p(out.data_ptr()), int(reps), ctypes.byref(best), times))
         return best.value, [times[i] for i in range(n)]
 
+    def set_small_k(self, max_k):
+        self.h.check(lib().tq_plan_set_small_k(self.ptr, int(max_k)))
+
     def set_config(self, cfg):
         self.h.check(lib().tq_plan_set_config(self.ptr, int(cfg)))
 
