@@ -138,10 +138,11 @@ enum : int32_t {
   kBPacked = 8,
 };
 
-// Pairs with K <= kStreamMaxK are written by the streaming small-K kernel
-// in chunks of kStreamChunk output elements instead of MFMA tiles.
+// Pairs with K <= kStreamMaxK are written by the streaming small-K kernel,
+// one work item per kStreamRows-row segment of the output block, instead of
+// MFMA tiles.
 constexpr int kStreamMaxK = 8;
-constexpr int kStreamChunk = 4096;
+constexpr int kStreamRows = 64;
 
 tq_status tq_launch_small_k(tq_handle_t h, tq_dtype dt, tq_precision prec,
                             const GemmDesc *descs, const int32_t *chunk_pair,

@@ -503,8 +503,8 @@ __global__ __launch_bounds__(256) void sort_scatter_kernel(
 }
 
 // Work items per position of the work list: MFMA tiles of pairs with
-// K > max_k (chunk = 0), or streaming chunks of pairs with K <= max_k
-// (chunk > 0, the chunk size in elements).
+// K > max_k (chunk = 0), or streaming row segments of pairs with K <= max_k
+// (chunk > 0, the segment height in rows).
 __global__ void tiles_per_position_kernel(int64_t n, const i64 *__restrict__ order,
                                           const GemmDesc *__restrict__ desc,
                                           int32_t bm, int32_t bn, int32_t max_k,
@@ -513,7 +513,7 @@ __global__ void tiles_per_position_kernel(int64_t n, const i64 *__restrict__ ord
   if (q >= n) return;
   const GemmDesc d = desc[order[q]];
   if (chunk > 0)
-    tiles[q] = d.k <= max_k ? ceil_div((i64)d.m * d.n, chunk) : 0;
+    tiles[q] = d.k <= max_k ? ceil_div(d.m, chunk) : 0;
   else
     tiles[q] = d.k > max_k ? ceil_div(d.m, bm) * ceil_div(d.n, bn) : 0;
 }
@@ -596,11 +596,11 @@ tq_status plan_build_tiles(tq_plan_t pl, int cfg) {
     int64_t total = 0;
     for (int64_t p = 0; p < np; ++p)
       if (pl->h_k[p] <= pl->stream_max_k)
-        total += ceil_div(pl->h_m[p] * pl->h_n[p], kStreamChunk);
+        total += ceil_div(pl->h_m[p], kStreamRows);
     if (total > 0x7fffffff)
       return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many output chunks");
     if (total > 0)
-      TQ_CHECK(expand_work(pl, 0, 0, kStreamChunk, total, pl->chunk_pair,
+      TQ_CHECK(expand_work(pl, 0, 0, kStreamRows, total, pl->chunk_pair,
                            pl->chunk_first));
     pl->n_chunks = total;
     pl->chunks_ready = true;

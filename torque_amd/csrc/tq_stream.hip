@@ -1,58 +1,54 @@
 // tq_stream.hip -- the small-K path of the grouped contraction.
 //
 // A pair with a short contracted extent (K <= kStreamMaxK) does 2K flops per
-// output element: its cost is writing C, not multiplying. Such pairs skip the
-// MFMA tiles and are written as a stream: workgroup chunk c covers
-// kStreamChunk consecutive elements of one pair's column-major output block,
-// lane-contiguous like a fill, each element the FP64 (or FP32) fused
-// multiply-add chain sum_k A[m,k] * B[n,k] over the pair's operand views
-// (read through L2; the operand panels are small next to the output).
+// output element: its cost is writing C, not multiplying, so it skips the
+// MFMA tiles. One wavefront takes a 64-row segment of the pair's column-major
+// output: each lane holds its row of A (K values) in registers and walks the
+// N columns; B[n, :] is the same for the whole wavefront (scalar loads), and
+// each column is one contiguous 64-element store.
 #include "tq_common.h"
 
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kPer = kStreamChunk / kThreads;
+constexpr int kWaves = kThreads / 64;
 
 template <class TIn, class TC, class TOut>
 __global__ __launch_bounds__(kThreads) void small_k_kernel(
-    const GemmDesc *__restrict__ descs, const int32_t *__restrict__ chunk_pair,
-    const int32_t *__restrict__ chunk_first, int64_t n_chunks,
+    const GemmDesc *__restrict__ descs, const int32_t *__restrict__ seg_pair,
+    const int32_t *__restrict__ seg_first, int64_t n_segs,
     const TIn *__restrict__ a, const TIn *__restrict__ a_packed,
     const TIn *__restrict__ b, const TIn *__restrict__ b_packed,
     TOut *__restrict__ c) {
-  for (int64_t ch = blockIdx.x; ch < n_chunks; ch += gridDim.x) {
-    const int p = chunk_pair[ch];
+  const int lane = threadIdx.x & 63;
+  // the segment index is the same on every lane of the wavefront: say so, so
+  // that the descriptor and B reads become scalar loads
+  const int wave_id = __builtin_amdgcn_readfirstlane(
+      (int)(blockIdx.x * kWaves + (threadIdx.x >> 6)));
+  const int n_waves = (int)(gridDim.x * kWaves);
+  for (int sg = wave_id; sg < n_segs; sg += n_waves) {
+    const int p = __builtin_amdgcn_readfirstlane(seg_pair[sg]);
     const GemmDesc d = descs[p];
-    const int64_t e0 = (int64_t)(ch - chunk_first[p]) * kStreamChunk;
-    const int64_t mn = (int64_t)d.m * d.n;
+    const int m = (int)(sg - seg_first[p]) * 64 + lane;
+    const bool live = m < d.m;
     const TIn *pa = ((d.flags & kAPacked) ? a_packed : a) + d.a_off;
     const TIn *pb = ((d.flags & kBPacked) ? b_packed : b) + d.b_off;
-    // element strides of m and k in A, of n and k in B
     const int64_t sam = (d.flags & kAKFast) ? d.lda : 1;
     const int64_t sak = (d.flags & kAKFast) ? 1 : d.lda;
     const int64_t sbn = (d.flags & kBKFast) ? d.ldb : 1;
     const int64_t sbk = (d.flags & kBKFast) ? 1 : d.ldb;
-    TOut *pc = c + d.c_off;
-    // first element of this thread, then step kThreads elements at a time
-    int64_t e = e0 + threadIdx.x;
-    int64_t n = e / d.m;
-    int64_t m = e - n * d.m;
-#pragma unroll 4
-    for (int q = 0; q < kPer; ++q) {
-      if (e >= mn) break;
-      const TIn *ra = pa + m * sam;
+    TC av[kStreamMaxK];
+#pragma unroll
+    for (int k = 0; k < kStreamMaxK; ++k)
+      av[k] = (live && k < d.k) ? (TC)pa[m * sam + k * sak] : (TC)0;
+    TOut *pc = c + d.c_off + m;
+    for (int n = 0; n < d.n; ++n) {
       const TIn *rb = pb + n * sbn;
-      TC acc = (TC)ra[0] * (TC)rb[0];
-      for (int k = 1; k < d.k; ++k) acc = fma((TC)ra[k * sak], (TC)rb[k * sbk], acc);
-      pc[e] = (TOut)acc;
-      e += kThreads;
-      m += kThreads;
-      if (m >= d.m) {
-        const uint32_t wrap = (uint32_t)m / (uint32_t)d.m;
-        n += wrap;
-        m -= (int64_t)wrap * d.m;
-      }
+      TC acc = av[0] * (TC)rb[0];
+#pragma unroll
+      for (int k = 1; k < kStreamMaxK; ++k)
+        if (k < d.k) acc = fma(av[k], (TC)rb[k * sbk], acc);
+      if (live) pc[(int64_t)d.m * n] = (TOut)acc;
     }
   }
 }
@@ -69,7 +65,8 @@ tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *chunk_pair
       occ = 1;
     per_cu = occ;
   }
-  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(n_chunks, (int64_t)h->n_cu * per_cu));
+  const int64_t grid = std::max<int64_t>(
+      1, std::min<int64_t>(ceil_div(n_chunks, kWaves), (int64_t)h->n_cu * per_cu));
   hipLaunchKernelGGL((small_k_kernel<TIn, TC, TOut>), dim3((unsigned)grid), dim3(kThreads),
                      0, h->stream, descs, chunk_pair, chunk_first, n_chunks,
                      reinterpret_cast<const TIn *>(a), reinterpret_cast<const TIn *>(ap),
