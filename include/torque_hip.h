@@ -195,7 +195,7 @@ tq_status tq_bs_contract_multiply(tq_plan_t plan, const void *a, const void *b,
 tq_status tq_plan_set_config(tq_plan_t plan, int32_t config);
 tq_status tq_plan_destroy(tq_plan_t plan);
 
-/* Pairs whose contracted extent K is at most max_k are written by the
+/* Pairs whose contracted extent K is at most max_k (0..8) are written by the
  * streaming small-K kernel (2K flops per output element: bound by writing C)
  * instead of MFMA tiles; 0 sends every pair to the MFMA GEMM. The default is
  * 8 (0 for the split-BF16 tier). */

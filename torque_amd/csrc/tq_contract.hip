@@ -181,7 +181,8 @@ tq_status tq_bs_contract_multiply(tq_plan_t pl, const void *a, const void *b,
 
 tq_status tq_plan_set_small_k(tq_plan_t pl, int32_t max_k) {
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
-  if (max_k < 0) return TQ_ERROR_INVALID_VALUE;
+  if (max_k < 0 || max_k > kStreamMaxK)
+    return tq_fail(pl->h, TQ_ERROR_INVALID_VALUE, "small-K threshold above 8");
   if (pl->prec == TQ_PREC_BF16X3 && max_k > 0)
     return tq_fail(pl->h, TQ_ERROR_UNSUPPORTED, "split-BF16 tier runs every pair on MFMA");
   pl->stream_max_k = max_k;
