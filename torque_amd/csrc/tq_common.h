@@ -142,7 +142,7 @@ enum : int32_t {
 // one work item per kStreamRows-row segment of the output block, instead of
 // MFMA tiles.
 constexpr int kStreamMaxK = 8;
-constexpr int kStreamRows = 128;
+constexpr int kStreamRows = 64;
 
 tq_status tq_launch_small_k(tq_handle_t h, tq_dtype dt, tq_precision prec,
                             const GemmDesc *descs, const int32_t *chunk_pair,

@@ -2,17 +2,50 @@
 //
 // A pair with a short contracted extent (K <= kStreamMaxK) does 2K flops per
 // output element: its cost is writing C, not multiplying, so it skips the
-// MFMA tiles. One wavefront takes a 128-row segment of the pair's
-// column-major output: each lane holds two rows of A (K values each) in
-// registers and walks the N columns; B[n, :] is the same for the whole
-// wavefront (scalar loads), and each column segment is written with
-// contiguous 16-byte (or 8-byte) stores.
+// MFMA tiles. One wavefront takes a 64-row segment of the pair's column-major
+// output: each lane holds its row of A (K values) in registers and walks the
+// N columns; B[n, :] is the same for the whole wavefront (scalar loads), and
+// each column is one contiguous 64-element store. The next segment's
+// descriptor and A rows are loaded while the current one is written.
 #include "tq_common.h"
 
 namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
+
+// Operands of one 64-row segment: the lane's row of A and the column walk.
+template <class TIn, class TC>
+struct Segment {
+  GemmDesc d;
+  int row;
+  bool live;
+  const TIn *pb;
+  int64_t sbn, sbk;
+  TC av[kStreamMaxK];
+};
+
+template <class TIn, class TC>
+__device__ inline void load_segment(Segment<TIn, TC> &s, int sg,
+                                    const GemmDesc *__restrict__ descs,
+                                    const int32_t *__restrict__ seg_pair,
+                                    const int32_t *__restrict__ seg_first,
+                                    const TIn *a, const TIn *a_packed, const TIn *b,
+                                    const TIn *b_packed, int lane) {
+  const int p = __builtin_amdgcn_readfirstlane(seg_pair[sg]);
+  s.d = descs[p];
+  s.row = (sg - seg_first[p]) * kStreamRows + lane;
+  s.live = s.row < s.d.m;
+  const TIn *pa = ((s.d.flags & kAPacked) ? a_packed : a) + s.d.a_off;
+  s.pb = ((s.d.flags & kBPacked) ? b_packed : b) + s.d.b_off;
+  const int64_t sam = (s.d.flags & kAKFast) ? s.d.lda : 1;
+  const int64_t sak = (s.d.flags & kAKFast) ? 1 : s.d.lda;
+  s.sbn = (s.d.flags & kBKFast) ? s.d.ldb : 1;
+  s.sbk = (s.d.flags & kBKFast) ? 1 : s.d.ldb;
+#pragma unroll
+  for (int k = 0; k < kStreamMaxK; ++k)
+    s.av[k] = (s.live && k < s.d.k) ? (TC)pa[s.row * sam + k * sak] : (TC)0;
+}
 
 template <class TIn, class TC, class TOut>
 __global__ __launch_bounds__(kThreads) void small_k_kernel(
@@ -27,55 +60,28 @@ __global__ __launch_bounds__(kThreads) void small_k_kernel(
   const int wave_id = __builtin_amdgcn_readfirstlane(
       (int)(blockIdx.x * kWaves + (threadIdx.x >> 6)));
   const int n_waves = (int)(gridDim.x * kWaves);
-  for (int sg = wave_id; sg < n_segs; sg += n_waves) {
-    const int p = __builtin_amdgcn_readfirstlane(seg_pair[sg]);
-    const GemmDesc d = descs[p];
-    const int m0 = (int)(sg - seg_first[p]) * kStreamRows;
-    // paired rows: every column of the block starts on an even element, so
-    // lane L writes rows 2L, 2L+1 as one 16-byte store; otherwise rows L and
-    // L + 64 as two 8-byte stores
-    const bool paired = sizeof(TOut) == 8 && ((d.c_off + m0) & 1) == 0 && (d.m & 1) == 0;
-    const int r0 = paired ? m0 + 2 * lane : m0 + lane;
-    const int r1 = paired ? r0 + 1 : r0 + 64;
-    const bool live0 = r0 < d.m, live1 = r1 < d.m;
-    const TIn *pa = ((d.flags & kAPacked) ? a_packed : a) + d.a_off;
-    const TIn *pb = ((d.flags & kBPacked) ? b_packed : b) + d.b_off;
-    const int64_t sam = (d.flags & kAKFast) ? d.lda : 1;
-    const int64_t sak = (d.flags & kAKFast) ? 1 : d.lda;
-    const int64_t sbn = (d.flags & kBKFast) ? d.ldb : 1;
-    const int64_t sbk = (d.flags & kBKFast) ? 1 : d.ldb;
-    TC a0[kStreamMaxK], a1[kStreamMaxK];
-#pragma unroll
-    for (int k = 0; k < kStreamMaxK; ++k) {
-      a0[k] = (live0 && k < d.k) ? (TC)pa[r0 * sam + k * sak] : (TC)0;
-      a1[k] = (live1 && k < d.k) ? (TC)pa[r1 * sam + k * sak] : (TC)0;
-    }
-    TOut *pc = c + d.c_off;
-    for (int n = 0; n < d.n; ++n) {
-      const TIn *rb = pb + n * sbn;
-      const TC bk0 = (TC)rb[0];
-      TC x0 = a0[0] * bk0, x1 = a1[0] * bk0;
+  int sg = wave_id;
+  if (sg >= n_segs) return;
+  Segment<TIn, TC> cur, nxt;
+  load_segment(cur, sg, descs, seg_pair, seg_first, a, a_packed, b, b_packed, lane);
+  while (true) {
+    const int sg_next = sg + n_waves;
+    // the next segment's loads overlap this segment's stores
+    if (sg_next < n_segs)
+      load_segment(nxt, sg_next, descs, seg_pair, seg_first, a, a_packed, b, b_packed,
+                   lane);
+    TOut *col = c + cur.d.c_off + cur.row;
+    for (int n = 0; n < cur.d.n; ++n) {
+      const TIn *rb = cur.pb + n * cur.sbn;
+      TC acc = cur.av[0] * (TC)rb[0];
 #pragma unroll
       for (int k = 1; k < kStreamMaxK; ++k)
-        if (k < d.k) {
-          const TC bk = (TC)rb[k * sbk];
-          x0 = fma(a0[k], bk, x0);
-          x1 = fma(a1[k], bk, x1);
-        }
-      TOut *col = pc + (int64_t)d.m * n;
-      if (paired) {
-        if (live0) {  // d.m even: live0 implies live1
-          typedef TOut pair_t __attribute__((ext_vector_type(2)));
-          pair_t v;
-          v.x = (TOut)x0;
-          v.y = (TOut)x1;
-          *reinterpret_cast<pair_t *>(col + r0) = v;
-        }
-      } else {
-        if (live0) col[r0] = (TOut)x0;
-        if (live1) col[r1] = (TOut)x1;
-      }
+        if (k < cur.d.k) acc = fma(cur.av[k], (TC)rb[k * cur.sbk], acc);
+      if (cur.live) col[(int64_t)cur.d.m * n] = (TOut)acc;
     }
+    if (sg_next >= n_segs) break;
+    sg = sg_next;
+    cur = nxt;
   }
 }
 
