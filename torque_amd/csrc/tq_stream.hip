@@ -5,47 +5,13 @@
 // MFMA tiles. One wavefront takes a 64-row segment of the pair's column-major
 // output: each lane holds its row of A (K values) in registers and walks the
 // N columns; B[n, :] is the same for the whole wavefront (scalar loads), and
-// each column is one contiguous 64-element store. The next segment's
-// descriptor and A rows are loaded while the current one is written.
+// each column is one contiguous 64-element store.
 #include "tq_common.h"
 
 namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
-
-// Operands of one 64-row segment: the lane's row of A and the column walk.
-template <class TIn, class TC>
-struct Segment {
-  GemmDesc d;
-  int row;
-  bool live;
-  const TIn *pb;
-  int64_t sbn, sbk;
-  TC av[kStreamMaxK];
-};
-
-template <class TIn, class TC>
-__device__ inline void load_segment(Segment<TIn, TC> &s, int sg,
-                                    const GemmDesc *__restrict__ descs,
-                                    const int32_t *__restrict__ seg_pair,
-                                    const int32_t *__restrict__ seg_first,
-                                    const TIn *a, const TIn *a_packed, const TIn *b,
-                                    const TIn *b_packed, int lane) {
-  const int p = __builtin_amdgcn_readfirstlane(seg_pair[sg]);
-  s.d = descs[p];
-  s.row = (sg - seg_first[p]) * kStreamRows + lane;
-  s.live = s.row < s.d.m;
-  const TIn *pa = ((s.d.flags & kAPacked) ? a_packed : a) + s.d.a_off;
-  s.pb = ((s.d.flags & kBPacked) ? b_packed : b) + s.d.b_off;
-  const int64_t sam = (s.d.flags & kAKFast) ? s.d.lda : 1;
-  const int64_t sak = (s.d.flags & kAKFast) ? 1 : s.d.lda;
-  s.sbn = (s.d.flags & kBKFast) ? s.d.ldb : 1;
-  s.sbk = (s.d.flags & kBKFast) ? 1 : s.d.ldb;
-#pragma unroll
-  for (int k = 0; k < kStreamMaxK; ++k)
-    s.av[k] = (s.live && k < s.d.k) ? (TC)pa[s.row * sam + k * sak] : (TC)0;
-}
 
 template <class TIn, class TC, class TOut>
 __global__ __launch_bounds__(kThreads) void small_k_kernel(
@@ -60,28 +26,30 @@ __global__ __launch_bounds__(kThreads) void small_k_kernel(
   const int wave_id = __builtin_amdgcn_readfirstlane(
       (int)(blockIdx.x * kWaves + (threadIdx.x >> 6)));
   const int n_waves = (int)(gridDim.x * kWaves);
-  int sg = wave_id;
-  if (sg >= n_segs) return;
-  Segment<TIn, TC> cur, nxt;
-  load_segment(cur, sg, descs, seg_pair, seg_first, a, a_packed, b, b_packed, lane);
-  while (true) {
-    const int sg_next = sg + n_waves;
-    // the next segment's loads overlap this segment's stores
-    if (sg_next < n_segs)
-      load_segment(nxt, sg_next, descs, seg_pair, seg_first, a, a_packed, b, b_packed,
-                   lane);
-    TOut *col = c + cur.d.c_off + cur.row;
-    for (int n = 0; n < cur.d.n; ++n) {
-      const TIn *rb = cur.pb + n * cur.sbn;
-      TC acc = cur.av[0] * (TC)rb[0];
+  for (int sg = wave_id; sg < n_segs; sg += n_waves) {
+    const int p = __builtin_amdgcn_readfirstlane(seg_pair[sg]);
+    const GemmDesc d = descs[p];
+    const int m = (int)(sg - seg_first[p]) * 64 + lane;
+    const bool live = m < d.m;
+    const TIn *pa = ((d.flags & kAPacked) ? a_packed : a) + d.a_off;
+    const TIn *pb = ((d.flags & kBPacked) ? b_packed : b) + d.b_off;
+    const int64_t sam = (d.flags & kAKFast) ? d.lda : 1;
+    const int64_t sak = (d.flags & kAKFast) ? 1 : d.lda;
+    const int64_t sbn = (d.flags & kBKFast) ? d.ldb : 1;
+    const int64_t sbk = (d.flags & kBKFast) ? 1 : d.ldb;
+    TC av[kStreamMaxK];
+#pragma unroll
+    for (int k = 0; k < kStreamMaxK; ++k)
+      av[k] = (live && k < d.k) ? (TC)pa[m * sam + k * sak] : (TC)0;
+    TOut *pc = c + d.c_off + m;
+    for (int n = 0; n < d.n; ++n) {
+      const TIn *rb = pb + n * sbn;
+      TC acc = av[0] * (TC)rb[0];
 #pragma unroll
       for (int k = 1; k < kStreamMaxK; ++k)
-        if (k < cur.d.k) acc = fma(cur.av[k], (TC)rb[k * cur.sbk], acc);
-      if (cur.live) col[(int64_t)cur.d.m * n] = (TOut)acc;
+        if (k < d.k) acc = fma(av[k], (TC)rb[k * sbk], acc);
+      if (live) pc[(int64_t)d.m * n] = (TOut)acc;
     }
-    if (sg_next >= n_segs) break;
-    sg = sg_next;
-    cur = nxt;
   }
 }
 
