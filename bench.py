@@ -42,6 +42,9 @@ def parse():
     p.add_argument("--cpu-sample-blocks", type=int, default=0,
                    help="E blocks in the CPU-oracle sample (0 = all)")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="finish with a plain 1 GiB device copy (reference for "
+                        "FETCH_SIZE / WRITE_SIZE counters)")
     return p.parse_args()
 
 
@@ -226,6 +229,12 @@ def main():
         "fp64_rel_err_vs_oracle": err64,
     }
     print(json.dumps(line), flush=True)
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("# calibration copy: 1 GiB read, 1 GiB written", flush=True)
 
 
 if __name__ == "__main__":
