@@ -30,6 +30,19 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
+def committed_traffic(name):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/<name>_traffic.md),
+    or None when no summary exists for this kernel and workload."""
+    path = os.path.join(ROOT, "profiles", name + "_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -116,14 +129,19 @@ def main():
     gemm_s = ev0.elapsed_time(ev1) / 1e3 / n_rep
     alg_bytes = p2.bytes
     achieved = alg_bytes / gemm_s / 1e9
+    n_tiles, n_segs = p2.work()
+    kernel = ("small_k_kernel" if n_tiles == 0 else
+              "grouped_gemm_kernel" if n_segs == 0 else "small_k_kernel + grouped_gemm_kernel")
+    traffic = committed_traffic("%s_peps_absorption_D%d_chi%d" % (kernel, args.D, args.chi))
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "grouped_gemm_kernel (E.a)", "kernel_ms": round(gemm_s * 1e3, 4),
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": kernel + " (E.a)", "kernel_ms": round(gemm_s * 1e3, 4),
                 "kernel_gflops": round(p2.flops / gemm_s / 1e9, 1),
                 "algorithmic_bytes": alg_bytes}
-    print("# E.a grouped GEMM %.3f ms: %.0f GB/s algorithmic (%.1f%% of HBM), %.0f GFLOP/s"
-          % (gemm_s * 1e3, achieved, 100 * achieved / HBM_PEAK_GBS, p2.flops / gemm_s / 1e9),
-          flush=True)
+    print("# E.a multiply (%s; %d tiles, %d segments) %.3f ms: %.0f GB/s algorithmic "
+          "(%.1f%% of HBM), %.0f GFLOP/s"
+          % (kernel, n_tiles, n_segs, gemm_s * 1e3, achieved, 100 * achieved / HBM_PEAK_GBS,
+             p2.flops / gemm_s / 1e9), flush=True)
 
     # ---- oracle sample: E blocks [0, s) x all of a, pairs contiguous in out
     from oracle import oracle as O

@@ -41,7 +41,8 @@ class PlanInfo(ctypes.Structure):
     _fields_ = [("out_rank", ctypes.c_int32), ("n_legs", ctypes.c_int32),
                 ("n_pairs", ctypes.c_int64), ("out_n_elem", ctypes.c_int64),
                 ("flops", ctypes.c_double), ("bytes", ctypes.c_double),
-                ("a_layout", ctypes.c_int32), ("b_layout", ctypes.c_int32)]
+                ("a_layout", ctypes.c_int32), ("b_layout", ctypes.c_int32),
+                ("n_tiles", ctypes.c_int64), ("n_segments", ctypes.c_int64)]
 
 
 class TorqueError(RuntimeError):

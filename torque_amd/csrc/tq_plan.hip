@@ -1072,6 +1072,10 @@ tq_status tq_plan_get_info(tq_plan_t pl, tq_plan_info *info) {
   info->bytes = pl->bytes;
   info->a_layout = pl->a_layout;
   info->b_layout = pl->b_layout;
+  if (pl->n_pairs > 0 && pl->ro > 0)
+    TQ_CHECK(plan_build_tiles(pl, pl->cfg >= 0 ? pl->cfg : pl->auto_cfg));
+  info->n_tiles = pl->n_tiles;
+  info->n_segments = pl->n_chunks;
   return TQ_SUCCESS;
 }
 

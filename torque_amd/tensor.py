@@ -337,6 +337,12 @@ class ContractionPlan:
             _i64p(self.out_strides), _i64p(self.out_offsets)))
         self.device = a.data.device
 
+    def work(self):
+        """(MFMA tiles, streamed 64-row segments) of the current configuration."""
+        info = PlanInfo()
+        self.h.check(lib().tq_plan_get_info(self.ptr, ctypes.byref(info)))
+        return info.n_tiles, info.n_segments
+
     def pairs(self):
         n = self.n_pairs
         arrs = [np.zeros(n, np.int64) for _ in range(7)]
