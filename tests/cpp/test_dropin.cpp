@@ -1,0 +1,209 @@
+// The reference's GPU test sections (test/gpu/block_sparse_test.cu:10-232,
+// test/gpu/dense_test.cu:9-103) written against the drop-in C++ classes:
+// the same tensors, calls and expected values, with tq_handle_t in place of
+// cublasHandle_t. Prints one line per failed check and exits non-zero.
+#include <cstdio>
+#include <vector>
+
+#include "torque/gpu/block_sparse.h"
+#include "torque/gpu/dense.h"
+
+static int g_fail = 0, g_checks = 0;
+#define CHECK(cond)                                                     \
+  do {                                                                  \
+    ++g_checks;                                                         \
+    if (!(cond)) {                                                      \
+      ++g_fail;                                                         \
+      std::printf("FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);     \
+    }                                                                   \
+  } while (0)
+
+using torque::umat;
+using torque::uvec;
+using torque::uword;
+using BST = torque::gpu::BlockSparseTensor<float>;
+using DT = torque::gpu::DenseTensor<float>;
+
+static void block_sparse_sections(tq_handle_t handle) {
+  {  // block sparse matrix initialization
+    std::vector<float> a = {1, 2, 3, 4};
+    BST tensor(a.data(), uvec{1, 1}, uvec{2, 2}, uvec{4, 3});
+    CHECK(tensor.query({1, 1}) == 1);
+    CHECK(tensor.query({2, 2}) == 4);
+    CHECK(tensor.query({3, 2}) == 0);
+    tensor.modify({2, 2}, 0);
+    CHECK(tensor.query({2, 2}) == 0);
+    tensor.modify({3, 2}, 5);
+    CHECK(tensor.query({3, 2}) == 5);
+    tensor.append_block(a.data(), {0, 0}, {1, 1}, {1, 2});
+    CHECK(tensor.query({1, 1}) == 5);
+  }
+  {  // block sparse matrix transposition
+    std::vector<float> a = {1, 2, 3, 4};
+    BST tensor(a.data(), uvec{2, 0}, uvec{3, 1}, uvec{4, 3});
+    tensor = tensor.hard_transpose({1, 0});
+    CHECK(tensor.query({0, 3}) == 2);
+    CHECK(tensor.query({1, 2}) == 3);
+    const auto original = tensor.hard_transpose({1, 0});
+    CHECK(original.query({3, 0}) == 2);
+    CHECK(original.query({2, 1}) == 3);
+  }
+  {  // block sparse tensor initialization
+    BST tensor(uvec{2, 2, 3});
+    CHECK(tensor.query({1, 1, 0}) == 0);
+    std::vector<float> a = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12};
+    tensor.append_block(a.data(), {0, 0, 0}, {1, 1, 2}, {1, 2, 4});
+    CHECK(tensor.query({0, 1, 2}) == 11);
+  }
+  {  // block sparse tensor transposition (overlapping blocks)
+    std::vector<float> a = {1, 2, 3, 4, 5, 6, 0, 0, 0, 0, 0, 0};
+    std::vector<float> b = {7, 8, 9, 10, 11, 12};
+    BST tensor(uvec{2, 2, 3});
+    tensor.append_block(a.data(), {0, 0, 0}, {1, 1, 2}, {1, 2, 4});
+    tensor.append_block(b.data(), {1, 0, 0}, {1, 1, 2}, {1, 1, 2});
+    const auto t = tensor.hard_transpose({1, 0, 2});
+    CHECK(t.query({1, 0, 0}) == 3);
+    CHECK(t.query({1, 0, 1}) == 0);
+    CHECK(t.query({1, 1, 2}) == 12);
+  }
+  {  // scalar
+    BST tensor(uvec{});
+    CHECK(tensor.rank == 0);
+    CHECK(tensor.query({}) == 0);
+    tensor.modify({}, 1);
+    CHECK(tensor.query({}) == 1);
+  }
+  {  // vector contraction
+    const std::vector<float> v1{0, 1}, v2{2, 3, 4};
+    BST t(uvec{5});
+    t.append_block(v1.data(), {0}, {1}, {1});
+    t.append_block(v2.data(), {2}, {4}, {1});
+    CHECK(t.contract(handle, t, umat({{0, 0}})).to_number() == 30);
+  }
+  {  // matrix multiplication against the dense tensor
+    const std::vector<float> vec{0, 1, 0, 0, 2, 3, 0, 0, 0, 0, 4, 5, 0, 0, 6, 7};
+    const std::vector<float> b1{0, 1, 2, 3}, b2{4, 5, 6, 7};
+    const DT dense(vec.data(), {4, 4});
+    BST sparse(uvec{4, 4});
+    sparse.append_block(b1.data(), {0, 0}, {1, 1}, {1, 2});
+    sparse.append_block(b2.data(), {2, 2}, {3, 3}, {1, 2});
+    const auto d2 = dense.contract(handle, dense, umat({1, 0}));
+    const auto s2 = sparse.contract(handle, sparse, umat({{1, 0}}));
+    for (uword i = 0; i < 4; ++i)
+      for (uword j = 0; j < 4; ++j) CHECK(d2.query({i, j}) == s2.query({i, j}));
+  }
+  {  // matrix inner product with overlapping blocks
+    const std::vector<float> b1{0, 1, 2, 3}, b2{0, 7, 8, 9};
+    BST t(uvec{3, 3});
+    t.append_block(b1.data(), {0, 0}, {1, 1}, {1, 2});
+    t.append_block(b2.data(), {1, 1}, {2, 2}, {1, 2});
+    // accu(ref % ref.t()) for ref = {{0,1,0},{2,3,7},{0,8,9}}^T
+    const float ref[3][3] = {{0, 2, 0}, {1, 3, 8}, {0, 7, 9}};
+    float expect = 0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) expect += ref[i][j] * ref[j][i];
+    CHECK(t.contract(handle, t, umat({{1, 0}, {0, 1}})).to_number() == expect);
+  }
+  {  // tensor-vector contraction
+    const std::vector<float> d1{0, 1, 2, 3}, d2{4, 5, 6, 7}, v{1, 2};
+    BST t(uvec{2, 2, 2});
+    t.append_block(d1.data(), {0, 0, 0}, {1, 1, 0}, {1, 2, 4});
+    t.append_block(d2.data(), {0, 0, 1}, {1, 1, 1}, {1, 2, 4});
+    BST vt(uvec{2});
+    vt.append_block(v.data(), {0}, {1}, {1});
+    const auto c1 = t.contract(handle, vt, umat{{0, 0}});
+    CHECK(c1.query({0, 0}) == 2);
+    CHECK(c1.query({1, 0}) == 8);
+    CHECK(c1.query({0, 1}) == 14);
+    CHECK(c1.query({1, 1}) == 20);
+    const auto c2 = t.contract(handle, vt, umat{{1, 0}});
+    CHECK(c2.query({0, 0}) == 4);
+    CHECK(c2.query({1, 0}) == 7);
+    CHECK(c2.query({0, 1}) == 16);
+    CHECK(c2.query({1, 1}) == 19);
+    const auto c3 = t.contract(handle, vt, umat{{2, 0}});
+    CHECK(c3.query({1, 0}) == 11);
+  }
+  {  // tensor-matrix contraction
+    const std::vector<float> d1{0, 1, 2, 3}, d2{4, 5, 6, 7}, d3{8, 9, 10, 11}, m{1, 2, 3, 4};
+    BST t(uvec{2, 2, 3});
+    t.append_block(d3.data(), {0, 0, 2}, {1, 1, 2}, {1, 2, 4});
+    t.append_block(d2.data(), {0, 0, 1}, {1, 1, 1}, {1, 2, 4});
+    t.append_block(d1.data(), {0, 0, 0}, {1, 1, 0}, {1, 2, 4});
+    BST mt(uvec{2, 2});
+    mt.append_block(m.data(), {0, 0}, {1, 1}, {1, 2});
+    const auto c = t.contract(handle, mt, umat{{0, 1}, {1, 0}});
+    CHECK(c.query({0}) == 19);
+    CHECK(c.query({1}) == 59);
+    CHECK(c.query({2}) == 99);
+  }
+  {  // errors keep the reference's messages
+    BST t(uvec{5}), u(uvec{4});
+    bool thrown = false;
+    try {
+      (void)t.contract(handle, u, umat{{0, 0}});
+    } catch (const Error &e) {
+      thrown = std::string(e.what()) ==
+               "Error: The dimensions from two tensors to be contracted do not match";
+    }
+    CHECK(thrown);
+  }
+}
+
+static void dense_sections() {
+  {  // dense matrix initialization
+    DT t(uvec{4, 3});
+    CHECK(t.query({1, 1}) == 0);
+    std::vector<float> a = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12};
+    t.initialize(a.data());
+    CHECK(t.query({1, 2}) == 10);
+  }
+  {  // dense matrix transposition
+    DT t(uvec{4, 3});
+    std::vector<float> a = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12};
+    t.initialize(a.data());
+    CHECK(t.query({1, 2}) == 10);
+    t = t.hard_transpose({1, 0});
+    CHECK(t.query({1, 2}) == 7);
+    const auto back = t.hard_transpose({1, 0});
+    for (uword i = 0; i < 3; ++i)
+      for (uword j = 0; j < 4; ++j) CHECK(t.query({i, j}) == back.query({j, i}));
+  }
+  {  // dense tensor transposition
+    std::vector<float> a = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12};
+    DT t(a.data(), {2, 2, 3});
+    CHECK(t.query({0, 0, 2}) == 9);
+    CHECK(t.query({0, 1, 2}) == 11);
+    t = t.hard_transpose({0, 2, 1});
+    CHECK(t.query({0, 2, 0}) == 9);
+    CHECK(t.query({0, 2, 1}) == 11);
+    const auto t2 = t.hard_transpose({1, 0, 2});
+    CHECK(t2.query({2, 0, 0}) == 9);
+    CHECK(t2.query({2, 0, 1}) == 11);
+  }
+  {  // scalar
+    DT t(uvec{});
+    CHECK(t.rank == 0);
+    CHECK(t.query({}) == 0);
+    t.modify({}, 1);
+    CHECK(t.query({}) == 1);
+  }
+}
+
+int main() {
+  tq_handle_t handle = nullptr;
+  if (tq_create(&handle, 0) != TQ_SUCCESS) {
+    std::printf("FAILED: no device\n");
+    return 2;
+  }
+  try {
+    block_sparse_sections(handle);
+    dense_sections();
+  } catch (const std::exception &e) {
+    std::printf("FAILED with exception: %s\n", e.what());
+    ++g_fail;
+  }
+  tq_destroy(handle);
+  std::printf("%d checks, %d failed\n", g_checks, g_fail);
+  return g_fail ? 1 : 0;
+}

@@ -63,10 +63,24 @@ def build_oracle():
     return os.path.join(ROOT, "oracle", "liboracle.so")
 
 
+CPP_TEST = os.path.join(ROOT, "tests", "cpp", "test_dropin")
+
+
+def build_cpp_tests():
+    """The C++ drop-in test program (host code only: g++ against the
+    headers in torque_amd/include and the C-ABI library)."""
+    src = CPP_TEST + ".cpp"
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
+          "-I" + os.path.join(PKG, "include"), src, "-o", CPP_TEST,
+          "-L" + PKG, "-ltorque_hip", "-Wl,-rpath,$ORIGIN/../../torque_amd"])
+    return CPP_TEST
+
+
 def build_all():
     lib = build_hip()
     orc = build_oracle()
-    return lib, orc
+    cpp = build_cpp_tests()
+    return lib, orc, cpp
 
 
 if __name__ == "__main__":
