@@ -214,6 +214,27 @@ tq_status tq_plan_autotune(tq_plan_t plan, const void *a, const void *b,
                            void *c, int32_t reps, int32_t *best,
                            float *times_ms);
 
+/* ---- batches and graphs (no reference counterpart) ------------------- */
+
+/* Joins n plans into one batch plan: every pair of every plan goes into one
+ * cost-sorted work list, so tq_bs_contract_execute(batch, a, b, c) packs all
+ * operands with one launch per side and multiplies every pair with one
+ * streaming launch and one grouped-GEMM launch. Plan i reads A at element
+ * offset a_offset[i] of `a`, B at b_offset[i] of `b`, and writes its output
+ * blocks at c_offset[i] of `c`. The plans must share dtype, precision and
+ * small-K threshold and have tensor (rank > 0) results; they may be destroyed
+ * after the call. The batch supports execute, autotune, set_config,
+ * capture and destroy. */
+tq_status tq_batch_create(tq_handle_t handle, int32_t n, const tq_plan_t *plans,
+                          const int64_t *a_offset, const int64_t *b_offset,
+                          const int64_t *c_offset, tq_plan_t *batch);
+
+/* Captures tq_bs_contract_execute(plan, a, b, c) into a HIP graph owned by
+ * the plan (the pointers are baked in); tq_plan_graph_launch replays it on
+ * the handle's stream with no host work per replay. */
+tq_status tq_plan_capture(tq_plan_t plan, const void *a, const void *b, void *c);
+tq_status tq_plan_graph_launch(tq_plan_t plan);
+
 /* Number of grouped-GEMM tile configurations and a description of one. */
 int32_t tq_gemm_config_count(tq_precision precision);
 tq_status tq_gemm_config_describe(tq_precision precision, int32_t config,

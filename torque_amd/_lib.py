@@ -85,6 +85,10 @@ def _proto(lib):
         "tq_bs_contract_pack": (S, [V, V, V]),
         "tq_bs_contract_multiply": (S, [V, V, V, V]),
         "tq_plan_set_config": (S, [V, ctypes.c_int32]),
+        "tq_batch_create": (S, [H, ctypes.c_int32, ctypes.POINTER(V), c_i64_p, c_i64_p,
+                                c_i64_p, ctypes.POINTER(V)]),
+        "tq_plan_capture": (S, [V, V, V, V]),
+        "tq_plan_graph_launch": (S, [V]),
         "tq_plan_set_small_k": (S, [V, ctypes.c_int32]),
         "tq_plan_autotune": (S, [V, V, V, V, ctypes.c_int32, c_i32_p,
                                  ctypes.POINTER(ctypes.c_float)]),

@@ -114,10 +114,23 @@ struct tq_plan_s {
   // rank-0 reduction scratch
   DevBuf partial, red_tmp;
 
+  // captured execute (tq_plan_capture)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  hipStream_t capture_stream = nullptr;
+  bool is_batch = false;
+
+  ~tq_plan_s() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
+  }
+
   // host copies
   std::vector<i64> h_m, h_n, h_k;
   std::vector<i64> h_out_begin, h_out_end, h_c_off;
 };
 
 tq_status plan_build_tiles(tq_plan_t pl, int cfg);
+tq_status tq_sort_by_cost(tq_handle_t h, const i64 *cost, int64_t n, i64 *order);
 int tq_plan_pick_config(tq_plan_t pl);

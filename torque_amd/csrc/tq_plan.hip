@@ -563,6 +563,26 @@ bool valid_desc(const tq_bs_desc *d) {
 
 }  // namespace
 
+// Stable device counting sort of n items by cost bucket, largest cost first:
+// order receives item indices.
+tq_status tq_sort_by_cost(tq_handle_t h, const i64 *cost, int64_t n, i64 *order) {
+  if (n <= 0) return TQ_SUCCESS;
+  const int64_t n_chunks = ceil_div(n, kSortChunk);
+  DevBuf hist, stmp;
+  TQ_HIP(h, hist.alloc(sizeof(i64) * (kBuckets * n_chunks + 1)));
+  TQ_HIP(h, stmp.alloc(tq_scan_tmp_bytes(kBuckets * n_chunks + 1)));
+  hipLaunchKernelGGL(sort_hist_kernel, dim3((unsigned)n_chunks), dim3(256), 0, h->stream,
+                     cost, n, n_chunks, hist.as<i64>());
+  TQ_CHECK(tq_scan_i64(h, hist.as<i64>(), hist.as<i64>(), kBuckets * n_chunks, nullptr,
+                       stmp.p));
+  hipLaunchKernelGGL(sort_scatter_kernel, dim3((unsigned)n_chunks), dim3(256), 0, h->stream,
+                     cost, n, n_chunks, hist.as<i64>(), order);
+  TQ_HIP(h, hipGetLastError());
+  // the temporaries are released on return
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  return TQ_SUCCESS;
+}
+
 // ---- plan object ----------------------------------------------------------
 
 // Expands the work list into per-item pair indices (tile_pair / chunk_pair)
@@ -1018,19 +1038,8 @@ tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
                         split, up));
 
     // ---- 5. work list: stable device sort by cost, largest first ----
-    const int64_t n_chunks = ceil_div(np, kSortChunk);
-    DevBuf hist, stmp;
-    TQ_HIP(h, hist.alloc(sizeof(i64) * (kBuckets * n_chunks + 1)));
-    TQ_HIP(h, stmp.alloc(tq_scan_tmp_bytes(kBuckets * n_chunks + 1)));
     TQ_HIP(h, pl->order.alloc(sizeof(i64) * np));
-    hipLaunchKernelGGL(sort_hist_kernel, dim3((unsigned)n_chunks), dim3(256), 0,
-                       h->stream, cost.as<i64>(), np, n_chunks, hist.as<i64>());
-    TQ_CHECK(tq_scan_i64(h, hist.as<i64>(), hist.as<i64>(), kBuckets * n_chunks,
-                         nullptr, stmp.p));
-    hipLaunchKernelGGL(sort_scatter_kernel, dim3((unsigned)n_chunks), dim3(256), 0,
-                       h->stream, cost.as<i64>(), np, n_chunks, hist.as<i64>(),
-                       pl->order.as<i64>());
-    TQ_HIP(h, hipGetLastError());
+    TQ_CHECK(tq_sort_by_cost(h, cost.as<i64>(), np, pl->order.as<i64>()));
   }
   pl->a_layout = pl->pack_a.mode == kModeDirect ? 0 : 2;
   pl->b_layout = pl->pack_b.mode == kModeDirect ? 0 : 2;
@@ -1082,6 +1091,8 @@ tq_status tq_plan_get_info(tq_plan_t pl, tq_plan_info *info) {
 tq_status tq_plan_get_output(tq_plan_t pl, int64_t *dims, int64_t *begin,
                              int64_t *end, int64_t *strides, int64_t *offsets) {
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  if (pl->is_batch)
+    return tq_fail(pl->h, TQ_ERROR_UNSUPPORTED, "batch: ask the member plans");
   const int ro = pl->ro;
   const int64_t np = pl->n_pairs;
   if (dims) std::copy(pl->out_dims.begin(), pl->out_dims.end(), dims);
@@ -1107,6 +1118,7 @@ tq_status tq_plan_get_pairs(tq_plan_t pl, int64_t *a_block, int64_t *b_block,
                             int64_t *n, int64_t *k) {
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
   tq_handle_t h = pl->h;
+  if (pl->is_batch) return tq_fail(h, TQ_ERROR_UNSUPPORTED, "batch: ask the member plans");
   const int64_t np = pl->n_pairs;
   if (np == 0) return TQ_SUCCESS;
   std::vector<int32_t> pa(np), pb(np);

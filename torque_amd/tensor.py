@@ -406,6 +406,17 @@ class ContractionPlan:
         res._meta_changed()
         return res
 
+    def capture(self, a_data, b_data, out):
+        """Capture execute(a, b, out) into a HIP graph (tq_plan_capture)."""
+        self.h.bind_stream()
+        self.h.check(lib().tq_plan_capture(
+            self.ptr, ctypes.c_void_p(a_data.data_ptr()), ctypes.c_void_p(b_data.data_ptr()),
+            ctypes.c_void_p(out.data_ptr())))
+
+    def graph_launch(self):
+        self.h.bind_stream()
+        self.h.check(lib().tq_plan_graph_launch(self.ptr))
+
     def destroy(self):
         if getattr(self, "ptr", None):
             lib().tq_plan_destroy(self.ptr)
@@ -416,6 +427,32 @@ class ContractionPlan:
             self.destroy()
         except Exception:
             pass
+
+
+class BatchPlan(ContractionPlan):
+    """Several contraction plans joined into one work list
+    (tq_batch_create). Plan i reads A at a_offsets[i] of the shared A buffer,
+    B at b_offsets[i], and writes C at c_offsets[i]."""
+
+    def __init__(self, handle, plans, a_offsets, b_offsets, c_offsets):
+        self.h = handle
+        self.dtype = plans[0].dtype
+        self.precision = plans[0].precision
+        arr = (ctypes.c_void_p * len(plans))(*[p.ptr for p in plans])
+        ao = np.ascontiguousarray(a_offsets, np.int64)
+        bo = np.ascontiguousarray(b_offsets, np.int64)
+        co = np.ascontiguousarray(c_offsets, np.int64)
+        ptr = ctypes.c_void_p()
+        handle.check(lib().tq_batch_create(handle.ptr, len(plans), arr, _i64p(ao), _i64p(bo),
+                                           _i64p(co), ctypes.byref(ptr)))
+        self.ptr = ptr
+        info = PlanInfo()
+        handle.check(lib().tq_plan_get_info(ptr, ctypes.byref(info)))
+        self.n_pairs = info.n_pairs
+        self.out_n_elem = info.out_n_elem
+        self.flops = info.flops
+        self.bytes = info.bytes
+        self.out_rank = info.out_rank
 
 
 def gemm_configs(precision="native"):
