@@ -284,7 +284,9 @@ class BlockSparseTensor:
         if plan is None:
             plan = ContractionPlan(h, self, tensor, legs, precision)
             if len(h.plans) > 64:
-                h.plans.pop(next(iter(h.plans))).destroy()
+                # drop the cache's reference only: a caller may still hold the
+                # plan, which frees itself when the last reference goes
+                h.plans.pop(next(iter(h.plans)))
             h.plans[key] = plan
         return plan
 
