@@ -106,9 +106,10 @@ def bench_batched(args):
     batch.capture(a_all, b_all, c)
     n_tiles, n_segs = batch.work()
     print("# batched: %d contractions, %d pairs, %.1f GFLOP, out %.1f M elem; gen %.1f s, "
-          "plans %.1f s; config %d; per config ms %s"
+          "plans %.1f s; configs (small, large tiles) %s; per config ms %s / %s"
           % (args.n_batch, batch.n_pairs, batch.flops / 1e9, batch.out_n_elem / 1e6, gen_s,
-             plan_s, best, ["%.2f" % x for x in times]), flush=True)
+             plan_s, best, ["%.2f" % x for x in times[0]], ["%.2f" % x for x in times[1]]),
+          flush=True)
 
     def time_graph(bp):
         for _ in range(args.warmup):
@@ -236,8 +237,9 @@ def main():
     best1, _ = p1.autotune(gA.data, gAc.data, aa)
     best2, times2 = p2.autotune(gE.data, aa, out)
     tune_s = time.perf_counter() - t0
-    print("# autotune %.2f s: A.A* config %d, E.a config %d; E.a ms per config %s"
-          % (tune_s, best1, best2, ["%.3f" % x for x in times2]), flush=True)
+    print("# autotune %.2f s: A.A* configs %s, E.a configs %s; E.a ms per config %s / %s"
+          % (tune_s, best1, best2, ["%.3f" % x for x in times2[0]],
+             ["%.3f" % x for x in times2[1]]), flush=True)
     print("# workload D=%d chi=%d: A.A* %d pairs, E.a %d pairs, %.3f GFLOP, out %.1f M elem, "
           "plans %.3f s" % (args.D, args.chi, p1.n_pairs, p2.n_pairs,
                             (p1.flops + p2.flops) / 1e9, p2.out_n_elem / 1e6, plan_s),

@@ -192,9 +192,15 @@ tq_status tq_bs_contract_execute(tq_plan_t plan, const void *a, const void *b,
 tq_status tq_bs_contract_pack(tq_plan_t plan, const void *a, const void *b);
 tq_status tq_bs_contract_multiply(tq_plan_t plan, const void *a, const void *b,
                                   void *c);
-/* Select the grouped-GEMM tile configuration by index (see
- * tq_gemm_config_count); -1 restores the plan's automatic choice. */
+/* MFMA pairs (K above the small-K threshold) fall in two tile classes, each
+ * launched with its own tile configuration: class 1 holds the pairs with
+ * M >= 128 and N >= 128, class 0 the rest. set_config selects the
+ * configuration of both classes by index (see tq_gemm_config_count),
+ * set_class_config of one; -1 restores the plan's automatic choice.
+ * get_config writes the two classes' current configurations. */
 tq_status tq_plan_set_config(tq_plan_t plan, int32_t config);
+tq_status tq_plan_set_class_config(tq_plan_t plan, int32_t cls, int32_t config);
+tq_status tq_plan_get_config(tq_plan_t plan, int32_t *config /* [2] */);
 tq_status tq_plan_destroy(tq_plan_t plan);
 
 /* Pairs whose contracted extent K is at most max_k (0..8) are written by the
@@ -204,11 +210,13 @@ tq_status tq_plan_destroy(tq_plan_t plan);
 tq_status tq_plan_set_small_k(tq_plan_t plan, int32_t max_k);
 
 /* Autotuner (no reference counterpart: the reference measures a cuTT plan
- * per call, src/gpu/block_sparse.cuh:836-863). Packs A and B once, then times
- * `reps` grouped-GEMM launches of every tile configuration with HIP events on
- * the handle's stream and keeps the fastest for this plan. times_ms (host,
- * tq_gemm_config_count(precision) entries, may be NULL) receives the mean
- * time of each configuration; *best (may be NULL) the chosen index. C is
+ * per call, src/gpu/block_sparse.cuh:836-863). Packs A and B once, then, for
+ * each tile class, times `reps` grouped-GEMM launches of every tile
+ * configuration with HIP events on the handle's stream and keeps the fastest
+ * for that class. times_ms (host, 2 * tq_gemm_config_count(precision)
+ * entries, class 0 first, may be NULL) receives the mean time of each
+ * (class, configuration), 0 for a class without tiles; best (host, 2
+ * entries, may be NULL) the chosen configuration of each class. C is
  * written by the timed launches. */
 tq_status tq_plan_autotune(tq_plan_t plan, const void *a, const void *b,
                            void *c, int32_t reps, int32_t *best,
@@ -219,7 +227,7 @@ tq_status tq_plan_autotune(tq_plan_t plan, const void *a, const void *b,
 /* Joins n plans into one batch plan: every pair of every plan goes into one
  * cost-sorted work list, so tq_bs_contract_execute(batch, a, b, c) packs all
  * operands with one launch per side and multiplies every pair with one
- * streaming launch and one grouped-GEMM launch. Plan i reads A at element
+ * streaming launch and one grouped-GEMM launch per tile class. Plan i reads A at element
  * offset a_offset[i] of `a`, B at b_offset[i] of `b`, and writes its output
  * blocks at c_offset[i] of `c`. The plans must share dtype, precision and
  * small-K threshold and have tensor (rank > 0) results; they may be destroyed

@@ -143,6 +143,11 @@ def test_every_gemm_config_agrees(cuda):
     for cfg in range(len(gemm_configs())):
         _, got = gpu_contract(A, da, B, db, legs, cuda, cfg=cfg)
         assert rel_fro(got, ref_c) < FP64_TOL, cfg
+        # the FP32 tier and FP32 data run the same configurations
+        _, got = gpu_contract(A, da, B, db, legs, cuda, cfg=cfg, precision="fp32")
+        assert rel_fro(got, ref_c) < 1e-5, cfg
+        _, got = gpu_contract(A, da, B, db, legs, cuda, cfg=cfg, dtype=torch.float32)
+        assert rel_fro(got, ref_c) < 1e-5, cfg
 
 
 @pytest.mark.parametrize("small_k", [0, 4, 8])
@@ -192,6 +197,39 @@ def test_batched_pairs_large_k(cuda):
         plan, got = gpu_contract(A, da, B, db, legs, cuda)
         check_maps(plan, ref)
         assert rel_fro(got, ref_c) < FP64_TOL
+
+
+def test_tile_classes(cuda):
+    # pairs on both MFMA tile classes (min(M, N) >= 128 and below), tuned and
+    # then forced to unrelated configurations per class
+    a, b, legs = W.batched(1, seed=9, dim_lo=60, dim_hi=96)[0]
+    A, da = from_workload(a)
+    B, db = from_workload(b)
+    ref_c, _ = O.contract(A, da, B, db, legs)
+    ga = torch_tensor(A, da, cuda, torch.float64)
+    gb = torch_tensor(B, db, cuda, torch.float64)
+    plan = ga.plan_contract(gb, legs)
+    p = plan.pairs()
+    mfma = p["k"] > 8
+    large = mfma & (p["m"] >= 128) & (p["n"] >= 128)
+    assert large.any() and (mfma & ~large).any()
+    out = torch.empty(plan.out_n_elem, dtype=torch.float64, device=cuda)
+    best, times = plan.autotune(ga.data, gb.data, out, reps=1)
+    assert plan.config() == best
+    assert all(t > 0 for t in times[0]) and all(t > 0 for t in times[1])
+    assert best[0] == int(np.argmin(times[0])) and best[1] == int(np.argmin(times[1]))
+    nc = len(gemm_configs())
+    for c0, c1 in [(6, 3), (3, 6), (0, nc - 1), (nc - 1, 0)]:
+        plan.set_class_config(0, c0)
+        plan.set_class_config(1, c1)
+        assert plan.config() == [c0, c1]
+        out.zero_()
+        plan.execute(ga.data, gb.data, out)
+        torch.cuda.synchronize()
+        assert rel_fro(out.cpu().numpy(), ref_c) < FP64_TOL, (c0, c1)
+    plan.set_config(-1)
+    with pytest.raises(TorqueError):
+        plan.set_class_config(2, 0)
 
 
 def test_rank0_outer_and_empty(cuda):

@@ -26,6 +26,13 @@ HIP_FLAGS = [
 ]
 
 
+# Per-source additions. The wave-tile GEMM keeps its MFMA accumulators in
+# VGPRs (AGPR forms make the compiler copy them across the k loop).
+FILE_FLAGS = {
+    "tq_wave_gemm.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+}
+
+
 def _run(cmd):
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
@@ -46,7 +53,7 @@ def build_hip(jobs=None):
         if (os.path.exists(o) and os.path.getmtime(o) > os.path.getmtime(s)
                 and os.path.getmtime(o) > newest_header):
             return o
-        _run([HIPCC] + HIP_FLAGS + ["-c", s, "-o", o])
+        _run([HIPCC] + HIP_FLAGS + FILE_FLAGS.get(os.path.basename(s), []) + ["-c", s, "-o", o])
         return o
 
     jobs = jobs or min(8, os.cpu_count() or 4)

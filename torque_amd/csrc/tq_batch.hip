@@ -152,9 +152,8 @@ tq_status tq_batch_create(tq_handle_t h, int32_t n, const tq_plan_t *plans,
   TQ_CHECK(tq_sort_by_cost(h, dcost.as<i64>(), pl->n_pairs, pl->order.as<i64>()));
   pl->a_layout = pl->pack_a.mode == kModeDirect ? 0 : 2;
   pl->b_layout = pl->pack_b.mode == kModeDirect ? 0 : 2;
-  pl->auto_cfg = tq_plan_pick_config(pl.get());
-  pl->cfg = -1;
-  if (pl->n_pairs > 0) TQ_CHECK(plan_build_tiles(pl.get(), pl->auto_cfg));
+  for (int c = 0; c < 2; ++c) pl->cls[c].auto_cfg = tq_plan_pick_config(pl.get(), c);
+  if (pl->n_pairs > 0) TQ_CHECK(plan_build_tiles(pl.get()));
   TQ_HIP(h, hipStreamSynchronize(h->stream));
   *batch = pl.release();
   return TQ_SUCCESS;
@@ -167,7 +166,7 @@ tq_status tq_plan_capture(tq_plan_t pl, const void *a, const void *b, void *c) {
     return tq_fail(h, TQ_ERROR_UNSUPPORTED,
                    "capture: scalar and empty results are not captured");
   // everything the execute needs is allocated before capture starts
-  TQ_CHECK(plan_build_tiles(pl, pl->cfg >= 0 ? pl->cfg : pl->auto_cfg));
+  TQ_CHECK(plan_build_tiles(pl));
   TQ_HIP(h, hipStreamSynchronize(h->stream));
   if (pl->graph_exec) {
     (void)hipGraphExecDestroy(pl->graph_exec);

@@ -150,8 +150,11 @@ tq_status tq_launch_small_k(tq_handle_t h, tq_dtype dt, tq_precision prec,
                             const void *a_packed, const void *b, const void *b_packed,
                             void *c);
 
+// Tile shape of a configuration; wave_tile = 1 when one wavefront (not a
+// four-wave workgroup) computes a tile.
 struct GemmConfig {
   int bm, bn, bk;
+  int wave_tile;
 };
 
 int tq_gemm_n_configs(tq_precision prec);

@@ -1,8 +1,9 @@
 // tq_contract.hip -- execution of a contraction plan, the rank-0 (network
 // scalar) branch, tile-configuration choice, and the dense contraction.
 //
-// Execute = [pack A] -> [pack B] -> one grouped GEMM launch (or, for a
-// rank-0 output, one pair-dot launch and a deterministic FP64 sum). This
+// Execute = [pack A] -> [pack B] -> [streaming small-K launch] -> one
+// grouped-GEMM launch per MFMA tile class (or, for a rank-0 output, one
+// pair-dot launch and a deterministic FP64 sum). This
 // replaces the per-A-block sequence of src/gpu/block_sparse.cuh:693-1011
 // (reshape, cuTT, strided-batched GEMM, reshape, D2D copy, dot + add<<<1,1>>>).
 #include <algorithm>
@@ -45,11 +46,11 @@ __global__ void store_scalar_kernel(const double *__restrict__ v, T *__restrict_
 
 }  // namespace
 
-// Analytic choice of the tile configuration: models each tile as MFMA work
-// on the padded tile plus its operand and output traffic, and picks the
-// configuration with the least modelled time. The autotuner measures the
-// candidates instead and overrides this through tq_plan_set_config.
-int tq_plan_pick_config(tq_plan_t pl) {
+// Analytic choice of the tile configuration for one MFMA tile class: models
+// each tile as MFMA work on the padded tile plus its operand and output
+// traffic, and picks the configuration with the least modelled time. The
+// autotuner measures the candidates instead and overrides this.
+int tq_plan_pick_config(tq_plan_t pl, int cls) {
   const int nc = tq_gemm_n_configs(pl->prec);
   const double es = pl->dtype == TQ_F64 ? 8.0 : 4.0;
   const double peak = pl->dtype == TQ_F32 || pl->prec == TQ_PREC_F32 ? 157e12
@@ -63,7 +64,7 @@ int tq_plan_pick_config(tq_plan_t pl) {
     const GemmConfig g = tq_gemm_config(pl->prec, c);
     double flop = 0, bytes = 0, tiles = 0;
     for (int64_t p = 0; p < pl->n_pairs; ++p) {
-      if (pl->h_k[p] <= pl->stream_max_k) continue;
+      if (pair_class(*pl, pl->h_m[p], pl->h_n[p], pl->h_k[p]) != cls) continue;
       const double tm = std::ceil((double)pl->h_m[p] / g.bm);
       const double tn = std::ceil((double)pl->h_n[p] / g.bn);
       const double kp = std::ceil((double)pl->h_k[p] / 4.0) * 4.0;
@@ -72,8 +73,8 @@ int tq_plan_pick_config(tq_plan_t pl) {
       bytes += tm * tn * (g.bm + g.bn) * kp * es * 0.5 +
                (double)pl->h_m[p] * pl->h_n[p] * es;
     }
-    // tail effect: the last wave of tiles on 4 workgroups per CU
-    const double waves = std::ceil(tiles / (n_cu * 4));
+    // tail effect: the last round of tiles on 4 workgroups per CU
+    const double waves = std::ceil(tiles / (n_cu * (g.wave_tile ? 16 : 4)));
     const double t = std::max(flop / peak, bytes / bw) + waves * 2e-6;
     if (t < best_t) {
       best_t = t;
@@ -83,13 +84,48 @@ int tq_plan_pick_config(tq_plan_t pl) {
   return best;
 }
 
+namespace {
+
+// One grouped-GEMM launch over the tiles of one MFMA tile class.
+tq_status launch_class(tq_plan_t pl, int cls, const void *a, const void *b, void *c) {
+  tq_handle_t h = pl->h;
+  const TileClass &tc = pl->cls[cls];
+  if (tc.n_tiles == 0) return TQ_SUCCESS;
+  if (pl->prec == TQ_PREC_BF16X3)
+    return tq_launch_bf16x3_gemm(h, tc.tiles_cfg, pl->desc.as<GemmDesc>(),
+                                 tc.tile_pair.as<int32_t>(), tc.tile_first.as<int32_t>(),
+                                 tc.n_tiles, pl->pack_a.panel.as<uint16_t>(),
+                                 pl->pack_b.panel.as<uint16_t>(), pl->pack_a.panel_elems,
+                                 pl->pack_b.panel_elems, reinterpret_cast<double *>(c));
+  return tq_launch_grouped_gemm(h, pl->dtype, pl->prec, tc.tiles_cfg, pl->desc.as<GemmDesc>(),
+                                tc.tile_pair.as<int32_t>(), tc.tile_first.as<int32_t>(),
+                                tc.n_tiles, a, pl->pack_a.panel.p, b, pl->pack_b.panel.p, c);
+}
+
+}  // namespace
+
 extern "C" {
 
 tq_status tq_plan_set_config(tq_plan_t pl, int32_t config) {
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
   if (config < -1 || config >= tq_gemm_n_configs(pl->prec))
     return TQ_ERROR_INVALID_VALUE;
-  pl->cfg = config;
+  pl->cls[0].cfg = pl->cls[1].cfg = config;
+  return TQ_SUCCESS;
+}
+
+tq_status tq_plan_set_class_config(tq_plan_t pl, int32_t cls, int32_t config) {
+  if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  if (cls < 0 || cls > 1 || config < -1 || config >= tq_gemm_n_configs(pl->prec))
+    return TQ_ERROR_INVALID_VALUE;
+  pl->cls[cls].cfg = config;
+  return TQ_SUCCESS;
+}
+
+tq_status tq_plan_get_config(tq_plan_t pl, int32_t *config) {
+  if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  if (!config) return TQ_ERROR_INVALID_VALUE;
+  for (int c = 0; c < 2; ++c) config[c] = pl->cls[c].current();
   return TQ_SUCCESS;
 }
 
@@ -128,27 +164,15 @@ tq_status tq_bs_contract_multiply(tq_plan_t pl, const void *a, const void *b,
   }
   if (!a || !b) return TQ_ERROR_INVALID_VALUE;
   if (pl->ro > 0) {
-    const int cfg = pl->cfg >= 0 ? pl->cfg : pl->auto_cfg;
-    TQ_CHECK(plan_build_tiles(pl, cfg));
+    TQ_CHECK(plan_build_tiles(pl));
     if (pl->n_chunks > 0)
       TQ_CHECK(tq_launch_small_k(h, pl->dtype, pl->prec, pl->desc.as<GemmDesc>(),
                                  pl->chunk_pair.as<int32_t>(),
                                  pl->chunk_first.as<int32_t>(), pl->n_chunks, a,
                                  pl->pack_a.panel.p, b, pl->pack_b.panel.p, c));
-    if (pl->n_tiles == 0) return TQ_SUCCESS;
-    if (pl->prec == TQ_PREC_BF16X3)
-      return tq_launch_bf16x3_gemm(h, cfg, pl->desc.as<GemmDesc>(),
-                                   pl->tile_pair.as<int32_t>(),
-                                   pl->tile_first.as<int32_t>(), pl->n_tiles,
-                                   pl->pack_a.panel.as<uint16_t>(),
-                                   pl->pack_b.panel.as<uint16_t>(),
-                                   pl->pack_a.panel_elems, pl->pack_b.panel_elems,
-                                   reinterpret_cast<double *>(c));
-    return tq_launch_grouped_gemm(h, pl->dtype, pl->prec, cfg,
-                                  pl->desc.as<GemmDesc>(),
-                                  pl->tile_pair.as<int32_t>(),
-                                  pl->tile_first.as<int32_t>(), pl->n_tiles, a,
-                                  pl->pack_a.panel.p, b, pl->pack_b.panel.p, c);
+    // large tiles first: the small-tile launch then fills the tail
+    TQ_CHECK(launch_class(pl, 1, a, b, c));
+    return launch_class(pl, 0, a, b, c);
   }
   // rank-0 output: the network scalar, reduced in FP64 on the GPU
   const int64_t np = pl->n_pairs;
@@ -187,8 +211,10 @@ tq_status tq_plan_set_small_k(tq_plan_t pl, int32_t max_k) {
     return tq_fail(pl->h, TQ_ERROR_UNSUPPORTED, "split-BF16 tier runs every pair on MFMA");
   pl->stream_max_k = max_k;
   pl->chunks_ready = false;
-  pl->tiles_ready = false;
-  pl->auto_cfg = tq_plan_pick_config(pl);
+  for (int c = 0; c < 2; ++c) {
+    pl->cls[c].ready = false;
+    pl->cls[c].auto_cfg = tq_plan_pick_config(pl, c);
+  }
   return TQ_SUCCESS;
 }
 
@@ -198,43 +224,51 @@ tq_status tq_plan_autotune(tq_plan_t pl, const void *a, const void *b, void *c,
   tq_handle_t h = pl->h;
   if (reps < 1) reps = 1;
   const int nc = tq_gemm_n_configs(pl->prec);
+  if (times_ms)
+    for (int q = 0; q < 2 * nc; ++q) times_ms[q] = 0.f;
   if (pl->n_pairs == 0 || pl->ro == 0) {
-    if (best) *best = pl->cfg >= 0 ? pl->cfg : pl->auto_cfg;
-    if (times_ms)
-      for (int q = 0; q < nc; ++q) times_ms[q] = 0.f;
+    if (best)
+      for (int k = 0; k < 2; ++k) best[k] = pl->cls[k].current();
     return TQ_SUCCESS;
   }
   TQ_CHECK(tq_bs_contract_pack(pl, a, b));
+  TQ_CHECK(plan_build_tiles(pl));
   hipEvent_t e0, e1;
   TQ_HIP(h, hipEventCreate(&e0));
   TQ_HIP(h, hipEventCreate(&e1));
-  int best_cfg = 0;
-  float best_t = 3.4e38f;
   tq_status st = TQ_SUCCESS;
-  for (int q = 0; q < nc && st == TQ_SUCCESS; ++q) {
-    pl->cfg = q;
-    st = tq_bs_contract_multiply(pl, a, b, c);  // builds the tile list, warms up
-    if (st != TQ_SUCCESS) break;
-    (void)hipEventRecord(e0, h->stream);
-    for (int r = 0; r < reps && st == TQ_SUCCESS; ++r)
-      st = tq_bs_contract_multiply(pl, a, b, c);
-    (void)hipEventRecord(e1, h->stream);
-    (void)hipEventSynchronize(e1);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    ms /= reps;
-    if (times_ms) times_ms[q] = ms;
-    if (ms < best_t) {
-      best_t = ms;
-      best_cfg = q;
+  // each class is timed alone: its launches do not depend on the other's
+  for (int k = 0; k < 2 && st == TQ_SUCCESS; ++k) {
+    TileClass &tc = pl->cls[k];
+    if (tc.n_tiles == 0) continue;
+    int best_cfg = tc.current();
+    float best_t = 3.4e38f;
+    for (int q = 0; q < nc && st == TQ_SUCCESS; ++q) {
+      tc.cfg = q;
+      st = plan_build_tiles(pl);
+      if (st == TQ_SUCCESS) st = launch_class(pl, k, a, b, c);  // warm-up
+      if (st != TQ_SUCCESS) break;
+      (void)hipEventRecord(e0, h->stream);
+      for (int r = 0; r < reps && st == TQ_SUCCESS; ++r) st = launch_class(pl, k, a, b, c);
+      (void)hipEventRecord(e1, h->stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      ms /= reps;
+      if (times_ms) times_ms[k * nc + q] = ms;
+      if (ms < best_t) {
+        best_t = ms;
+        best_cfg = q;
+      }
     }
+    tc.cfg = best_cfg;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (st != TQ_SUCCESS) return st;
-  pl->cfg = best_cfg;
-  if (best) *best = best_cfg;
-  return plan_build_tiles(pl, best_cfg);
+  if (best)
+    for (int k = 0; k < 2; ++k) best[k] = pl->cls[k].current();
+  return plan_build_tiles(pl);
 }
 
 tq_status tq_bs_contract_execute(tq_plan_t pl, const void *a, const void *b,

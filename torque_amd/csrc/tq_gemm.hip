@@ -14,6 +14,8 @@
 // persistent and walk the tile list with an XCD-aware stride so that
 // neighbouring tiles (sharing operand panels) land on the same L2.
 //
+// The wave-tile variant without LDS is in tq_wave_gemm.hip.
+//
 // Arithmetic: FP64 on v_mfma_f64_16x16x4_f64; FP32 on v_mfma_f32_16x16x4_f32
 // (for FP32 data, or FP64 data at the FP32 tier); split-BF16 (three BF16
 // pieces per operand, six piece products on v_mfma_f32_16x16x32_bf16 with
@@ -277,16 +279,30 @@ tq_status dispatch(tq_handle_t h, int cfg, const GemmDesc *descs,
 
 int tq_bf16x3_n_configs();
 GemmConfig tq_bf16x3_config(int cfg);
+int tq_wave_n_configs();
+GemmConfig tq_wave_config(int cfg);
+tq_status tq_launch_wave_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec, int cfg,
+                              const GemmDesc *descs, const int32_t *tile_pair,
+                              const int32_t *tile_first, int64_t n_tiles, const void *a,
+                              const void *a_packed, const void *b, const void *b_packed,
+                              void *c);
 
+// FP64 / FP32 configurations: the workgroup-tile kernel's (0 .. kNCfg-1),
+// then the wave-tile kernel's (tq_wave_gemm.hip).
 int tq_gemm_n_configs(tq_precision prec) {
   if (prec == TQ_PREC_BF16X3) return tq_bf16x3_n_configs();
-  return kNCfg;
+  return kNCfg + tq_wave_n_configs();
 }
 
 GemmConfig tq_gemm_config(tq_precision prec, int cfg) {
   if (prec == TQ_PREC_BF16X3) return tq_bf16x3_config(cfg);
-  if (cfg < 0 || cfg >= kNCfg) cfg = 0;
-  return GemmConfig{kCfgs[cfg].bm, kCfgs[cfg].bn, kCfgs[cfg].bk};
+  if (cfg >= kNCfg) {
+    GemmConfig g = tq_wave_config(cfg - kNCfg);
+    g.wave_tile = 1;
+    return g;
+  }
+  if (cfg < 0) cfg = 0;
+  return GemmConfig{kCfgs[cfg].bm, kCfgs[cfg].bn, kCfgs[cfg].bk, 0};
 }
 
 tq_status tq_launch_grouped_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec,
@@ -298,6 +314,9 @@ tq_status tq_launch_grouped_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec,
   if (n_tiles <= 0) return TQ_SUCCESS;
   if (prec == TQ_PREC_BF16X3)
     return tq_fail(h, TQ_ERROR_INVALID_VALUE, "split-BF16 tier runs on split panels");
+  if (cfg >= kNCfg)
+    return tq_launch_wave_gemm(h, dt, prec, cfg - kNCfg, descs, tile_pair, tile_first, n_tiles,
+                               a, a_packed, b, b_packed, c);
   if (dt == TQ_F32)
     return dispatch<float, float, float>(h, cfg, descs, tile_pair, tile_first,
                                          n_tiles, a, a_packed, b, b_packed, c);

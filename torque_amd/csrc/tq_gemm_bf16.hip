@@ -167,7 +167,7 @@ int tq_bf16x3_n_configs() { return kNCfg; }
 
 GemmConfig tq_bf16x3_config(int cfg) {
   if (cfg < 0 || cfg >= kNCfg) cfg = 0;
-  return GemmConfig{kCfgs[cfg].bm, kCfgs[cfg].bn, kBK};
+  return GemmConfig{kCfgs[cfg].bm, kCfgs[cfg].bn, kBK, 0};
 }
 
 tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, const GemmDesc *descs,

@@ -73,6 +73,20 @@ struct PackSide {
   int64_t panel_elems = 0;  // elements per plane
 };
 
+// Smallest min(M, N) of a pair in the large MFMA tile class.
+constexpr int kLargeMin = 128;
+
+// One MFMA tile class of a plan: its configuration and tile list.
+struct TileClass {
+  DevBuf tile_pair, tile_first;
+  int64_t n_tiles = 0;
+  int tiles_cfg = -1;
+  bool ready = false;
+  int cfg = -1;  // user-selected configuration (-1 = automatic)
+  int auto_cfg = 0;
+  int current() const { return cfg >= 0 ? cfg : auto_cfg; }
+};
+
 struct tq_plan_s {
   tq_handle_t h = nullptr;
   tq_dtype dtype = TQ_F64;
@@ -98,18 +112,16 @@ struct tq_plan_s {
   DevBuf order;      // int64 work list (pair ids, largest cost first)
   PackSide pack_a, pack_b;
 
-  // grouped-GEMM tile list for the current configuration (pairs with
-  // K > stream_max_k) and the streaming chunk list (K <= stream_max_k)
-  DevBuf tile_pair, tile_first;
-  int64_t n_tiles = 0;
+  // Pairs with K <= stream_max_k go to the streaming small-K kernel (chunk
+  // list); the others to MFMA tiles in two classes, each with its own tile
+  // configuration: class 1 when min(M, N) >= large_min, class 0 otherwise.
   int stream_max_k = 0;
   DevBuf chunk_pair, chunk_first;
   int64_t n_chunks = 0;
   bool chunks_ready = false;
-  int tiles_cfg = -1;
-  bool tiles_ready = false;
-  int cfg = -1;       // user-selected configuration (-1 = automatic)
-  int auto_cfg = 0;
+  int large_min = kLargeMin;
+  TileClass cls[2];
+  int64_t n_tiles = 0;  // both classes
 
   // rank-0 reduction scratch
   DevBuf partial, red_tmp;
@@ -131,6 +143,14 @@ struct tq_plan_s {
   std::vector<i64> h_out_begin, h_out_end, h_c_off;
 };
 
-tq_status plan_build_tiles(tq_plan_t pl, int cfg);
+// Builds the streaming segment list and both tile classes' lists for their
+// current configurations (no work when already built).
+tq_status plan_build_tiles(tq_plan_t pl);
+// Analytic configuration choice for one tile class.
+int tq_plan_pick_config(tq_plan_t pl, int cls);
+// 0, 1: MFMA tile class of a pair; -1: streaming.
+inline int pair_class(const tq_plan_s &pl, int64_t m, int64_t n, int64_t k) {
+  if (k <= pl.stream_max_k) return -1;
+  return (m >= pl.large_min && n >= pl.large_min) ? 1 : 0;
+}
 tq_status tq_sort_by_cost(tq_handle_t h, const i64 *cost, int64_t n, i64 *order);
-int tq_plan_pick_config(tq_plan_t pl);

@@ -502,20 +502,24 @@ __global__ __launch_bounds__(256) void sort_scatter_kernel(
   }
 }
 
-// Work items per position of the work list: MFMA tiles of pairs with
-// K > max_k (chunk = 0), or streaming row segments of pairs with K <= max_k
-// (chunk > 0, the segment height in rows).
+// Work items per position of the work list: streaming row segments of
+// pairs with K <= max_k (cls = -1; bm = segment height), or MFMA tiles of the
+// pairs in tile class cls (0: min(M, N) < large_min, 1: otherwise).
 __global__ void tiles_per_position_kernel(int64_t n, const i64 *__restrict__ order,
                                           const GemmDesc *__restrict__ desc,
                                           int32_t bm, int32_t bn, int32_t max_k,
-                                          int32_t chunk, i64 *__restrict__ tiles) {
+                                          int32_t large_min, int32_t cls,
+                                          i64 *__restrict__ tiles) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const GemmDesc d = desc[order[q]];
-  if (chunk > 0)
-    tiles[q] = d.k <= max_k ? ceil_div(d.m, chunk) : 0;
+  const int c = d.k <= max_k ? -1 : (d.m >= large_min && d.n >= large_min) ? 1 : 0;
+  if (c != cls)
+    tiles[q] = 0;
+  else if (cls < 0)
+    tiles[q] = ceil_div(d.m, bm);
   else
-    tiles[q] = d.k > max_k ? ceil_div(d.m, bm) * ceil_div(d.n, bn) : 0;
+    tiles[q] = ceil_div(d.m, bm) * ceil_div(d.n, bn);
 }
 
 __global__ void expand_tiles_kernel(int64_t n, const i64 *__restrict__ order,
@@ -587,7 +591,7 @@ tq_status tq_sort_by_cost(tq_handle_t h, const i64 *cost, int64_t n, i64 *order)
 
 // Expands the work list into per-item pair indices (tile_pair / chunk_pair)
 // and each pair's first item; `total` is known on the host from the shapes.
-static tq_status expand_work(tq_plan_t pl, int32_t bm, int32_t bn, int32_t chunk,
+static tq_status expand_work(tq_plan_t pl, int32_t bm, int32_t bn, int32_t cls,
                              int64_t total, DevBuf &item_pair, DevBuf &item_first) {
   tq_handle_t h = pl->h;
   const int64_t np = pl->n_pairs;
@@ -598,7 +602,8 @@ static tq_status expand_work(tq_plan_t pl, int32_t bm, int32_t bn, int32_t chunk
   TQ_HIP(h, item_first.alloc(sizeof(int32_t) * std::max<int64_t>(np, 1)));
   hipLaunchKernelGGL(tiles_per_position_kernel, dim3(grid_for(np)), dim3(256), 0,
                      h->stream, np, pl->order.as<i64>(), pl->desc.as<GemmDesc>(), bm, bn,
-                     (int32_t)pl->stream_max_k, chunk, items.as<i64>());
+                     (int32_t)pl->stream_max_k, (int32_t)pl->large_min, cls,
+                     items.as<i64>());
   TQ_CHECK(tq_scan_i64(h, items.as<i64>(), items.as<i64>(), np, items.as<i64>() + np,
                        tmp.p));
   hipLaunchKernelGGL(expand_tiles_kernel, dim3(grid_for(np)), dim3(256), 0, h->stream,
@@ -609,35 +614,40 @@ static tq_status expand_work(tq_plan_t pl, int32_t bm, int32_t bn, int32_t chunk
   return TQ_SUCCESS;
 }
 
-tq_status plan_build_tiles(tq_plan_t pl, int cfg) {
+tq_status plan_build_tiles(tq_plan_t pl) {
   tq_handle_t h = pl->h;
   const int64_t np = pl->n_pairs;
   if (!pl->chunks_ready) {
     int64_t total = 0;
     for (int64_t p = 0; p < np; ++p)
-      if (pl->h_k[p] <= pl->stream_max_k)
+      if (pair_class(*pl, pl->h_m[p], pl->h_n[p], pl->h_k[p]) < 0)
         total += ceil_div(pl->h_m[p], kStreamRows);
     if (total > 0x7fffffff)
-      return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many output chunks");
+      return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many output segments");
     if (total > 0)
-      TQ_CHECK(expand_work(pl, 0, 0, kStreamRows, total, pl->chunk_pair,
-                           pl->chunk_first));
+      TQ_CHECK(expand_work(pl, kStreamRows, 0, -1, total, pl->chunk_pair, pl->chunk_first));
     pl->n_chunks = total;
     pl->chunks_ready = true;
   }
-  if (pl->tiles_cfg == cfg && pl->tiles_ready) return TQ_SUCCESS;
-  const GemmConfig c = tq_gemm_config(pl->prec, cfg);
-  int64_t total = 0;
-  for (int64_t p = 0; p < np; ++p)
-    if (pl->h_k[p] > pl->stream_max_k)
-      total += ceil_div(pl->h_m[p], c.bm) * ceil_div(pl->h_n[p], c.bn);
-  if (total > 0x7fffffff)
-    return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many GEMM tiles");
-  if (total > 0)
-    TQ_CHECK(expand_work(pl, c.bm, c.bn, 0, total, pl->tile_pair, pl->tile_first));
-  pl->n_tiles = total;
-  pl->tiles_cfg = cfg;
-  pl->tiles_ready = true;
+  pl->n_tiles = 0;
+  for (int c = 0; c < 2; ++c) {
+    TileClass &tc = pl->cls[c];
+    const int cfg = tc.current();
+    if (!(tc.ready && tc.tiles_cfg == cfg)) {
+      const GemmConfig g = tq_gemm_config(pl->prec, cfg);
+      int64_t total = 0;
+      for (int64_t p = 0; p < np; ++p)
+        if (pair_class(*pl, pl->h_m[p], pl->h_n[p], pl->h_k[p]) == c)
+          total += ceil_div(pl->h_m[p], g.bm) * ceil_div(pl->h_n[p], g.bn);
+      if (total > 0x7fffffff)
+        return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many GEMM tiles");
+      if (total > 0) TQ_CHECK(expand_work(pl, g.bm, g.bn, c, total, tc.tile_pair, tc.tile_first));
+      tc.n_tiles = total;
+      tc.tiles_cfg = cfg;
+      tc.ready = true;
+    }
+    pl->n_tiles += tc.n_tiles;
+  }
   return TQ_SUCCESS;
 }
 
@@ -1063,9 +1073,8 @@ tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
   // short contractions are written as streams; the split-BF16 tier keeps
   // every pair on the BF16 MFMA
   pl->stream_max_k = pl->prec == TQ_PREC_BF16X3 ? 0 : kStreamMaxK;
-  pl->auto_cfg = tq_plan_pick_config(pl.get());
-  pl->cfg = -1;
-  if (np > 0 && ro > 0) TQ_CHECK(plan_build_tiles(pl.get(), pl->auto_cfg));
+  for (int c = 0; c < 2; ++c) pl->cls[c].auto_cfg = tq_plan_pick_config(pl.get(), c);
+  if (np > 0 && ro > 0) TQ_CHECK(plan_build_tiles(pl.get()));
   *plan_out = pl.release();
   return TQ_SUCCESS;
 }
@@ -1081,8 +1090,7 @@ tq_status tq_plan_get_info(tq_plan_t pl, tq_plan_info *info) {
   info->bytes = pl->bytes;
   info->a_layout = pl->a_layout;
   info->b_layout = pl->b_layout;
-  if (pl->n_pairs > 0 && pl->ro > 0)
-    TQ_CHECK(plan_build_tiles(pl, pl->cfg >= 0 ? pl->cfg : pl->auto_cfg));
+  if (pl->n_pairs > 0 && pl->ro > 0) TQ_CHECK(plan_build_tiles(pl));
   info->n_tiles = pl->n_tiles;
   info->n_segments = pl->n_chunks;
   return TQ_SUCCESS;

@@ -1,0 +1,287 @@
+// tq_wave_gemm.hip -- grouped GEMM with one wavefront per output tile and
+// operands loaded straight from global memory into MFMA registers.
+//
+// Same contract as the workgroup-tile kernel in tq_gemm.hip (every pair p:
+// C_p(MxN) = A_p(MxK) B_p(NxK)^T, C_p column-major with ldc = M, operands in
+// place with either index fastest or from packed panels), different
+// schedule: each wavefront owns a (16 TM) x (16 TN) tile of one pair and
+// walks K on its own, so there is no LDS and no workgroup barrier. Four
+// independent wavefronts share a workgroup only to fill the SIMDs.
+//
+// The FP64 MFMA (v_mfma_f64_16x16x4_f64) takes 64 cycles on a CDNA4 SIMD,
+// so a 2x2 register tile consumes operands at about 32 B/clk per CU: what the
+// L1/L2 path delivers without staging in LDS. Small block pairs (the bulk of
+// a U(1) contraction's pairs) then cost no barrier or staging round trips.
+//
+// k mapping: MFMA step s of a k tile sums k = k0 + 4s + g over the four
+// 16-lane groups g, the grouping the workgroup-tile kernel uses, so every
+// configuration of both kernels adds the same products in the same order and
+// gives bit-identical results. For a fixed step the 16 lanes of a group read
+// 16 consecutive rows (one cache line when the row index is fastest) and the
+// four groups 4 consecutive k of each row (the next steps read the rest of
+// the same lines when k is fastest).
+#include "tq_gemm_common.h"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ inline d4 mfma(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline f4 mfma(float a, float b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <class TC>
+struct Acc;
+template <>
+struct Acc<double> {
+  typedef d4 type;
+};
+template <>
+struct Acc<float> {
+  typedef f4 type;
+};
+
+__device__ inline int64_t uniform64(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Where one wave tile reads and writes (all wave-uniform).
+template <class TIn>
+struct WaveTile {
+  const TIn *pa, *pb;
+  int64_t sra, ska, srb, skb;  // element strides along (row, k) of A and B
+  int64_t c_off;
+  int m, n, k, m0, n0;
+};
+
+template <class TIn, int BM, int BN>
+__device__ inline WaveTile<TIn> wave_tile(int64_t t, const GemmDesc *__restrict__ descs,
+                                          const int32_t *__restrict__ tile_pair,
+                                          const int32_t *__restrict__ tile_first,
+                                          const TIn *a, const TIn *a_packed, const TIn *b,
+                                          const TIn *b_packed) {
+  WaveTile<TIn> w;
+  const int p = __builtin_amdgcn_readfirstlane(tile_pair[t]);
+  const GemmDesc d = descs[p];
+  const int lt = (int)(t - tile_first[p]);
+  const int tiles_m = (d.m + BM - 1) / BM;
+  w.m = d.m;
+  w.n = d.n;
+  w.k = d.k;
+  w.m0 = (lt % tiles_m) * BM;
+  w.n0 = (lt / tiles_m) * BN;
+  w.c_off = d.c_off;
+  const bool akf = d.flags & kAKFast, bkf = d.flags & kBKFast;
+  w.sra = akf ? d.lda : 1;
+  w.ska = akf ? 1 : d.lda;
+  w.srb = bkf ? d.ldb : 1;
+  w.skb = bkf ? 1 : d.ldb;
+  w.pa = ((d.flags & kAPacked) ? a_packed : a) + d.a_off;
+  w.pb = ((d.flags & kBPacked) ? b_packed : b) + d.b_off;
+  return w;
+}
+
+// Loads this lane's KPL k values (k0 + 4s + g) of each of the T row blocks
+// of one operand for the k tile starting at k0. Rows past the block are
+// clamped to its last row (their products only reach output rows that are
+// never stored) and k past K to the last k; the values at k >= K are zeroed
+// where they are used (select_k), so no load result is consumed before the
+// next k tile and every load stays in flight under the current tile's MFMAs.
+// Offsets are products of non-negative 32-bit values (one v_mad_u64_u32).
+template <class TIn, class TC, int T, int KPL>
+__device__ inline void load_operand(TC (&r)[T][KPL], const TIn *__restrict__ p, int64_t sr,
+                                    int64_t sk, int rows, int row0, int kk, int k0, int lane) {
+  const int kb = k0 + (lane >> 4);
+  uint64_t koff[KPL];
+#pragma unroll
+  for (int s = 0; s < KPL; ++s)
+    koff[s] = (uint64_t)(uint32_t)min(kb + 4 * s, kk - 1) * (uint32_t)sk;
+#pragma unroll
+  for (int i = 0; i < T; ++i) {
+    const int row = min(row0 + 16 * i + (lane & 15), rows - 1);
+    const TIn *q = p + (uint64_t)(uint32_t)row * (uint32_t)sr;
+#pragma unroll
+    for (int s = 0; s < KPL; ++s) r[i][s] = (TC)q[koff[s]];
+  }
+}
+
+// Operand value for MFMA step s: zero where k = k0 + 4s + g is past K
+// (kvalid = K - k0 - g).
+template <class TC>
+__device__ inline TC select_k(TC v, int s, int kvalid) {
+  return 4 * s < kvalid ? v : (TC)0;
+}
+
+template <class TIn, class TC, class TOut, int TM, int TN, int KPL>
+__global__ __launch_bounds__(256) void wave_gemm_kernel(
+    const GemmDesc *__restrict__ descs, const int32_t *__restrict__ tile_pair,
+    const int32_t *__restrict__ tile_first, int64_t n_tiles, const TIn *__restrict__ a,
+    const TIn *__restrict__ a_packed, const TIn *__restrict__ b,
+    const TIn *__restrict__ b_packed, TOut *__restrict__ c) {
+  typedef typename Acc<TC>::type AccT;
+  constexpr int BM = 16 * TM, BN = 16 * TN, BK = 4 * KPL;
+  constexpr bool kF64 = sizeof(TC) == 8;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t stride = (int64_t)gridDim.x * 4;
+  int64_t t = uniform64(tqg::xcd_virtual_id(blockIdx.x, gridDim.x) * 4 + wave);
+  if (t >= n_tiles) return;
+
+  WaveTile<TIn> cur =
+      wave_tile<TIn, BM, BN>(t, descs, tile_pair, tile_first, a, a_packed, b, b_packed);
+  // (na, nb) always hold the prefetched next k tile; (ra, rb) the current one
+  TC ra[TM][KPL], rb[TN][KPL], na[TM][KPL], nb[TN][KPL];
+  load_operand<TIn, TC, TM, KPL>(na, cur.pa, cur.sra, cur.ska, cur.m, cur.m0, cur.k, 0, lane);
+  load_operand<TIn, TC, TN, KPL>(nb, cur.pb, cur.srb, cur.skb, cur.n, cur.n0, cur.k, 0, lane);
+  while (true) {
+    // the next wave tile is known from the start, so the prefetch of its first
+    // k tile is an unconditional load (the last tile prefetches itself again)
+    const int64_t t_next = t + stride;
+    const bool more = t_next < n_tiles;
+    WaveTile<TIn> nxt = cur;
+    if (more)
+      nxt = wave_tile<TIn, BM, BN>(t_next, descs, tile_pair, tile_first, a, a_packed, b,
+                                   b_packed);
+    AccT acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = AccT{0, 0, 0, 0};
+    for (int k0 = 0; k0 < cur.k; k0 += BK) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int s = 0; s < KPL; ++s) ra[i][s] = na[i][s];
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int s = 0; s < KPL; ++s) rb[j][s] = nb[j][s];
+      const bool last = k0 + BK >= cur.k;
+      const WaveTile<TIn> &src = last ? nxt : cur;
+      const int kn = last ? 0 : k0 + BK;
+      load_operand<TIn, TC, TM, KPL>(na, src.pa, src.sra, src.ska, src.m, src.m0, src.k, kn,
+                                     lane);
+      load_operand<TIn, TC, TN, KPL>(nb, src.pb, src.srb, src.skb, src.n, src.n0, src.k, kn,
+                                     lane);
+      // keep these loads ahead of the MFMAs: their latency hides under them
+      __builtin_amdgcn_sched_barrier(0);
+      const int kvalid = cur.k - k0 - (lane >> 4);
+#pragma unroll
+      for (int s = 0; s < KPL; ++s) {
+        TC av[TM], bv[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) av[i] = select_k(ra[i][s], s, kvalid);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) bv[j] = select_k(rb[j][s], s, kvalid);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) acc[i][j] = mfma(av[i], bv[j], acc[i][j]);
+      }
+    }
+    // column-major store of the tile's valid rows and columns
+    TOut *cb = c + cur.c_off;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int col = cur.n0 + 16 * j + (lane & 15);
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = cur.m0 + 16 * i + tqg::acc_row<kF64>(lane, r);
+          if (row < cur.m && col < cur.n) cb[row + (int64_t)cur.m * col] = (TOut)acc[i][j][r];
+        }
+    }
+    if (!more) break;
+    t = t_next;
+    cur = nxt;
+  }
+}
+
+struct WaveCfg {
+  int tm, tn, kpl;
+};
+// keep in step with the switch in dispatch()
+constexpr WaveCfg kWaveCfgs[] = {{1, 1, 4}, {2, 1, 4}, {1, 2, 4}, {2, 2, 4},
+                                 {2, 2, 2}, {4, 2, 2}, {2, 4, 2}, {4, 4, 2}};
+constexpr int kNWave = sizeof(kWaveCfgs) / sizeof(kWaveCfgs[0]);
+
+template <class TIn, class TC, class TOut, int TM, int TN, int KPL>
+tq_status launch_one(tq_handle_t h, const GemmDesc *descs, const int32_t *tile_pair,
+                     const int32_t *tile_first, int64_t n_tiles, const void *a,
+                     const void *ap, const void *b, const void *bp, void *c) {
+  // persistent grid: the workgroups that fit at once, a multiple of the 8 XCDs
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &occ, (wave_gemm_kernel<TIn, TC, TOut, TM, TN, KPL>), 256, 0) != hipSuccess ||
+        occ < 1)
+      occ = 1;
+    per_cu = occ;
+  }
+  int64_t grid = std::min<int64_t>(ceil_div(n_tiles, (int64_t)4), (int64_t)h->n_cu * per_cu);
+  if (grid >= 8) grid -= grid % 8;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL((wave_gemm_kernel<TIn, TC, TOut, TM, TN, KPL>), dim3((unsigned)grid),
+                     dim3(256), 0, h->stream, descs, tile_pair, tile_first, n_tiles,
+                     reinterpret_cast<const TIn *>(a), reinterpret_cast<const TIn *>(ap),
+                     reinterpret_cast<const TIn *>(b), reinterpret_cast<const TIn *>(bp),
+                     reinterpret_cast<TOut *>(c));
+  TQ_HIP(h, hipGetLastError());
+  return TQ_SUCCESS;
+}
+
+template <class TIn, class TC, class TOut>
+tq_status dispatch(tq_handle_t h, int cfg, const GemmDesc *descs, const int32_t *tile_pair,
+                   const int32_t *tile_first, int64_t n_tiles, const void *a, const void *ap,
+                   const void *b, const void *bp, void *c) {
+#define TQ_WCFG(I, TM, TN, KPL)                                                          \
+  case I:                                                                                \
+    return launch_one<TIn, TC, TOut, TM, TN, KPL>(h, descs, tile_pair, tile_first,      \
+                                                  n_tiles, a, ap, b, bp, c);
+  switch (cfg) {
+    TQ_WCFG(0, 1, 1, 4)
+    TQ_WCFG(1, 2, 1, 4)
+    TQ_WCFG(2, 1, 2, 4)
+    TQ_WCFG(3, 2, 2, 4)
+    TQ_WCFG(4, 2, 2, 2)
+    TQ_WCFG(5, 4, 2, 2)
+    TQ_WCFG(6, 2, 4, 2)
+    TQ_WCFG(7, 4, 4, 2)
+  }
+#undef TQ_WCFG
+  return tq_fail(h, TQ_ERROR_INVALID_VALUE, "gemm: unknown wave tile configuration");
+}
+
+}  // namespace
+
+int tq_wave_n_configs() { return kNWave; }
+
+GemmConfig tq_wave_config(int cfg) {
+  if (cfg < 0 || cfg >= kNWave) cfg = 0;
+  const WaveCfg &w = kWaveCfgs[cfg];
+  return GemmConfig{16 * w.tm, 16 * w.tn, 4 * w.kpl, 1};
+}
+
+tq_status tq_launch_wave_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec, int cfg,
+                              const GemmDesc *descs, const int32_t *tile_pair,
+                              const int32_t *tile_first, int64_t n_tiles, const void *a,
+                              const void *a_packed, const void *b, const void *b_packed,
+                              void *c) {
+  if (n_tiles <= 0) return TQ_SUCCESS;
+  if (dt == TQ_F32)
+    return dispatch<float, float, float>(h, cfg, descs, tile_pair, tile_first, n_tiles, a,
+                                         a_packed, b, b_packed, c);
+  if (prec == TQ_PREC_F32)
+    return dispatch<double, float, double>(h, cfg, descs, tile_pair, tile_first, n_tiles, a,
+                                           a_packed, b, b_packed, c);
+  return dispatch<double, double, double>(h, cfg, descs, tile_pair, tile_first, n_tiles, a,
+                                          a_packed, b, b_packed, c);
+}

@@ -358,16 +358,26 @@ class ContractionPlan:
         return order
 
     def autotune(self, a_data, b_data, out, reps=3):
-        """Time every tile configuration (tq_plan_autotune); returns
-        (best index, per-configuration mean ms)."""
+        """Time every tile configuration of each MFMA tile class
+        (tq_plan_autotune); returns ([class 0, class 1] chosen configuration,
+        [class 0 mean ms per configuration, class 1 ...])."""
         self.h.bind_stream()
         n = lib().tq_gemm_config_count(_lib.PRECISIONS[self.precision])
-        times = (ctypes.c_float * max(n, 1))()
-        best = ctypes.c_int32()
+        times = (ctypes.c_float * max(2 * n, 1))()
+        best = (ctypes.c_int32 * 2)()
         self.h.check(lib().tq_plan_autotune(
             self.ptr, ctypes.c_void_p(a_data.data_ptr()), ctypes.c_void_p(b_data.data_ptr()),
-            ctypes.c_void_p(out.data_ptr()), int(reps), ctypes.byref(best), times))
-        return best.value, [times[i] for i in range(n)]
+            ctypes.c_void_p(out.data_ptr()), int(reps), best, times))
+        return [best[0], best[1]], [[times[c * n + i] for i in range(n)] for c in range(2)]
+
+    def config(self):
+        """Current tile configuration of [class 0, class 1]."""
+        cfg = (ctypes.c_int32 * 2)()
+        self.h.check(lib().tq_plan_get_config(self.ptr, cfg))
+        return [cfg[0], cfg[1]]
+
+    def set_class_config(self, cls, cfg):
+        self.h.check(lib().tq_plan_set_class_config(self.ptr, int(cls), int(cfg)))
 
     def set_small_k(self, max_k):
         self.h.check(lib().tq_plan_set_small_k(self.ptr, int(max_k)))
