@@ -215,9 +215,10 @@ tq_status tq_plan_set_small_k(tq_plan_t plan, int32_t max_k);
  * configuration with HIP events on the handle's stream and keeps the fastest
  * for that class. times_ms (host, 2 * tq_gemm_config_count(precision)
  * entries, class 0 first, may be NULL) receives the mean time of each
- * (class, configuration), 0 for a class without tiles; best (host, 2
- * entries, may be NULL) the chosen configuration of each class. C is
- * written by the timed launches. */
+ * (class, configuration), 0 for a class without tiles and -1 for a
+ * configuration the plan cannot run (wave-tile configurations need leading
+ * dimensions below 2^22); best (host, 2 entries, may be NULL) the chosen
+ * configuration of each class. C is written by the timed launches. */
 tq_status tq_plan_autotune(tq_plan_t plan, const void *a, const void *b,
                            void *c, int32_t reps, int32_t *best,
                            float *times_ms);

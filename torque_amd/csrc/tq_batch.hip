@@ -136,6 +136,7 @@ tq_status tq_batch_create(tq_handle_t h, int32_t n, const tq_plan_t *plans,
       pl->h_m.push_back(d.m);
       pl->h_n.push_back(d.n);
       pl->h_k.push_back(d.k);
+      pl->max_ld = std::max<int64_t>(pl->max_ld, std::max(d.lda, d.ldb));
     }
     pl->flops += p.flops;
     pl->bytes += p.bytes;

@@ -150,6 +150,10 @@ tq_status tq_launch_small_k(tq_handle_t h, tq_dtype dt, tq_precision prec,
                             const void *a_packed, const void *b, const void *b_packed,
                             void *c);
 
+// The wave-tile kernel addresses operands with 32-bit byte offsets from a
+// per-tile base; it runs only when every leading dimension is below this.
+constexpr int64_t kWaveMaxLd = int64_t(1) << 22;
+
 // Tile shape of a configuration; wave_tile = 1 when one wavefront (not a
 // four-wave workgroup) computes a tile.
 struct GemmConfig {

@@ -62,6 +62,7 @@ int tq_plan_pick_config(tq_plan_t pl, int cls) {
   double best_t = 1e300;
   for (int c = 0; c < nc; ++c) {
     const GemmConfig g = tq_gemm_config(pl->prec, c);
+    if (g.wave_tile && !pl->wave_ok()) continue;
     double flop = 0, bytes = 0, tiles = 0;
     for (int64_t p = 0; p < pl->n_pairs; ++p) {
       if (pair_class(*pl, pl->h_m[p], pl->h_n[p], pl->h_k[p]) != cls) continue;
@@ -106,18 +107,29 @@ tq_status launch_class(tq_plan_t pl, int cls, const void *a, const void *b, void
 
 extern "C" {
 
+namespace {
+
+tq_status check_config(tq_plan_t pl, int32_t config) {
+  if (config < -1 || config >= tq_gemm_n_configs(pl->prec)) return TQ_ERROR_INVALID_VALUE;
+  if (config >= 0 && tq_gemm_config(pl->prec, config).wave_tile && !pl->wave_ok())
+    return tq_fail(pl->h, TQ_ERROR_UNSUPPORTED,
+                   "gemm: wave-tile configurations need leading dimensions below 2^22");
+  return TQ_SUCCESS;
+}
+
+}  // namespace
+
 tq_status tq_plan_set_config(tq_plan_t pl, int32_t config) {
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
-  if (config < -1 || config >= tq_gemm_n_configs(pl->prec))
-    return TQ_ERROR_INVALID_VALUE;
+  TQ_CHECK(check_config(pl, config));
   pl->cls[0].cfg = pl->cls[1].cfg = config;
   return TQ_SUCCESS;
 }
 
 tq_status tq_plan_set_class_config(tq_plan_t pl, int32_t cls, int32_t config) {
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
-  if (cls < 0 || cls > 1 || config < -1 || config >= tq_gemm_n_configs(pl->prec))
-    return TQ_ERROR_INVALID_VALUE;
+  if (cls < 0 || cls > 1) return TQ_ERROR_INVALID_VALUE;
+  TQ_CHECK(check_config(pl, config));
   pl->cls[cls].cfg = config;
   return TQ_SUCCESS;
 }
@@ -244,6 +256,10 @@ tq_status tq_plan_autotune(tq_plan_t pl, const void *a, const void *b, void *c,
     int best_cfg = tc.current();
     float best_t = 3.4e38f;
     for (int q = 0; q < nc && st == TQ_SUCCESS; ++q) {
+      if (tq_gemm_config(pl->prec, q).wave_tile && !pl->wave_ok()) {
+        if (times_ms) times_ms[k * nc + q] = -1.f;  // not applicable
+        continue;
+      }
       tc.cfg = q;
       st = plan_build_tiles(pl);
       if (st == TQ_SUCCESS) st = launch_class(pl, k, a, b, c);  // warm-up

@@ -120,6 +120,8 @@ struct tq_plan_s {
   int64_t n_chunks = 0;
   bool chunks_ready = false;
   int large_min = kLargeMin;
+  int64_t max_ld = 0;  // largest lda / ldb of the GEMM descriptors
+  bool wave_ok() const { return max_ld < kWaveMaxLd; }
   TileClass cls[2];
   int64_t n_tiles = 0;  // both classes
 

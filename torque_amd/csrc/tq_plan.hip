@@ -1047,6 +1047,14 @@ tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
                                                                          : kModeDirect,
                         split, up));
 
+    // packing rewrote descriptors: the largest leading dimension gates the
+    // wave-tile kernel's 32-bit offsets
+    TQ_HIP(h, hipMemcpyAsync(hd.data(), pl->desc.p, sizeof(GemmDesc) * np,
+                             hipMemcpyDeviceToHost, h->stream));
+    TQ_HIP(h, hipStreamSynchronize(h->stream));
+    for (int64_t p = 0; p < np; ++p)
+      pl->max_ld = std::max<int64_t>(pl->max_ld, std::max(hd[p].lda, hd[p].ldb));
+
     // ---- 5. work list: stable device sort by cost, largest first ----
     TQ_HIP(h, pl->order.alloc(sizeof(i64) * np));
     TQ_CHECK(tq_sort_by_cost(h, cost.as<i64>(), np, pl->order.as<i64>()));

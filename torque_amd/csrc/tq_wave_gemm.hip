@@ -51,11 +51,13 @@ __device__ inline int64_t uniform64(int64_t v) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// Where one wave tile reads and writes (all wave-uniform).
+// Where one wave tile reads and writes (all wave-uniform). pa and pb point
+// at the tile's first row of A and of B; element (row, k) of the tile lies
+// at row * sr + k * sk from there.
 template <class TIn>
 struct WaveTile {
   const TIn *pa, *pb;
-  int64_t sra, ska, srb, skb;  // element strides along (row, k) of A and B
+  int32_t sra, ska, srb, skb;
   int64_t c_off;
   int m, n, k, m0, n0;
 };
@@ -82,40 +84,69 @@ __device__ inline WaveTile<TIn> wave_tile(int64_t t, const GemmDesc *__restrict_
   w.ska = akf ? 1 : d.lda;
   w.srb = bkf ? d.ldb : 1;
   w.skb = bkf ? 1 : d.ldb;
-  w.pa = ((d.flags & kAPacked) ? a_packed : a) + d.a_off;
-  w.pb = ((d.flags & kBPacked) ? b_packed : b) + d.b_off;
+  w.pa = ((d.flags & kAPacked) ? a_packed : a) + d.a_off + (int64_t)w.m0 * w.sra;
+  w.pb = ((d.flags & kBPacked) ? b_packed : b) + d.b_off + (int64_t)w.n0 * w.srb;
   return w;
 }
 
-// Loads this lane's KPL k values (k0 + 4s + g) of each of the T row blocks
-// of one operand for the k tile starting at k0. Rows past the block are
-// clamped to its last row (their products only reach output rows that are
-// never stored) and k past K to the last k; the values at k >= K are zeroed
-// where they are used (select_k), so no load result is consumed before the
-// next k tile and every load stays in flight under the current tile's MFMAs.
-// Offsets are products of non-negative 32-bit values (one v_mad_u64_u32).
-template <class TIn, class TC, int T, int KPL>
-__device__ inline void load_operand(TC (&r)[T][KPL], const TIn *__restrict__ p, int64_t sr,
-                                    int64_t sk, int rows, int row0, int kk, int k0, int lane) {
-  const int kb = k0 + (lane >> 4);
-  uint64_t koff[KPL];
-#pragma unroll
-  for (int s = 0; s < KPL; ++s)
-    koff[s] = (uint64_t)(uint32_t)min(kb + 4 * s, kk - 1) * (uint32_t)sk;
+// Byte offsets, from element (row0, k0) of the operand, of this lane's KPL
+// k values (k0 + 4s + g) in each of its T row blocks, for a k tile that lies
+// inside K. Rows past the block are clamped to its last row (their products
+// only reach output rows that are never stored). Offsets stay below 2^32
+// because the leading dimensions are below kWaveMaxLd.
+template <class TIn, int T, int KPL>
+__device__ inline void tile_offsets(uint32_t (&off)[T][KPL], int32_t sr, int32_t sk, int rows,
+                                    int row0, int lane) {
+  const uint32_t g = lane >> 4;
 #pragma unroll
   for (int i = 0; i < T; ++i) {
-    const int row = min(row0 + 16 * i + (lane & 15), rows - 1);
-    const TIn *q = p + (uint64_t)(uint32_t)row * (uint32_t)sr;
+    const uint32_t rl = (uint32_t)(min(row0 + 16 * i + (lane & 15), rows - 1) - row0);
 #pragma unroll
-    for (int s = 0; s < KPL; ++s) r[i][s] = (TC)q[koff[s]];
+    for (int s = 0; s < KPL; ++s)
+      off[i][s] = (rl * (uint32_t)sr + (g + 4 * s) * (uint32_t)sk) * (uint32_t)sizeof(TIn);
   }
 }
 
-// Operand value for MFMA step s: zero where k = k0 + 4s + g is past K
-// (kvalid = K - k0 - g).
-template <class TC>
-__device__ inline TC select_k(TC v, int s, int kvalid) {
-  return 4 * s < kvalid ? v : (TC)0;
+// Loads one k tile of one operand: base is element (row0, k0) (wave-uniform,
+// so the loads take a scalar base and a 32-bit lane offset). In a k tile
+// that crosses K (rem = K - k0 < 4 KPL) k is clamped to K - 1 and the MFMAs
+// zero those values (tail_select); either way the loads are one straight run
+// whose results nothing consumes before the next k tile, so they stay in
+// flight under the current tile's MFMAs.
+template <class TIn, class TC, int T, int KPL>
+__device__ inline void load_k(TC (&r)[T][KPL], const TIn *base, const uint32_t (&off)[T][KPL],
+                              int32_t sk, int rem, int lane) {
+  const char *pb = reinterpret_cast<const char *>(base);
+  uint32_t o[T][KPL];
+#pragma unroll
+  for (int i = 0; i < T; ++i)
+#pragma unroll
+    for (int s = 0; s < KPL; ++s) o[i][s] = off[i][s];
+  if (rem < 4 * KPL) {
+    // move each clamped k back onto k = rem - 1 of the same row
+    const int g = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < KPL; ++s) {
+      const int over = max(0, g + 4 * s - (rem - 1));
+      const uint32_t back = (uint32_t)over * (uint32_t)sk * (uint32_t)sizeof(TIn);
+#pragma unroll
+      for (int i = 0; i < T; ++i) o[i][s] -= back;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < T; ++i)
+#pragma unroll
+    for (int s = 0; s < KPL; ++s) r[i][s] = (TC)*reinterpret_cast<const TIn *>(pb + o[i][s]);
+}
+
+// Zeroes this lane's operand values past K (kvalid = K - k0 - g) in the k
+// tile that crosses K.
+template <class TC, int T, int KPL>
+__device__ inline void tail_select(TC (&r)[T][KPL], int kvalid) {
+#pragma unroll
+  for (int s = 0; s < KPL; ++s)
+#pragma unroll
+    for (int i = 0; i < T; ++i) r[i][s] = 4 * s < kvalid ? r[i][s] : (TC)0;
 }
 
 template <class TIn, class TC, class TOut, int TM, int TN, int KPL>
@@ -135,13 +166,16 @@ __global__ __launch_bounds__(256) void wave_gemm_kernel(
 
   WaveTile<TIn> cur =
       wave_tile<TIn, BM, BN>(t, descs, tile_pair, tile_first, a, a_packed, b, b_packed);
+  uint32_t offa[TM][KPL], offb[TN][KPL];
+  tile_offsets<TIn, TM, KPL>(offa, cur.sra, cur.ska, cur.m, cur.m0, lane);
+  tile_offsets<TIn, TN, KPL>(offb, cur.srb, cur.skb, cur.n, cur.n0, lane);
   // (na, nb) always hold the prefetched next k tile; (ra, rb) the current one
   TC ra[TM][KPL], rb[TN][KPL], na[TM][KPL], nb[TN][KPL];
-  load_operand<TIn, TC, TM, KPL>(na, cur.pa, cur.sra, cur.ska, cur.m, cur.m0, cur.k, 0, lane);
-  load_operand<TIn, TC, TN, KPL>(nb, cur.pb, cur.srb, cur.skb, cur.n, cur.n0, cur.k, 0, lane);
+  load_k<TIn, TC, TM, KPL>(na, cur.pa, offa, cur.ska, cur.k, lane);
+  load_k<TIn, TC, TN, KPL>(nb, cur.pb, offb, cur.skb, cur.k, lane);
   while (true) {
     // the next wave tile is known from the start, so the prefetch of its first
-    // k tile is an unconditional load (the last tile prefetches itself again)
+    // k tile needs no wait (the last tile prefetches itself again)
     const int64_t t_next = t + stride;
     const bool more = t_next < n_tiles;
     WaveTile<TIn> nxt = cur;
@@ -162,28 +196,34 @@ __global__ __launch_bounds__(256) void wave_gemm_kernel(
       for (int j = 0; j < TN; ++j)
 #pragma unroll
         for (int s = 0; s < KPL; ++s) rb[j][s] = nb[j][s];
-      const bool last = k0 + BK >= cur.k;
-      const WaveTile<TIn> &src = last ? nxt : cur;
-      const int kn = last ? 0 : k0 + BK;
-      load_operand<TIn, TC, TM, KPL>(na, src.pa, src.sra, src.ska, src.m, src.m0, src.k, kn,
-                                     lane);
-      load_operand<TIn, TC, TN, KPL>(nb, src.pb, src.srb, src.skb, src.n, src.n0, src.k, kn,
-                                     lane);
+      const int kn = k0 + BK;
+      if (kn < cur.k) {
+        load_k<TIn, TC, TM, KPL>(na, cur.pa + (int64_t)kn * cur.ska, offa, cur.ska, cur.k - kn,
+                                 lane);
+        load_k<TIn, TC, TN, KPL>(nb, cur.pb + (int64_t)kn * cur.skb, offb, cur.skb, cur.k - kn,
+                                 lane);
+      } else {
+        // the current tile's loads are all issued: its offsets become the
+        // next tile's
+        tile_offsets<TIn, TM, KPL>(offa, nxt.sra, nxt.ska, nxt.m, nxt.m0, lane);
+        tile_offsets<TIn, TN, KPL>(offb, nxt.srb, nxt.skb, nxt.n, nxt.n0, lane);
+        load_k<TIn, TC, TM, KPL>(na, nxt.pa, offa, nxt.ska, nxt.k, lane);
+        load_k<TIn, TC, TN, KPL>(nb, nxt.pb, offb, nxt.skb, nxt.k, lane);
+      }
       // keep these loads ahead of the MFMAs: their latency hides under them
       __builtin_amdgcn_sched_barrier(0);
-      const int kvalid = cur.k - k0 - (lane >> 4);
+      if (kn > cur.k) {
+        tail_select<TC, TM, KPL>(ra, cur.k - k0 - (lane >> 4));
+        tail_select<TC, TN, KPL>(rb, cur.k - k0 - (lane >> 4));
+      }
+      // every MFMA block runs (blocks past M or N compute on clamped rows and
+      // are not stored), so the MFMA stream has no branches
 #pragma unroll
-      for (int s = 0; s < KPL; ++s) {
-        TC av[TM], bv[TN];
-#pragma unroll
-        for (int i = 0; i < TM; ++i) av[i] = select_k(ra[i][s], s, kvalid);
-#pragma unroll
-        for (int j = 0; j < TN; ++j) bv[j] = select_k(rb[j][s], s, kvalid);
+      for (int s = 0; s < KPL; ++s)
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
-          for (int j = 0; j < TN; ++j) acc[i][j] = mfma(av[i], bv[j], acc[i][j]);
-      }
+          for (int j = 0; j < TN; ++j) acc[i][j] = mfma(ra[i][s], rb[j][s], acc[i][j]);
     }
     // column-major store of the tile's valid rows and columns
     TOut *cb = c + cur.c_off;
