@@ -244,10 +244,14 @@ tq_status tq_batch_create(tq_handle_t handle, int32_t n, const tq_plan_t *plans,
 tq_status tq_plan_capture(tq_plan_t plan, const void *a, const void *b, void *c);
 tq_status tq_plan_graph_launch(tq_plan_t plan);
 
-/* Number of grouped-GEMM tile configurations and a description of one. */
+/* Number of grouped-GEMM tile configurations and a description of one:
+ * the tile's M and N extents, its k step, and wave_tile = 1 when one
+ * wavefront (not a four-wave workgroup) computes a tile. Any output pointer
+ * may be NULL. */
 int32_t tq_gemm_config_count(tq_precision precision);
 tq_status tq_gemm_config_describe(tq_precision precision, int32_t config,
-                                  int32_t *bm, int32_t *bn, int32_t *bk);
+                                  int32_t *bm, int32_t *bn, int32_t *bk,
+                                  int32_t *wave_tile);
 
 /* ---- dense contraction --------------------------------------------- */
 
@@ -272,6 +276,57 @@ tq_status tq_nrm2_f64(tq_handle_t handle, int64_t n, const double *x,
                       double *result);
 /* x <- x * alpha on the GPU (normalisation of environment tensors). */
 tq_status tq_scale_f64(tq_handle_t handle, int64_t n, double *x, double alpha);
+
+/* ---- learned tile-configuration cost model (no reference counterpart) -- */
+
+/* A multilayer perceptron (TQ_COST_FEATURES inputs, two hidden layers of
+ * TQ_COST_HIDDEN ReLU units, one output) predicting the log of a tile
+ * class's grouped-GEMM time under one configuration. It is trained on the
+ * GPU (full-batch Adam on the mean squared error) by the library's own
+ * forward, backward and update kernels, on timings the autotuner records;
+ * the autotuner stays the fallback and measures the model's regret. */
+#define TQ_COST_FEATURES 34
+#define TQ_COST_HIDDEN 64
+typedef struct tq_costmodel_s *tq_costmodel_t;
+
+/* Features of a set of block pairs (M, N, K per pair) under one tile
+ * configuration, host-only: [0,8) [8,16) [16,24) flop-weighted histograms
+ * of M, N, K over the buckets <16, 16-31, ..., 512-1023, >=1024; 24 log10
+ * flops; 25 log10 pairs; 26 log10 tiles; 27 useful / padded flops; 28-30
+ * log2 of the tile's M, N, K extents; 31 one when a wavefront computes a
+ * tile; 32 log10(tiles per concurrently resident tile); 33 log10 output
+ * bytes. All zero for an empty set. n_cu <= 0 means 256. */
+tq_status tq_cost_features(const int64_t *m, const int64_t *n, const int64_t *k,
+                           int64_t n_pairs, tq_precision precision, int32_t config,
+                           int32_t n_cu, float *features);
+/* The same for the pairs of MFMA tile class cls of a plan. */
+tq_status tq_plan_features(tq_plan_t plan, int32_t cls, int32_t config, float *features);
+
+int64_t tq_costmodel_n_params(void);
+/* He-normal weights from `seed`, zero biases. */
+tq_status tq_costmodel_create(tq_handle_t handle, uint64_t seed, tq_costmodel_t *model);
+tq_status tq_costmodel_destroy(tq_costmodel_t model);
+/* Parameters as one float array (tq_costmodel_n_params entries): W1
+ * [features][hidden], b1, W2 [hidden][hidden], b2, w3, b3. */
+tq_status tq_costmodel_get_params(tq_costmodel_t model, float *params);
+tq_status tq_costmodel_set_params(tq_costmodel_t model, const float *params);
+/* Gradient (tq_costmodel_n_params floats, may be NULL) and value of the
+ * mean squared error of the raw network over n samples (x row-major
+ * [n][TQ_COST_FEATURES], y [n]), computed on the GPU. */
+tq_status tq_costmodel_gradient(tq_costmodel_t model, const float *x, const float *y,
+                                int64_t n, float *grad, float *loss);
+/* Trains on (x, y) for `steps` Adam steps (learning rate lr) after setting
+ * the input and target normalisation from the data; *loss (may be NULL)
+ * receives the final normalised mean squared error. */
+tq_status tq_costmodel_fit(tq_costmodel_t model, const float *x, const float *y, int64_t n,
+                           int32_t steps, float lr, float *loss);
+/* Predictions for n samples; raw = 1 skips the normalisation (network
+ * output for the given inputs), raw = 0 applies the fit's. */
+tq_status tq_costmodel_predict(tq_costmodel_t model, const float *x, int64_t n, int32_t raw,
+                               float *y);
+/* Sets each tile class of the plan to the configuration with the least
+ * predicted time; best (may be NULL) receives the two choices. */
+tq_status tq_plan_choose_config(tq_plan_t plan, tq_costmodel_t model, int32_t *best);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,7 @@ PRECISIONS = {"native": TQ_PREC_NATIVE, "fp64": TQ_PREC_NATIVE,
 
 c_i64_p = ctypes.POINTER(ctypes.c_int64)
 c_i32_p = ctypes.POINTER(ctypes.c_int32)
+c_f32_p = ctypes.POINTER(ctypes.c_float)
 
 
 class BsDesc(ctypes.Structure):
@@ -84,6 +85,19 @@ def _proto(lib):
         "tq_bs_contract_execute": (S, [V, V, V, V]),
         "tq_bs_contract_pack": (S, [V, V, V]),
         "tq_bs_contract_multiply": (S, [V, V, V, V]),
+        "tq_cost_features": (S, [c_i64_p, c_i64_p, c_i64_p, ctypes.c_int64, ctypes.c_int,
+                                 ctypes.c_int32, ctypes.c_int32, c_f32_p]),
+        "tq_plan_features": (S, [V, ctypes.c_int32, ctypes.c_int32, c_f32_p]),
+        "tq_costmodel_n_params": (ctypes.c_int64, []),
+        "tq_costmodel_create": (S, [H, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+        "tq_costmodel_destroy": (S, [V]),
+        "tq_costmodel_get_params": (S, [V, c_f32_p]),
+        "tq_costmodel_set_params": (S, [V, c_f32_p]),
+        "tq_costmodel_gradient": (S, [V, c_f32_p, c_f32_p, ctypes.c_int64, c_f32_p, c_f32_p]),
+        "tq_costmodel_fit": (S, [V, c_f32_p, c_f32_p, ctypes.c_int64, ctypes.c_int32,
+                                 ctypes.c_float, c_f32_p]),
+        "tq_costmodel_predict": (S, [V, c_f32_p, ctypes.c_int64, ctypes.c_int32, c_f32_p]),
+        "tq_plan_choose_config": (S, [V, V, c_i32_p]),
         "tq_plan_set_config": (S, [V, ctypes.c_int32]),
         "tq_plan_set_class_config": (S, [V, ctypes.c_int32, ctypes.c_int32]),
         "tq_plan_get_config": (S, [V, c_i32_p]),
@@ -96,7 +110,8 @@ def _proto(lib):
                                  ctypes.POINTER(ctypes.c_float)]),
         "tq_plan_destroy": (S, [V]),
         "tq_gemm_config_count": (ctypes.c_int32, [S]),
-        "tq_gemm_config_describe": (S, [S, ctypes.c_int32, c_i32_p, c_i32_p, c_i32_p]),
+        "tq_gemm_config_describe": (S, [S, ctypes.c_int32, c_i32_p, c_i32_p, c_i32_p,
+                                        c_i32_p]),
         "tq_dense_contract": (S, [H, S, ctypes.c_int32, c_i64_p, ctypes.c_int32, c_i64_p,
                                   ctypes.c_int32, c_i32_p, V, V, V]),
         "tq_dot_f64": (S, [H, ctypes.c_int64, V, V, ctypes.POINTER(ctypes.c_double)]),

@@ -334,13 +334,15 @@ int32_t tq_gemm_config_count(tq_precision precision) {
 }
 
 tq_status tq_gemm_config_describe(tq_precision precision, int32_t config,
-                                  int32_t *bm, int32_t *bn, int32_t *bk) {
+                                  int32_t *bm, int32_t *bn, int32_t *bk,
+                                  int32_t *wave_tile) {
   if (config < 0 || config >= tq_gemm_n_configs(precision))
     return TQ_ERROR_INVALID_VALUE;
   GemmConfig c = tq_gemm_config(precision, config);
   if (bm) *bm = c.bm;
   if (bn) *bn = c.bn;
   if (bk) *bk = c.bk;
+  if (wave_tile) *wave_tile = c.wave_tile;
   return TQ_SUCCESS;
 }
 

@@ -467,15 +467,17 @@ class BatchPlan(ContractionPlan):
         self.out_rank = info.out_rank
 
 
-def gemm_configs(precision="native"):
+def gemm_configs(precision="native", with_kind=False):
+    """(bm, bn, bk) of every tile configuration, plus wave_tile when
+    with_kind (1: one wavefront per tile)."""
     L = lib()
     pr = _lib.PRECISIONS[precision]
     out = []
     for c in range(L.tq_gemm_config_count(pr)):
-        bm, bn, bk = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        bm, bn, bk, wt = (ctypes.c_int32() for _ in range(4))
         check(L.tq_gemm_config_describe(pr, c, ctypes.byref(bm), ctypes.byref(bn),
-                                        ctypes.byref(bk)))
-        out.append((bm.value, bn.value, bk.value))
+                                        ctypes.byref(bk), ctypes.byref(wt)))
+        out.append((bm.value, bn.value, bk.value) + ((wt.value,) if with_kind else ()))
     return out
 
 
