@@ -264,6 +264,46 @@ tq_status tq_dense_contract(tq_handle_t handle, tq_dtype dtype, int32_t rank_a,
                             const int32_t *legs, const void *a, const void *b,
                             void *c);
 
+/* ---- element-sparse tensors ---------------------------------------- */
+
+/* An element-sparse tensor (src/gpu/sparse.cuh:84-441): nnz stored elements,
+ * element i at linear index indices[i] under the index table `table`
+ * (element strides per leg; a soft-transposed tensor carries a permuted
+ * table, src/gpu/sparse.cuh:380-392). */
+typedef struct tq_sp_desc {
+  int32_t rank;
+  const int64_t *dims;    /* [rank] host: dimension                           */
+  const int64_t *table;   /* [rank] host: index table; NULL = compact table   */
+  int64_t nnz;
+  const int64_t *indices; /* [nnz] device: linear index of each element       */
+  const void *values;     /* [nnz] device: values (dtype)                     */
+} tq_sp_desc;
+
+/* C = contract(A, B, legs) of element-sparse tensors (legs n_legs x 2, row
+ * k = (leg of A, leg of B)). Replaces SparseTensor<T>::contract
+ * (src/gpu/sparse.cuh:270-375 and its handle_indices kernel,
+ * src/gpu/sparse.cu:88-194; CPU semantics src/tensor/sparse.h:244-356).
+ * C has dims [A free legs, B free legs] (written to out_dims, host, may be
+ * NULL), the compact column-major index table, and holds every output
+ * index that receives a product, in ascending order, each with the sum of
+ * its products (FP64 accumulation); a rank-0 result is the one element
+ * {index 0: sum}. The library allocates *out_indices (int64) and
+ * *out_values (dtype) on the device (at least one element each); the
+ * caller releases both with tq_free. Synchronous. */
+tq_status tq_sparse_contract(tq_handle_t handle, tq_dtype dtype, const tq_sp_desc *a,
+                             const tq_sp_desc *b, int32_t n_legs, const int32_t *legs,
+                             int64_t *out_dims, int64_t *out_nnz, int64_t **out_indices,
+                             void **out_values);
+
+/* Re-encodes nnz linear indices (device) of a tensor with dims/table (host,
+ * table NULL = compact) under the compact table of the permuted dimension
+ * dims[perm[k]]: the index part of SparseTensor::hard_transpose
+ * (src/gpu/sparse.cuh:398-432; the values keep their order). in_indices
+ * and out_indices may alias. */
+tq_status tq_sparse_permute_indices(tq_handle_t handle, int32_t rank, const int64_t *dims,
+                                    const int64_t *table, const int32_t *perm, int64_t nnz,
+                                    const int64_t *in_indices, int64_t *out_indices);
+
 /* ---- FP64 reductions ------------------------------------------------ */
 
 /* result = sum_i x[i] * y[i] in FP64 on the GPU (reference: cublasDdot +

@@ -460,15 +460,46 @@ int32_t orc_bs_densify_f64(const orc_bs *t, const double *data, double *out) {
   return 0;
 }
 
-int64_t orc_sparse_contract_f64(int32_t rank_a, const int64_t *dims_a,
-                                int64_t nnz_a, const int64_t *idx_a,
-                                const double *val_a, int32_t rank_b,
-                                const int64_t *dims_b, int64_t nnz_b,
-                                const int64_t *idx_b, const double *val_b,
-                                int32_t n_legs, const int32_t *legs,
-                                int64_t *out_idx, double *out_val) {
-  // src/tensor/sparse.h:244-356 (each tensor decoded with its own table; the
-  // reference reads A's table for B at :285-286)
+// util::index_to_indices with a sort index (src/util/space.h:44-61): the
+// coordinates of linear index idx under a (possibly soft-transposed) table,
+// peeled off from the largest stride down. Among equal strides the legs of
+// extent 1 are peeled last, so they receive coordinate 0.
+std::vector<int> table_order(int n, const i64 *dims, const i64 *table) {
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    if (table[x] != table[y]) return table[x] < table[y];
+    return (dims[x] > 1) < (dims[y] > 1);
+  });
+  return order;
+}
+
+void decode_table(i64 idx, int n, const std::vector<int> &order, const i64 *table, i64 *coord) {
+  for (int j = n - 1; j > 0; --j) {
+    coord[order[j]] = idx / table[order[j]];
+    idx %= table[order[j]];
+  }
+  if (n > 0) coord[order[0]] = idx;
+}
+
+std::vector<i64> table_or_compact(int rank, const int64_t *dims, const int64_t *table) {
+  std::vector<i64> t(rank);
+  i64 acc = 1;
+  for (int l = 0; l < rank; ++l) {
+    t[l] = table ? table[l] : acc;
+    acc *= dims[l];
+  }
+  return t;
+}
+
+int64_t orc_sparse_contract_f64(int32_t rank_a, const int64_t *dims_a, const int64_t *table_a,
+                                int64_t nnz_a, const int64_t *idx_a, const double *val_a,
+                                int32_t rank_b, const int64_t *dims_b, const int64_t *table_b,
+                                int64_t nnz_b, const int64_t *idx_b, const double *val_b,
+                                int32_t n_legs, const int32_t *legs, int64_t *out_idx,
+                                double *out_val) {
+  // src/tensor/sparse.h:244-356, each tensor decoded with its own table (the
+  // reference reads A's table for B at :285-286); A-major accumulation order
   std::vector<int> al(n_legs), bl(n_legs);
   for (int k = 0; k < n_legs; ++k) {
     al[k] = legs[2 * k];
@@ -481,12 +512,20 @@ int64_t orc_sparse_contract_f64(int32_t rank_a, const int64_t *dims_a,
   for (int l : bf) od.push_back(dims_b[l]);
   std::vector<i64> ot(od.size());
   orc_index_table((int)od.size(), od.data(), ot.data());
+  const std::vector<i64> ta = table_or_compact(rank_a, dims_a, table_a);
+  const std::vector<i64> tb = table_or_compact(rank_b, dims_b, table_b);
+  const std::vector<int> oa = table_order(rank_a, dims_a, ta.data());
+  const std::vector<int> ob = table_order(rank_b, dims_b, tb.data());
+  // B decoded once
+  std::vector<i64> dball((size_t)nnz_b * rank_b);
+  for (i64 j = 0; j < nnz_b; ++j)
+    decode_table(idx_b[j], rank_b, ob, tb.data(), dball.data() + (size_t)j * rank_b);
   std::map<i64, double> acc;
-  std::vector<i64> da(rank_a), db(rank_b);
+  std::vector<i64> da(rank_a);
   for (i64 i = 0; i < nnz_a; ++i) {
-    decode(idx_a[i], rank_a, dims_a, da.data());
+    decode_table(idx_a[i], rank_a, oa, ta.data(), da.data());
     for (i64 j = 0; j < nnz_b; ++j) {
-      decode(idx_b[j], rank_b, dims_b, db.data());
+      const i64 *db = dball.data() + (size_t)j * rank_b;
       bool ok = true;
       for (int k = 0; k < n_legs && ok; ++k) ok = da[al[k]] == db[bl[k]];
       if (!ok) continue;
@@ -494,8 +533,17 @@ int64_t orc_sparse_contract_f64(int32_t rank_a, const int64_t *dims_a,
       size_t q = 0;
       for (int l : af) key += da[l] * ot[q++];
       for (int l : bf) key += db[l] * ot[q++];
-      acc[key] += val_a[i] * val_b[j];
+      const double prod = val_a[i] * val_b[j];
+      acc[key] += prod;
     }
+  }
+  if (od.empty()) {
+    // full contraction: the single element {0: sum} (src/tensor/sparse.h:328-352)
+    if (out_idx) {
+      out_idx[0] = 0;
+      out_val[0] = acc.empty() ? 0.0 : acc.begin()->second;
+    }
+    return 1;
   }
   if (out_idx) {
     i64 q = 0;
@@ -506,6 +554,25 @@ int64_t orc_sparse_contract_f64(int32_t rank_a, const int64_t *dims_a,
     }
   }
   return (int64_t)acc.size();
+}
+
+int32_t orc_sparse_permute_indices(int32_t rank, const int64_t *dims, const int64_t *table,
+                                   const int32_t *perm, int64_t nnz, const int64_t *in,
+                                   int64_t *out) {
+  // src/tensor/sparse.h:371-405 with each stored index decoded (the
+  // reference decodes the storage position i, :391-393)
+  const std::vector<i64> t = table_or_compact(rank, dims, table);
+  const std::vector<int> order = table_order(rank, dims, t.data());
+  std::vector<i64> nd(rank), nt(rank), c(rank);
+  for (int k = 0; k < rank; ++k) nd[k] = dims[perm[k]];
+  orc_index_table(rank, nd.data(), nt.data());
+  for (i64 i = 0; i < nnz; ++i) {
+    decode_table(in[i], rank, order, t.data(), c.data());
+    i64 o = 0;
+    for (int k = 0; k < rank; ++k) o += c[perm[k]] * nt[k];
+    out[i] = o;
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -97,16 +97,23 @@ int32_t orc_dense_contract_f64(int32_t rank_a, const int64_t *dims_a,
 int32_t orc_bs_densify_f64(const orc_bs *t, const double *data, double *out);
 
 /* SparseTensor::contract, src/tensor/sparse.h:244-356: element-sparse
- * tensors given by sorted linear indices (compact column-major tables).
- * Output indices are sorted and unique; returns the output nnz, or -1. When
- * out_idx is NULL only the count is returned. */
-int64_t orc_sparse_contract_f64(int32_t rank_a, const int64_t *dims_a,
-                                int64_t nnz_a, const int64_t *idx_a,
-                                const double *val_a, int32_t rank_b,
-                                const int64_t *dims_b, int64_t nnz_b,
-                                const int64_t *idx_b, const double *val_b,
-                                int32_t n_legs, const int32_t *legs,
-                                int64_t *out_idx, double *out_val);
+ * tensors given by linear indices under their index tables (NULL = compact
+ * column-major). Output indices are sorted and unique; a rank-0 result is
+ * the one element {0: sum}. Returns the output nnz, or -1 on a dimension
+ * mismatch. When out_idx is NULL only the count is returned. */
+int64_t orc_sparse_contract_f64(int32_t rank_a, const int64_t *dims_a, const int64_t *table_a,
+                                int64_t nnz_a, const int64_t *idx_a, const double *val_a,
+                                int32_t rank_b, const int64_t *dims_b, const int64_t *table_b,
+                                int64_t nnz_b, const int64_t *idx_b, const double *val_b,
+                                int32_t n_legs, const int32_t *legs, int64_t *out_idx,
+                                double *out_val);
+
+/* Index part of SparseTensor::hard_transpose, src/tensor/sparse.h:371-405:
+ * each index decoded under (dims, table) and re-encoded under the compact
+ * table of dims[perm[k]]. */
+int32_t orc_sparse_permute_indices(int32_t rank, const int64_t *dims, const int64_t *table,
+                                   const int32_t *perm, int64_t nnz, const int64_t *in,
+                                   int64_t *out);
 
 #ifdef __cplusplus
 }

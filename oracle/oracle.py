@@ -44,9 +44,13 @@ def lib():
         L.orc_bs_densify_f64.restype = ctypes.c_int32
         L.orc_bs_densify_f64.argtypes = [B, _f64p, _f64p]
         L.orc_sparse_contract_f64.restype = ctypes.c_int64
-        L.orc_sparse_contract_f64.argtypes = [ctypes.c_int32, _i64p, ctypes.c_int64, _i64p,
-                                              _f64p, ctypes.c_int32, _i64p, ctypes.c_int64,
-                                              _i64p, _f64p, ctypes.c_int32, _i32p, _i64p, _f64p]
+        L.orc_sparse_contract_f64.argtypes = [ctypes.c_int32, _i64p, _i64p, ctypes.c_int64,
+                                              _i64p, _f64p, ctypes.c_int32, _i64p, _i64p,
+                                              ctypes.c_int64, _i64p, _f64p, ctypes.c_int32,
+                                              _i32p, _i64p, _f64p]
+        L.orc_sparse_permute_indices.restype = ctypes.c_int32
+        L.orc_sparse_permute_indices.argtypes = [ctypes.c_int32, _i64p, _i64p, _i32p,
+                                                 ctypes.c_int64, _i64p, _i64p]
         L.orc_index_table.argtypes = [ctypes.c_int32, _i64p, _i64p]
         L.orc_index_to_indices.argtypes = [ctypes.c_int64, ctypes.c_int32, _i64p, _i64p, _i64p]
         L.orc_nest_sum.argtypes = [ctypes.c_int64, _i64p, ctypes.c_int64, _i64p]
@@ -174,17 +178,23 @@ def densify(t, data):
     return out
 
 
-def sparse_contract(dims_a, idx_a, val_a, dims_b, idx_b, val_b, legs):
+def sparse_contract(dims_a, idx_a, val_a, dims_b, idx_b, val_b, legs, table_a=None,
+                    table_b=None):
+    """Element-sparse contraction; tables default to the compact ones.
+    Returns (sorted unique output indices, values)."""
     L = lib()
     da = np.ascontiguousarray(dims_a, np.int64)
     db = np.ascontiguousarray(dims_b, np.int64)
+    ta = None if table_a is None else np.ascontiguousarray(table_a, np.int64)
+    tb = None if table_b is None else np.ascontiguousarray(table_b, np.int64)
     ia = np.ascontiguousarray(idx_a, np.int64)
     ib = np.ascontiguousarray(idx_b, np.int64)
     va = np.ascontiguousarray(val_a, np.float64)
     vb = np.ascontiguousarray(val_b, np.float64)
     lg = _legs(legs)
-    args = [da.size, _p(da, _i64p), ia.size, _p(ia, _i64p), _p(va, _f64p), db.size,
-            _p(db, _i64p), ib.size, _p(ib, _i64p), _p(vb, _f64p), lg.shape[0], _p(lg, _i32p)]
+    args = [da.size, _p(da, _i64p), _p(ta, _i64p), ia.size, _p(ia, _i64p), _p(va, _f64p),
+            db.size, _p(db, _i64p), _p(tb, _i64p), ib.size, _p(ib, _i64p), _p(vb, _f64p),
+            lg.shape[0], _p(lg, _i32p)]
     n = L.orc_sparse_contract_f64(*args, None, None)
     if n < 0:
         raise ValueError("The dimensions from two tensors to be contracted do not match")
@@ -192,6 +202,18 @@ def sparse_contract(dims_a, idx_a, val_a, dims_b, idx_b, val_b, legs):
     ov = np.zeros(max(n, 1), np.float64)
     L.orc_sparse_contract_f64(*args, _p(oi, _i64p), _p(ov, _f64p))
     return oi[:n], ov[:n]
+
+
+def sparse_permute_indices(dims, indices, perm, table=None):
+    """Indices of SparseTensor::hard_transpose(perm)."""
+    d = np.ascontiguousarray(dims, np.int64)
+    t = None if table is None else np.ascontiguousarray(table, np.int64)
+    p = np.ascontiguousarray(perm, np.int32)
+    i = np.ascontiguousarray(indices, np.int64)
+    o = np.zeros(max(i.size, 1), np.int64)
+    lib().orc_sparse_permute_indices(d.size, _p(d, _i64p), _p(t, _i64p), _p(p, _i32p), i.size,
+                                     _p(i, _i64p), _p(o, _i64p))
+    return o[:i.size]
 
 
 def index_table(dims):

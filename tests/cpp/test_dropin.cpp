@@ -1,5 +1,6 @@
 // The reference's GPU test sections (test/gpu/block_sparse_test.cu:10-232,
-// test/gpu/dense_test.cu:9-103) written against the drop-in C++ classes:
+// test/gpu/dense_test.cu:9-103, test/gpu/sparse_test.cu:45-333) written
+// against the drop-in C++ classes:
 // the same tensors, calls and expected values, with tq_handle_t in place of
 // cublasHandle_t. Prints one line per failed check and exits non-zero.
 #include <cstdio>
@@ -7,6 +8,7 @@
 
 #include "torque/gpu/block_sparse.h"
 #include "torque/gpu/dense.h"
+#include "torque/gpu/sparse.h"
 
 static int g_fail = 0, g_checks = 0;
 #define CHECK(cond)                                                     \
@@ -23,6 +25,7 @@ using torque::uvec;
 using torque::uword;
 using BST = torque::gpu::BlockSparseTensor<float>;
 using DT = torque::gpu::DenseTensor<float>;
+using ST = torque::gpu::SparseTensor<float>;
 
 static void block_sparse_sections(tq_handle_t handle) {
   {  // block sparse matrix initialization
@@ -190,6 +193,111 @@ static void dense_sections() {
   }
 }
 
+static void sparse_sections(tq_handle_t handle) {
+  using torque::util::generate_index_table;
+  {  // sparse matrix initialization
+    ST t(uvec{4, 3});
+    CHECK(t.query({1, 1}) == 0);
+    std::vector<float> a = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12};
+    t.initialize(a.data(), {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11});
+    CHECK(t.query({1, 2}) == 10);
+  }
+  {  // sparse matrix transposition
+    ST t(uvec{4, 3});
+    std::vector<float> a = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12};
+    t.initialize(a.data(), {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11});
+    CHECK(t.query({1, 2}) == 10);
+    t.soft_transpose({1, 0});
+    CHECK(t.query({1, 2}) == 7);
+    const auto tt = t.hard_transpose({1, 0});
+    for (uword i = 0; i < 3; ++i)
+      for (uword j = 0; j < 4; ++j) CHECK(t.query({i, j}) == tt.query({j, i}));
+  }
+  {  // sparse tensor initialization
+    ST t(uvec{2, 2, 3});
+    CHECK(t.query({1, 1, 0}) == 0);
+    std::vector<float> a = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12};
+    t.initialize(a.data(), {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11});
+    CHECK(t.query({0, 1, 2}) == 11);
+  }
+  {  // sparse tensor transposition
+    std::vector<float> a = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12};
+    const uvec dim{2, 2, 3};
+    ST t(a.data(), {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11}, generate_index_table(dim), dim);
+    CHECK(t.query({0, 0, 2}) == 9);
+    CHECK(t.query({0, 1, 2}) == 11);
+    t.soft_transpose({0, 2, 1});
+    CHECK(t.query({0, 2, 0}) == 9);
+    CHECK(t.query({0, 2, 1}) == 11);
+    const auto t2 = t.hard_transpose({1, 0, 2});
+    CHECK(t2.query({2, 0, 0}) == 9);
+    CHECK(t2.query({2, 0, 1}) == 11);
+  }
+  {  // scalar
+    ST t(uvec{});
+    CHECK(t.rank == 0);
+    CHECK(t.query({}) == 0);
+    t.modify({}, 1);
+    CHECK(t.query({}) == 1);
+  }
+  {  // vector contraction
+    std::vector<float> v = {0, 1, 2, 3, 4};
+    const uvec dim{5};
+    const ST t(v.data(), {0, 1, 2, 3, 4}, generate_index_table(dim), dim);
+    CHECK(t.contract(handle, t, umat({0, 0})).to_number() == 30);
+  }
+  {  // matrix multiplication
+    std::vector<float> v = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+    const uvec dim{3, 3};
+    const ST t(v.data(), {0, 1, 2, 3, 4, 5, 6, 7, 8}, generate_index_table(dim), dim);
+    const auto sq = t.contract(handle, t, umat({1, 0}));
+    const float ref[3][3] = {{15, 42, 69}, {18, 54, 90}, {21, 66, 111}};
+    for (uword i = 0; i < 3; ++i)
+      for (uword j = 0; j < 3; ++j) CHECK(sq.query({i, j}) == ref[i][j]);
+  }
+  {  // matrix inner product
+    std::vector<float> v = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+    const uvec dim{3, 3};
+    const ST t(v.data(), {0, 1, 2, 3, 4, 5, 6, 7, 8}, generate_index_table(dim), dim);
+    float expect = 0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) expect += v[i + 3 * j] * v[j + 3 * i];
+    CHECK(t.contract(handle, t, umat({{1, 0}, {0, 1}})).to_number() == expect);
+  }
+  {  // tensor-vector contraction
+    std::vector<float> td = {0, 1, 2, 3, 4, 5, 6, 7};
+    const uvec tdim{2, 2, 2};
+    const ST t(td.data(), {0, 1, 2, 3, 4, 5, 6, 7}, generate_index_table(tdim), tdim);
+    std::vector<float> vd = {1, 2};
+    const ST v(vd.data(), {0, 1}, generate_index_table({2}), {2});
+    const auto c = t.contract(handle, v, umat{{0, 0}});
+    CHECK(c.query({0, 0}) == 2);
+    CHECK(c.query({1, 0}) == 8);
+    CHECK(c.query({0, 1}) == 14);
+    CHECK(c.query({1, 1}) == 20);
+    const auto c2 = t.contract(handle, v, umat{{1, 0}});
+    CHECK(c2.query({0, 0}) == 4);
+    CHECK(c2.query({1, 0}) == 7);
+    CHECK(c2.query({0, 1}) == 16);
+    CHECK(c2.query({1, 1}) == 19);
+    const auto c3 = t.contract(handle, v, umat{{2, 0}});
+    CHECK(c3.query({1, 0}) == 11);
+  }
+  {  // tensor-matrix contraction
+    std::vector<float> td = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11};
+    const uvec tdim{2, 2, 3};
+    const ST t(td.data(), {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11}, generate_index_table(tdim),
+               tdim);
+    std::vector<float> md = {1, 2, 3, 4};
+    const uvec mdim{2, 2};
+    const ST m(md.data(), {0, 1, 2, 3}, generate_index_table(mdim), mdim);
+    const auto c = t.contract(handle, m, umat{{0, 1}, {1, 0}});
+    CHECK(c.query({0}) == 19);
+    CHECK(c.query({1}) == 59);
+    CHECK(c.query({2}) == 99);
+  }
+}
+
 int main() {
   tq_handle_t handle = nullptr;
   if (tq_create(&handle, 0) != TQ_SUCCESS) {
@@ -199,6 +307,7 @@ int main() {
   try {
     block_sparse_sections(handle);
     dense_sections();
+    sparse_sections(handle);
   } catch (const std::exception &e) {
     std::printf("FAILED with exception: %s\n", e.what());
     ++g_fail;

@@ -38,6 +38,12 @@ class BsDesc(ctypes.Structure):
                 ("strides", c_i64_p), ("offsets", c_i64_p)]
 
 
+class SpDesc(ctypes.Structure):
+    _fields_ = [("rank", ctypes.c_int32), ("dims", c_i64_p), ("table", c_i64_p),
+                ("nnz", ctypes.c_int64), ("indices", ctypes.c_void_p),
+                ("values", ctypes.c_void_p)]
+
+
 class PlanInfo(ctypes.Structure):
     _fields_ = [("out_rank", ctypes.c_int32), ("n_legs", ctypes.c_int32),
                 ("n_pairs", ctypes.c_int64), ("out_n_elem", ctypes.c_int64),
@@ -114,6 +120,11 @@ def _proto(lib):
                                         c_i32_p]),
         "tq_dense_contract": (S, [H, S, ctypes.c_int32, c_i64_p, ctypes.c_int32, c_i64_p,
                                   ctypes.c_int32, c_i32_p, V, V, V]),
+        "tq_sparse_contract": (S, [H, S, ctypes.POINTER(SpDesc), ctypes.POINTER(SpDesc),
+                                   ctypes.c_int32, c_i32_p, c_i64_p, c_i64_p,
+                                   ctypes.POINTER(V), ctypes.POINTER(V)]),
+        "tq_sparse_permute_indices": (S, [H, ctypes.c_int32, c_i64_p, c_i64_p, c_i32_p,
+                                          ctypes.c_int64, V, V]),
         "tq_dot_f64": (S, [H, ctypes.c_int64, V, V, ctypes.POINTER(ctypes.c_double)]),
         "tq_nrm2_f64": (S, [H, ctypes.c_int64, V, ctypes.POINTER(ctypes.c_double)]),
         "tq_scale_f64": (S, [H, ctypes.c_int64, V, ctypes.c_double]),

@@ -585,6 +585,183 @@ class DenseTensor:
         return res
 
 
+class SparseTensor:
+    """Element-sparse tensor (src/gpu/sparse.cuh:84-441): `indices[i]` is the
+    linear index of stored element i under `index_table`, `data[i]` its
+    value. Indices and values live on the device (int64 and the dtype);
+    contract and hard_transpose run in libtorque_hip.so."""
+
+    def __init__(self, dimension, source_data=None, indices=None, index_table_=None,
+                 dtype=torch.float64, device=None):
+        dim = np.asarray(dimension, dtype=np.int64).reshape(-1)
+        self.rank = int(dim.size)
+        self.dimension = dim
+        self.index_table = (index_table(dim) if index_table_ is None
+                            else np.asarray(index_table_, dtype=np.int64).reshape(-1).copy())
+        self.dtype = dtype
+        dev = torch.device(device) if device is not None else (
+            torch.device("cuda") if torch.cuda.is_available() else torch.device("cpu"))
+        if source_data is None:
+            # src/gpu/sparse.cuh:98-107: no elements, data = {0}
+            self.indices = torch.zeros(0, dtype=torch.int64, device=dev)
+            self.data = torch.zeros(1, dtype=dtype, device=dev)
+        else:
+            if self.rank > 0 and indices is None:
+                raise TorqueError(_lib.TQ_ERROR_INVALID_VALUE, "Source data not allocated!")
+            idx = torch.as_tensor(np.asarray(indices if indices is not None else [0],
+                                             dtype=np.int64)).reshape(-1)
+            n = idx.numel() if self.rank > 0 else 1
+            self.indices = idx.to(dev) if self.rank > 0 else \
+                torch.zeros(0, dtype=torch.int64, device=dev)
+            self.data = torch.as_tensor(source_data).reshape(-1)[:n].to(dev, dtype).clone()
+
+    @property
+    def nnz(self):
+        return int(self.indices.numel())
+
+    def _desc(self):
+        dims = np.ascontiguousarray(self.dimension, np.int64)
+        table = np.ascontiguousarray(self.index_table, np.int64)
+        self._keep = (dims, table)
+        return _lib.SpDesc(self.rank, _i64p(dims), _i64p(table), self.nnz,
+                           ctypes.c_void_p(self.indices.data_ptr() if self.nnz else 0),
+                           ctypes.c_void_p(self.data.data_ptr() if self.nnz else 0))
+
+    def initialize(self, source_data, indices):
+        """src/gpu/sparse.cuh:174-187."""
+        idx = torch.as_tensor(np.asarray(indices, dtype=np.int64)).reshape(-1)
+        self.indices = idx.to(self.data.device)
+        self.data = torch.as_tensor(source_data).reshape(-1)[:idx.numel()].to(
+            self.data.device, self.dtype).clone()
+
+    def _position(self, idx):
+        if idx.size != self.rank:
+            raise TorqueError(_lib.TQ_ERROR_RANK_MISMATCH, "Rank does not match")
+        key = int(np.sum(idx * self.index_table))
+        hit = torch.nonzero(self.indices == key).reshape(-1)
+        return key, (int(hit[0]) if hit.numel() else -1)
+
+    def query(self, indices):
+        """src/gpu/sparse.cuh:240-266: the stored value, 0 if absent."""
+        idx = np.asarray(indices, dtype=np.int64).reshape(-1)
+        if self.rank == 0:
+            if idx.size != 0:
+                raise TorqueError(_lib.TQ_ERROR_RANK_MISMATCH, "Rank does not match")
+            return float(self.data[0])
+        _, pos = self._position(idx)
+        return float(self.data[pos]) if pos >= 0 else 0.0
+
+    def modify(self, indices, number):
+        """src/gpu/sparse.cuh:191-236: overwrite, remove (number == 0, the
+        last element moves into the hole) or append."""
+        idx = np.asarray(indices, dtype=np.int64).reshape(-1)
+        if self.rank == 0:
+            if idx.size != 0:
+                raise TorqueError(_lib.TQ_ERROR_RANK_MISMATCH, "Rank does not match")
+            self.data[0] = number
+            return
+        key, pos = self._position(idx)
+        if pos >= 0:
+            if number == 0:
+                last = self.nnz - 1
+                self.data[pos] = self.data[last]
+                self.indices[pos] = self.indices[last]
+                self.indices = self.indices[:last].clone()
+                self.data = self.data[:last].clone()
+            else:
+                self.data[pos] = number
+        elif number != 0:
+            # appended after the stored values (src/gpu/sparse.cuh:226-227
+            # pushes onto a buffer that still holds the placeholder value of
+            # a tensor built from its dimension alone)
+            dev, old = self.data.device, self.nnz
+            self.indices = torch.cat([self.indices, torch.tensor([key], dtype=torch.int64,
+                                                                 device=dev)])
+            self.data = torch.cat([self.data[:old],
+                                   torch.tensor([number], dtype=self.dtype, device=dev)])
+
+    def to_number(self):
+        assert self.rank == 0
+        return float(self.data[0])
+
+    def soft_transpose(self, permutation):
+        """src/gpu/sparse.cuh:380-392: permutes dimension and index table only."""
+        perm = np.asarray(permutation, dtype=np.int64).reshape(-1)
+        if perm.size != self.rank:
+            raise TorqueError(_lib.TQ_ERROR_RANK_MISMATCH,
+                              "The number of permutation does not match the rank of tensor")
+        self.index_table = self.index_table[perm]
+        self.dimension = self.dimension[perm]
+
+    def hard_transpose(self, permutation):
+        """src/gpu/sparse.cuh:398-432: same values in the same order, indices
+        re-encoded under the compact table of the permuted dimension
+        (tq_sparse_permute_indices; each stored index is decoded, where the
+        reference decodes the storage position, src/gpu/sparse.cuh:420-422)."""
+        perm = np.ascontiguousarray(np.asarray(permutation, dtype=np.int32).reshape(-1))
+        if perm.size != self.rank:
+            raise TorqueError(_lib.TQ_ERROR_RANK_MISMATCH,
+                              "The number of permutation does not match the rank of tensor")
+        _require_gpu(self.data)
+        h = default_handle(self.data.device.index)
+        dims = np.ascontiguousarray(self.dimension, np.int64)
+        table = np.ascontiguousarray(self.index_table, np.int64)
+        out = torch.empty_like(self.indices)
+        h.check(lib().tq_sparse_permute_indices(
+            h.ptr, self.rank, _i64p(dims), _i64p(table), _i32p(perm), self.nnz,
+            ctypes.c_void_p(self.indices.data_ptr() if self.nnz else 0),
+            ctypes.c_void_p(out.data_ptr() if self.nnz else 0)))
+        res = SparseTensor.__new__(SparseTensor)
+        res.rank, res.dtype = self.rank, self.dtype
+        res.dimension = self.dimension[perm.astype(np.int64)]
+        res.index_table = index_table(res.dimension)
+        res.indices = out
+        res.data = self.data.clone()
+        return res
+
+    def contract(self, tensor, contracting_indices):
+        """SparseTensor::contract (src/gpu/sparse.cuh:270-375) through
+        tq_sparse_contract: every output index that receives a product,
+        ascending, with the summed value."""
+        legs = np.ascontiguousarray(np.asarray(contracting_indices, dtype=np.int32).reshape(-1, 2))
+        for a_leg, b_leg in legs:
+            if self.dimension[a_leg] != tensor.dimension[b_leg]:
+                raise TorqueError(_lib.TQ_ERROR_DIMENSION_MISMATCH,
+                                  "The dimensions from two tensors to be contracted do not match")
+        _require_gpu(self.data)
+        _require_gpu(tensor.data)
+        h = default_handle(self.data.device.index)
+        out_rank = self.rank + tensor.rank - 2 * legs.shape[0]
+        out_dims = np.zeros(max(out_rank, 1), np.int64)
+        nnz = ctypes.c_int64()
+        pi, pv = ctypes.c_void_p(), ctypes.c_void_p()
+        da, db = self._desc(), tensor._desc()
+        h.check(lib().tq_sparse_contract(h.ptr, _DT[self.dtype], ctypes.byref(da),
+                                         ctypes.byref(db), legs.shape[0], _i32p(legs),
+                                         _i64p(out_dims), ctypes.byref(nnz),
+                                         ctypes.byref(pi), ctypes.byref(pv)))
+        n = nnz.value
+        dev = self.data.device
+        res = SparseTensor.__new__(SparseTensor)
+        res.rank, res.dtype = out_rank, self.dtype
+        res.dimension = out_dims[:out_rank].copy()
+        res.index_table = index_table(res.dimension)
+        res.indices = torch.empty(n if out_rank > 0 else 0, dtype=torch.int64, device=dev)
+        res.data = torch.empty(max(n, 1) if out_rank == 0 else n, dtype=self.dtype, device=dev)
+        try:
+            if out_rank > 0 and n > 0:
+                h.check(lib().tq_memcpy(h.ptr, ctypes.c_void_p(res.indices.data_ptr()), pi,
+                                        8 * n, 2))
+            if res.data.numel():
+                h.check(lib().tq_memcpy(h.ptr, ctypes.c_void_p(res.data.data_ptr()), pv,
+                                        res.data.element_size() * res.data.numel(), 2))
+            h.synchronize()
+        finally:
+            lib().tq_free(h.ptr, pi)
+            lib().tq_free(h.ptr, pv)
+        return res
+
+
 def dot_f64(x, y):
     h = default_handle(x.device.index)
     r = ctypes.c_double()
