@@ -55,7 +55,9 @@ def parse():
     p.add_argument("--cpu-sample-blocks", type=int, default=0,
                    help="E blocks in the CPU-oracle sample (0 = all)")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--workload", default="peps", choices=["peps", "batched"],
+    p.add_argument("--no-permute", action="store_true",
+                   help="skip the standalone 200M-element rank-6 permutation")
+    p.add_argument("--workload", default="peps", choices=["peps", "batched", "permute"],
                    help="peps: the D, chi absorption step (default); batched: --n-batch "
                         "independent rank-4 x rank-4 contractions in one captured launch")
     p.add_argument("--n-batch", type=int, default=512)
@@ -204,6 +206,64 @@ def bench_batched(args):
     print(json.dumps(line), flush=True)
 
 
+PERMUTE_DIMS = (10, 20, 25, 40, 50, 20)  # 200,000,000 elements
+PERMUTE_PERMS = ((5, 4, 3, 2, 1, 0), (1, 0, 2, 3, 4, 5), (0, 2, 1, 3, 4, 5),
+                 (2, 3, 4, 5, 0, 1), (3, 5, 0, 4, 2, 1))
+
+
+def bench_permute(dev, reps=10, dims=PERMUTE_DIMS, perms=PERMUTE_PERMS):
+    """Standalone rank-6 permutation of a 200M-element FP64 tensor
+    (tq_permute), per permutation: kernel time from HIP events on the
+    handle's stream, algorithmic bytes 2 * 8 * N per launch, and a bitwise
+    round trip (permute, inverse permute) over the whole tensor."""
+    import ctypes
+
+    import torch
+
+    from torque_amd import default_handle, lib
+
+    n = int(np.prod(dims))
+    x = torch.randn(n, dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    h = default_handle(dev.index)
+    stream = torch.cuda.current_stream(dev)
+    out = []
+    for perm in perms:
+        d = np.ascontiguousarray(dims, np.int64)
+        p = np.ascontiguousarray(perm, np.int32)
+        inv = np.ascontiguousarray(np.argsort(perm), np.int32)
+        d2 = np.ascontiguousarray(np.asarray(dims)[list(perm)], np.int64)
+
+        def run(src, dst, dd, pp):
+            h.check(lib().tq_permute(h.ptr, 1, len(dims), dd.ctypes.data_as(ctypes.POINTER(
+                ctypes.c_int64)), None, pp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr())))
+
+        run(x, y, d, p)
+        run(y, z, d2, inv)
+        torch.cuda.synchronize()
+        exact = bool(torch.equal(x, z))
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            run(x, y, d, p)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        gbs = 2 * 8 * n / (ms * 1e-3) / 1e9
+        out.append({"perm": list(perm), "ms": round(ms, 4), "gbs": round(gbs, 1),
+                    "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "round_trip_bitwise": exact})
+        print("# permute %s: %.3f ms, %.0f GB/s (%.1f%% of HBM), round trip %s"
+              % (list(perm), ms, gbs, 100 * gbs / HBM_PEAK_GBS, "exact" if exact else "DIFFERS"),
+              flush=True)
+    del x, y, z
+    torch.cuda.empty_cache()
+    return {"dims": list(dims), "elements": n, "dtype": "f64",
+            "algorithmic_bytes_per_launch": 16 * n, "perms": out}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -212,6 +272,12 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "permute":
+        import torch
+        dev = torch.device("cuda:0")
+        torch.cuda.set_device(dev)
+        print(json.dumps(bench_permute(dev)), flush=True)
+        return
     import torch
 
     from torque_amd import BlockSparseTensor, TorqueError
@@ -375,6 +441,7 @@ def main():
         print("# %s step %.3f ms, %.1f GFLOP/s, rel err %.2e" %
               (tier, tdt * 1e3, flops / tdt / 1e9, rel(got)), flush=True)
 
+    permute = None if args.no_permute else bench_permute(dev)
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "GFLOP/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 4),
@@ -391,6 +458,7 @@ def main():
         "cpu_baseline": cpu_baseline,
         "tiers": tiers,
         "fp64_rel_err_vs_oracle": err64,
+        "permute_rank6_200M": permute,
     }
     print(json.dumps(line), flush=True)
     if args.copy_calibration:

@@ -148,6 +148,18 @@ tq_status tq_bs_permute(tq_handle_t handle, tq_dtype dtype,
                         const tq_bs_desc *in_desc, const int32_t *perm,
                         const void *in, void *out, int64_t *out_offsets);
 
+/* Splits every block into prod(divisor) equal sub-blocks (block extent on
+ * leg l divided by divisor[l]; a remainder is an error, "current slice does
+ * not support residues"). Sub-block j of block b, with j decoded leg 0
+ * fastest over the divisor, becomes output block b * prod(divisor) + j; its
+ * begin/end points go to out_begin/out_end (host, block-major, may be NULL)
+ * and its data is the compact column-major copy of that region of block b.
+ * Replaces BlockSparseTensor::slice (src/gpu/block_sparse.cuh:1124-1176),
+ * which copies the data buffer unchanged under the new block list. */
+tq_status tq_bs_slice(tq_handle_t handle, tq_dtype dtype, const tq_bs_desc *in_desc,
+                      const int64_t *divisor, const void *in, void *out,
+                      int64_t *out_begin, int64_t *out_end);
+
 /* ---- block-sparse contraction -------------------------------------- */
 
 /* Plan C = contract(A, B, legs): legs is n_legs x 2, row k = (leg of A,

@@ -153,6 +153,18 @@ static void block_sparse_sections(tq_handle_t handle) {
   }
 }
 
+static void slice_section() {
+  // slice as the reference's benchmarks use it (block_sparse_benchmark.cu:73-83):
+  // the sliced tensor answers every query as before
+  std::vector<float> a(16);
+  for (int i = 0; i < 16; ++i) a[i] = (float)(i + 1);
+  BST t(a.data(), uvec{0, 0}, uvec{3, 3}, uvec{4, 4});
+  const auto s = t.slice({2, 2});
+  CHECK(s.n_blocks() == 4);
+  for (uword i = 0; i < 4; ++i)
+    for (uword j = 0; j < 4; ++j) CHECK(s.query({i, j}) == t.query({i, j}));
+}
+
 static void dense_sections() {
   {  // dense matrix initialization
     DT t(uvec{4, 3});
@@ -308,6 +320,7 @@ int main() {
     block_sparse_sections(handle);
     dense_sections();
     sparse_sections(handle);
+    slice_section();
   } catch (const std::exception &e) {
     std::printf("FAILED with exception: %s\n", e.what());
     ++g_fail;

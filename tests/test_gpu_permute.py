@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import O, permuted_strides, random_blocks, tensor, torch_tensor
+from helpers import O, W, permuted_strides, random_blocks, tensor, torch_tensor
 from torque_amd import BlockSparseTensor, DenseTensor
 
 pytestmark = pytest.mark.gpu
@@ -107,3 +107,74 @@ def test_block_sparse_permute_large_blocks(cuda):
     out = gt.hard_transpose([0, 3, 1, 2])
     ref = O.bs_permute(t, E.data, [0, 3, 1, 2])
     assert np.array_equal(out.data.cpu().numpy()[: ref.size], ref)
+
+
+# ---- slice (src/gpu/block_sparse.cuh:1124-1176) ---------------------------
+
+def expected_sub_blocks(begin, end, divisor):
+    """Sub-block begin/end points in the order of the reference's loop:
+    block-major, sub-block j decoded leg 0 fastest over the divisor."""
+    divisor = np.asarray(divisor)
+    n_sub = int(np.prod(divisor))
+    bs, es = [], []
+    for b, e in zip(begin, end):
+        sub = (e - b + 1) // divisor
+        for j in range(n_sub):
+            q = np.array(np.unravel_index(j, tuple(divisor), order="F"))
+            bs.append(b + q * sub)
+            es.append(b + q * sub + sub - 1)
+    return np.array(bs, np.int64), np.array(es, np.int64)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_slice_keeps_values(cuda, seed):
+    rng = np.random.default_rng(40 + seed)
+    dims = [12, 8, 6]
+    # block extents divisible by the divisor (2, 2, 3); overlapping blocks and
+    # permuted index tables on the second seed
+    n = 5
+    ext = np.array([[2, 2, 3], [4, 2, 6], [6, 4, 3], [2, 8, 6], [12, 4, 3]], np.int64)[:n]
+    begin = np.array([[int(rng.integers(0, d - x + 1)) for d, x in zip(dims, row)]
+                      for row in ext], np.int64)
+    end = begin + ext - 1
+    strides = permuted_strides(begin, end, rng) if seed else None
+    t, d = tensor(dims, begin, end, rng, strides)
+    g = torch_tensor(t, d, cuda)
+    s = g.slice([2, 2, 3])
+    eb, ee = expected_sub_blocks(begin, end, [2, 2, 3])
+    assert np.array_equal(s.begin_points, eb) and np.array_equal(s.end_points, ee)
+    # every sub-block holds the values of its region: same dense image, and
+    # with compact tables each sub-block equals the parent's region
+    so = O.BS(dims, s.begin_points, s.end_points)
+    got = s.data.cpu().numpy()
+    assert np.array_equal(O.densify(so, got), O.densify(t, d))
+    parent = O.densify(O.BS(dims, begin[:1], end[:1], None if strides is None else strides[:1]),
+                       d[:int(np.prod(ext[0]))]).reshape(dims, order="F")
+    for j in range(12):
+        b, e = eb[j], ee[j]
+        region = parent[b[0]:e[0] + 1, b[1]:e[1] + 1, b[2]:e[2] + 1]
+        off = int(s.block_offsets[j])
+        assert np.array_equal(got[off:off + region.size], region.reshape(-1, order="F"))
+
+
+def test_slice_then_contract(cuda):
+    # block_sparse_benchmark.cu:73-83: contracting sliced tensors gives the
+    # same dense result
+    a, b, legs = W.c1(3)
+    ga = torch_tensor(O.BS(a.dims, a.begin, a.end), a.data, cuda)
+    gb = torch_tensor(O.BS(b.dims, b.begin, b.end), b.data, cuda)
+    ref = ga.contract(gb, legs)
+    sa, sb = ga.slice([2, 2, 2, 2]), gb.slice([2, 2, 2, 2])
+    got = sa.contract(sb, legs)
+    dref = O.densify(O.BS(ref.dimension, ref.begin_points, ref.end_points),
+                     ref.data.cpu().numpy())
+    dgot = O.densify(O.BS(got.dimension, got.begin_points, got.end_points),
+                     got.data.cpu().numpy())
+    assert np.abs(dgot - dref).max() <= 1e-12 * np.abs(dref).max()
+
+
+def test_slice_residue_error(cuda):
+    from torque_amd import TorqueError
+    t = BlockSparseTensor([6, 6], np.arange(9.0), [[0, 0]], [[2, 2]], device=cuda)
+    with pytest.raises(TorqueError, match="residues"):
+        t.slice([2, 1])

@@ -81,6 +81,7 @@ def _proto(lib):
         "tq_memset_zero": (S, [H, V, ctypes.c_size_t]),
         "tq_permute": (S, [H, S, ctypes.c_int32, c_i64_p, c_i64_p, c_i32_p, V, V]),
         "tq_bs_permute": (S, [H, S, ctypes.POINTER(BsDesc), c_i32_p, V, V, c_i64_p]),
+        "tq_bs_slice": (S, [H, S, ctypes.POINTER(BsDesc), c_i64_p, V, V, c_i64_p, c_i64_p]),
         "tq_bs_contract_plan": (S, [H, ctypes.POINTER(BsDesc), ctypes.POINTER(BsDesc),
                                     ctypes.c_int32, c_i32_p, S, S, ctypes.POINTER(V)]),
         "tq_plan_get_info": (S, [V, ctypes.POINTER(PlanInfo)]),

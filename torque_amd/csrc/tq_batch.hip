@@ -68,7 +68,7 @@ tq_status merge_side(tq_handle_t h, tq_plan_s &out, bool is_b, int32_t n,
   int64_t total = 0;
   for (size_t q = 0; q < jobs.size(); ++q) {
     wg[q] = (int32_t)total;
-    total += ceil_div(jobs[q].n_elem, kGatherPerWG);
+    total += pack_job_wgs(jobs[q]);
   }
   if (total > 0x7fffffff)
     return tq_fail(h, TQ_ERROR_UNSUPPORTED, "batch: pack too large");

@@ -81,9 +81,12 @@ __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) {
 // where e is the column-major linear index over dims (leg 0 fastest).
 struct GemmDesc;
 
-// For the split-BF16 pack (kdim > 0) the leading legs multiply to kdim and
-// element e lands at dst_off + (e / kdim) * kpad + e % kdim of each of the
-// three BF16 planes.
+// For the split-BF16 pack (kdim > 0) the first nk legs are the contracted
+// ones (they multiply to kdim) and the rest the rows (n_elem / kdim of
+// them); each of the three BF16 planes is k-chunked: value (row, k) lands at
+// dst_off + (k / 8) * rows * 8 + row * 8 + k % 8, with k padded with zeros
+// to kpad (a multiple of 8), so the 8 values of one row in one chunk form a
+// 16-byte vector.
 struct CopyJob {
   i64 src_off;
   i64 dst_off;
@@ -91,10 +94,16 @@ struct CopyJob {
   int32_t rank;
   int32_t kdim;
   int32_t kpad;
-  int32_t pad;
+  int32_t nk;
   int32_t dims[TQ_MAX_RANK];
   i64 strides[TQ_MAX_RANK];
 };
+
+// Workgroups of the pack kernels per job: kGatherPerWG elements each for a
+// plain gather, one tile of 64 rows x 32 padded k for a split-BF16 job.
+constexpr int kSplitRows = 64;
+constexpr int kSplitK = 32;
+__host__ __device__ inline int64_t pack_job_wgs(const CopyJob &j);
 
 // Split-BF16 pack: like tq_launch_gather_jobs, reading FP64 and writing the
 // three BF16 pieces of each value to planes dst, dst + plane, dst + 2*plane.
@@ -115,6 +124,12 @@ tq_status tq_launch_gather_jobs(tq_handle_t h, tq_dtype dt, const CopyJob *jobs,
                                 int64_t total_wg, const void *src, void *dst);
 // Elements each workgroup of the gather kernel covers.
 constexpr int kGatherPerWG = 2048;
+
+__host__ __device__ inline int64_t pack_job_wgs(const CopyJob &j) {
+  if (j.n_elem <= 0) return 0;
+  if (j.kdim <= 0) return ceil_div(j.n_elem, kGatherPerWG);
+  return ceil_div(j.n_elem / j.kdim, kSplitRows) * ceil_div(j.kpad, kSplitK);
+}
 
 // ---- grouped GEMM (tq_gemm.hip) ----------------------------------------
 

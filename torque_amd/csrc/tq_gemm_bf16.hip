@@ -1,16 +1,24 @@
 // tq_gemm_bf16.hip -- the split-BF16 tier of the grouped GEMM.
 //
-// Each FP64 operand value x is stored as three BF16 pieces hi + mid + lo
-// (planes written by the split pack, k fastest, rows padded to multiples of
-// 8). The product keeps the six piece products of order >= 2^-16 relative:
-//   hi*hi + hi*mid + mid*hi + hi*lo + mid*mid + lo*hi
+// Each FP64 operand value x is stored as three BF16 pieces hi + mid + lo in
+// three planes written by the split pack (tq_permute.hip). A panel of R rows
+// is k-chunked: value (row, k) of a plane sits at (k / 8) * R * 8 + row * 8 +
+// k % 8, so the 8 values of one row in one chunk are one 16-byte vector and
+// consecutive rows are consecutive vectors. The product keeps the six piece
+// products of order >= 2^-16 relative,
+//   lo*hi + mid*mid + hi*lo + mid*hi + hi*mid + hi*hi   (smallest first),
 // on v_mfma_f32_16x16x32_bf16 with FP32 accumulation; the dropped terms
 // (mid*lo, lo*mid, lo*lo) are below 2^-23 of |a||b| and the pieces represent
-// each operand to 2^-24 relative, so the error per product term is about
-// 2^-22 |a||b| plus FP32 accumulation. Results are written as FP64.
+// each operand to 2^-24 relative. Results are written as FP64.
 //
-// Same work list, tile mapping and persistence as the FP64 kernel
-// (tq_gemm.hip); operands are always the packed split panels.
+// Schedule: persistent workgroups of four wavefronts walk the same tile list
+// as the FP64 kernel (tq_gemm.hip). A k step covers 32 values (four chunks):
+// the workgroup stages the step's A and B vectors of all three planes in LDS
+// ([plane][chunk][row] vectors, so the 16 lanes reading one MFMA operand read
+// 256 contiguous bytes), and the vectors of the next step (or of the next
+// tile's first step) are loaded into registers while the current step's
+// MFMAs run. The six piece products are six passes over the wavefront's
+// register tile, so consecutive MFMAs never depend on each other.
 #include "tq_gemm_common.h"
 
 namespace {
@@ -19,34 +27,71 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 using tqg::kThreads;
-constexpr int kBK = 32;
-constexpr int kLDK = kBK + 8;  // LDS row stride in BF16 elements (80 bytes)
+constexpr int kBK = 32;     // k per step
+constexpr int kChunks = 4;  // 8-value chunks per step
 
-// Loads a ROWS x 32 tile of the three planes into LDS (16-byte chunks:
-// thread t covers row t / 4, k chunk (t % 4) * 8, stepping 64 rows).
+// One operand's vectors of one k step: 3 planes x 4 chunks x ROWS rows of
+// 16 bytes; vector v of the step is (row v % ROWS, chunk (v / ROWS) % 4,
+// plane v / (4 ROWS)), so consecutive threads read consecutive rows.
 template <int ROWS>
-__device__ inline void load_planes(const uint16_t *__restrict__ base, int64_t plane,
-                                   int ld, int rows, int row0, int kpad, int k0,
-                                   uint4 *lds) {
-  const int t = threadIdx.x;
-  const int kc = (t & 3) * 8;
+struct StepLoader {
+  static constexpr int kVec = 3 * kChunks * ROWS;
+  static constexpr int kN = (kVec + kThreads - 1) / kThreads;
+  uint4 v[kN];
+
+  // base: plane 0 of the panel; rows R of the panel; rows past R are read
+  // from row R - 1 (their products only reach unstored output); chunks past
+  // the padded K are zeros.
+  __device__ inline void load(const uint16_t *__restrict__ base, int64_t plane, int rows,
+                              int row0, int n_chunks, int chunk0) {
 #pragma unroll
-  for (int r = t >> 2; r < ROWS; r += kThreads / 4) {
-    const bool ok = (row0 + r < rows) && (k0 + kc < kpad);
-    const uint16_t *p = base + (int64_t)(row0 + r) * ld + k0 + kc;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (ok) v = *reinterpret_cast<const uint4 *>(p + q * plane);
-      lds[(q * ROWS * kLDK + r * kLDK + kc) / 8] = v;
+    for (int q = 0; q < kN; ++q) {
+      const int e = threadIdx.x + q * kThreads;
+      const int r = e % ROWS;
+      const int c = (e / ROWS) % kChunks;
+      const int p = e / (ROWS * kChunks);
+      const int row = min(row0 + r, rows - 1);
+      const int chunk = chunk0 + c;
+      uint4 x = make_uint4(0, 0, 0, 0);
+      if (e < kVec && chunk < n_chunks)
+        x = *reinterpret_cast<const uint4 *>(base + p * plane +
+                                             ((int64_t)chunk * rows + row) * 8);
+      v[q] = x;
     }
   }
+
+  __device__ inline void store(uint4 *lds) const {
+#pragma unroll
+    for (int q = 0; q < kN; ++q) {
+      const int e = threadIdx.x + q * kThreads;
+      if (e < kVec) lds[e] = v[q];
+    }
+  }
+};
+
+struct TileRef {
+  GemmDesc d;
+  int m0, n0;
+};
+
+template <int BM, int BN>
+__device__ inline TileRef fetch(int64_t t, const GemmDesc *__restrict__ descs,
+                                const int32_t *__restrict__ tile_pair,
+                                const int32_t *__restrict__ tile_first) {
+  TileRef r;
+  const int p = tile_pair[t];
+  r.d = descs[p];
+  const int lt = (int)(t - tile_first[p]);
+  const int tiles_m = (r.d.m + BM - 1) / BM;
+  r.m0 = (lt % tiles_m) * BM;
+  r.n0 = (lt / tiles_m) * BN;
+  return r;
 }
 
 template <int BM, int BN>
 struct Smem {
-  static constexpr size_t kAB = 2 * 3 * (size_t)(BM + BN) * kLDK;
-  static constexpr size_t kStage = 8 * 16 * (size_t)(BM + 1);
+  static constexpr size_t kAB = sizeof(uint4) * 3 * kChunks * (size_t)(BM + BN);
+  static constexpr size_t kStage = sizeof(double) * 16 * (size_t)(BM + 1);
   static constexpr size_t kBytes = kAB > kStage ? kAB : kStage;
 };
 
@@ -62,70 +107,78 @@ __global__ __launch_bounds__(kThreads) void bf16x3_gemm_kernel(
   static_assert(TM >= 1 && TN >= 1, "tile shape");
   __shared__ __attribute__((aligned(16))) char smem[Smem<BM, BN>::kBytes];
   uint4 *As = reinterpret_cast<uint4 *>(smem);
-  uint4 *Bs = As + 3 * BM * kLDK / 8;
+  uint4 *Bs = As + 3 * kChunks * BM;
   double *stage = reinterpret_cast<double *>(smem);
-  const __bf16 *Ab = reinterpret_cast<const __bf16 *>(As);
-  const __bf16 *Bb = reinterpret_cast<const __bf16 *>(Bs);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
-  const int64_t vid = tqg::xcd_virtual_id(blockIdx.x, gridDim.x);
+  const int lrow = lane & 15, lchunk = lane >> 4;
+  // A-plane / B-plane of each piece product, smallest terms first
+  constexpr int kPa[6] = {2, 1, 0, 1, 0, 0};
+  constexpr int kPb[6] = {0, 1, 2, 0, 1, 0};
 
-  for (int64_t t = vid; t < n_tiles; t += gridDim.x) {
-    const int p = tile_pair[t];
-    const GemmDesc d = descs[p];
-    const int lt = (int)(t - tile_first[p]);
-    const int tiles_m = (d.m + BM - 1) / BM;
-    const int m0 = (lt % tiles_m) * BM;
-    const int n0 = (lt / tiles_m) * BN;
-    const int kpad = (d.k + 7) & ~7;
-    const uint16_t *pa = a + d.a_off;
-    const uint16_t *pb = b + d.b_off;
-    const int tm_live = min(TM, max(0, (d.m - m0 - wm0 + 15) / 16));
-    const int tn_live = min(TN, max(0, (d.n - n0 - wn0 + 15) / 16));
+  int64_t t = tqg::xcd_virtual_id(blockIdx.x, gridDim.x);
+  if (t >= n_tiles) return;
+  TileRef cur = fetch<BM, BN>(t, descs, tile_pair, tile_first);
+  StepLoader<BM> la;
+  StepLoader<BN> lb;
+  la.load(a + cur.d.a_off, a_plane, cur.d.m, cur.m0, cur.d.lda >> 3, 0);
+  lb.load(b + cur.d.b_off, b_plane, cur.d.n, cur.n0, cur.d.ldb >> 3, 0);
+  while (true) {
+    const GemmDesc &d = cur.d;
+    // panels store kpad = lda (A) / ldb (B) values per row: kpad / 8 chunks
+    const int n_chunks = d.lda >> 3, n_chunks_b = d.ldb >> 3;
+    const int tm_live = min(TM, max(0, (d.m - cur.m0 - wm0 + 15) / 16));
+    const int tn_live = min(TN, max(0, (d.n - cur.n0 - wn0 + 15) / 16));
     f4 acc[TM][TN];
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-    for (int k0 = 0; k0 < kpad; k0 += kBK) {
+    const int64_t t_next = t + gridDim.x;
+    TileRef nxt = cur;
+    for (int ch0 = 0; ch0 < n_chunks; ch0 += kChunks) {
+      __syncthreads();  // the previous step (or epilogue staging) is read
+      la.store(As);
+      lb.store(Bs);
       __syncthreads();
-      load_planes<BM>(pa, a_plane, d.lda, d.m, m0, kpad, k0, As);
-      load_planes<BN>(pb, b_plane, d.ldb, d.n, n0, kpad, k0, Bs);
-      __syncthreads();
-      const int kk = 8 * (lane >> 4);
-      bf16x8 ah[TM], am[TM], al[TM];
+      if (ch0 + kChunks < n_chunks) {
+        la.load(a + d.a_off, a_plane, d.m, cur.m0, n_chunks, ch0 + kChunks);
+        lb.load(b + d.b_off, b_plane, d.n, cur.n0, n_chunks_b, ch0 + kChunks);
+      } else if (t_next < n_tiles) {
+        // last step: the next tile's first step loads under these MFMAs and
+        // the epilogue
+        nxt = fetch<BM, BN>(t_next, descs, tile_pair, tile_first);
+        la.load(a + nxt.d.a_off, a_plane, nxt.d.m, nxt.m0, nxt.d.lda >> 3, 0);
+        lb.load(b + nxt.d.b_off, b_plane, nxt.d.n, nxt.n0, nxt.d.ldb >> 3, 0);
+      }
+      bf16x8 af[3][TM], bfr[3][TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        const int r = (wm0 + i * 16 + (lane & 15)) * kLDK + kk;
-        ah[i] = *reinterpret_cast<const bf16x8 *>(Ab + r);
-        am[i] = *reinterpret_cast<const bf16x8 *>(Ab + BM * kLDK + r);
-        al[i] = *reinterpret_cast<const bf16x8 *>(Ab + 2 * BM * kLDK + r);
+      for (int p = 0; p < 3; ++p) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+          af[p][i] = *reinterpret_cast<const bf16x8 *>(
+              &As[(p * kChunks + lchunk) * BM + wm0 + 16 * i + lrow]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          bfr[p][j] = *reinterpret_cast<const bf16x8 *>(
+              &Bs[(p * kChunks + lchunk) * BN + wn0 + 16 * j + lrow]);
       }
 #pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        if (j >= tn_live) continue;
-        const int r = (wn0 + j * 16 + (lane & 15)) * kLDK + kk;
-        const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(Bb + r);
-        const bf16x8 bm = *reinterpret_cast<const bf16x8 *>(Bb + BN * kLDK + r);
-        const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(Bb + 2 * BN * kLDK + r);
+      for (int q = 0; q < 6; ++q)
 #pragma unroll
-        for (int i = 0; i < TM; ++i) {
-          if (i >= tm_live) continue;
-          f4 x = acc[i][j];
-          // smallest terms first
-          x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, x, 0, 0, 0);
-          x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[i], bm, x, 0, 0, 0);
-          x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, x, 0, 0, 0);
-          x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[i], bh, x, 0, 0, 0);
-          x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bm, x, 0, 0, 0);
-          x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, x, 0, 0, 0);
-          acc[i][j] = x;
-        }
-      }
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            if (i < tm_live && j < tn_live)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[kPa[q]][i], bfr[kPb[q]][j],
+                                                                  acc[i][j], 0, 0, 0);
     }
-    tqg::store_tile<double, false, BM, BN, WAVES_M, TM, TN>(acc, c, d, m0, n0, stage);
+    tqg::store_tile<double, false, BM, BN, WAVES_M, TM, TN>(acc, c, d, cur.m0, cur.n0, stage);
+    if (t_next >= n_tiles) break;
+    t = t_next;
+    cur = nxt;
   }
 }
 
@@ -146,8 +199,9 @@ tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *tile_pair,
   static int per_cu = 0;
   if (per_cu == 0) {
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (bf16x3_gemm_kernel<BM, BN, WAVES_M>), kThreads, 0) !=
-            hipSuccess || occ < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (bf16x3_gemm_kernel<BM, BN, WAVES_M>),
+                                                     kThreads, 0) != hipSuccess ||
+        occ < 1)
       occ = 1;
     per_cu = occ;
   }
@@ -185,7 +239,7 @@ tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, const GemmDesc *descs,
     TQ_CFG(1, 128, 64, 2)
     TQ_CFG(2, 64, 128, 2)
     TQ_CFG(3, 128, 128, 2)
-    TQ_CFG(4, 256, 32, 2)
+    TQ_CFG(4, 256, 32, 4)
     TQ_CFG(5, 32, 32, 2)
     TQ_CFG(6, 128, 16, 4)
     TQ_CFG(7, 256, 16, 4)

@@ -368,6 +368,11 @@ __device__ inline double bf16_value(uint16_t h) {
   return (double)__uint_as_float((uint32_t)h << 16);
 }
 
+// One workgroup per tile of 64 rows x 32 padded k of a job: thread t takes
+// row t % 64 and chunk t / 64 of the tile, reads its 8 values (lanes run
+// along the rows, so the reads coalesce when a row leg is fastest in memory),
+// splits them and writes one 16-byte vector to each plane (a wavefront
+// writes 1 KiB of contiguous output per plane). Values past kdim are zeros.
 __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
     const CopyJob *__restrict__ jobs, const int32_t *__restrict__ job_wg_start,
     int64_t n_jobs, const double *__restrict__ src, uint16_t *__restrict__ dst,
@@ -383,32 +388,78 @@ __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
         hi = mid - 1;
     }
     const CopyJob &job = jobs[lo];
-    const int64_t e0 = (int64_t)(c - job_wg_start[lo]) * kGatherPerWG;
-    const uint32_t kd = (uint32_t)job.kdim;
+    const int nk = job.nk;
+    const int64_t kd = job.kdim;
+    const int64_t rows = job.n_elem / kd;
+    const int64_t k_tiles = ceil_div(job.kpad, kSplitK);
+    const int64_t tile = c - job_wg_start[lo];
+    const int64_t row = (tile / k_tiles) * kSplitRows + (threadIdx.x & (kSplitRows - 1));
+    const int64_t chunk = (tile % k_tiles) * (kSplitK / 8) + (threadIdx.x >> 6);
+    if (row >= rows || chunk * 8 >= job.kpad) continue;  // no barrier in this loop
+    // source offset of (row, k = 8 * chunk): rows decode over legs nk.., k
+    // over legs 0..nk-1 (leg 0 fastest); k digits advance with carries
+    i64 off = job.src_off;
+    uint32_t rem = (uint32_t)row;
 #pragma unroll
-    for (int q = 0; q < kPerThread; ++q) {
-      const int64_t e = e0 + (int64_t)q * kGatherThreads + threadIdx.x;
-      if (e >= job.n_elem) break;
-      uint32_t rem = (uint32_t)e;
-      i64 off = job.src_off;
-      for (int l = 0; l < job.rank; ++l) {
-        const uint32_t d = (uint32_t)job.dims[l];
-        const uint32_t qd = rem / d;
-        off += (i64)(rem - qd * d) * job.strides[l];
-        rem = qd;
-      }
-      const uint32_t row = (uint32_t)e / kd;
-      const i64 o = job.dst_off + (i64)row * job.kpad + ((uint32_t)e - row * kd);
-      const double x = src[off];
-      const uint16_t h = bf16_rne((float)x);
-      const double r1 = x - bf16_value(h);
-      const uint16_t m = bf16_rne((float)r1);
-      const double r2 = r1 - bf16_value(m);
-      const uint16_t l = bf16_rne((float)r2);
-      dst[o] = h;
-      dst[plane + o] = m;
-      dst[2 * plane + o] = l;
+    for (int l = 0; l < TQ_MAX_RANK; ++l) {
+      if (l < nk || l >= job.rank) continue;
+      const uint32_t d = (uint32_t)job.dims[l];
+      const uint32_t q = rem / d;
+      off += (i64)(rem - q * d) * job.strides[l];
+      rem = q;
     }
+    int32_t dig[TQ_MAX_RANK];
+    rem = (uint32_t)(chunk * 8);
+#pragma unroll
+    for (int l = 0; l < TQ_MAX_RANK; ++l) {
+      dig[l] = 0;
+      if (l >= nk) continue;
+      const uint32_t d = (uint32_t)job.dims[l];
+      const uint32_t q = rem / d;
+      dig[l] = (int32_t)(rem - q * d);
+      off += (i64)dig[l] * job.strides[l];
+      rem = q;
+    }
+    uint16_t h[8], m[8], w[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const double x = (chunk * 8 + e < kd) ? src[off] : 0.0;
+      h[e] = bf16_rne((float)x);
+      const double r1 = x - bf16_value(h[e]);
+      m[e] = bf16_rne((float)r1);
+      const double r2 = r1 - bf16_value(m[e]);
+      w[e] = bf16_rne((float)r2);
+      // next k: carry through the contracted legs
+      bool carry = true;
+#pragma unroll
+      for (int l = 0; l < TQ_MAX_RANK; ++l) {
+        if (!carry || l >= nk) continue;
+        off += job.strides[l];
+        if (++dig[l] < job.dims[l]) {
+          carry = false;
+        } else {
+          off -= (i64)job.dims[l] * job.strides[l];
+          dig[l] = 0;
+        }
+      }
+    }
+    const int64_t o = job.dst_off + chunk * rows * 8 + row * 8;
+    uint4 vh, vm, vw;
+    vh.x = h[0] | ((uint32_t)h[1] << 16);
+    vh.y = h[2] | ((uint32_t)h[3] << 16);
+    vh.z = h[4] | ((uint32_t)h[5] << 16);
+    vh.w = h[6] | ((uint32_t)h[7] << 16);
+    vm.x = m[0] | ((uint32_t)m[1] << 16);
+    vm.y = m[2] | ((uint32_t)m[3] << 16);
+    vm.z = m[4] | ((uint32_t)m[5] << 16);
+    vm.w = m[6] | ((uint32_t)m[7] << 16);
+    vw.x = w[0] | ((uint32_t)w[1] << 16);
+    vw.y = w[2] | ((uint32_t)w[3] << 16);
+    vw.z = w[4] | ((uint32_t)w[5] << 16);
+    vw.w = w[6] | ((uint32_t)w[7] << 16);
+    *reinterpret_cast<uint4 *>(dst + o) = vh;
+    *reinterpret_cast<uint4 *>(dst + plane + o) = vm;
+    *reinterpret_cast<uint4 *>(dst + 2 * plane + o) = vw;
   }
 }
 
@@ -553,6 +604,82 @@ tq_status tq_bs_permute(tq_handle_t h, tq_dtype dtype, const tq_bs_desc *d,
   TQ_CHECK(tq_launch_gather_jobs(h, dtype, djobs.as<CopyJob>(),
                                  dwg32.as<int32_t>(), nb, total_elem, in, out));
   // the temporaries are freed on return: wait for the kernels using them
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  return TQ_SUCCESS;
+}
+
+tq_status tq_bs_slice(tq_handle_t h, tq_dtype dtype, const tq_bs_desc *d,
+                      const int64_t *divisor, const void *in, void *out,
+                      int64_t *out_begin, int64_t *out_end) {
+  if (!h) return TQ_ERROR_NOT_INITIALIZED;
+  if (!d || !divisor) return TQ_ERROR_INVALID_VALUE;
+  const int r = d->rank;
+  if (r < 1 || r > TQ_MAX_RANK) return TQ_ERROR_UNSUPPORTED;
+  if (dtype != TQ_F32 && dtype != TQ_F64) return TQ_ERROR_UNSUPPORTED;
+  const i64 nb = d->n_blocks;
+  if (nb > 0 && (!d->begin || !d->end || !d->strides || !d->offsets)) return TQ_ERROR_INVALID_VALUE;
+  i64 n_sub = 1;
+  for (int l = 0; l < r; ++l) {
+    if (divisor[l] < 1) return TQ_ERROR_INVALID_VALUE;
+    n_sub *= divisor[l];
+  }
+  // sub-block extents; every block extent must divide evenly
+  // (src/gpu/block_sparse.cuh:1126-1141)
+  for (i64 b = 0; b < nb; ++b)
+    for (int l = 0; l < r; ++l)
+      if ((d->end[b * r + l] - d->begin[b * r + l] + 1) % divisor[l] != 0)
+        return tq_fail(h, TQ_ERROR_INVALID_VALUE, "current slice does not support residues");
+  const i64 n_jobs = nb * n_sub;
+  if (n_jobs == 0) return TQ_SUCCESS;
+  if (!in || !out) return TQ_ERROR_INVALID_VALUE;
+  // one job per sub-block: sub-block j of block b (j decoded leg 0 fastest
+  // over the divisor, :1155-1163) is copied compact, in order, from its
+  // region of the parent block (read with the parent's index table)
+  const size_t job_bytes = sizeof(CopyJob) * n_jobs;
+  const size_t wg_off = (job_bytes + 15) / 16 * 16;
+  const size_t bytes = wg_off + sizeof(int32_t) * (n_jobs + 1);
+  void *host = nullptr;
+  TQ_CHECK(tq_stage(h, bytes, &host));
+  CopyJob *jobs = reinterpret_cast<CopyJob *>(host);
+  int32_t *wg = reinterpret_cast<int32_t *>(reinterpret_cast<char *>(host) + wg_off);
+  i64 dst = 0, wg_total = 0;
+  for (i64 b = 0; b < nb; ++b) {
+    i64 sub[TQ_MAX_RANK];
+    for (int l = 0; l < r; ++l) sub[l] = (d->end[b * r + l] - d->begin[b * r + l] + 1) / divisor[l];
+    for (i64 j = 0; j < n_sub; ++j) {
+      CopyJob &jb = jobs[b * n_sub + j];
+      jb = CopyJob{};
+      i64 rem = j, src = d->offsets[b], n = 1;
+      for (int l = 0; l < r; ++l) {
+        const i64 q = rem % divisor[l];
+        rem /= divisor[l];
+        const i64 lo = q * sub[l];
+        src += lo * d->strides[b * r + l];
+        if (out_begin) out_begin[(b * n_sub + j) * r + l] = d->begin[b * r + l] + lo;
+        if (out_end) out_end[(b * n_sub + j) * r + l] = d->begin[b * r + l] + lo + sub[l] - 1;
+        jb.dims[l] = (int32_t)sub[l];
+        jb.strides[l] = d->strides[b * r + l];
+        n *= sub[l];
+      }
+      if (n >= (i64(1) << 31))
+        return tq_fail(h, TQ_ERROR_UNSUPPORTED, "slice: sub-block above 2^31 elements");
+      jb.rank = r;
+      jb.src_off = src;
+      jb.dst_off = dst;
+      jb.n_elem = n;
+      wg[b * n_sub + j] = (int32_t)wg_total;
+      wg_total += ceil_div(n, kGatherPerWG);
+      dst += n;
+    }
+  }
+  if (wg_total > 0x7fffffff) return tq_fail(h, TQ_ERROR_UNSUPPORTED, "slice: too many elements");
+  wg[n_jobs] = (int32_t)wg_total;
+  DevBuf dmeta;
+  TQ_HIP(h, dmeta.alloc(bytes));
+  TQ_HIP(h, hipMemcpyAsync(dmeta.p, host, bytes, hipMemcpyHostToDevice, h->stream));
+  TQ_CHECK(tq_launch_gather_jobs(h, dtype, dmeta.as<CopyJob>(),
+                                 reinterpret_cast<const int32_t *>(dmeta.as<char>() + wg_off),
+                                 n_jobs, wg_total, in, out));
   TQ_HIP(h, hipStreamSynchronize(h->stream));
   return TQ_SUCCESS;
 }

@@ -20,8 +20,8 @@ struct GeomArgs {
 };
 
 // Whole-block panels. Plain panels list legs [free..., contracted in pair
-// order...]; split-BF16 panels list [contracted..., free...] (k fastest) and
-// pad each row of kdim values to kpad = kdim rounded up to 8.
+// order...]; split-BF16 panels list [contracted..., free...] for the decode
+// and store the three planes k-chunked (CopyJob), kdim padded to kpad.
 struct PanelArgs {
   int64_t n_blocks;
   int32_t rank;

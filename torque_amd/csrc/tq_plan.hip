@@ -258,8 +258,9 @@ __global__ void set_c_offsets_kernel(int64_t n, const i64 *__restrict__ c_off,
 __device__ inline i64 round8(i64 x) { return (x + 7) & ~(i64)7; }
 
 // Per-block panel sizes (unused blocks get 0). A plain panel is the block
-// permuted to [free..., contracted in pair order...]; a split-BF16 panel is
-// [contracted..., free...] with each row of kdim values padded to kpad.
+// permuted to [free..., contracted in pair order...]; a split-BF16 panel has
+// rows = the free legs and kpad = kdim rounded up to 8 contracted values per
+// row, stored k-chunked (CopyJob).
 __global__ void block_panel_sizes_kernel(PanelArgs g,
                                          const uint8_t *__restrict__ used,
                                          i64 *__restrict__ sizes) {
@@ -299,10 +300,11 @@ __global__ void block_panel_jobs_kernel(PanelArgs g,
   if (g.split) {
     j.kdim = (int32_t)kd;
     j.kpad = (int32_t)round8(kd);
+    j.nk = g.nk;
   }
   j.n_elem = used[b] ? n : 0;
   jobs[b] = j;
-  wg[b] = ceil_div(j.n_elem, kGatherPerWG);
+  wg[b] = pack_job_wgs(j);
 }
 
 // Redirect each pair's operand into its block panel: the pair's sub-box of
@@ -325,7 +327,8 @@ __global__ void block_panel_desc_kernel(BoxArgs g,
   }
   for (int k = 0; k < g.nl; ++k) kfull *= se[g.self_legs[k]] - sb[g.self_legs[k]] + 1;
   // plain panel: column stride of contracted index starts at rows;
-  // split panel: contracted index is fastest (stride 1), rows at kpad
+  // split panel: the pair reads the block's whole contracted extent (pairs
+  // that do not get pair panels), so its offset stays at the panel start
   i64 off = panel_off[i], cstr = g.split ? 1 : rows;
   for (int k = 0; k < g.nl; ++k) {
     const int ls = g.self_legs[k], lo_ = g.other_legs[k];
@@ -349,8 +352,8 @@ __global__ void block_panel_desc_kernel(BoxArgs g,
 }
 
 // Per-pair panels (general overlapping blocks): panel p holds the pair's
-// sub-block permuted to [free..., contracted...], or for split-BF16 to
-// [contracted..., free...] with rows padded to a multiple of 8.
+// sub-block permuted to [free..., contracted...], or for split-BF16 the
+// k-chunked planes of its rows (free legs) x contracted values.
 __global__ void pair_panel_sizes_kernel(int64_t n_pairs, int32_t is_b, int32_t split,
                                         const GemmDesc *__restrict__ desc,
                                         i64 *__restrict__ sizes) {
@@ -405,12 +408,13 @@ __global__ void pair_panel_jobs_kernel(BoxArgs g,
   if (g.split) {
     job.kdim = (int32_t)kk;
     job.kpad = (int32_t)round8(kk);
+    job.nk = g.nl;
   }
   job.src_off = src;
   job.dst_off = panel_off[p];
   job.n_elem = n;
   jobs[p] = job;
-  wg[p] = ceil_div(n, kGatherPerWG);
+  wg[p] = pack_job_wgs(job);
   GemmDesc d = desc[p];
   const i64 ld = g.split ? round8(kk) : (rows > 0 ? rows : 1);
   if (g.is_b) {

@@ -277,6 +277,34 @@ class BlockSparseTensor {
     return out;
   }
 
+  // src/gpu/block_sparse.cuh:1124-1176 through tq_bs_slice: every block
+  // split into prod(divisor) sub-blocks, each holding the values of its
+  // region (the reference keeps the old data buffer under the new blocks)
+  BlockSparseTensor slice(const uvec &divisor) const {
+    if (divisor.n_elem() != rank) throw Error("Rank does not match");
+    uword n_sub = 1;
+    for (uword l = 0; l < rank; ++l) n_sub *= divisor[l];
+    const uword nb = n_blocks() * n_sub;
+    std::vector<int64_t> div(divisor.vec().begin(), divisor.vec().end());
+    std::vector<int64_t> bb(nb * rank), ee(nb * rank);
+    BlockSparseTensor out;
+    out.rank = rank;
+    out.dimension = dimension;
+    out.allocate(std::max<uword>(total(), 1));
+    Desc d(*this);
+    check(tq_bs_slice(default_handle(), dtype_of<T>(), &d.d, div.data(), data, out.data,
+                      bb.data(), ee.data()),
+          default_handle());
+    umat b(rank, nb), e(rank, nb);
+    for (uword j = 0; j < nb; ++j)
+      for (uword l = 0; l < rank; ++l) {
+        b(l, j) = (uword)bb[j * rank + l];
+        e(l, j) = (uword)ee[j * rank + l];
+      }
+    out.set_blocks(b, e, nullptr);
+    return out;
+  }
+
  private:
   T *data = nullptr;
 

@@ -274,6 +274,27 @@ class BlockSparseTensor:
         res._meta_changed()
         return res
 
+    def slice(self, divisor):
+        """BlockSparseTensor::slice (src/gpu/block_sparse.cuh:1124-1176)
+        through tq_bs_slice: every block split into prod(divisor) equal
+        sub-blocks (sub-block j decoded leg 0 fastest over the divisor), each
+        holding the compact copy of its region of the parent block."""
+        div = np.ascontiguousarray(np.asarray(divisor, dtype=np.int64).reshape(-1))
+        if div.size != self.rank:
+            raise TorqueError(_lib.TQ_ERROR_RANK_MISMATCH, "Rank does not match")
+        _require_gpu(self.data)
+        h = default_handle(self.data.device.index)
+        nb = self.n_blocks * int(np.prod(div))
+        nbeg = np.zeros((nb, self.rank), np.int64)
+        nend = np.zeros((nb, self.rank), np.int64)
+        out = torch.empty(max(int(self.block_n_elem.sum()), 1), dtype=self.dtype,
+                          device=self.data.device)
+        h.check(lib().tq_bs_slice(h.ptr, _DT[self.dtype], ctypes.byref(self.desc()),
+                                  _i64p(div), ctypes.c_void_p(self.data.data_ptr()),
+                                  ctypes.c_void_p(out.data_ptr()), _i64p(nbeg), _i64p(nend)))
+        return BlockSparseTensor(self.dimension, out, nbeg, nend, dtype=self.dtype,
+                                 device=self.data.device)
+
     def plan_contract(self, tensor, contracting_indices, precision="native", handle=None):
         """Build (or reuse) the device plan of contract(tensor, legs)."""
         legs = np.asarray(contracting_indices, dtype=np.int32).reshape(-1, 2)
