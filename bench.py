@@ -57,7 +57,10 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-permute", action="store_true",
                    help="skip the standalone 200M-element rank-6 permutation")
-    p.add_argument("--workload", default="peps", choices=["peps", "batched", "permute"],
+    p.add_argument("--no-large", action="store_true",
+                   help="skip the D=10, chi=256 enlarged-corner product")
+    p.add_argument("--workload", default="peps",
+                   choices=["peps", "batched", "permute", "large"],
                    help="peps: the D, chi absorption step (default); batched: --n-batch "
                         "independent rank-4 x rank-4 contractions in one captured launch")
     p.add_argument("--n-batch", type=int, default=512)
@@ -264,6 +267,119 @@ def bench_permute(dev, reps=10, dims=PERMUTE_DIMS, perms=PERMUTE_PERMS):
             "algorithmic_bytes_per_launch": 16 * n, "perms": out}
 
 
+def bench_large(dev, steps=5, warmup=2, tiers=("fp32", "bf16x3"), oracle_max_block=1368,
+                seed=0):
+    """The corner product of a D=10, chi=256 CTMRG step in the fused-leg view
+    (enlarged corners as block-diagonal matrices over the fused (chi, D, D)
+    leg: 19 GEMMs of 4..3600, 392 GFLOP): FP64 and the tiers, each autotuned
+    by a full sweep, timed with HIP events over whole executes (pack +
+    multiply) and over the multiply alone; relative error against the oracle
+    on every block up to oracle_max_block; and the committed cost model's
+    choice against the sweep (regret, time to schedule)."""
+    import torch
+
+    from oracle import oracle as O
+    from torque_amd import BlockSparseTensor, costmodel
+    from torque_amd import workloads as W
+
+    c1, c2, legs = W.enlarged_corners(10, 256, seed=seed)
+    A = BlockSparseTensor(c1.dims, c1.data, c1.begin, c1.end, device=dev)
+    B = BlockSparseTensor(c2.dims, c2.data, c2.begin, c2.end, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    sizes = (c1.end[:, 0] - c1.begin[:, 0] + 1).astype(np.int64)
+    offs = np.concatenate([[0], np.cumsum(sizes * sizes)])
+    sel = np.nonzero(sizes <= oracle_max_block)[0]
+
+    def sub(t):
+        data = np.concatenate([t.data[offs[i]:offs[i + 1]] for i in sel])
+        return O.BS(t.dims, t.begin[sel], t.end[sel]), data
+
+    TA, dA = sub(c1)
+    TB, dB = sub(c2)
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    ref, _ = O.contract(TA, dA, TB, dB, legs, threads=threads)
+    cpu_s = time.perf_counter() - t0
+    sample_flops = float(2 * (sizes[sel].astype(np.float64) ** 3).sum())
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    res = {}
+    fp64 = None
+    for prec in ("native",) + tuple(tiers):
+        p = A.plan_contract(B, legs, precision=prec)
+        out = torch.empty(max(p.out_n_elem, 1), dtype=torch.float64, device=dev)
+        t0 = time.perf_counter()
+        best, times = p.autotune(A.data, B.data, out, reps=2)
+        torch.cuda.synchronize()
+        sweep_s = time.perf_counter() - t0
+        ms = timed(lambda: p.execute(A.data, B.data, out))
+        p.pack(A.data, B.data)
+        gemm_ms = timed(lambda: p.multiply(A.data, B.data, out))
+        host = out.cpu().numpy()
+        got = np.concatenate([host[p.out_offsets[i]:p.out_offsets[i] + sizes[i] ** 2]
+                              for i in sel])
+        err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        name = "fp64" if prec == "native" else prec
+        e = {"ms": round(ms, 4), "gflops": round(p.flops / ms / 1e6, 1),
+             "gemm_ms": round(gemm_ms, 4), "gemm_gflops": round(p.flops / gemm_ms / 1e6, 1),
+             "rel_err_vs_oracle": err, "gemm_config": best,
+             "autotune_sweep_s": round(sweep_s, 3)}
+        if prec == "native":
+            fp64 = (ms, gemm_ms)
+            e["frac_fp64_mfma_peak"] = round(p.flops / ms / 1e9 / FP64_PEAK_TFLOPS, 4)
+            e["gemm_frac_fp64_mfma_peak"] = round(p.flops / gemm_ms / 1e9 / FP64_PEAK_TFLOPS, 4)
+            model = costmodel.load_default()
+            if model is not None:
+                t0 = time.perf_counter()
+                chosen = model.choose(p)
+                choose_s = time.perf_counter() - t0
+                regret = {}
+                for cls in range(2):
+                    t = np.array(times[cls])
+                    if (t > 0).any():
+                        regret["class%d" % cls] = round(float(t[chosen[cls]] / t[t > 0].min() - 1),
+                                                        4)
+                e["cost_model"] = {"chosen": chosen, "regret_vs_sweep": regret,
+                                   "time_to_schedule_s": round(choose_s, 4),
+                                   "sweep_time_to_schedule_s": round(sweep_s, 3)}
+        else:
+            e["speedup_vs_fp64"] = round(fp64[0] / ms, 3)
+            e["gemm_speedup_vs_fp64"] = round(fp64[1] / gemm_ms, 3)
+        res[name] = e
+        print("# corner product %s: %.3f ms (%.1f GFLOP/s), multiply alone %.3f ms "
+              "(%.1f GFLOP/s), rel err %.2e, configs %s"
+              % (name, ms, p.flops / ms / 1e6, gemm_ms, p.flops / gemm_ms / 1e6, err, best),
+              flush=True)
+        p.destroy()
+        del out
+    if "cost_model" in res.get("fp64", {}):
+        print("# cost model: chose %s, regret %s, %.4f s to schedule (sweep %.2f s)"
+              % (res["fp64"]["cost_model"]["chosen"], res["fp64"]["cost_model"]["regret_vs_sweep"],
+                 res["fp64"]["cost_model"]["time_to_schedule_s"],
+                 res["fp64"]["cost_model"]["sweep_time_to_schedule_s"]), flush=True)
+    del A, B
+    torch.cuda.empty_cache()
+    return {"workload": "enlarged_corner_product_D10_chi256_fused",
+            "blocks": sizes.tolist(), "gflop": round(float(2 * (sizes.astype(np.float64) ** 3).sum())
+                                                     / 1e9, 3),
+            "tiers": res,
+            "cpu_oracle": {"gflops": round(sample_flops / cpu_s / 1e9, 3), "cores": threads,
+                           "sample": "blocks up to %d (%.2f GFLOP), %.2f s"
+                                     % (oracle_max_block, sample_flops / 1e9, cpu_s)}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -272,11 +388,12 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         return bench_batched(args)
-    if args.workload == "permute":
+    if args.workload in ("permute", "large"):
         import torch
         dev = torch.device("cuda:0")
         torch.cuda.set_device(dev)
-        print(json.dumps(bench_permute(dev)), flush=True)
+        fn = bench_permute if args.workload == "permute" else bench_large
+        print(json.dumps(fn(dev)), flush=True)
         return
     import torch
 
@@ -442,6 +559,7 @@ def main():
               (tier, tdt * 1e3, flops / tdt / 1e9, rel(got)), flush=True)
 
     permute = None if args.no_permute else bench_permute(dev)
+    large = None if args.no_large else bench_large(dev)
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "GFLOP/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 4),
@@ -459,6 +577,7 @@ def main():
         "tiers": tiers,
         "fp64_rel_err_vs_oracle": err64,
         "permute_rank6_200M": permute,
+        "enlarged_corner_D10_chi256": large,
     }
     print(json.dumps(line), flush=True)
     if args.copy_calibration:

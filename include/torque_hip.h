@@ -218,7 +218,8 @@ tq_status tq_plan_destroy(tq_plan_t plan);
 /* Pairs whose contracted extent K is at most max_k (0..8) are written by the
  * streaming small-K kernel (2K flops per output element: bound by writing C)
  * instead of MFMA tiles; 0 sends every pair to the MFMA GEMM. The default is
- * 8 (0 for the split-BF16 tier). */
+ * 8. The split-BF16 tier streams such pairs from its split panels (each
+ * value rebuilt as hi + mid + lo in FP64, FP64 products). */
 tq_status tq_plan_set_small_k(tq_plan_t plan, int32_t max_k);
 
 /* Autotuner (no reference counterpart: the reference measures a cuTT plan
@@ -362,6 +363,13 @@ tq_status tq_costmodel_destroy(tq_costmodel_t model);
  * [features][hidden], b1, W2 [hidden][hidden], b2, w3, b3. */
 tq_status tq_costmodel_get_params(tq_costmodel_t model, float *params);
 tq_status tq_costmodel_set_params(tq_costmodel_t model, const float *params);
+/* The normalisation a fit sets (identity until then): x_mean and x_std
+ * (TQ_COST_FEATURES floats each) and y = {mean, std} of the log-times; with
+ * the parameters they save and restore a trained model. */
+tq_status tq_costmodel_get_scaling(tq_costmodel_t model, float *x_mean, float *x_std,
+                                   float *y);
+tq_status tq_costmodel_set_scaling(tq_costmodel_t model, const float *x_mean,
+                                   const float *x_std, const float *y);
 /* Gradient (tq_costmodel_n_params floats, may be NULL) and value of the
  * mean squared error of the raw network over n samples (x row-major
  * [n][TQ_COST_FEATURES], y [n]), computed on the GPU. */

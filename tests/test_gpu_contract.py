@@ -168,6 +168,9 @@ def test_small_k_stream_and_mfma_agree(cuda, small_k):
     assert rel_fro(got, ref_c) < FP64_TOL
     _, got32 = gpu_contract(a, da, b, db, legs, cuda, small_k=small_k, precision="fp32")
     assert rel_fro(got32, ref_c) < 1e-5
+    # the split-BF16 tier streams short pairs from its split panels
+    _, got3 = gpu_contract(a, da, b, db, legs, cuda, small_k=small_k, precision="bf16x3")
+    assert rel_fro(got3, ref_c) < 1e-6
     with pytest.raises(TorqueError, match="small-K"):
         plan.set_small_k(9)
 
@@ -186,6 +189,11 @@ def test_peps_absorption_d6_chi64(cuda):
     check_maps(plan2, ref2)
     assert rel_fro(got, ref_c) < FP64_TOL
     assert plan2.a_layout == 2 and plan2.b_layout == 2  # both permuted to GEMM layout
+    # split-BF16 tier: every E.a pair has K <= 4 and is streamed from split panels
+    plan3, got3 = gpu_contract(TE, dE, ta, ref_aa, l2, cuda, precision="bf16x3")
+    check_maps(plan3, ref2)
+    assert plan3.work()[0] == 0 and plan3.work()[1] > 0
+    assert rel_fro(got3, ref_c) < 1e-6
 
 
 def test_batched_pairs_large_k(cuda):

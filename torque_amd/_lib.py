@@ -100,6 +100,8 @@ def _proto(lib):
         "tq_costmodel_destroy": (S, [V]),
         "tq_costmodel_get_params": (S, [V, c_f32_p]),
         "tq_costmodel_set_params": (S, [V, c_f32_p]),
+        "tq_costmodel_get_scaling": (S, [V, c_f32_p, c_f32_p, c_f32_p]),
+        "tq_costmodel_set_scaling": (S, [V, c_f32_p, c_f32_p, c_f32_p]),
         "tq_costmodel_gradient": (S, [V, c_f32_p, c_f32_p, ctypes.c_int64, c_f32_p, c_f32_p]),
         "tq_costmodel_fit": (S, [V, c_f32_p, c_f32_p, ctypes.c_int64, ctypes.c_int32,
                                  ctypes.c_float, c_f32_p]),

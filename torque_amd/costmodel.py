@@ -9,6 +9,7 @@ from the autotuner: `record` autotunes a plan and returns its
 `CostModel.choose` configures a plan from the predictions alone.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -124,9 +125,122 @@ class CostModel:
                                                 _fp(y)))
         return y
 
+    def scaling(self):
+        xm = np.zeros(N_FEATURES, np.float32)
+        xs = np.zeros(N_FEATURES, np.float32)
+        y = np.zeros(2, np.float32)
+        self.h.check(lib().tq_costmodel_get_scaling(self.ptr, _fp(xm), _fp(xs), _fp(y)))
+        return xm, xs, y
+
+    def save(self, path, **meta):
+        """Parameters and normalisation (plus any metadata arrays) to .npz."""
+        xm, xs, y = self.scaling()
+        np.savez(path, params=self.params(), x_mean=xm, x_std=xs, y=y, **meta)
+
+    def load(self, path):
+        z = np.load(path)
+        self.set_params(z["params"])
+        xm, xs, y = (np.ascontiguousarray(z[k], np.float32) for k in ("x_mean", "x_std", "y"))
+        self.h.check(lib().tq_costmodel_set_scaling(self.ptr, _fp(xm), _fp(xs), _fp(y)))
+        return z
+
     def choose(self, plan):
         """Configures each tile class of `plan` from the predictions; returns
         the two chosen configurations."""
         best = (ctypes.c_int32 * 2)()
         self.h.check(lib().tq_plan_choose_config(plan.ptr, self.ptr, best))
         return [best[0], best[1]]
+
+
+# ---- training set and persisted model --------------------------------------
+
+DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                            "costmodel_gfx950.npz")
+
+
+def _block_diagonal(rng, n_lo, n_hi, s_lo, s_hi):
+    """Two block-diagonal matrices with n random sectors of sizes s_lo..s_hi
+    (the fused-leg shape of an enlarged-corner product)."""
+    from . import workloads as W
+    n = int(rng.integers(n_lo, n_hi + 1))
+    sizes = rng.integers(s_lo, s_hi + 1, n)
+    off = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    begin = np.stack([off, off], 1)
+    end = begin + np.stack([sizes - 1, sizes - 1], 1)
+    dim = int(sizes.sum())
+    a = W.Tensor([dim, dim], begin, end, W.random_data(begin, end, rng))
+    b = W.Tensor([dim, dim], begin.copy(), end.copy(), W.random_data(begin, end, rng))
+    return a, b, [[1, 0]]
+
+
+def training_contractions(n, seed):
+    """n contractions for the autotuner to time: U(1) rank-4 x rank-4 pairs
+    with leg dimensions spanning 8..160, and (every fourth) block-diagonal
+    fused-leg products with 2..12 sectors of 16..2048."""
+    from . import workloads as W
+    rng = np.random.default_rng(seed)
+    ranges = [(8, 40), (16, 64), (24, 80), (40, 96), (60, 120), (12, 160)]
+    out = []
+    for s in range(n):
+        if s % 4 == 3:
+            lo = int(rng.choice([16, 64, 256, 512, 1024]))
+            out.append(_block_diagonal(rng, 2, 12, lo, 2 * lo))
+        else:
+            lo, hi = ranges[s % len(ranges)]
+            out.append(W.batched(1, seed=int(rng.integers(1 << 30)), dim_lo=lo, dim_hi=hi)[0])
+    return out
+
+
+def collect(contractions, device, reps=3):
+    """Autotuner records (x, log-ms) of FP64 plans of the given contractions."""
+    import torch
+
+    from .tensor import BlockSparseTensor
+    xs, ys = [], []
+    for a, b, legs in contractions:
+        A = BlockSparseTensor(a.dims, a.data, a.begin, a.end, device=device)
+        B = BlockSparseTensor(b.dims, b.data, b.begin, b.end, device=device)
+        plan = A.plan_contract(B, legs)
+        c = torch.empty(max(plan.out_n_elem, 1), dtype=torch.float64, device=device)
+        x, y, _ = record(plan, A.data, B.data, c, reps=reps)
+        xs.append(x)
+        ys.append(y)
+        plan.destroy()
+    return np.concatenate(xs), np.concatenate(ys)
+
+
+def train(device, n=160, seed=20261203, steps=2000, lr=3e-3, path=DEFAULT_PATH):
+    """Times n training contractions with the autotuner, fits a model on the
+    GPU and saves it (parameters, normalisation, the records) to `path`."""
+    import time
+    t0 = time.perf_counter()
+    x, y = collect(training_contractions(n, seed), device)
+    t1 = time.perf_counter()
+    model = CostModel(seed=0)
+    loss = model.fit(x, y, steps=steps, lr=lr)
+    t2 = time.perf_counter()
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    model.save(path, x=x, y_log_ms=y, loss=np.float32(loss), seed=np.int64(seed),
+               n_contractions=np.int64(n), record_seconds=np.float32(t1 - t0),
+               fit_seconds=np.float32(t2 - t1))
+    return model, loss, x.shape[0], t1 - t0, t2 - t1
+
+
+def load_default(handle=None, path=DEFAULT_PATH):
+    """The committed model trained on this GPU's autotuner records, or None."""
+    if not os.path.exists(path):
+        return None
+    m = CostModel(handle)
+    m.load(path)
+    return m
+
+
+if __name__ == "__main__":
+    import sys
+
+    import torch
+    out = sys.argv[1] if len(sys.argv) > 1 else DEFAULT_PATH
+    dev = torch.device("cuda:0")
+    _, loss, n_rows, rec_s, fit_s = train(dev, path=out)
+    print("cost model: %d records (%.1f s of autotuning), fit %.2f s, normalised MSE %.4f -> %s"
+          % (n_rows, rec_s, fit_s, loss, out), flush=True)

@@ -159,11 +159,13 @@ enum : int32_t {
 constexpr int kStreamMaxK = 8;
 constexpr int kStreamRows = 64;
 
+// For the split-BF16 tier the operands are the split panels (a_packed /
+// b_packed, planes a_plane / b_plane elements apart).
 tq_status tq_launch_small_k(tq_handle_t h, tq_dtype dt, tq_precision prec,
                             const GemmDesc *descs, const int32_t *chunk_pair,
                             const int32_t *chunk_first, int64_t n_chunks, const void *a,
                             const void *a_packed, const void *b, const void *b_packed,
-                            void *c);
+                            void *c, int64_t a_plane, int64_t b_plane);
 
 // The wave-tile kernel addresses operands with 32-bit byte offsets from a
 // per-tile base; it runs only when every leading dimension is below this.

@@ -181,7 +181,8 @@ tq_status tq_bs_contract_multiply(tq_plan_t pl, const void *a, const void *b,
       TQ_CHECK(tq_launch_small_k(h, pl->dtype, pl->prec, pl->desc.as<GemmDesc>(),
                                  pl->chunk_pair.as<int32_t>(),
                                  pl->chunk_first.as<int32_t>(), pl->n_chunks, a,
-                                 pl->pack_a.panel.p, b, pl->pack_b.panel.p, c));
+                                 pl->pack_a.panel.p, b, pl->pack_b.panel.p, c,
+                                 pl->pack_a.panel_elems, pl->pack_b.panel_elems));
     // large tiles first: the small-tile launch then fills the tail
     TQ_CHECK(launch_class(pl, 1, a, b, c));
     return launch_class(pl, 0, a, b, c);
@@ -219,8 +220,6 @@ tq_status tq_plan_set_small_k(tq_plan_t pl, int32_t max_k) {
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
   if (max_k < 0 || max_k > kStreamMaxK)
     return tq_fail(pl->h, TQ_ERROR_INVALID_VALUE, "small-K threshold above 8");
-  if (pl->prec == TQ_PREC_BF16X3 && max_k > 0)
-    return tq_fail(pl->h, TQ_ERROR_UNSUPPORTED, "split-BF16 tier runs every pair on MFMA");
   pl->stream_max_k = max_k;
   pl->chunks_ready = false;
   for (int c = 0; c < 2; ++c) {

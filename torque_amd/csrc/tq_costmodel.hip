@@ -17,6 +17,7 @@
 // gradients (forward and backward of every sample of the wave, accumulated
 // in registers), a fixed-order reduction of the per-wave partial gradients
 // (deterministic), and the Adam update.
+#include <algorithm>
 #include <cmath>
 #include <memory>
 #include <random>
@@ -347,6 +348,31 @@ tq_status tq_costmodel_set_params(tq_costmodel_t md, const float *params) {
   if (!params) return TQ_ERROR_INVALID_VALUE;
   TQ_CHECK(upload_f(md->h, md->params, params, kParams));
   TQ_HIP(md->h, hipStreamSynchronize(md->h->stream));
+  return TQ_SUCCESS;
+}
+
+tq_status tq_costmodel_get_scaling(tq_costmodel_t md, float *x_mean, float *x_std,
+                                   float *y) {
+  if (!md) return TQ_ERROR_NOT_INITIALIZED;
+  if (!x_mean || !x_std || !y) return TQ_ERROR_INVALID_VALUE;
+  std::copy(md->x_mean.begin(), md->x_mean.end(), x_mean);
+  std::copy(md->x_std.begin(), md->x_std.end(), x_std);
+  y[0] = md->y_mean;
+  y[1] = md->y_std;
+  return TQ_SUCCESS;
+}
+
+tq_status tq_costmodel_set_scaling(tq_costmodel_t md, const float *x_mean, const float *x_std,
+                                   const float *y) {
+  if (!md) return TQ_ERROR_NOT_INITIALIZED;
+  if (!x_mean || !x_std || !y) return TQ_ERROR_INVALID_VALUE;
+  for (int i = 0; i < kF; ++i)
+    if (!(x_std[i] > 0.f)) return TQ_ERROR_INVALID_VALUE;
+  if (!(y[1] > 0.f)) return TQ_ERROR_INVALID_VALUE;
+  md->x_mean.assign(x_mean, x_mean + kF);
+  md->x_std.assign(x_std, x_std + kF);
+  md->y_mean = y[0];
+  md->y_std = y[1];
   return TQ_SUCCESS;
 }
 

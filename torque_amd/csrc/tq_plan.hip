@@ -1082,9 +1082,9 @@ tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
                            hipMemcpyDeviceToHost, h->stream));
   TQ_HIP(h, hipStreamSynchronize(h->stream));
 
-  // short contractions are written as streams; the split-BF16 tier keeps
-  // every pair on the BF16 MFMA
-  pl->stream_max_k = pl->prec == TQ_PREC_BF16X3 ? 0 : kStreamMaxK;
+  // short contractions are written as streams (the split-BF16 tier streams
+  // them from its split panels)
+  pl->stream_max_k = kStreamMaxK;
   for (int c = 0; c < 2; ++c) pl->cls[c].auto_cfg = tq_plan_pick_config(pl.get(), c);
   if (np > 0 && ro > 0) TQ_CHECK(plan_build_tiles(pl.get()));
   *plan_out = pl.release();

@@ -6,6 +6,13 @@
 // output: each lane holds its row of A (K values) in registers and walks the
 // N columns; B[n, :] is the same for the whole wavefront (scalar loads), and
 // each column is one contiguous 64-element store.
+//
+// The split-BF16 tier streams the same pairs from its split panels: a pair
+// with K <= 8 has one 8-value chunk per row, so a row is one 16-byte vector
+// per plane; each operand value is rebuilt as hi + mid + lo in FP64 (the
+// tier's operand rounding, below 2^-24 relative) and the products are FP64
+// FMAs -- BF16 MFMA tiles of 32 k would leave three quarters of each step
+// empty on a pair whose cost is its output.
 #include "tq_common.h"
 
 namespace {
@@ -53,6 +60,75 @@ __global__ __launch_bounds__(kThreads) void small_k_kernel(
   }
 }
 
+__device__ inline double bf16_bits_value(uint32_t h) {
+  return (double)__uint_as_float(h << 16);
+}
+
+// The 8 values of one split-panel row chunk: hi + mid + lo per value.
+__device__ inline void split_row(const uint16_t *p, int64_t plane, double (&v)[8]) {
+  const uint4 h = *reinterpret_cast<const uint4 *>(p);
+  const uint4 m = *reinterpret_cast<const uint4 *>(p + plane);
+  const uint4 l = *reinterpret_cast<const uint4 *>(p + 2 * plane);
+  const uint32_t hw[4] = {h.x, h.y, h.z, h.w}, mw[4] = {m.x, m.y, m.z, m.w},
+                 lw[4] = {l.x, l.y, l.z, l.w};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    v[2 * q] = bf16_bits_value(hw[q] & 0xffffu) + bf16_bits_value(mw[q] & 0xffffu) +
+               bf16_bits_value(lw[q] & 0xffffu);
+    v[2 * q + 1] = bf16_bits_value(hw[q] >> 16) + bf16_bits_value(mw[q] >> 16) +
+                   bf16_bits_value(lw[q] >> 16);
+  }
+}
+
+// B rows are rebuilt 64 at a time, one per lane, into the wavefront's LDS
+// slot, then read back by every lane as broadcasts: the rebuild costs one
+// pass per 64 output columns instead of one per column and lane.
+__global__ __launch_bounds__(kThreads) void small_k_split_kernel(
+    const GemmDesc *__restrict__ descs, const int32_t *__restrict__ seg_pair,
+    const int32_t *__restrict__ seg_first, int64_t n_segs, const uint16_t *__restrict__ a,
+    int64_t a_plane, const uint16_t *__restrict__ b, int64_t b_plane,
+    double *__restrict__ c) {
+  __shared__ double brow[kWaves][64][9];  // 9: rows on distinct bank pairs
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wave_id = __builtin_amdgcn_readfirstlane(
+      (int)(blockIdx.x * kWaves + wave));
+  const int n_waves = (int)(gridDim.x * kWaves);
+  for (int sg = wave_id; sg < n_segs; sg += n_waves) {
+    const int p = __builtin_amdgcn_readfirstlane(seg_pair[sg]);
+    const GemmDesc d = descs[p];
+    const int m = (int)(sg - seg_first[p]) * 64 + lane;
+    const bool live = m < d.m;
+    double av[8];
+    // K <= 8: the panel holds one chunk, row r at r * 8
+    split_row(a + d.a_off + (int64_t)min(m, d.m - 1) * 8, a_plane, av);
+    double *pc = c + d.c_off + m;
+    for (int n0 = 0; n0 < d.n; n0 += 64) {
+      double bv[8];
+      split_row(b + d.b_off + (int64_t)min(n0 + lane, d.n - 1) * 8, b_plane, bv);
+      // the slot's previous rows were read by this wavefront before this
+      // point; publish the new ones to the wavefront
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int k = 0; k < 8; ++k) brow[wave][lane][k] = bv[k];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int nn = min(64, d.n - n0);
+      for (int j = 0; j < nn; ++j) {
+        double acc = av[0] * brow[wave][j][0];
+#pragma unroll
+        for (int k = 1; k < 8; ++k)
+          if (k < d.k) acc = fma(av[k], brow[wave][j][k], acc);
+        if (live) pc[(int64_t)d.m * (n0 + j)] = acc;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+}
+
 template <class TIn, class TC, class TOut>
 tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *chunk_pair,
                  const int32_t *chunk_first, int64_t n_chunks, const void *a,
@@ -82,8 +158,28 @@ tq_status tq_launch_small_k(tq_handle_t h, tq_dtype dt, tq_precision prec,
                             const GemmDesc *descs, const int32_t *chunk_pair,
                             const int32_t *chunk_first, int64_t n_chunks, const void *a,
                             const void *a_packed, const void *b, const void *b_packed,
-                            void *c) {
+                            void *c, int64_t a_plane, int64_t b_plane) {
   if (n_chunks <= 0) return TQ_SUCCESS;
+  if (prec == TQ_PREC_BF16X3) {
+    static int per_cu = 0;
+    if (per_cu == 0) {
+      int occ = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, small_k_split_kernel, kThreads,
+                                                       0) != hipSuccess ||
+          occ < 1)
+        occ = 1;
+      per_cu = occ;
+    }
+    const int64_t grid = std::max<int64_t>(
+        1, std::min<int64_t>(ceil_div(n_chunks, kWaves), (int64_t)h->n_cu * per_cu));
+    hipLaunchKernelGGL(small_k_split_kernel, dim3((unsigned)grid), dim3(kThreads), 0,
+                       h->stream, descs, chunk_pair, chunk_first, n_chunks,
+                       reinterpret_cast<const uint16_t *>(a_packed), a_plane,
+                       reinterpret_cast<const uint16_t *>(b_packed), b_plane,
+                       reinterpret_cast<double *>(c));
+    TQ_HIP(h, hipGetLastError());
+    return TQ_SUCCESS;
+  }
   if (dt == TQ_F32)
     return launch<float, float, float>(h, descs, chunk_pair, chunk_first, n_chunks, a,
                                        a_packed, b, b_packed, c);

@@ -75,7 +75,7 @@ class Handle:
 
     def __del__(self):
         try:
-            for p in self.plans.values():
+            for p in list(self.plans.values()):
                 p.destroy()
             if self.ptr:
                 lib().tq_destroy(self.ptr)
@@ -451,6 +451,11 @@ class ContractionPlan:
         self.h.check(lib().tq_plan_graph_launch(self.ptr))
 
     def destroy(self):
+        """Frees the device plan now and drops it from the handle's cache."""
+        cache = getattr(getattr(self, "h", None), "plans", None)
+        if cache:
+            for k in [k for k, v in cache.items() if v is self]:
+                del cache[k]
         if getattr(self, "ptr", None):
             lib().tq_plan_destroy(self.ptr)
             self.ptr = None

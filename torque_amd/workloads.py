@@ -15,6 +15,8 @@ The workloads:
       sector sizes.
   batched  many independent rank-4 x rank-4 contractions over two legs with
       random uneven sectors.
+  enlarged_corners(D, chi) the corner product of a CTMRG step in the
+      fused-leg view: block-diagonal matrices over the fused (chi, D, D) leg.
 """
 from math import comb
 
@@ -200,3 +202,34 @@ def densify(t):
         off += n
     return dense
 
+
+def fused_leg(legs, signs):
+    """Sectors of the fusion of U(1) legs (charge = sum of sign * charge),
+    ascending in charge: a list of (charge, begin, size)."""
+    sizes = {0: 1}
+    for lg, sg in zip(legs, signs):
+        nxt = {}
+        for q0, n0 in sizes.items():
+            for q, _, n in lg:
+                nxt[q0 + sg * q] = nxt.get(q0 + sg * q, 0) + n0 * n
+        sizes = nxt
+    qs = sorted(sizes)
+    return leg(qs, [sizes[q] for q in qs])
+
+
+def enlarged_corners(D=10, chi=256, seed=0):
+    """The corner product of a CTMRG step in the fused-leg view: two
+    enlarged corners C1, C2, each a matrix over the fused leg (chi, D ket,
+    D bra) of dimension chi * D^2, block-diagonal in the fused U(1) charge
+    (one block per charge sector), contracted over their shared fused leg
+    (C1 leg 1 with C2 leg 0). Charges -3..3 (chi > 128) or -2..2 on each
+    leg. Entries N(0,1). Returns (C1, C2, legs)."""
+    rng = np.random.default_rng(seed)
+    lchi, lD, _ = peps_legs(D, chi)
+    f = fused_leg([lchi, lD, lD], [1, 1, -1])
+    dim = leg_dim(f)
+    begin = np.array([[b, b] for _, b, _ in f], np.int64)
+    end = np.array([[b + n - 1, b + n - 1] for _, b, n in f], np.int64)
+    c1 = Tensor([dim, dim], begin, end, random_data(begin, end, rng))
+    c2 = Tensor([dim, dim], begin.copy(), end.copy(), random_data(begin, end, rng))
+    return c1, c2, [[1, 0]]
