@@ -174,30 +174,29 @@ def _block_diagonal(rng, n_lo, n_hi, s_lo, s_hi):
 
 
 def training_contractions(n, seed):
-    """n contractions for the autotuner to time: U(1) rank-4 x rank-4 pairs
+    """Yields n contractions for the autotuner to time: U(1) rank-4 x rank-4 pairs
     with leg dimensions spanning 8..160, and (every fourth) block-diagonal
     fused-leg products with 2..12 sectors of 16..2048."""
     from . import workloads as W
     rng = np.random.default_rng(seed)
     ranges = [(8, 40), (16, 64), (24, 80), (40, 96), (60, 120), (12, 160)]
-    out = []
-    for s in range(n):
+    for s in range(n):  # generated one at a time: the large ones hold 100s of MB
         if s % 4 == 3:
             lo = int(rng.choice([16, 64, 256, 512, 1024]))
-            out.append(_block_diagonal(rng, 2, 12, lo, 2 * lo))
+            yield _block_diagonal(rng, 2, 12, lo, 2 * lo)
         else:
             lo, hi = ranges[s % len(ranges)]
-            out.append(W.batched(1, seed=int(rng.integers(1 << 30)), dim_lo=lo, dim_hi=hi)[0])
-    return out
+            yield W.batched(1, seed=int(rng.integers(1 << 30)), dim_lo=lo, dim_hi=hi)[0]
 
 
-def collect(contractions, device, reps=3):
-    """Autotuner records (x, log-ms) of FP64 plans of the given contractions."""
+def collect(contractions, device, reps=3, progress=None):
+    """Autotuner records (x, log-ms) of FP64 plans of the given contractions
+    (any iterable); progress(i) is called after each one."""
     import torch
 
     from .tensor import BlockSparseTensor
     xs, ys = [], []
-    for a, b, legs in contractions:
+    for i, (a, b, legs) in enumerate(contractions):
         A = BlockSparseTensor(a.dims, a.data, a.begin, a.end, device=device)
         B = BlockSparseTensor(b.dims, b.data, b.begin, b.end, device=device)
         plan = A.plan_contract(B, legs)
@@ -206,15 +205,18 @@ def collect(contractions, device, reps=3):
         xs.append(x)
         ys.append(y)
         plan.destroy()
+        if progress:
+            progress(i + 1)
     return np.concatenate(xs), np.concatenate(ys)
 
 
-def train(device, n=160, seed=20261203, steps=2000, lr=3e-3, path=DEFAULT_PATH):
+def train(device, n=160, seed=20261203, steps=2000, lr=3e-3, path=DEFAULT_PATH,
+          progress=None):
     """Times n training contractions with the autotuner, fits a model on the
     GPU and saves it (parameters, normalisation, the records) to `path`."""
     import time
     t0 = time.perf_counter()
-    x, y = collect(training_contractions(n, seed), device)
+    x, y = collect(training_contractions(n, seed), device, progress=progress)
     t1 = time.perf_counter()
     model = CostModel(seed=0)
     loss = model.fit(x, y, steps=steps, lr=lr)
@@ -241,6 +243,10 @@ if __name__ == "__main__":
     import torch
     out = sys.argv[1] if len(sys.argv) > 1 else DEFAULT_PATH
     dev = torch.device("cuda:0")
-    _, loss, n_rows, rec_s, fit_s = train(dev, path=out)
+    def report(i):
+        if i % 20 == 0:
+            print("recorded %d contractions" % i, flush=True)
+
+    _, loss, n_rows, rec_s, fit_s = train(dev, path=out, progress=report)
     print("cost model: %d records (%.1f s of autotuning), fit %.2f s, normalised MSE %.4f -> %s"
           % (n_rows, rec_s, fit_s, loss, out), flush=True)

@@ -6,7 +6,8 @@
 // k % 8, so the 8 values of one row in one chunk are one 16-byte vector and
 // consecutive rows are consecutive vectors. The product keeps the six piece
 // products of order >= 2^-16 relative,
-//   lo*hi + mid*mid + hi*lo + mid*hi + hi*mid + hi*hi   (smallest first),
+//   hi*lo + mid*mid + hi*mid + lo*hi + mid*hi + hi*hi   (A piece * B piece,
+// in this order per k step: grouped by B plane, smaller products first),
 // on v_mfma_f32_16x16x32_bf16 with FP32 accumulation; the dropped terms
 // (mid*lo, lo*mid, lo*lo) are below 2^-23 of |a||b| and the pieces represent
 // each operand to 2^-24 relative. Results are written as FP64.
@@ -18,7 +19,8 @@
 // 256 contiguous bytes), and the vectors of the next step (or of the next
 // tile's first step) are loaded into registers while the current step's
 // MFMAs run. The six piece products are six passes over the wavefront's
-// register tile, so consecutive MFMAs never depend on each other.
+// register tile, so consecutive MFMAs never depend on each other; the three
+// A planes stay in registers and the B planes are read one at a time.
 #include "tq_gemm_common.h"
 
 namespace {
@@ -95,8 +97,10 @@ struct Smem {
   static constexpr size_t kBytes = kAB > kStage ? kAB : kStage;
 };
 
-template <int BM, int BN, int WAVES_M>
-__global__ __launch_bounds__(kThreads) void bf16x3_gemm_kernel(
+// MINW: wavefronts per SIMD the register allocation must allow (2 makes
+// the 128 x 128 tile fit two workgroups per CU at the price of a few spills).
+template <int BM, int BN, int WAVES_M, int MINW>
+__global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
     const GemmDesc *__restrict__ descs, const int32_t *__restrict__ tile_pair,
     const int32_t *__restrict__ tile_first, int64_t n_tiles,
     const uint16_t *__restrict__ a, const uint16_t *__restrict__ b,
@@ -113,9 +117,6 @@ __global__ __launch_bounds__(kThreads) void bf16x3_gemm_kernel(
   const int wave = threadIdx.x >> 6;
   const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
   const int lrow = lane & 15, lchunk = lane >> 4;
-  // A-plane / B-plane of each piece product, smallest terms first
-  constexpr int kPa[6] = {2, 1, 0, 1, 0, 0};
-  constexpr int kPb[6] = {0, 1, 2, 0, 1, 0};
 
   int64_t t = tqg::xcd_virtual_id(blockIdx.x, gridDim.x);
   if (t >= n_tiles) return;
@@ -153,27 +154,34 @@ __global__ __launch_bounds__(kThreads) void bf16x3_gemm_kernel(
         la.load(a + nxt.d.a_off, a_plane, nxt.d.m, nxt.m0, nxt.d.lda >> 3, 0);
         lb.load(b + nxt.d.b_off, b_plane, nxt.d.n, nxt.n0, nxt.d.ldb >> 3, 0);
       }
-      bf16x8 af[3][TM], bfr[3][TN];
+      // all three A planes stay in registers; B planes are read one at a
+      // time (fewer live registers), each followed by its products
+      bf16x8 af[3][TM];
 #pragma unroll
-      for (int p = 0; p < 3; ++p) {
+      for (int p = 0; p < 3; ++p)
 #pragma unroll
         for (int i = 0; i < TM; ++i)
           af[p][i] = *reinterpret_cast<const bf16x8 *>(
               &As[(p * kChunks + lchunk) * BM + wm0 + 16 * i + lrow]);
 #pragma unroll
+      for (int pbq = 2; pbq >= 0; --pbq) {
+        bf16x8 bfr[TN];
+#pragma unroll
         for (int j = 0; j < TN; ++j)
-          bfr[p][j] = *reinterpret_cast<const bf16x8 *>(
-              &Bs[(p * kChunks + lchunk) * BN + wn0 + 16 * j + lrow]);
+          bfr[j] = *reinterpret_cast<const bf16x8 *>(
+              &Bs[(pbq * kChunks + lchunk) * BN + wn0 + 16 * j + lrow]);
+        // A planes paired with this B plane, smaller products first:
+        // b_lo: hi; b_mid: mid, hi; b_hi: lo, mid, hi
+#pragma unroll
+        for (int pa = 2 - pbq; pa >= 0; --pa)
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+              if (i < tm_live && j < tn_live)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[pa][i], bfr[j],
+                                                                    acc[i][j], 0, 0, 0);
       }
-#pragma unroll
-      for (int q = 0; q < 6; ++q)
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
-#pragma unroll
-          for (int j = 0; j < TN; ++j)
-            if (i < tm_live && j < tn_live)
-              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[kPa[q]][i], bfr[kPb[q]][j],
-                                                                  acc[i][j], 0, 0, 0);
     }
     tqg::store_tile<double, false, BM, BN, WAVES_M, TM, TN>(acc, c, d, cur.m0, cur.n0, stage);
     if (t_next >= n_tiles) break;
@@ -187,10 +195,10 @@ struct Cfg {
 };
 // keep in step with the switch in tq_launch_bf16x3_gemm()
 constexpr Cfg kCfgs[] = {{64, 64}, {128, 64}, {64, 128}, {128, 128}, {256, 32},
-                         {32, 32}, {128, 16},  {256, 16}};
+                         {32, 32}, {128, 16},  {256, 16}, {128, 128}};
 constexpr int kNCfg = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
-template <int BM, int BN, int WAVES_M>
+template <int BM, int BN, int WAVES_M, int MINW>
 tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *tile_pair,
                  const int32_t *tile_first, int64_t n_tiles, const uint16_t *a,
                  const uint16_t *b, int64_t ap, int64_t bp, double *c) {
@@ -199,7 +207,8 @@ tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *tile_pair,
   static int per_cu = 0;
   if (per_cu == 0) {
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (bf16x3_gemm_kernel<BM, BN, WAVES_M>),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ,
+                                                     (bf16x3_gemm_kernel<BM, BN, WAVES_M, MINW>),
                                                      kThreads, 0) != hipSuccess ||
         occ < 1)
       occ = 1;
@@ -208,7 +217,7 @@ tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *tile_pair,
   int64_t grid = std::min<int64_t>(n_tiles, (int64_t)h->n_cu * per_cu);
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((bf16x3_gemm_kernel<BM, BN, WAVES_M>), dim3((unsigned)grid),
+  hipLaunchKernelGGL((bf16x3_gemm_kernel<BM, BN, WAVES_M, MINW>), dim3((unsigned)grid),
                      dim3(kThreads), 0, h->stream, descs, tile_pair, tile_first, n_tiles,
                      a, b, ap, bp, c);
   TQ_HIP(h, hipGetLastError());
@@ -231,18 +240,19 @@ tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, const GemmDesc *descs,
                                 int64_t b_plane, double *c) {
   if (n_tiles <= 0) return TQ_SUCCESS;
   switch (cfg) {
-#define TQ_CFG(I, BM, BN, WM)                                                        \
+#define TQ_CFG(I, BM, BN, WM, MINW)                                                  \
   case I:                                                                            \
-    return launch<BM, BN, WM>(h, descs, tile_pair, tile_first, n_tiles, a_split,     \
-                              b_split, a_plane, b_plane, c);
-    TQ_CFG(0, 64, 64, 2)
-    TQ_CFG(1, 128, 64, 2)
-    TQ_CFG(2, 64, 128, 2)
-    TQ_CFG(3, 128, 128, 2)
-    TQ_CFG(4, 256, 32, 4)
-    TQ_CFG(5, 32, 32, 2)
-    TQ_CFG(6, 128, 16, 4)
-    TQ_CFG(7, 256, 16, 4)
+    return launch<BM, BN, WM, MINW>(h, descs, tile_pair, tile_first, n_tiles, a_split, \
+                                    b_split, a_plane, b_plane, c);
+    TQ_CFG(0, 64, 64, 2, 1)
+    TQ_CFG(1, 128, 64, 2, 1)
+    TQ_CFG(2, 64, 128, 2, 1)
+    TQ_CFG(3, 128, 128, 2, 1)
+    TQ_CFG(4, 256, 32, 4, 1)
+    TQ_CFG(5, 32, 32, 2, 1)
+    TQ_CFG(6, 128, 16, 4, 1)
+    TQ_CFG(7, 256, 16, 4, 1)
+    TQ_CFG(8, 128, 128, 2, 2)
 #undef TQ_CFG
   }
   return tq_fail(h, TQ_ERROR_INVALID_VALUE, "gemm: unknown tile configuration");

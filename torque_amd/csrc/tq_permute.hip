@@ -368,6 +368,85 @@ __device__ inline double bf16_value(uint16_t h) {
   return (double)__uint_as_float((uint32_t)h << 16);
 }
 
+// Split tile of a job whose first contracted leg is contiguous in memory
+// (k fastest): four lanes share a row's 8-value chunk, two values each, so
+// a wavefront reads 16 rows x 64 contiguous bytes and writes 256 contiguous
+// bytes per plane (a 32-bit word of two BF16 pieces per lane).
+__device__ inline void split_kfast_tile(const CopyJob &job, int64_t rows, int64_t row0,
+                                        int64_t chunk, const double *__restrict__ src,
+                                        uint16_t *__restrict__ dst, int64_t plane) {
+  if (chunk * 8 >= job.kpad) return;
+  const int lane = threadIdx.x & 63;
+  const int nk = job.nk;
+  const int64_t k0 = chunk * 8 + 2 * (lane & 3);
+  // offsets of k0 and k0 + 1 over the contracted legs (-1: past kdim)
+  i64 koff[2];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    koff[e] = -1;
+    if (k0 + e < job.kdim) {
+      uint32_t rem = (uint32_t)(k0 + e);
+      i64 o = 0;
+#pragma unroll
+      for (int l = 0; l < TQ_MAX_RANK; ++l) {
+        if (l >= nk) continue;
+        const uint32_t d = (uint32_t)job.dims[l];
+        const uint32_t q = rem / d;
+        o += (i64)(rem - q * d) * job.strides[l];
+        rem = q;
+      }
+      koff[e] = o;
+    }
+  }
+  // the row offsets of the four row groups, then all eight loads in flight
+  // together, then the splits and stores
+  constexpr int kG = kSplitRows / 16;
+  i64 roff[kG];
+#pragma unroll
+  for (int rg = 0; rg < kG; ++rg) {
+    const int64_t row = row0 + rg * 16 + (lane >> 2);
+    i64 o = job.src_off;
+    uint32_t rem = (uint32_t)row;
+#pragma unroll
+    for (int l = 0; l < TQ_MAX_RANK; ++l) {
+      if (l < nk || l >= job.rank) continue;
+      const uint32_t d = (uint32_t)job.dims[l];
+      const uint32_t q = rem / d;
+      o += (i64)(rem - q * d) * job.strides[l];
+      rem = q;
+    }
+    roff[rg] = row < rows ? o : -1;
+  }
+  double xv[kG][2];
+#pragma unroll
+  for (int rg = 0; rg < kG; ++rg)
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+      xv[rg][e] = (roff[rg] >= 0 && koff[e] >= 0) ? src[roff[rg] + koff[e]] : 0.0;
+#pragma unroll
+  for (int rg = 0; rg < kG; ++rg) {
+    const int64_t row = row0 + rg * 16 + (lane >> 2);
+    if (row >= rows) continue;
+    uint32_t wh = 0, wm = 0, wl = 0;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const double x = xv[rg][e];
+      const uint16_t h = bf16_rne((float)x);
+      const double r1 = x - bf16_value(h);
+      const uint16_t m = bf16_rne((float)r1);
+      const double r2 = r1 - bf16_value(m);
+      const uint16_t w = bf16_rne((float)r2);
+      wh |= (uint32_t)h << (16 * e);
+      wm |= (uint32_t)m << (16 * e);
+      wl |= (uint32_t)w << (16 * e);
+    }
+    const int64_t o = job.dst_off + chunk * rows * 8 + row * 8 + 2 * (lane & 3);
+    *reinterpret_cast<uint32_t *>(dst + o) = wh;
+    *reinterpret_cast<uint32_t *>(dst + plane + o) = wm;
+    *reinterpret_cast<uint32_t *>(dst + 2 * plane + o) = wl;
+  }
+}
+
 // One workgroup per tile of 64 rows x 32 padded k of a job: thread t takes
 // row t % 64 and chunk t / 64 of the tile, reads its 8 values (lanes run
 // along the rows, so the reads coalesce when a row leg is fastest in memory),
@@ -393,6 +472,11 @@ __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
     const int64_t rows = job.n_elem / kd;
     const int64_t k_tiles = ceil_div(job.kpad, kSplitK);
     const int64_t tile = c - job_wg_start[lo];
+    if (nk > 0 && job.strides[0] == 1 && job.dims[0] > 1) {
+      split_kfast_tile(job, rows, (tile / k_tiles) * kSplitRows,
+                       (tile % k_tiles) * (kSplitK / 8) + (threadIdx.x >> 6), src, dst, plane);
+      continue;
+    }
     const int64_t row = (tile / k_tiles) * kSplitRows + (threadIdx.x & (kSplitRows - 1));
     const int64_t chunk = (tile % k_tiles) * (kSplitK / 8) + (threadIdx.x >> 6);
     if (row >= rows || chunk * 8 >= job.kpad) continue;  // no barrier in this loop
@@ -420,16 +504,12 @@ __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
       off += (i64)dig[l] * job.strides[l];
       rem = q;
     }
-    uint16_t h[8], m[8], w[8];
+    // the 8 source offsets first (carrying through the contracted legs), then
+    // all 8 loads in flight together, then the splits
+    i64 offs[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const double x = (chunk * 8 + e < kd) ? src[off] : 0.0;
-      h[e] = bf16_rne((float)x);
-      const double r1 = x - bf16_value(h[e]);
-      m[e] = bf16_rne((float)r1);
-      const double r2 = r1 - bf16_value(m[e]);
-      w[e] = bf16_rne((float)r2);
-      // next k: carry through the contracted legs
+      offs[e] = off;
       bool carry = true;
 #pragma unroll
       for (int l = 0; l < TQ_MAX_RANK; ++l) {
@@ -442,6 +522,19 @@ __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
           dig[l] = 0;
         }
       }
+    }
+    double xv[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) xv[e] = (chunk * 8 + e < kd) ? src[offs[e]] : 0.0;
+    uint16_t h[8], m[8], w[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const double x = xv[e];
+      h[e] = bf16_rne((float)x);
+      const double r1 = x - bf16_value(h[e]);
+      m[e] = bf16_rne((float)r1);
+      const double r2 = r1 - bf16_value(m[e]);
+      w[e] = bf16_rne((float)r2);
     }
     const int64_t o = job.dst_off + chunk * rows * 8 + row * 8;
     uint4 vh, vm, vw;
