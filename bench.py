@@ -261,10 +261,23 @@ def bench_permute(dev, reps=10, dims=PERMUTE_DIMS, perms=PERMUTE_PERMS):
         print("# permute %s: %.3f ms, %.0f GB/s (%.1f%% of HBM), round trip %s"
               % (list(perm), ms, gbs, 100 * gbs / HBM_PEAK_GBS, "exact" if exact else "DIFFERS"),
               flush=True)
+    # calibration: a plain device copy of the same buffer (torch), the
+    # practical ceiling of a read-once write-once kernel on this GPU
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    y.copy_(x)
+    e0.record(stream)
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    copy_gbs = 2 * 8 * n / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9
+    print("# plain device copy of the same buffer: %.0f GB/s" % copy_gbs, flush=True)
     del x, y, z
     torch.cuda.empty_cache()
     return {"dims": list(dims), "elements": n, "dtype": "f64",
-            "algorithmic_bytes_per_launch": 16 * n, "perms": out}
+            "algorithmic_bytes_per_launch": 16 * n, "perms": out,
+            "copy_calibration_gbs": round(copy_gbs, 1)}
 
 
 def bench_large(dev, steps=5, warmup=2, tiers=("fp32", "bf16x3"), oracle_max_block=1368,

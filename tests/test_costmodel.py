@@ -59,8 +59,8 @@ def test_features_match_oracle():
     m = rng.integers(1, 900, 60)
     n = rng.integers(1, 700, 60)
     k = rng.integers(9, 1500, 60)
-    for q, (bm, bn, bk, wave) in enumerate(gemm_configs(with_kind=True)):
+    for q, (bm, bn, bk, wave, min_waves) in enumerate(gemm_configs(with_kind=True)):
         got = shape_features(m, n, k, q, n_cu=256)
-        ref = M.features(m, n, k, bm, bn, bk, wave, n_cu=256)
+        ref = M.features(m, n, k, bm, bn, bk, wave, n_cu=256, min_waves=min_waves)
         assert np.allclose(got, ref, rtol=1e-5, atol=1e-6), (q, np.abs(got - ref).max())
     assert not shape_features([], [], [], 0).any()

@@ -120,7 +120,7 @@ def _proto(lib):
         "tq_plan_destroy": (S, [V]),
         "tq_gemm_config_count": (ctypes.c_int32, [S]),
         "tq_gemm_config_describe": (S, [S, ctypes.c_int32, c_i32_p, c_i32_p, c_i32_p,
-                                        c_i32_p]),
+                                        c_i32_p, c_i32_p]),
         "tq_dense_contract": (S, [H, S, ctypes.c_int32, c_i64_p, ctypes.c_int32, c_i64_p,
                                   ctypes.c_int32, c_i32_p, V, V, V]),
         "tq_sparse_contract": (S, [H, S, ctypes.POINTER(SpDesc), ctypes.POINTER(SpDesc),

@@ -172,10 +172,13 @@ tq_status tq_launch_small_k(tq_handle_t h, tq_dtype dt, tq_precision prec,
 constexpr int64_t kWaveMaxLd = int64_t(1) << 22;
 
 // Tile shape of a configuration; wave_tile = 1 when one wavefront (not a
-// four-wave workgroup) computes a tile.
+// four-wave workgroup) computes a tile; min_waves = wavefronts per SIMD the
+// kernel's register budget keeps resident (2 for the variants compiled to
+// fit two).
 struct GemmConfig {
   int bm, bn, bk;
   int wave_tile;
+  int min_waves;
 };
 
 int tq_gemm_n_configs(tq_precision prec);

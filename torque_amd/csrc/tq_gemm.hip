@@ -302,7 +302,7 @@ GemmConfig tq_gemm_config(tq_precision prec, int cfg) {
     return g;
   }
   if (cfg < 0) cfg = 0;
-  return GemmConfig{kCfgs[cfg].bm, kCfgs[cfg].bn, kCfgs[cfg].bk, 0};
+  return GemmConfig{kCfgs[cfg].bm, kCfgs[cfg].bn, kCfgs[cfg].bk, 0, 1};
 }
 
 tq_status tq_launch_grouped_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec,
@@ -335,7 +335,7 @@ int32_t tq_gemm_config_count(tq_precision precision) {
 
 tq_status tq_gemm_config_describe(tq_precision precision, int32_t config,
                                   int32_t *bm, int32_t *bn, int32_t *bk,
-                                  int32_t *wave_tile) {
+                                  int32_t *wave_tile, int32_t *min_waves) {
   if (config < 0 || config >= tq_gemm_n_configs(precision))
     return TQ_ERROR_INVALID_VALUE;
   GemmConfig c = tq_gemm_config(precision, config);
@@ -343,6 +343,7 @@ tq_status tq_gemm_config_describe(tq_precision precision, int32_t config,
   if (bn) *bn = c.bn;
   if (bk) *bk = c.bk;
   if (wave_tile) *wave_tile = c.wave_tile;
+  if (min_waves) *min_waves = c.min_waves;
   return TQ_SUCCESS;
 }
 

@@ -191,11 +191,11 @@ __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
 }
 
 struct Cfg {
-  int bm, bn;
+  int bm, bn, min_waves;
 };
 // keep in step with the switch in tq_launch_bf16x3_gemm()
-constexpr Cfg kCfgs[] = {{64, 64}, {128, 64}, {64, 128}, {128, 128}, {256, 32},
-                         {32, 32}, {128, 16},  {256, 16}, {128, 128}};
+constexpr Cfg kCfgs[] = {{64, 64, 1},  {128, 64, 1}, {64, 128, 1}, {128, 128, 1}, {256, 32, 1},
+                         {32, 32, 1},  {128, 16, 1}, {256, 16, 1}, {128, 128, 2}};
 constexpr int kNCfg = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
 template <int BM, int BN, int WAVES_M, int MINW>
@@ -230,7 +230,7 @@ int tq_bf16x3_n_configs() { return kNCfg; }
 
 GemmConfig tq_bf16x3_config(int cfg) {
   if (cfg < 0 || cfg >= kNCfg) cfg = 0;
-  return GemmConfig{kCfgs[cfg].bm, kCfgs[cfg].bn, kBK, 0};
+  return GemmConfig{kCfgs[cfg].bm, kCfgs[cfg].bn, kBK, 0, kCfgs[cfg].min_waves};
 }
 
 tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, const GemmDesc *descs,

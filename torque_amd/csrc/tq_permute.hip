@@ -28,21 +28,30 @@ template <class W>
 __device__ inline void gather_range(const CopyJob &job, int64_t e0,
                                     const W *__restrict__ src,
                                     W *__restrict__ dst) {
-  // elements e0 + q*256 + tid, q < 8
+  // elements e0 + q*256 + tid, q < 8: all source offsets, then all loads in
+  // flight together, then the stores
   const int r = job.rank;
+  i64 off[kPerThread];
 #pragma unroll
   for (int q = 0; q < kPerThread; ++q) {
     const int64_t e = e0 + (int64_t)q * kGatherThreads + threadIdx.x;
-    if (e >= job.n_elem) break;
     uint32_t rem = (uint32_t)e;
-    i64 off = job.src_off;
+    i64 o = job.src_off;
     for (int l = 0; l < r; ++l) {
       const uint32_t d = (uint32_t)job.dims[l];
       const uint32_t qd = rem / d;
-      off += (i64)(rem - qd * d) * job.strides[l];
+      o += (i64)(rem - qd * d) * job.strides[l];
       rem = qd;
     }
-    dst[job.dst_off + e] = src[off];
+    off[q] = e < job.n_elem ? o : -1;
+  }
+  W v[kPerThread];
+#pragma unroll
+  for (int q = 0; q < kPerThread; ++q) v[q] = off[q] >= 0 ? src[off[q]] : W(0);
+#pragma unroll
+  for (int q = 0; q < kPerThread; ++q) {
+    const int64_t e = e0 + (int64_t)q * kGatherThreads + threadIdx.x;
+    if (off[q] >= 0) dst[job.dst_off + e] = v[q];
   }
 }
 
@@ -93,17 +102,37 @@ struct TileDesc {
   i64 b_in_str[TQ_MAX_RANK];
   i64 b_out_str[TQ_MAX_RANK];
   i64 p_og, p_ig;  // extents of the two groups
+  int32_t t_og, t_ig;  // tile extents (t_og <= 64, t_ig <= 512, product <= 4096)
+  uint32_t m_og, m_ig;  // ceil(2^32 / t): e / t == umulhi(e, m) for e < 2^32 / t
   i64 n_og_tiles, n_ig_tiles;
   i64 n_tiles;
 };
 
+constexpr int kTileElems = 4096;  // elements of one tile
+constexpr int kMaxTig = 512;
+
+__device__ inline int udiv_magic(uint32_t e, uint32_t m, int t) {
+  return t == 1 ? (int)e : (int)__umulhi(e, m);
+}
+
+// Tiles of t_og output-group x t_ig input-group elements through LDS. Each
+// thread owns 16 elements of the tile, numbered linearly: for the read the
+// input-group index runs fastest (lanes read along the input), for the write
+// the output-group index does (lanes write along the output), so edge tiles
+// and short groups keep every lane busy. All loads of a thread are issued
+// before its LDS stores.
 template <class W>
 __global__ __launch_bounds__(256) void tiled_permute_kernel(
     TileDesc td, const W *__restrict__ src, W *__restrict__ dst) {
-  __shared__ W tile[kT][kT + 1];
+  // row r of the tile at r * ld, ld = t_ig + 1 or + 2 (odd: rows on distinct
+  // bank pairs for the column-wise reads of the write phase)
+  __shared__ W tile[kTileElems + 2 * kT];
   __shared__ i64 in_og[kT];
-  __shared__ i64 in_ig[kT];
-  __shared__ i64 out_ig[kT];
+  __shared__ i64 in_ig[kMaxTig];
+  __shared__ i64 out_ig[kMaxTig];
+  constexpr int kPer = kTileElems / 256;
+  const int tid = threadIdx.x;
+  const int ld = td.t_ig + ((td.t_ig & 1) ? 2 : 1);
   for (i64 t = blockIdx.x; t < td.n_tiles; t += gridDim.x) {
     i64 rest = t;
     const i64 c_og = rest % td.n_og_tiles;
@@ -119,56 +148,63 @@ __global__ __launch_bounds__(256) void tiled_permute_kernel(
       out_base += dg * td.b_out_str[l];
       bidx = q;
     }
-    const i64 og0 = c_og * kT, ig0 = c_ig * kT;
-    const int tid = threadIdx.x;
-    if (tid < kT) {
-      i64 og = og0 + tid, off = 0;
-      if (og < td.p_og) {
-        for (int l = 0; l < td.n_og; ++l) {
-          const i64 d = td.og_dims[l];
-          const i64 q = og / d;
-          off += (og - q * d) * td.og_in_str[l];
-          og = q;
-        }
+    const i64 og0 = c_og * td.t_og, ig0 = c_ig * td.t_ig;
+    const int n_og = (int)min<i64>(td.t_og, td.p_og - og0);
+    const int n_ig = (int)min<i64>(td.t_ig, td.p_ig - ig0);
+    if (tid < n_og) {
+      uint32_t og = (uint32_t)(og0 + tid);
+      i64 off = 0;
+      for (int l = 0; l < td.n_og; ++l) {
+        const uint32_t d = (uint32_t)td.og_dims[l];
+        const uint32_t q = og / d;
+        off += (i64)(og - q * d) * td.og_in_str[l];
+        og = q;
       }
       in_og[tid] = off;
-    } else if (tid < 2 * kT) {
-      const int u = tid - kT;
-      i64 ig = ig0 + u, oi = 0, oo = 0;
-      if (ig < td.p_ig) {
-        for (int l = 0; l < td.n_ig; ++l) {
-          const i64 d = td.ig_dims[l];
-          const i64 q = ig / d;
-          const i64 dg = ig - q * d;
-          oi += dg * td.ig_in_str[l];
-          oo += dg * td.ig_out_str[l];
-          ig = q;
-        }
+    }
+    for (int u = tid; u < n_ig; u += 256) {
+      // group indices stay below 2^31 (p_ig <= 2^31 is checked on the host)
+      uint32_t ig = (uint32_t)(ig0 + u);
+      i64 oi = 0, oo = 0;
+      for (int l = 0; l < td.n_ig; ++l) {
+        const uint32_t d = (uint32_t)td.ig_dims[l];
+        const uint32_t q = ig / d;
+        const uint32_t dg = ig - q * d;
+        oi += (i64)dg * td.ig_in_str[l];
+        oo += (i64)dg * td.ig_out_str[l];
+        ig = q;
       }
       in_ig[u] = oi;
       out_ig[u] = oo;
     }
     __syncthreads();
-    const int n_og = (int)min<i64>(kT, td.p_og - og0);
-    const int n_ig = (int)min<i64>(kT, td.p_ig - ig0);
-    // read: input-group index fastest across lanes
-    {
-      const int u = tid & (kT - 1);
-      if (u < n_ig) {
-        const i64 iu = in_ig[u];
-        for (int r = tid >> 6; r < n_og; r += 4)
-          tile[r][u] = src[in_base + in_og[r] + iu];
+    // two halves of 8 loads each in flight (fewer live registers)
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      constexpr int kH = kPer / 2;
+      W v[kH];
+#pragma unroll
+      for (int q = 0; q < kH; ++q) {
+        const uint32_t e = (uint32_t)(tid + 256 * (half * kH + q));
+        const int r = udiv_magic(e, td.m_ig, td.t_ig);
+        const int u = (int)e - r * td.t_ig;
+        v[q] = (r < n_og && u < n_ig) ? src[in_base + in_og[r] + in_ig[u]] : W(0);
+      }
+#pragma unroll
+      for (int q = 0; q < kH; ++q) {
+        const uint32_t e = (uint32_t)(tid + 256 * (half * kH + q));
+        const int r = udiv_magic(e, td.m_ig, td.t_ig);
+        const int u = (int)e - r * td.t_ig;
+        if (r < n_og && u < n_ig) tile[r * ld + u] = v[q];
       }
     }
     __syncthreads();
-    // write: output-group index fastest across lanes (contiguous output)
-    {
-      const int r = tid & (kT - 1);
-      if (r < n_og) {
-        const i64 ob = out_base + og0 + r;
-        for (int u = tid >> 6; u < n_ig; u += 4)
-          dst[ob + out_ig[u]] = tile[r][u];
-      }
+#pragma unroll 4
+    for (int q = 0; q < kPer; ++q) {
+      const uint32_t e = (uint32_t)(tid + 256 * q);
+      const int u = udiv_magic(e, td.m_og, td.t_og);
+      const int r = (int)e - u * td.t_og;
+      if (r < n_og && u < n_ig) dst[out_base + og0 + r + out_ig[u]] = tile[r * ld + u];
     }
     __syncthreads();
   }
@@ -258,9 +294,13 @@ tq_status dense_permute_t(tq_handle_t h, const Legs &L, i64 n_total,
     used[a] = true;
     ++a;
   }
+  // tile extents: up to 64 along the output group; along the input group as
+  // many as fill a tile of kTileElems (at most kMaxTig)
+  const int t_og = (int)std::min<i64>(kT, p_og);
+  const i64 want_ig = std::min<i64>(kMaxTig, std::max<i64>(kT, kTileElems / t_og));
   // input group: legs by increasing input stride, starting at f
   i64 p_ig = 1;
-  while (p_ig < kT) {
+  while (p_ig < want_ig) {
     int best = -1;
     for (int k = 0; k < L.r; ++k)
       if (!used[k] && (best < 0 || L.s[k] < L.s[best])) best = k;
@@ -281,10 +321,18 @@ tq_status dense_permute_t(tq_handle_t h, const Legs &L, i64 n_total,
     ++td.n_b;
     n_b *= L.d[k];
   }
+  if (p_og >= (i64(1) << 31) || p_ig >= (i64(1) << 31))
+    return tq_fail(h, TQ_ERROR_UNSUPPORTED, "permute: leg group above 2^31 elements");
   td.p_og = p_og;
   td.p_ig = p_ig;
-  td.n_og_tiles = ceil_div(p_og, kT);
-  td.n_ig_tiles = ceil_div(p_ig, kT);
+  td.t_og = t_og;
+  td.t_ig = (int)std::min<i64>(p_ig, std::min<i64>(kMaxTig, kTileElems / t_og));
+  td.m_og = (uint32_t)(((uint64_t)1 << 32) / (uint64_t)td.t_og +
+                       ((((uint64_t)1 << 32) % (uint64_t)td.t_og) ? 1 : 0));
+  td.m_ig = (uint32_t)(((uint64_t)1 << 32) / (uint64_t)td.t_ig +
+                       ((((uint64_t)1 << 32) % (uint64_t)td.t_ig) ? 1 : 0));
+  td.n_og_tiles = ceil_div(p_og, td.t_og);
+  td.n_ig_tiles = ceil_div(p_ig, td.t_ig);
   td.n_tiles = td.n_og_tiles * td.n_ig_tiles * n_b;
   const i64 grid = std::min<i64>(td.n_tiles, (i64)h->n_cu * 16);
   hipLaunchKernelGGL(tiled_permute_kernel<W>, dim3((unsigned)grid), dim3(256),

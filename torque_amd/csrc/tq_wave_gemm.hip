@@ -149,8 +149,10 @@ __device__ inline void tail_select(TC (&r)[T][KPL], int kvalid) {
     for (int i = 0; i < T; ++i) r[i][s] = 4 * s < kvalid ? r[i][s] : (TC)0;
 }
 
-template <class TIn, class TC, class TOut, int TM, int TN, int KPL>
-__global__ __launch_bounds__(256) void wave_gemm_kernel(
+// MINW: wavefronts per SIMD the register allocation must allow; 2 keeps a
+// second wavefront resident to cover the first one's operand waits.
+template <class TIn, class TC, class TOut, int TM, int TN, int KPL, int MINW>
+__global__ __launch_bounds__(256, MINW) void wave_gemm_kernel(
     const GemmDesc *__restrict__ descs, const int32_t *__restrict__ tile_pair,
     const int32_t *__restrict__ tile_first, int64_t n_tiles, const TIn *__restrict__ a,
     const TIn *__restrict__ a_packed, const TIn *__restrict__ b,
@@ -245,14 +247,15 @@ __global__ __launch_bounds__(256) void wave_gemm_kernel(
 }
 
 struct WaveCfg {
-  int tm, tn, kpl;
+  int tm, tn, kpl, min_waves;
 };
 // keep in step with the switch in dispatch()
-constexpr WaveCfg kWaveCfgs[] = {{1, 1, 4}, {2, 1, 4}, {1, 2, 4}, {2, 2, 4},
-                                 {2, 2, 2}, {4, 2, 2}, {2, 4, 2}, {4, 4, 2}};
+constexpr WaveCfg kWaveCfgs[] = {{1, 1, 4, 1}, {2, 1, 4, 1}, {1, 2, 4, 1},
+                                 {2, 2, 4, 1}, {2, 2, 2, 1}, {4, 2, 2, 1},
+                                 {2, 4, 2, 1}, {4, 4, 2, 1}, {4, 4, 2, 2}};
 constexpr int kNWave = sizeof(kWaveCfgs) / sizeof(kWaveCfgs[0]);
 
-template <class TIn, class TC, class TOut, int TM, int TN, int KPL>
+template <class TIn, class TC, class TOut, int TM, int TN, int KPL, int MINW>
 tq_status launch_one(tq_handle_t h, const GemmDesc *descs, const int32_t *tile_pair,
                      const int32_t *tile_first, int64_t n_tiles, const void *a,
                      const void *ap, const void *b, const void *bp, void *c) {
@@ -261,7 +264,7 @@ tq_status launch_one(tq_handle_t h, const GemmDesc *descs, const int32_t *tile_p
   if (per_cu == 0) {
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &occ, (wave_gemm_kernel<TIn, TC, TOut, TM, TN, KPL>), 256, 0) != hipSuccess ||
+            &occ, (wave_gemm_kernel<TIn, TC, TOut, TM, TN, KPL, MINW>), 256, 0) != hipSuccess ||
         occ < 1)
       occ = 1;
     per_cu = occ;
@@ -269,7 +272,7 @@ tq_status launch_one(tq_handle_t h, const GemmDesc *descs, const int32_t *tile_p
   int64_t grid = std::min<int64_t>(ceil_div(n_tiles, (int64_t)4), (int64_t)h->n_cu * per_cu);
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((wave_gemm_kernel<TIn, TC, TOut, TM, TN, KPL>), dim3((unsigned)grid),
+  hipLaunchKernelGGL((wave_gemm_kernel<TIn, TC, TOut, TM, TN, KPL, MINW>), dim3((unsigned)grid),
                      dim3(256), 0, h->stream, descs, tile_pair, tile_first, n_tiles,
                      reinterpret_cast<const TIn *>(a), reinterpret_cast<const TIn *>(ap),
                      reinterpret_cast<const TIn *>(b), reinterpret_cast<const TIn *>(bp),
@@ -282,19 +285,20 @@ template <class TIn, class TC, class TOut>
 tq_status dispatch(tq_handle_t h, int cfg, const GemmDesc *descs, const int32_t *tile_pair,
                    const int32_t *tile_first, int64_t n_tiles, const void *a, const void *ap,
                    const void *b, const void *bp, void *c) {
-#define TQ_WCFG(I, TM, TN, KPL)                                                          \
+#define TQ_WCFG(I, TM, TN, KPL, MINW)                                                    \
   case I:                                                                                \
-    return launch_one<TIn, TC, TOut, TM, TN, KPL>(h, descs, tile_pair, tile_first,      \
-                                                  n_tiles, a, ap, b, bp, c);
+    return launch_one<TIn, TC, TOut, TM, TN, KPL, MINW>(h, descs, tile_pair, tile_first, \
+                                                        n_tiles, a, ap, b, bp, c);
   switch (cfg) {
-    TQ_WCFG(0, 1, 1, 4)
-    TQ_WCFG(1, 2, 1, 4)
-    TQ_WCFG(2, 1, 2, 4)
-    TQ_WCFG(3, 2, 2, 4)
-    TQ_WCFG(4, 2, 2, 2)
-    TQ_WCFG(5, 4, 2, 2)
-    TQ_WCFG(6, 2, 4, 2)
-    TQ_WCFG(7, 4, 4, 2)
+    TQ_WCFG(0, 1, 1, 4, 1)
+    TQ_WCFG(1, 2, 1, 4, 1)
+    TQ_WCFG(2, 1, 2, 4, 1)
+    TQ_WCFG(3, 2, 2, 4, 1)
+    TQ_WCFG(4, 2, 2, 2, 1)
+    TQ_WCFG(5, 4, 2, 2, 1)
+    TQ_WCFG(6, 2, 4, 2, 1)
+    TQ_WCFG(7, 4, 4, 2, 1)
+    TQ_WCFG(8, 4, 4, 2, 2)
   }
 #undef TQ_WCFG
   return tq_fail(h, TQ_ERROR_INVALID_VALUE, "gemm: unknown wave tile configuration");
@@ -307,7 +311,7 @@ int tq_wave_n_configs() { return kNWave; }
 GemmConfig tq_wave_config(int cfg) {
   if (cfg < 0 || cfg >= kNWave) cfg = 0;
   const WaveCfg &w = kWaveCfgs[cfg];
-  return GemmConfig{16 * w.tm, 16 * w.tn, 4 * w.kpl, 1};
+  return GemmConfig{16 * w.tm, 16 * w.tn, 4 * w.kpl, 1, w.min_waves};
 }
 
 tq_status tq_launch_wave_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec, int cfg,

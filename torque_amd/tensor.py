@@ -494,16 +494,17 @@ class BatchPlan(ContractionPlan):
 
 
 def gemm_configs(precision="native", with_kind=False):
-    """(bm, bn, bk) of every tile configuration, plus wave_tile when
-    with_kind (1: one wavefront per tile)."""
+    """(bm, bn, bk) of every tile configuration, plus (wave_tile, min_waves)
+    when with_kind (wave_tile 1: one wavefront per tile; min_waves: the
+    wavefronts per SIMD its registers keep resident)."""
     L = lib()
     pr = _lib.PRECISIONS[precision]
     out = []
     for c in range(L.tq_gemm_config_count(pr)):
-        bm, bn, bk, wt = (ctypes.c_int32() for _ in range(4))
+        bm, bn, bk, wt, mw = (ctypes.c_int32() for _ in range(5))
         check(L.tq_gemm_config_describe(pr, c, ctypes.byref(bm), ctypes.byref(bn),
-                                        ctypes.byref(bk), ctypes.byref(wt)))
-        out.append((bm.value, bn.value, bk.value) + ((wt.value,) if with_kind else ()))
+                                        ctypes.byref(bk), ctypes.byref(wt), ctypes.byref(mw)))
+        out.append((bm.value, bn.value, bk.value) + ((wt.value, mw.value) if with_kind else ()))
     return out
 
 
