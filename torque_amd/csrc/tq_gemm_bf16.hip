@@ -129,8 +129,8 @@ __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
     const GemmDesc &d = cur.d;
     // panels store kpad = lda (A) / ldb (B) values per row: kpad / 8 chunks
     const int n_chunks = d.lda >> 3, n_chunks_b = d.ldb >> 3;
-    const int tm_live = min(TM, max(0, (d.m - cur.m0 - wm0 + 15) / 16));
-    const int tn_live = min(TN, max(0, (d.n - cur.n0 - wn0 + 15) / 16));
+    // every MFMA block runs (blocks past M or N compute on clamped rows and
+    // are not stored), so the MFMA stream has no branches
     f4 acc[TM][TN];
 #pragma unroll
     for (int i = 0; i < TM; ++i)
@@ -178,9 +178,8 @@ __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
           for (int i = 0; i < TM; ++i)
 #pragma unroll
             for (int j = 0; j < TN; ++j)
-              if (i < tm_live && j < tn_live)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[pa][i], bfr[j],
-                                                                    acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[pa][i], bfr[j],
+                                                                  acc[i][j], 0, 0, 0);
       }
     }
     tqg::store_tile<double, false, BM, BN, WAVES_M, TM, TN>(acc, c, d, cur.m0, cur.n0, stage);
