@@ -259,9 +259,10 @@ tq_status tq_plan_graph_launch(tq_plan_t plan);
 
 /* Number of grouped-GEMM tile configurations and a description of one:
  * the tile's M and N extents, its k step, wave_tile = 1 when one
- * wavefront (not a four-wave workgroup) computes a tile, and min_waves, the
- * wavefronts per SIMD the kernel's register budget keeps resident (2 for the
- * variants compiled to fit two, else 1). Any output pointer may be NULL. */
+ * wavefront (not a workgroup) computes a tile, and min_waves, the
+ * wavefronts per SIMD the kernel keeps resident (2 for the variants whose
+ * registers fit two, or whose eight-wave workgroup places two on each SIMD,
+ * else 1). Any output pointer may be NULL. */
 int32_t tq_gemm_config_count(tq_precision precision);
 tq_status tq_gemm_config_describe(tq_precision precision, int32_t config,
                                   int32_t *bm, int32_t *bn, int32_t *bk,
@@ -347,10 +348,10 @@ typedef struct tq_costmodel_s *tq_costmodel_t;
  * configuration, host-only: [0,8) [8,16) [16,24) flop-weighted histograms
  * of M, N, K over the buckets <16, 16-31, ..., 512-1023, >=1024; 24 log10
  * flops; 25 log10 pairs; 26 log10 tiles; 27 useful / padded flops; 28-30
- * log2 of the tile's M, N, K extents; 31 for a configuration in which a
- * wavefront computes a tile, the wavefronts per SIMD its registers keep
- * resident (1 or 2), else 0; 32 log10(tiles per concurrently resident tile); 33 log10 output
- * bytes. All zero for an empty set. n_cu <= 0 means 256. */
+ * log2 of the tile's M, N, K extents; 31 2 * wave_tile + min_waves (as
+ * tq_gemm_config_describe reports them); 32 log10(tiles per concurrently
+ * resident tile); 33 log10 output bytes. All zero for an empty set.
+ * n_cu <= 0 means 256. */
 tq_status tq_cost_features(const int64_t *m, const int64_t *n, const int64_t *k,
                            int64_t n_pairs, tq_precision precision, int32_t config,
                            int32_t n_cu, float *features);

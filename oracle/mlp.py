@@ -106,7 +106,7 @@ def features(m, n, k, bm, bn, bk, wave_tile, n_cu=256, min_waves=1):
     f[26] = np.log10(tiles)
     f[27] = tot / padded
     f[28], f[29], f[30] = np.log2(bm), np.log2(bn), np.log2(bk)
-    f[31] = wave_tile * min_waves
+    f[31] = 2 * wave_tile + min_waves
     f[32] = np.log10(tiles / (n_cu * (16 if wave_tile else 4)))
     f[33] = np.log10((8 * m * n).sum())
     return f

@@ -330,3 +330,41 @@ def test_bf16x3_tier(cuda, maker):
         check_maps(plan, ref)
         if plan.n_pairs:
             assert rel_fro(got, ref_c) < 1e-6, cfg
+
+
+def block_diagonal_pair(rng, n_blocks, lo, hi):
+    """A (R x K) and B (K x N) block-diagonal matrices whose i-th blocks are
+    r_i x k_i and k_i x n_i with sizes uniform in [lo, hi]: n_blocks
+    independent GEMMs of uneven shapes."""
+    r, k, n = (rng.integers(lo, hi + 1, n_blocks) for _ in range(3))
+
+    def mat(rows, cols):
+        b0 = np.concatenate([[0], np.cumsum(rows)[:-1]])
+        b1 = np.concatenate([[0], np.cumsum(cols)[:-1]])
+        begin = np.stack([b0, b1], 1).astype(np.int64)
+        end = np.stack([b0 + rows - 1, b1 + cols - 1], 1).astype(np.int64)
+        return W.Tensor([rows.sum(), cols.sum()], begin, end,
+                        rng.standard_normal(int((rows * cols).sum())))
+
+    return mat(r, k), mat(k, n), [[1, 0]]
+
+
+def test_bf16x3_many_tiles_per_workgroup(cuda):
+    # more tiles than resident workgroups, so every persistent workgroup runs
+    # several tiles and the double-buffered kernel's loads cross from one
+    # tile's last k step to the next tile's first; K is not a multiple of 32
+    rng = np.random.default_rng(91)
+    a, b, legs = block_diagonal_pair(rng, 96, 150, 420)
+    (A, da), (B, db) = from_workload(a), from_workload(b)
+    ref_c, ref = O.contract(A, da, B, db, legs, threads=8)
+    plan, c64 = gpu_contract(A, da, B, db, legs, cuda)
+    check_maps(plan, ref)
+    assert rel_fro(c64, ref_c) < FP64_TOL
+    first = None
+    for cfg in range(len(gemm_configs("bf16x3"))):
+        _, got = gpu_contract(A, da, B, db, legs, cuda, precision="bf16x3", cfg=cfg)
+        assert rel_fro(got, ref_c) < 1e-6, cfg
+        # every configuration adds the piece products in one order
+        if first is None:
+            first = got
+        assert np.array_equal(got, first), cfg

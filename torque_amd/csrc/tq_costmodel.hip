@@ -268,7 +268,7 @@ int tq_cost_features_host(const int64_t *m, const int64_t *n, const int64_t *k,
   feat[28] = (float)std::log2((double)g.bm);
   feat[29] = (float)std::log2((double)g.bn);
   feat[30] = (float)std::log2((double)g.bk);
-  feat[31] = (float)(g.wave_tile * g.min_waves);
+  feat[31] = (float)(2 * g.wave_tile + g.min_waves);
   feat[32] = (float)std::log10(tiles / concurrency);
   feat[33] = (float)std::log10(out_bytes);
   return 1;

@@ -35,38 +35,46 @@ constexpr int kChunks = 4;  // 8-value chunks per step
 // One operand's vectors of one k step: 3 planes x 4 chunks x ROWS rows of
 // 16 bytes; vector v of the step is (row v % ROWS, chunk (v / ROWS) % 4,
 // plane v / (4 ROWS)), so consecutive threads read consecutive rows.
-template <int ROWS>
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <int ROWS, int THREADS = kThreads>
 struct StepLoader {
   static constexpr int kVec = 3 * kChunks * ROWS;
-  static constexpr int kN = (kVec + kThreads - 1) / kThreads;
-  uint4 v[kN];
+  static constexpr int kN = (kVec + THREADS - 1) / THREADS;
+  static_assert(kN <= 32, "one mask bit per vector");
+  u32x4 v[kN];
+  uint32_t ok;  // bit q: v[q] is in range (otherwise it is stored as zeros)
 
   // base: plane 0 of the panel; rows R of the panel; rows past R are read
   // from row R - 1 (their products only reach unstored output); chunks past
-  // the padded K are zeros.
+  // the padded K are zeros. Branch-free and without waiting on the loads:
+  // out-of-range vectors load a clamped in-range vector and become zeros
+  // when stored. (Native vector types: the loaded values stay in registers.)
   __device__ inline void load(const uint16_t *__restrict__ base, int64_t plane, int rows,
                               int row0, int n_chunks, int chunk0) {
+    ok = 0;
+    if (n_chunks <= 0) return;  // no k values: the panel holds nothing to read
+    const int last = n_chunks - 1;
 #pragma unroll
     for (int q = 0; q < kN; ++q) {
-      const int e = threadIdx.x + q * kThreads;
+      const int e = threadIdx.x + q * THREADS;
       const int r = e % ROWS;
       const int c = (e / ROWS) % kChunks;
-      const int p = e / (ROWS * kChunks);
+      const int p = min(e / (ROWS * kChunks), 2);
       const int row = min(row0 + r, rows - 1);
       const int chunk = chunk0 + c;
-      uint4 x = make_uint4(0, 0, 0, 0);
-      if (e < kVec && chunk < n_chunks)
-        x = *reinterpret_cast<const uint4 *>(base + p * plane +
-                                             ((int64_t)chunk * rows + row) * 8);
-      v[q] = x;
+      ok |= (uint32_t)(chunk < n_chunks && (kVec % THREADS == 0 || e < kVec)) << q;
+      v[q] = *reinterpret_cast<const u32x4 *>(
+          base + p * plane + ((int64_t)min(chunk, last) * rows + row) * 8);
     }
   }
 
   __device__ inline void store(uint4 *lds) const {
 #pragma unroll
     for (int q = 0; q < kN; ++q) {
-      const int e = threadIdx.x + q * kThreads;
-      if (e < kVec) lds[e] = v[q];
+      const int e = threadIdx.x + q * THREADS;
+      const u32x4 x = ((ok >> q) & 1) ? v[q] : u32x4{0, 0, 0, 0};
+      if (kVec % THREADS == 0 || e < kVec) reinterpret_cast<u32x4 *>(lds)[e] = x;
     }
   }
 };
@@ -83,11 +91,48 @@ __device__ inline TileRef fetch(int64_t t, const GemmDesc *__restrict__ descs,
   TileRef r;
   const int p = tile_pair[t];
   r.d = descs[p];
-  const int lt = (int)(t - tile_first[p]);
-  const int tiles_m = (r.d.m + BM - 1) / BM;
-  r.m0 = (lt % tiles_m) * BM;
-  r.n0 = (lt / tiles_m) * BN;
+  const tqg::TilePos tp =
+      tqg::tile_position((int)(t - tile_first[p]), (r.d.m + BM - 1) / BM, (r.d.n + BN - 1) / BN,
+                         tqg::tile_group<BM, BN>(gridDim.x));
+  r.m0 = tp.mt * BM;
+  r.n0 = tp.nt * BN;
   return r;
+}
+
+// The six piece products of one k step for a wavefront's TM x TN blocks:
+// all three A planes stay in registers; B planes are read one at a time
+// (fewer live registers), each followed by its products. Every configuration
+// adds the products to an accumulator in this one order, so all of them give
+// the same bits. a_row / b_row: the LDS row of the wavefront's first block
+// for this lane.
+template <int BM, int BN, int TM, int TN>
+__device__ inline void step_products(const uint4 *As, const uint4 *Bs, int a_row, int b_row,
+                                     int lchunk, f4 (&acc)[TM][TN]) {
+  bf16x8 af[3][TM];
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+      af[p][i] =
+          *reinterpret_cast<const bf16x8 *>(&As[(p * kChunks + lchunk) * BM + a_row + 16 * i]);
+#pragma unroll
+  for (int pbq = 2; pbq >= 0; --pbq) {
+    bf16x8 bfr[TN];
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+      bfr[j] =
+          *reinterpret_cast<const bf16x8 *>(&Bs[(pbq * kChunks + lchunk) * BN + b_row + 16 * j]);
+    // A planes paired with this B plane, smaller products first:
+    // b_lo: a_hi; b_mid: a_mid, a_hi; b_hi: a_lo, a_mid, a_hi
+#pragma unroll
+    for (int pa = 2 - pbq; pa >= 0; --pa)
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          acc[i][j] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[pa][i], bfr[j], acc[i][j], 0, 0, 0);
+  }
 }
 
 template <int BM, int BN>
@@ -154,33 +199,7 @@ __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
         la.load(a + nxt.d.a_off, a_plane, nxt.d.m, nxt.m0, nxt.d.lda >> 3, 0);
         lb.load(b + nxt.d.b_off, b_plane, nxt.d.n, nxt.n0, nxt.d.ldb >> 3, 0);
       }
-      // all three A planes stay in registers; B planes are read one at a
-      // time (fewer live registers), each followed by its products
-      bf16x8 af[3][TM];
-#pragma unroll
-      for (int p = 0; p < 3; ++p)
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
-          af[p][i] = *reinterpret_cast<const bf16x8 *>(
-              &As[(p * kChunks + lchunk) * BM + wm0 + 16 * i + lrow]);
-#pragma unroll
-      for (int pbq = 2; pbq >= 0; --pbq) {
-        bf16x8 bfr[TN];
-#pragma unroll
-        for (int j = 0; j < TN; ++j)
-          bfr[j] = *reinterpret_cast<const bf16x8 *>(
-              &Bs[(pbq * kChunks + lchunk) * BN + wn0 + 16 * j + lrow]);
-        // A planes paired with this B plane, smaller products first:
-        // b_lo: hi; b_mid: mid, hi; b_hi: lo, mid, hi
-#pragma unroll
-        for (int pa = 2 - pbq; pa >= 0; --pa)
-#pragma unroll
-          for (int i = 0; i < TM; ++i)
-#pragma unroll
-            for (int j = 0; j < TN; ++j)
-              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[pa][i], bfr[j],
-                                                                  acc[i][j], 0, 0, 0);
-      }
+      step_products<BM, BN, TM, TN>(As, Bs, wm0 + lrow, wn0 + lrow, lchunk, acc);
     }
     tqg::store_tile<double, false, BM, BN, WAVES_M, TM, TN>(acc, c, d, cur.m0, cur.n0, stage);
     if (t_next >= n_tiles) break;
@@ -189,26 +208,228 @@ __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
   }
 }
 
+// k steps of a tile: a tile with no k values still runs one step over zeros.
+__device__ inline int tile_steps(const GemmDesc &d) {
+  return max(1, ((d.lda >> 3) + kChunks - 1) / kChunks);
+}
+
+// Double-buffered variant for large tiles at one workgroup per CU: LDS holds
+// two k steps, so each step needs one barrier, and that barrier orders LDS
+// only (the registers' loads of the step after next stay in flight across
+// it). A load cursor runs one step ahead of the MFMAs through the flattened
+// sequence of (tile, k step) of this workgroup, so the last step of a tile
+// already has the next tile's first step in LDS. The epilogue stages through
+// the buffer the last step read.
+template <int BM, int BN, int WAVES_M>
+__global__ __launch_bounds__(kThreads, 1) void bf16x3_db_kernel(
+    const GemmDesc *__restrict__ descs, const int32_t *__restrict__ tile_pair,
+    const int32_t *__restrict__ tile_first, int64_t n_tiles,
+    const uint16_t *__restrict__ a, const uint16_t *__restrict__ b,
+    int64_t a_plane, int64_t b_plane, double *__restrict__ c) {
+  constexpr int WAVES_N = 4 / WAVES_M;
+  constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int kBuf = 3 * kChunks * (BM + BN);  // 16-byte vectors per buffer
+  static_assert(TM >= 1 && TN >= 1, "tile shape");
+  static_assert(sizeof(double) * 16 * (BM + 1) <= sizeof(uint4) * kBuf,
+                "epilogue staging fits one buffer");
+  __shared__ uint4 smem[2 * kBuf];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
+  const int lrow = lane & 15, lchunk = lane >> 4;
+
+  int64_t t = tqg::xcd_virtual_id(blockIdx.x, gridDim.x);
+  if (t >= n_tiles) return;
+  TileRef cur = fetch<BM, BN>(t, descs, tile_pair, tile_first);
+
+  // load cursor: tile lt (lref), k step ls
+  TileRef lref = cur;
+  int64_t lt = t;
+  int ls = 0;
+  StepLoader<BM> la;
+  StepLoader<BN> lb;
+  auto load = [&]() {
+    la.load(a + lref.d.a_off, a_plane, lref.d.m, lref.m0, lref.d.lda >> 3, ls * kChunks);
+    lb.load(b + lref.d.b_off, b_plane, lref.d.n, lref.n0, lref.d.ldb >> 3, ls * kChunks);
+  };
+  auto advance = [&]() -> bool {
+    if (++ls < tile_steps(lref.d)) return true;
+    lt += gridDim.x;
+    if (lt >= n_tiles) return false;
+    lref = fetch<BM, BN>(lt, descs, tile_pair, tile_first);
+    ls = 0;
+    return true;
+  };
+  load();
+  la.store(smem);
+  lb.store(smem + 3 * kChunks * BM);
+  bool have = advance();  // the registers hold a step not yet in LDS
+  if (have) load();
+  tqg::lds_barrier();
+  int buf = 0;
+  while (true) {
+    const int steps = tile_steps(cur.d);
+    f4 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int s = 0; s < steps; ++s) {
+      const uint4 *As = smem + buf * kBuf;
+      if (have) {
+        // the other buffer was last read before the previous barrier
+        uint4 *An = smem + (buf ^ 1) * kBuf;
+        la.store(An);
+        lb.store(An + 3 * kChunks * BM);
+        have = advance();
+        if (have) load();
+      }
+      step_products<BM, BN, TM, TN>(As, As + 3 * kChunks * BM, wm0 + lrow, wn0 + lrow, lchunk,
+                                    acc);
+      tqg::lds_barrier();
+      buf ^= 1;
+    }
+    // buf now holds the next tile's first step; buf ^ 1 is free for staging
+    tqg::store_tile<double, false, BM, BN, WAVES_M, TM, TN>(
+        acc, c, cur.d, cur.m0, cur.n0, reinterpret_cast<double *>(smem + (buf ^ 1) * kBuf));
+    t += gridDim.x;
+    if (t >= n_tiles) break;
+    cur = fetch<BM, BN>(t, descs, tile_pair, tile_first);
+  }
+}
+
+// As bf16x3_db_kernel, with the loads two k steps ahead of the MFMAs: two
+// register sets take turns, so a load issued at the start of step g is stored
+// to LDS at the start of step g + 2 (two steps of MFMAs hide its latency where
+// the single set gives one). The work of the workgroup is one flat sequence
+// of k steps over its tiles, walked two steps per loop turn (one per
+// register set); a tile ends with its epilogue inside the step that finishes
+// it. NW wavefronts: with eight, a 256 x 128 tile keeps 64 x 64 wavefront
+// tiles and two wavefronts per SIMD at one workgroup per CU.
+template <int BM, int BN, int WAVES_M, int NW>
+__global__ __launch_bounds__(NW * 64, 1) void bf16x3_db2_kernel(
+    const GemmDesc *__restrict__ descs, const int32_t *__restrict__ tile_pair,
+    const int32_t *__restrict__ tile_first, int64_t n_tiles,
+    const uint16_t *__restrict__ a, const uint16_t *__restrict__ b,
+    int64_t a_plane, int64_t b_plane, double *__restrict__ c) {
+  constexpr int WAVES_N = NW / WAVES_M;
+  constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int kBuf = 3 * kChunks * (BM + BN);  // 16-byte vectors per buffer
+  static_assert(TM >= 1 && TN >= 1, "tile shape");
+  static_assert(sizeof(double) * 16 * (BM + 1) <= sizeof(uint4) * kBuf,
+                "epilogue staging fits one buffer");
+  __shared__ uint4 smem[2 * kBuf];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
+  const int lrow = lane & 15, lchunk = lane >> 4;
+
+  int64_t t = tqg::xcd_virtual_id(blockIdx.x, gridDim.x);
+  if (t >= n_tiles) return;
+  TileRef cur = fetch<BM, BN>(t, descs, tile_pair, tile_first);
+
+  // load cursor: the next step to load is step ls of tile lt (lref)
+  TileRef lref = cur;
+  int64_t lt = t;
+  int ls = 0;
+  bool lvalid = true;
+  // loads the step at the cursor into (la, lb) and advances the cursor;
+  // false once the workgroup's steps are all loaded
+  auto load_next = [&](StepLoader<BM, NW * 64> &la, StepLoader<BN, NW * 64> &lb) -> bool {
+    if (!lvalid) return false;
+    la.load(a + lref.d.a_off, a_plane, lref.d.m, lref.m0, lref.d.lda >> 3, ls * kChunks);
+    lb.load(b + lref.d.b_off, b_plane, lref.d.n, lref.n0, lref.d.ldb >> 3, ls * kChunks);
+    if (++ls >= tile_steps(lref.d)) {
+      lt += gridDim.x;
+      lvalid = lt < n_tiles;
+      if (lvalid) lref = fetch<BM, BN>(lt, descs, tile_pair, tile_first);
+      ls = 0;
+    }
+    return true;
+  };
+  StepLoader<BM, NW * 64> xa, ya;
+  StepLoader<BN, NW * 64> xb, yb;
+  bool hx = load_next(xa, xb);  // step 0
+  bool hy = load_next(ya, yb);  // step 1
+  xa.store(smem);
+  xb.store(smem + 3 * kChunks * BM);
+  hx = load_next(xa, xb);  // step 2
+  tqg::lds_barrier();
+
+  int buf = 0, s = 0, steps = tile_steps(cur.d);
+  f4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  // one k step: the register set (la, lb) holds the step after this one; it
+  // goes to the other buffer (last read before the previous barrier) and is
+  // refilled with the step three ahead. False once the workgroup is done.
+  auto step = [&](StepLoader<BM, NW * 64> &la, StepLoader<BN, NW * 64> &lb, bool &h) -> bool {
+    if (h) {
+      uint4 *An = smem + (buf ^ 1) * kBuf;
+      la.store(An);
+      lb.store(An + 3 * kChunks * BM);
+      h = load_next(la, lb);
+    }
+    const uint4 *As = smem + buf * kBuf;
+    step_products<BM, BN, TM, TN>(As, As + 3 * kChunks * BM, wm0 + lrow, wn0 + lrow, lchunk,
+                                  acc);
+    tqg::lds_barrier();
+    buf ^= 1;
+    if (++s < steps) return true;
+    // buf holds the next step; buf ^ 1 is free for staging
+    tqg::store_tile<double, false, BM, BN, WAVES_M, TM, TN, NW>(
+        acc, c, cur.d, cur.m0, cur.n0, reinterpret_cast<double *>(smem + (buf ^ 1) * kBuf));
+    t += gridDim.x;
+    if (t >= n_tiles) return false;
+    cur = fetch<BM, BN>(t, descs, tile_pair, tile_first);
+    s = 0;
+    steps = tile_steps(cur.d);
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    return true;
+  };
+  while (step(ya, yb, hy) && step(xa, xb, hx)) {
+  }
+}
+
 struct Cfg {
   int bm, bn, min_waves;
 };
-// keep in step with the switch in tq_launch_bf16x3_gemm()
-constexpr Cfg kCfgs[] = {{64, 64, 1},  {128, 64, 1}, {64, 128, 1}, {128, 128, 1}, {256, 32, 1},
-                         {32, 32, 1},  {128, 16, 1}, {256, 16, 1}, {128, 128, 2}};
+// keep in step with the switch in tq_launch_bf16x3_gemm(); configurations 9
+// and 10 are the double-buffered kernel, 11 and 12 its two-step-ahead form
+constexpr Cfg kCfgs[] = {{64, 64, 1},   {128, 64, 1},  {64, 128, 1},  {128, 128, 1},
+                         {256, 32, 1},  {32, 32, 1},   {128, 16, 1},  {256, 16, 1},
+                         {128, 128, 2}, {256, 128, 1}, {128, 256, 1}, {256, 128, 2},
+                         {128, 256, 2}};
 constexpr int kNCfg = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
-template <int BM, int BN, int WAVES_M, int MINW>
+typedef void (*KernelFn)(const GemmDesc *, const int32_t *, const int32_t *, int64_t,
+                         const uint16_t *, const uint16_t *, int64_t, int64_t, double *);
+
+template <int BM, int BN, int WAVES_M, int MINW, int DB>
 tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *tile_pair,
                  const int32_t *tile_first, int64_t n_tiles, const uint16_t *a,
                  const uint16_t *b, int64_t ap, int64_t bp, double *c) {
+  KernelFn kernel;
+  const int threads = DB == 2 ? 512 : kThreads;
+  if constexpr (DB == 2)
+    kernel = bf16x3_db2_kernel<BM, BN, WAVES_M, 8>;
+  else if constexpr (DB == 1)
+    kernel = bf16x3_db_kernel<BM, BN, WAVES_M>;
+  else
+    kernel = bf16x3_gemm_kernel<BM, BN, WAVES_M, MINW>;
   // persistent grid: exactly the workgroups that can be resident at once
   // (occupancy from registers and LDS), a multiple of the 8 XCDs
   static int per_cu = 0;
   if (per_cu == 0) {
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ,
-                                                     (bf16x3_gemm_kernel<BM, BN, WAVES_M, MINW>),
-                                                     kThreads, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, threads, 0) != hipSuccess ||
         occ < 1)
       occ = 1;
     per_cu = occ;
@@ -216,9 +437,8 @@ tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *tile_pair,
   int64_t grid = std::min<int64_t>(n_tiles, (int64_t)h->n_cu * per_cu);
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((bf16x3_gemm_kernel<BM, BN, WAVES_M, MINW>), dim3((unsigned)grid),
-                     dim3(kThreads), 0, h->stream, descs, tile_pair, tile_first, n_tiles,
-                     a, b, ap, bp, c);
+  hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(threads), 0, h->stream, descs,
+                     tile_pair, tile_first, n_tiles, a, b, ap, bp, c);
   TQ_HIP(h, hipGetLastError());
   return TQ_SUCCESS;
 }
@@ -239,19 +459,23 @@ tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, const GemmDesc *descs,
                                 int64_t b_plane, double *c) {
   if (n_tiles <= 0) return TQ_SUCCESS;
   switch (cfg) {
-#define TQ_CFG(I, BM, BN, WM, MINW)                                                  \
-  case I:                                                                            \
-    return launch<BM, BN, WM, MINW>(h, descs, tile_pair, tile_first, n_tiles, a_split, \
-                                    b_split, a_plane, b_plane, c);
-    TQ_CFG(0, 64, 64, 2, 1)
-    TQ_CFG(1, 128, 64, 2, 1)
-    TQ_CFG(2, 64, 128, 2, 1)
-    TQ_CFG(3, 128, 128, 2, 1)
-    TQ_CFG(4, 256, 32, 4, 1)
-    TQ_CFG(5, 32, 32, 2, 1)
-    TQ_CFG(6, 128, 16, 4, 1)
-    TQ_CFG(7, 256, 16, 4, 1)
-    TQ_CFG(8, 128, 128, 2, 2)
+#define TQ_CFG(I, BM, BN, WM, MINW, DB)                                                  \
+  case I:                                                                                \
+    return launch<BM, BN, WM, MINW, DB>(h, descs, tile_pair, tile_first, n_tiles, a_split, \
+                                        b_split, a_plane, b_plane, c);
+    TQ_CFG(0, 64, 64, 2, 1, 0)
+    TQ_CFG(1, 128, 64, 2, 1, 0)
+    TQ_CFG(2, 64, 128, 2, 1, 0)
+    TQ_CFG(3, 128, 128, 2, 1, 0)
+    TQ_CFG(4, 256, 32, 4, 1, 0)
+    TQ_CFG(5, 32, 32, 2, 1, 0)
+    TQ_CFG(6, 128, 16, 4, 1, 0)
+    TQ_CFG(7, 256, 16, 4, 1, 0)
+    TQ_CFG(8, 128, 128, 2, 2, 0)
+    TQ_CFG(9, 256, 128, 2, 1, 1)
+    TQ_CFG(10, 128, 256, 2, 1, 1)
+    TQ_CFG(11, 256, 128, 4, 1, 2)
+    TQ_CFG(12, 128, 256, 2, 1, 2)
 #undef TQ_CFG
   }
   return tq_fail(h, TQ_ERROR_INVALID_VALUE, "gemm: unknown tile configuration");

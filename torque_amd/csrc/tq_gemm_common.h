@@ -17,6 +17,42 @@ __device__ inline int64_t xcd_virtual_id(int64_t bid, int64_t grid) {
   return (bid % 8) * per + bid / 8;
 }
 
+// Position (tile row, tile column) of tile lt of a pair with tiles_m x
+// tiles_n tiles: bands of `group` tile rows are walked column by column, so a
+// run of consecutive tiles (what one XCD holds at once) covers a block of the
+// output and reuses both operand panels from L2, where a plain m-fastest walk
+// reuses only the B panel.
+struct TilePos {
+  int mt, nt;
+};
+__device__ inline TilePos tile_position(int lt, int tiles_m, int tiles_n, int group) {
+  const int per_group = group * tiles_n;
+  const int g = lt / per_group;
+  const int first = g * group;
+  const int gs = min(tiles_m - first, group);
+  const int r = lt - g * per_group;
+  return TilePos{first + r % gs, r / gs};
+}
+
+// Band height for tile_position: the workgroups of one XCD (a persistent
+// grid's eighth) hold `per_xcd` BM x BN tiles at once; a band of
+// sqrt(per_xcd * BN / BM) tile rows makes that block of the output square.
+template <int BM, int BN>
+__device__ inline int tile_group(int64_t grid) {
+  const float per_xcd = (float)max<int64_t>(grid / 8, 1);
+  const int g = (int)(sqrtf(per_xcd * BN / BM) + 0.5f);
+  return min(max(g, 1), 32);
+}
+
+// Workgroup barrier ordering LDS accesses only: the fences are restricted to
+// the local address space, so global loads in flight stay in flight across it
+// (a full __syncthreads() waits for every outstanding global access).
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Row of accumulator register r in a 16x16 MFMA block for this lane: the
 // FP64 MFMA interleaves rows (lane/16 + 4r), the FP32-accumulating ones keep
 // four consecutive rows per lane (4*(lane/16) + r). The column is lane % 16.
@@ -31,12 +67,14 @@ __device__ inline int acc_row(int lane, int r) {
 // linear column-major order, so a wavefront stores 64 consecutive elements
 // (contiguous across columns when the tile spans the whole column).
 // `stage` holds 16 * (BM + 1) TOut values.
+// NW: wavefronts in the workgroup (WAVES_M along M, NW / WAVES_M along N).
 template <class TOut, bool kF64Layout, int BM, int BN, int WAVES_M, int TM, int TN,
-          class AccT>
+          int NW = 4, class AccT>
 __device__ inline void store_tile(const AccT (&acc)[TM][TN], TOut *__restrict__ c,
                                   const GemmDesc &d, int m0, int n0, TOut *stage) {
+  constexpr int kThreads = NW * 64;
   constexpr int WM = BM / WAVES_M;
-  constexpr int WN = BN / (4 / WAVES_M);
+  constexpr int WN = BN / (NW / WAVES_M);
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
