@@ -132,7 +132,7 @@ tq_status tq_batch_create(tq_handle_t h, int32_t n, const tq_plan_t *plans,
       d.b_off += (d.flags & kBPacked) ? pb_base[i] : b_offset[i];
       d.c_off += c_offset[i];
       desc.push_back(d);
-      cost.push_back((i64)d.m * d.n * (d.k + 4));
+      cost.push_back(pair_order_key(d.m, d.n, d.k));
       pl->h_m.push_back(d.m);
       pl->h_n.push_back(d.n);
       pl->h_k.push_back(d.k);

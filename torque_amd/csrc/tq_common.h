@@ -157,6 +157,17 @@ enum : int32_t {
 // one work item per kStreamRows-row segment of the output block, instead of
 // MFMA tiles.
 constexpr int kStreamMaxK = 8;
+
+// Work-list sort key of a block pair (larger first). A pair streamed by the
+// small-K kernel costs about its output writes plus its K-long dot products,
+// M * N * (K + 4). A pair cut into fixed-size MFMA tiles costs about K + 4
+// per tile, and the tile list is dealt round-robin, so ordering pairs by K
+// balances the tiles; it also keeps the pairs of one contracted sector (same
+// K, hence the same operand blocks) next to each other in their (A block,
+// B block) order, so tiles that share operands run together and share L2.
+__host__ __device__ inline int64_t pair_order_key(int64_t m, int64_t n, int64_t k) {
+  return k <= kStreamMaxK ? m * n * (k + 4) : k + 4;
+}
 constexpr int kStreamRows = 64;
 
 // For the split-BF16 tier the operands are the split panels (a_packed /

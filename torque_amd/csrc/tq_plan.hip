@@ -244,8 +244,7 @@ __global__ void geometry_kernel(GeomArgs g, const int32_t *__restrict__ pair_a,
   if (M * K > 0x7fffffff || N * K > 0x7fffffff) need |= kTooLarge;
   desc[p] = d;
   mn[p] = M * N;
-  // work estimate: MFMA work plus output writes
-  cost[p] = M * N * (K + 4);
+  cost[p] = pair_order_key(M, N, K);
   if (need) atomicOr(flags_any, need);
 }
 
