@@ -100,9 +100,17 @@ struct CopyJob {
 };
 
 // Workgroups of the pack kernels per job: kGatherPerWG elements each for a
-// plain gather, one tile of 64 rows x 32 padded k for a split-BF16 job.
+// plain gather. A split-BF16 job is cut into its (row, 8-value chunk) cells,
+// kSplitCells per workgroup, numbered rows fastest; when its first
+// contracted leg is contiguous in memory (k fastest) into tiles of kSplitRows
+// rows x kSplitK padded k instead, so that lanes read whole cache lines of
+// a row.
+constexpr int kSplitCells = 256;
 constexpr int kSplitRows = 64;
 constexpr int kSplitK = 32;
+__host__ __device__ inline bool split_kfast(const CopyJob &j) {
+  return j.nk > 0 && j.strides[0] == 1 && j.dims[0] > 1;
+}
 __host__ __device__ inline int64_t pack_job_wgs(const CopyJob &j);
 
 // Split-BF16 pack: like tq_launch_gather_jobs, reading FP64 and writing the
@@ -128,7 +136,9 @@ constexpr int kGatherPerWG = 2048;
 __host__ __device__ inline int64_t pack_job_wgs(const CopyJob &j) {
   if (j.n_elem <= 0) return 0;
   if (j.kdim <= 0) return ceil_div(j.n_elem, kGatherPerWG);
-  return ceil_div(j.n_elem / j.kdim, kSplitRows) * ceil_div(j.kpad, kSplitK);
+  const int64_t rows = j.n_elem / j.kdim;
+  if (split_kfast(j)) return ceil_div(rows, kSplitRows) * ceil_div(j.kpad, kSplitK);
+  return ceil_div(rows * (j.kpad / 8), kSplitCells);
 }
 
 // ---- grouped GEMM (tq_gemm.hip) ----------------------------------------

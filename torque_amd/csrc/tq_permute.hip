@@ -55,6 +55,25 @@ __device__ inline void gather_range(const CopyJob &job, int64_t e0,
   }
 }
 
+// Job of workgroup chunk c: the last j < n with start[j] <= c (start[0] = 0).
+// A 64-ary search: each round every lane of the wavefront probes one
+// candidate and a ballot keeps the last one at or below c, so a list of n
+// jobs costs ceil(log64 n) dependent loads (three for 2^18 jobs) where a
+// binary search costs log2 n. Called by whole wavefronts.
+__device__ inline int64_t find_job(const int32_t *__restrict__ start, int64_t n, int32_t c) {
+  const int lane = threadIdx.x & 63;
+  int64_t lo = 0, hi = n;  // the answer lies in [lo, hi); start[lo] <= c
+  while (hi - lo > 1) {
+    const int64_t step = (hi - lo + 63) >> 6;
+    const int64_t j = lo + lane * step;
+    const uint64_t m = __ballot(j < hi && start[j] <= c);
+    const int64_t nlo = lo + (int64_t)(63 - __clzll(m)) * step;
+    hi = min(hi, nlo + step);
+    lo = nlo;
+  }
+  return lo;
+}
+
 // Persistent gather over a job list: chunk c of the whole list belongs to job
 // j with job_wg_start[j] <= c < job_wg_start[j+1]; the chunk count is read
 // from job_wg_start[n_jobs] so no host round trip is needed.
@@ -64,15 +83,7 @@ __global__ __launch_bounds__(kGatherThreads) void gather_jobs_kernel(
     int64_t n_jobs, const W *__restrict__ src, W *__restrict__ dst) {
   const int32_t total = job_wg_start[n_jobs];
   for (int32_t c = blockIdx.x; c < total; c += gridDim.x) {
-    // binary search: last j with job_wg_start[j] <= c
-    int64_t lo = 0, hi = n_jobs - 1;
-    while (lo < hi) {
-      int64_t mid = (lo + hi + 1) >> 1;
-      if (job_wg_start[mid] <= c)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
+    const int64_t lo = find_job(job_wg_start, n_jobs, c);
     const CopyJob &job = jobs[lo];
     const int64_t e0 = (int64_t)(c - job_wg_start[lo]) * kGatherPerWG;
     gather_range(job, e0, src, dst);
@@ -416,6 +427,42 @@ __device__ inline double bf16_value(uint16_t h) {
   return (double)__uint_as_float((uint32_t)h << 16);
 }
 
+// e / t for any 32-bit e, from m = ceil(2^32 / t): umulhi(e, m) exceeds the
+// quotient by at most one.
+struct Div32 {
+  uint32_t t, m;
+  __device__ explicit Div32(uint32_t d)
+      : t(d), m((uint32_t)((((uint64_t)1 << 32) + d - 1) / d)) {}
+  __device__ inline uint32_t div(uint32_t e) const {
+    if (t == 1) return e;
+    const uint32_t q = __umulhi(e, m);
+    return (uint64_t)q * t > e ? q - 1 : q;
+  }
+};
+
+// Splits a value into its three BF16 pieces.
+__device__ inline void split3(double x, uint16_t &h, uint16_t &m, uint16_t &w) {
+  h = bf16_rne((float)x);
+  const double r1 = x - bf16_value(h);
+  m = bf16_rne((float)r1);
+  const double r2 = r1 - bf16_value(m);
+  w = bf16_rne((float)r2);
+}
+
+// Source offset of row `row` over the row legs (nk..rank-1) of a job.
+__device__ inline i64 split_row_offset(const CopyJob &job, uint32_t row) {
+  i64 o = job.src_off;
+#pragma unroll
+  for (int l = 0; l < TQ_MAX_RANK; ++l) {
+    if (l < job.nk || l >= job.rank) continue;
+    const uint32_t d = (uint32_t)job.dims[l];
+    const uint32_t q = row / d;
+    o += (i64)(row - q * d) * job.strides[l];
+    row = q;
+  }
+  return o;
+}
+
 // Split tile of a job whose first contracted leg is contiguous in memory
 // (k fastest): four lanes share a row's 8-value chunk, two values each, so
 // a wavefront reads 16 rows x 64 contiguous bytes and writes 256 contiguous
@@ -495,53 +542,46 @@ __device__ inline void split_kfast_tile(const CopyJob &job, int64_t rows, int64_
   }
 }
 
-// One workgroup per tile of 64 rows x 32 padded k of a job: thread t takes
-// row t % 64 and chunk t / 64 of the tile, reads its 8 values (lanes run
-// along the rows, so the reads coalesce when a row leg is fastest in memory),
-// splits them and writes one 16-byte vector to each plane (a wavefront
-// writes 1 KiB of contiguous output per plane). Values past kdim are zeros.
+// Split-BF16 pack. The work items of a job are its (row, 8-value chunk)
+// cells, numbered rows fastest (cell = chunk * rows + row), and a workgroup
+// takes the next kSplitCells of them: one per thread, which reads the cell's
+// 8 values (lanes run along the rows, so the reads coalesce when a row leg
+// is fastest in memory), splits them and writes one 16-byte vector to each
+// plane (a wavefront writes 1 KiB of contiguous output per plane). When the
+// first contracted leg is contiguous a workgroup takes a tile of kSplitRows
+// rows x kSplitK padded k instead (split_kfast_tile). Values past kdim are
+// zeros. Numbering the cells across the
+// whole job keeps every lane busy for jobs of few rows or few chunks.
 __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
     const CopyJob *__restrict__ jobs, const int32_t *__restrict__ job_wg_start,
     int64_t n_jobs, const double *__restrict__ src, uint16_t *__restrict__ dst,
     int64_t plane) {
   const int32_t total = job_wg_start[n_jobs];
   for (int32_t c = blockIdx.x; c < total; c += gridDim.x) {
-    int64_t lo = 0, hi = n_jobs - 1;
-    while (lo < hi) {
-      int64_t mid = (lo + hi + 1) >> 1;
-      if (job_wg_start[mid] <= c)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
+    const int64_t lo = find_job(job_wg_start, n_jobs, c);
     const CopyJob &job = jobs[lo];
     const int nk = job.nk;
     const int64_t kd = job.kdim;
-    const int64_t rows = job.n_elem / kd;
-    const int64_t k_tiles = ceil_div(job.kpad, kSplitK);
-    const int64_t tile = c - job_wg_start[lo];
-    if (nk > 0 && job.strides[0] == 1 && job.dims[0] > 1) {
+    // a split job holds at most 2^31 values (the planner packs no larger
+    // panel), so its cells index in 32 bits
+    const uint32_t rows = (uint32_t)(job.n_elem / kd);
+    const uint32_t cells = rows * (uint32_t)(job.kpad >> 3);
+    const uint32_t tile = (uint32_t)(c - job_wg_start[lo]);
+    if (split_kfast(job)) {
+      const uint32_t k_tiles = (uint32_t)ceil_div(job.kpad, kSplitK);
       split_kfast_tile(job, rows, (tile / k_tiles) * kSplitRows,
                        (tile % k_tiles) * (kSplitK / 8) + (threadIdx.x >> 6), src, dst, plane);
       continue;
     }
-    const int64_t row = (tile / k_tiles) * kSplitRows + (threadIdx.x & (kSplitRows - 1));
-    const int64_t chunk = (tile % k_tiles) * (kSplitK / 8) + (threadIdx.x >> 6);
-    if (row >= rows || chunk * 8 >= job.kpad) continue;  // no barrier in this loop
+    const Div32 rdiv(rows);
+    const uint32_t cell = tile * kSplitCells + threadIdx.x;
+    if (cell >= cells) continue;  // no barrier in this loop
+    const uint32_t chunk = rdiv.div(cell), row = cell - chunk * rows;
     // source offset of (row, k = 8 * chunk): rows decode over legs nk.., k
     // over legs 0..nk-1 (leg 0 fastest); k digits advance with carries
-    i64 off = job.src_off;
-    uint32_t rem = (uint32_t)row;
-#pragma unroll
-    for (int l = 0; l < TQ_MAX_RANK; ++l) {
-      if (l < nk || l >= job.rank) continue;
-      const uint32_t d = (uint32_t)job.dims[l];
-      const uint32_t q = rem / d;
-      off += (i64)(rem - q * d) * job.strides[l];
-      rem = q;
-    }
+    i64 off = split_row_offset(job, row);
     int32_t dig[TQ_MAX_RANK];
-    rem = (uint32_t)(chunk * 8);
+    uint32_t rem = chunk * 8;
 #pragma unroll
     for (int l = 0; l < TQ_MAX_RANK; ++l) {
       dig[l] = 0;
@@ -573,18 +613,11 @@ __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
     }
     double xv[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) xv[e] = (chunk * 8 + e < kd) ? src[offs[e]] : 0.0;
+    for (int e = 0; e < 8; ++e) xv[e] = ((int64_t)chunk * 8 + e < kd) ? src[offs[e]] : 0.0;
     uint16_t h[8], m[8], w[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const double x = xv[e];
-      h[e] = bf16_rne((float)x);
-      const double r1 = x - bf16_value(h[e]);
-      m[e] = bf16_rne((float)r1);
-      const double r2 = r1 - bf16_value(m[e]);
-      w[e] = bf16_rne((float)r2);
-    }
-    const int64_t o = job.dst_off + chunk * rows * 8 + row * 8;
+    for (int e = 0; e < 8; ++e) split3(xv[e], h[e], m[e], w[e]);
+    const int64_t o = job.dst_off + ((int64_t)chunk * rows + row) * 8;
     uint4 vh, vm, vw;
     vh.x = h[0] | ((uint32_t)h[1] << 16);
     vh.y = h[2] | ((uint32_t)h[3] << 16);
