@@ -147,6 +147,7 @@ tq_status tq_batch_create(tq_handle_t h, int32_t n, const tq_plan_t *plans,
   if (pl->n_pairs > 0x7fffffff)
     return tq_fail(h, TQ_ERROR_UNSUPPORTED, "batch: too many block pairs");
   TQ_CHECK(upload(h, pl->desc, desc));
+  pl->h_desc = desc;
   DevBuf dcost;
   TQ_CHECK(upload(h, dcost, cost));
   TQ_HIP(h, pl->order.alloc(sizeof(i64) * std::max<int64_t>(pl->n_pairs, 1)));

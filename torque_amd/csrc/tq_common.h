@@ -163,6 +163,18 @@ enum : int32_t {
   kBPacked = 8,
 };
 
+// A run: pairs p0 .. p0 + np - 1, consecutive in pair order, that share their
+// A operand (same a_off, M, K, lda and A flags) and whose output blocks
+// follow each other in C. Together they are one GEMM
+// C(M x n) = A(M x K) [B_p0; ...; B_p(np-1)]^T with n the sum of their N and
+// the output contiguous (ldc = M); the wave-tile kernel finds the B operand of
+// each output column in the run's pairs.
+struct RunDesc {
+  i64 a_off, c_off;
+  int32_t m, n, k, lda, flags;
+  int32_t p0, np;
+};
+
 // Pairs with K <= kStreamMaxK are written by the streaming small-K kernel,
 // one work item per kStreamRows-row segment of the output block, instead of
 // MFMA tiles.
@@ -206,11 +218,12 @@ int tq_gemm_n_configs(tq_precision prec);
 GemmConfig tq_gemm_config(tq_precision prec, int cfg);
 
 // Launch the grouped GEMM over `tile_pair` (pair index per tile, in work-list
-// order) with tile_start[p] = first tile of pair p.
+// order) with tile_start[p] = first tile of pair p; or, with runs != null (a
+// wave-tile configuration only), over runs: tile_pair then holds run indices.
 tq_status tq_launch_grouped_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec,
-                                 int cfg, const GemmDesc *descs,
+                                 int cfg, const GemmDesc *descs, const RunDesc *runs,
                                  const int32_t *tile_pair,
-                                 const int32_t *tile_start, int64_t n_tiles,
+                                 const int32_t *tile_first, int64_t n_tiles,
                                  const void *a, const void *a_packed,
                                  const void *b, const void *b_packed, void *c);
 

@@ -99,6 +99,7 @@ tq_status launch_class(tq_plan_t pl, int cls, const void *a, const void *b, void
                                  pl->pack_b.panel.as<uint16_t>(), pl->pack_a.panel_elems,
                                  pl->pack_b.panel_elems, reinterpret_cast<double *>(c));
   return tq_launch_grouped_gemm(h, pl->dtype, pl->prec, tc.tiles_cfg, pl->desc.as<GemmDesc>(),
+                                tc.runs ? pl->run_desc.as<RunDesc>() : nullptr,
                                 tc.tile_pair.as<int32_t>(), tc.tile_first.as<int32_t>(),
                                 tc.n_tiles, a, pl->pack_a.panel.p, b, pl->pack_b.panel.p, c);
 }
@@ -222,6 +223,7 @@ tq_status tq_plan_set_small_k(tq_plan_t pl, int32_t max_k) {
     return tq_fail(pl->h, TQ_ERROR_INVALID_VALUE, "small-K threshold above 8");
   pl->stream_max_k = max_k;
   pl->chunks_ready = false;
+  pl->runs_built = false;  // runs hold pairs of one class
   for (int c = 0; c < 2; ++c) {
     pl->cls[c].ready = false;
     pl->cls[c].auto_cfg = tq_plan_pick_config(pl, c);

@@ -282,10 +282,10 @@ GemmConfig tq_bf16x3_config(int cfg);
 int tq_wave_n_configs();
 GemmConfig tq_wave_config(int cfg);
 tq_status tq_launch_wave_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec, int cfg,
-                              const GemmDesc *descs, const int32_t *tile_pair,
-                              const int32_t *tile_first, int64_t n_tiles, const void *a,
-                              const void *a_packed, const void *b, const void *b_packed,
-                              void *c);
+                              const GemmDesc *descs, const RunDesc *runs,
+                              const int32_t *tile_pair, const int32_t *tile_first,
+                              int64_t n_tiles, const void *a, const void *a_packed,
+                              const void *b, const void *b_packed, void *c);
 
 // FP64 / FP32 configurations: the workgroup-tile kernel's (0 .. kNCfg-1),
 // then the wave-tile kernel's (tq_wave_gemm.hip).
@@ -306,7 +306,7 @@ GemmConfig tq_gemm_config(tq_precision prec, int cfg) {
 }
 
 tq_status tq_launch_grouped_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec,
-                                 int cfg, const GemmDesc *descs,
+                                 int cfg, const GemmDesc *descs, const RunDesc *runs,
                                  const int32_t *tile_pair,
                                  const int32_t *tile_first, int64_t n_tiles,
                                  const void *a, const void *a_packed,
@@ -315,8 +315,10 @@ tq_status tq_launch_grouped_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec,
   if (prec == TQ_PREC_BF16X3)
     return tq_fail(h, TQ_ERROR_INVALID_VALUE, "split-BF16 tier runs on split panels");
   if (cfg >= kNCfg)
-    return tq_launch_wave_gemm(h, dt, prec, cfg - kNCfg, descs, tile_pair, tile_first, n_tiles,
-                               a, a_packed, b, b_packed, c);
+    return tq_launch_wave_gemm(h, dt, prec, cfg - kNCfg, descs, runs, tile_pair, tile_first,
+                               n_tiles, a, a_packed, b, b_packed, c);
+  if (runs)
+    return tq_fail(h, TQ_ERROR_INVALID_VALUE, "gemm: runs need a wave-tile configuration");
   if (dt == TQ_F32)
     return dispatch<float, float, float>(h, cfg, descs, tile_pair, tile_first,
                                          n_tiles, a, a_packed, b, b_packed, c);

@@ -82,6 +82,7 @@ struct TileClass {
   int64_t n_tiles = 0;
   int tiles_cfg = -1;
   bool ready = false;
+  bool runs = false;  // the tile list indexes runs (wave-tile configurations)
   int cfg = -1;  // user-selected configuration (-1 = automatic)
   int auto_cfg = 0;
   int current() const { return cfg >= 0 ? cfg : auto_cfg; }
@@ -139,6 +140,13 @@ struct tq_plan_s {
     if (graph) (void)hipGraphDestroy(graph);
     if (capture_stream) (void)hipStreamDestroy(capture_stream);
   }
+
+  // runs of pairs sharing an A operand with adjacent outputs (built on the
+  // host from h_desc when a class first uses a wave-tile configuration)
+  std::vector<GemmDesc> h_desc;
+  std::vector<RunDesc> h_runs;
+  DevBuf run_desc;
+  bool runs_built = false;
 
   // host copies
   std::vector<i64> h_m, h_n, h_k;

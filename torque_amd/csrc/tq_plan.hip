@@ -617,6 +617,83 @@ static tq_status expand_work(tq_plan_t pl, int32_t bm, int32_t bn, int32_t cls,
   return TQ_SUCCESS;
 }
 
+// Runs: maximal stretches of consecutive pairs (pair order = output order)
+// of one MFMA tile class that share their A operand and whose output blocks
+// follow each other, each with its tile class; then listed by K, largest
+// first (stable), as the pairs of the work list are.
+static tq_status build_runs(tq_plan_t pl) {
+  tq_handle_t h = pl->h;
+  const int64_t np = pl->n_pairs;
+  if ((int64_t)pl->h_desc.size() != np)
+    return tq_fail(h, TQ_ERROR_INVALID_VALUE, "contract: host descriptors missing");
+  std::vector<RunDesc> runs;
+  int prev_cls = -2;
+  for (int64_t p = 0; p < np; ++p) {
+    const GemmDesc &d = pl->h_desc[p];
+    const int cls = pair_class(*pl, d.m, d.n, d.k);
+    if (cls < 0) {
+      prev_cls = -2;
+      continue;
+    }
+    const int32_t af = d.flags & (kAKFast | kAPacked);
+    if (!runs.empty() && cls == prev_cls) {
+      RunDesc &r = runs.back();
+      const GemmDesc &last = pl->h_desc[r.p0 + r.np - 1];
+      if (r.p0 + r.np == p && r.a_off == d.a_off && r.m == d.m && r.k == d.k &&
+          r.lda == d.lda && r.flags == af && last.c_off + (i64)last.m * last.n == d.c_off &&
+          (i64)r.n + d.n <= 0x7fffffff) {
+        r.n += d.n;
+        ++r.np;
+        continue;
+      }
+    }
+    runs.push_back(RunDesc{d.a_off, d.c_off, d.m, d.n, d.k, d.lda, af, (int32_t)p, 1});
+    prev_cls = cls;
+  }
+  std::stable_sort(runs.begin(), runs.end(), [](const RunDesc &x, const RunDesc &y) {
+    return 63 - __builtin_clzll((unsigned long long)pair_order_key(x.m, x.n, x.k)) >
+           63 - __builtin_clzll((unsigned long long)pair_order_key(y.m, y.n, y.k));
+  });
+  pl->h_runs.swap(runs);
+  TQ_HIP(h, pl->run_desc.alloc(sizeof(RunDesc) * std::max<size_t>(pl->h_runs.size(), 1)));
+  if (!pl->h_runs.empty())
+    TQ_HIP(h, hipMemcpyAsync(pl->run_desc.p, pl->h_runs.data(),
+                             sizeof(RunDesc) * pl->h_runs.size(), hipMemcpyHostToDevice,
+                             h->stream));
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  pl->runs_built = true;
+  return TQ_SUCCESS;
+}
+
+// Tile list of the runs of class cls under a bm x bn tile: run index per tile
+// and each run's first tile.
+static tq_status build_run_tiles(tq_plan_t pl, int cls, int32_t bm, int32_t bn,
+                                 TileClass &tc) {
+  tq_handle_t h = pl->h;
+  const int64_t nr = (int64_t)pl->h_runs.size();
+  std::vector<int32_t> first(std::max<int64_t>(nr, 1), 0), tiles;
+  for (int64_t u = 0; u < nr; ++u) {
+    const RunDesc &r = pl->h_runs[u];
+    first[u] = (int32_t)tiles.size();
+    // every pair of a run is in one class
+    if (pair_class(*pl, r.m, pl->h_desc[r.p0].n, r.k) != cls) continue;
+    const int64_t nt = ceil_div(r.m, bm) * ceil_div(r.n, bn);
+    if ((int64_t)tiles.size() + nt > 0x7fffffff)
+      return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many GEMM tiles");
+    tiles.insert(tiles.end(), (size_t)nt, (int32_t)u);
+  }
+  TQ_HIP(h, tc.tile_pair.alloc(sizeof(int32_t) * std::max<size_t>(tiles.size(), 1)));
+  TQ_HIP(h, tc.tile_first.alloc(sizeof(int32_t) * first.size()));
+  if (!tiles.empty())
+    TQ_HIP(h, hipMemcpyAsync(tc.tile_pair.p, tiles.data(), sizeof(int32_t) * tiles.size(),
+                             hipMemcpyHostToDevice, h->stream));
+  TQ_HIP(h, hipMemcpyAsync(tc.tile_first.p, first.data(), sizeof(int32_t) * first.size(),
+                           hipMemcpyHostToDevice, h->stream));
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  tc.n_tiles = (int64_t)tiles.size();
+  return TQ_SUCCESS;
+}
+
 tq_status plan_build_tiles(tq_plan_t pl) {
   tq_handle_t h = pl->h;
   const int64_t np = pl->n_pairs;
@@ -638,6 +715,27 @@ tq_status plan_build_tiles(tq_plan_t pl) {
     const int cfg = tc.current();
     if (!(tc.ready && tc.tiles_cfg == cfg)) {
       const GemmConfig g = tq_gemm_config(pl->prec, cfg);
+      // a wave-tile configuration tiles runs of pairs (FP64 and FP32 tiers)
+      // when runs join pairs of this class (otherwise pair tiles are the same
+      // work with cheaper B addressing)
+      if (g.wave_tile && pl->prec != TQ_PREC_BF16X3) {
+        if (!pl->runs_built) TQ_CHECK(build_runs(pl));
+        int64_t n_runs = 0, n_cls = 0;
+        for (const RunDesc &r : pl->h_runs)
+          if (pair_class(*pl, r.m, pl->h_desc[r.p0].n, r.k) == c) {
+            ++n_runs;
+            n_cls += r.np;
+          }
+        if (n_runs < n_cls) {
+          TQ_CHECK(build_run_tiles(pl, c, g.bm, g.bn, tc));
+          tc.runs = true;
+          tc.tiles_cfg = cfg;
+          tc.ready = true;
+          pl->n_tiles += tc.n_tiles;
+          continue;
+        }
+      }
+      tc.runs = false;
       int64_t total = 0;
       for (int64_t p = 0; p < np; ++p)
         if (pair_class(*pl, pl->h_m[p], pl->h_n[p], pl->h_k[p]) == c)
@@ -1057,6 +1155,7 @@ tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
     TQ_HIP(h, hipStreamSynchronize(h->stream));
     for (int64_t p = 0; p < np; ++p)
       pl->max_ld = std::max<int64_t>(pl->max_ld, std::max(hd[p].lda, hd[p].ldb));
+    pl->h_desc = hd;
 
     // ---- 5. work list: stable device sort by cost, largest first ----
     TQ_HIP(h, pl->order.alloc(sizeof(i64) * np));

@@ -89,6 +89,32 @@ __device__ inline WaveTile<TIn> wave_tile(int64_t t, const GemmDesc *__restrict_
   return w;
 }
 
+// The A side and output of tile t of a run (B comes from run_columns).
+template <class TIn, int BM, int BN>
+__device__ inline WaveTile<TIn> run_tile(int64_t t, const RunDesc *__restrict__ runs,
+                                         const int32_t *__restrict__ tile_pair,
+                                         const int32_t *__restrict__ tile_first, const TIn *a,
+                                         const TIn *a_packed, RunDesc &r) {
+  WaveTile<TIn> w;
+  const int u = __builtin_amdgcn_readfirstlane(tile_pair[t]);
+  r = runs[u];
+  const int lt = (int)(t - tile_first[u]);
+  const int tiles_m = (r.m + BM - 1) / BM;
+  w.m = r.m;
+  w.n = r.n;
+  w.k = r.k;
+  w.m0 = (lt % tiles_m) * BM;
+  w.n0 = (lt / tiles_m) * BN;
+  w.c_off = r.c_off;
+  const bool akf = r.flags & kAKFast;
+  w.sra = akf ? r.lda : 1;
+  w.ska = akf ? 1 : r.lda;
+  w.srb = w.skb = 0;
+  w.pa = ((r.flags & kAPacked) ? a_packed : a) + r.a_off + (int64_t)w.m0 * w.sra;
+  w.pb = nullptr;
+  return w;
+}
+
 // Byte offsets, from element (row0, k0) of the operand, of this lane's KPL
 // k values (k0 + 4s + g) in each of its T row blocks, for a k tile that lies
 // inside K. Rows past the block are clamped to its last row (their products
@@ -139,6 +165,57 @@ __device__ inline void load_k(TC (&r)[T][KPL], const TIn *base, const uint32_t (
     for (int s = 0; s < KPL; ++s) r[i][s] = (TC)*reinterpret_cast<const TIn *>(pb + o[i][s]);
 }
 
+// The B operand of a run tile, per lane: for each of the lane's TN column
+// blocks, element (column, k = 0) of its column and the k stride. Columns
+// past the run are clamped to its last column (their products only reach
+// unstored output). The run's pairs are walked in order, summing their N.
+template <class TIn, int TN>
+struct LaneCols {
+  const TIn *p[TN];
+  int32_t sk[TN];
+};
+
+template <class TIn, int TN>
+__device__ inline void run_columns(LaneCols<TIn, TN> &lc, const RunDesc &r, int n0,
+                                   const GemmDesc *__restrict__ descs, const TIn *b,
+                                   const TIn *b_packed, int lane) {
+  int col[TN];
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    col[j] = min(n0 + 16 * j + (lane & 15), r.n - 1);
+    lc.p[j] = b;
+    lc.sk[j] = 1;
+  }
+  int start = 0;
+  for (int q = 0; q < r.np; ++q) {
+    const GemmDesc d = descs[r.p0 + q];
+    const bool bkf = d.flags & kBKFast;
+    const TIn *base = ((d.flags & kBPacked) ? b_packed : b) + d.b_off;
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+      if (col[j] >= start && col[j] < start + d.n) {
+        lc.p[j] = base + (int64_t)(col[j] - start) * (bkf ? d.ldb : 1);
+        lc.sk[j] = bkf ? 1 : d.ldb;
+      }
+    start += d.n;
+  }
+}
+
+// Loads one k tile from per-lane columns: the lane's KPL k values (k0 + 4s +
+// g) of each of its T columns, k clamped to K - 1 in the k tile that crosses
+// K (tail_select zeroes those values).
+template <class TIn, class TC, int T, int KPL>
+__device__ inline void load_k_cols(TC (&r)[T][KPL], const LaneCols<TIn, T> &lc, int k0, int K,
+                                   int lane) {
+  const int g = lane >> 4;
+#pragma unroll
+  for (int s = 0; s < KPL; ++s) {
+    const int64_t kk = min(k0 + 4 * s + g, K - 1);
+#pragma unroll
+    for (int j = 0; j < T; ++j) r[j][s] = (TC)lc.p[j][kk * lc.sk[j]];
+  }
+}
+
 // Zeroes this lane's operand values past K (kvalid = K - k0 - g) in the k
 // tile that crosses K.
 template <class TC, int T, int KPL>
@@ -150,13 +227,14 @@ __device__ inline void tail_select(TC (&r)[T][KPL], int kvalid) {
 }
 
 // MINW: wavefronts per SIMD the register allocation must allow; 2 keeps a
-// second wavefront resident to cover the first one's operand waits.
-template <class TIn, class TC, class TOut, int TM, int TN, int KPL, int MINW>
+// second wavefront resident to cover the first one's operand waits. RUNS:
+// the tile list indexes runs (RunDesc) rather than pairs.
+template <class TIn, class TC, class TOut, int TM, int TN, int KPL, int MINW, bool RUNS>
 __global__ __launch_bounds__(256, MINW) void wave_gemm_kernel(
-    const GemmDesc *__restrict__ descs, const int32_t *__restrict__ tile_pair,
-    const int32_t *__restrict__ tile_first, int64_t n_tiles, const TIn *__restrict__ a,
-    const TIn *__restrict__ a_packed, const TIn *__restrict__ b,
-    const TIn *__restrict__ b_packed, TOut *__restrict__ c) {
+    const GemmDesc *__restrict__ descs, const RunDesc *__restrict__ runs,
+    const int32_t *__restrict__ tile_pair, const int32_t *__restrict__ tile_first,
+    int64_t n_tiles, const TIn *__restrict__ a, const TIn *__restrict__ a_packed,
+    const TIn *__restrict__ b, const TIn *__restrict__ b_packed, TOut *__restrict__ c) {
   typedef typename Acc<TC>::type AccT;
   constexpr int BM = 16 * TM, BN = 16 * TN, BK = 4 * KPL;
   constexpr bool kF64 = sizeof(TC) == 8;
@@ -166,24 +244,40 @@ __global__ __launch_bounds__(256, MINW) void wave_gemm_kernel(
   int64_t t = uniform64(tqg::xcd_virtual_id(blockIdx.x, gridDim.x) * 4 + wave);
   if (t >= n_tiles) return;
 
-  WaveTile<TIn> cur =
-      wave_tile<TIn, BM, BN>(t, descs, tile_pair, tile_first, a, a_packed, b, b_packed);
+  RunDesc run_cur, run_nxt;
+  WaveTile<TIn> cur;
+  if constexpr (RUNS)
+    cur = run_tile<TIn, BM, BN>(t, runs, tile_pair, tile_first, a, a_packed, run_cur);
+  else
+    cur = wave_tile<TIn, BM, BN>(t, descs, tile_pair, tile_first, a, a_packed, b, b_packed);
   uint32_t offa[TM][KPL], offb[TN][KPL];
+  LaneCols<TIn, TN> cols;
   tile_offsets<TIn, TM, KPL>(offa, cur.sra, cur.ska, cur.m, cur.m0, lane);
-  tile_offsets<TIn, TN, KPL>(offb, cur.srb, cur.skb, cur.n, cur.n0, lane);
+  if constexpr (RUNS)
+    run_columns<TIn, TN>(cols, run_cur, cur.n0, descs, b, b_packed, lane);
+  else
+    tile_offsets<TIn, TN, KPL>(offb, cur.srb, cur.skb, cur.n, cur.n0, lane);
   // (na, nb) always hold the prefetched next k tile; (ra, rb) the current one
   TC ra[TM][KPL], rb[TN][KPL], na[TM][KPL], nb[TN][KPL];
   load_k<TIn, TC, TM, KPL>(na, cur.pa, offa, cur.ska, cur.k, lane);
-  load_k<TIn, TC, TN, KPL>(nb, cur.pb, offb, cur.skb, cur.k, lane);
+  if constexpr (RUNS)
+    load_k_cols<TIn, TC, TN, KPL>(nb, cols, 0, cur.k, lane);
+  else
+    load_k<TIn, TC, TN, KPL>(nb, cur.pb, offb, cur.skb, cur.k, lane);
   while (true) {
     // the next wave tile is known from the start, so the prefetch of its first
     // k tile needs no wait (the last tile prefetches itself again)
     const int64_t t_next = t + stride;
     const bool more = t_next < n_tiles;
     WaveTile<TIn> nxt = cur;
-    if (more)
-      nxt = wave_tile<TIn, BM, BN>(t_next, descs, tile_pair, tile_first, a, a_packed, b,
-                                   b_packed);
+    if constexpr (RUNS) run_nxt = run_cur;
+    if (more) {
+      if constexpr (RUNS)
+        nxt = run_tile<TIn, BM, BN>(t_next, runs, tile_pair, tile_first, a, a_packed, run_nxt);
+      else
+        nxt = wave_tile<TIn, BM, BN>(t_next, descs, tile_pair, tile_first, a, a_packed, b,
+                                     b_packed);
+    }
     AccT acc[TM][TN];
 #pragma unroll
     for (int i = 0; i < TM; ++i)
@@ -202,15 +296,23 @@ __global__ __launch_bounds__(256, MINW) void wave_gemm_kernel(
       if (kn < cur.k) {
         load_k<TIn, TC, TM, KPL>(na, cur.pa + (int64_t)kn * cur.ska, offa, cur.ska, cur.k - kn,
                                  lane);
-        load_k<TIn, TC, TN, KPL>(nb, cur.pb + (int64_t)kn * cur.skb, offb, cur.skb, cur.k - kn,
-                                 lane);
+        if constexpr (RUNS)
+          load_k_cols<TIn, TC, TN, KPL>(nb, cols, kn, cur.k, lane);
+        else
+          load_k<TIn, TC, TN, KPL>(nb, cur.pb + (int64_t)kn * cur.skb, offb, cur.skb,
+                                   cur.k - kn, lane);
       } else {
         // the current tile's loads are all issued: its offsets become the
         // next tile's
         tile_offsets<TIn, TM, KPL>(offa, nxt.sra, nxt.ska, nxt.m, nxt.m0, lane);
-        tile_offsets<TIn, TN, KPL>(offb, nxt.srb, nxt.skb, nxt.n, nxt.n0, lane);
         load_k<TIn, TC, TM, KPL>(na, nxt.pa, offa, nxt.ska, nxt.k, lane);
-        load_k<TIn, TC, TN, KPL>(nb, nxt.pb, offb, nxt.skb, nxt.k, lane);
+        if constexpr (RUNS) {
+          run_columns<TIn, TN>(cols, run_nxt, nxt.n0, descs, b, b_packed, lane);
+          load_k_cols<TIn, TC, TN, KPL>(nb, cols, 0, nxt.k, lane);
+        } else {
+          tile_offsets<TIn, TN, KPL>(offb, nxt.srb, nxt.skb, nxt.n, nxt.n0, lane);
+          load_k<TIn, TC, TN, KPL>(nb, nxt.pb, offb, nxt.skb, nxt.k, lane);
+        }
       }
       // keep these loads ahead of the MFMAs: their latency hides under them
       __builtin_amdgcn_sched_barrier(0);
@@ -243,6 +345,7 @@ __global__ __launch_bounds__(256, MINW) void wave_gemm_kernel(
     if (!more) break;
     t = t_next;
     cur = nxt;
+    if constexpr (RUNS) run_cur = run_nxt;
   }
 }
 
@@ -255,16 +358,17 @@ constexpr WaveCfg kWaveCfgs[] = {{1, 1, 4, 1}, {2, 1, 4, 1}, {1, 2, 4, 1},
                                  {2, 4, 2, 1}, {4, 4, 2, 1}, {4, 4, 2, 2}};
 constexpr int kNWave = sizeof(kWaveCfgs) / sizeof(kWaveCfgs[0]);
 
-template <class TIn, class TC, class TOut, int TM, int TN, int KPL, int MINW>
-tq_status launch_one(tq_handle_t h, const GemmDesc *descs, const int32_t *tile_pair,
-                     const int32_t *tile_first, int64_t n_tiles, const void *a,
-                     const void *ap, const void *b, const void *bp, void *c) {
+template <class TIn, class TC, class TOut, int TM, int TN, int KPL, int MINW, bool RUNS>
+tq_status launch_one(tq_handle_t h, const GemmDesc *descs, const RunDesc *runs,
+                     const int32_t *tile_pair, const int32_t *tile_first, int64_t n_tiles,
+                     const void *a, const void *ap, const void *b, const void *bp, void *c) {
   // persistent grid: the workgroups that fit at once, a multiple of the 8 XCDs
   static int per_cu = 0;
   if (per_cu == 0) {
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &occ, (wave_gemm_kernel<TIn, TC, TOut, TM, TN, KPL, MINW>), 256, 0) != hipSuccess ||
+            &occ, (wave_gemm_kernel<TIn, TC, TOut, TM, TN, KPL, MINW, RUNS>), 256, 0) !=
+            hipSuccess ||
         occ < 1)
       occ = 1;
     per_cu = occ;
@@ -272,23 +376,37 @@ tq_status launch_one(tq_handle_t h, const GemmDesc *descs, const int32_t *tile_p
   int64_t grid = std::min<int64_t>(ceil_div(n_tiles, (int64_t)4), (int64_t)h->n_cu * per_cu);
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((wave_gemm_kernel<TIn, TC, TOut, TM, TN, KPL, MINW>), dim3((unsigned)grid),
-                     dim3(256), 0, h->stream, descs, tile_pair, tile_first, n_tiles,
-                     reinterpret_cast<const TIn *>(a), reinterpret_cast<const TIn *>(ap),
-                     reinterpret_cast<const TIn *>(b), reinterpret_cast<const TIn *>(bp),
-                     reinterpret_cast<TOut *>(c));
+  hipLaunchKernelGGL((wave_gemm_kernel<TIn, TC, TOut, TM, TN, KPL, MINW, RUNS>),
+                     dim3((unsigned)grid), dim3(256), 0, h->stream, descs, runs, tile_pair,
+                     tile_first, n_tiles, reinterpret_cast<const TIn *>(a),
+                     reinterpret_cast<const TIn *>(ap), reinterpret_cast<const TIn *>(b),
+                     reinterpret_cast<const TIn *>(bp), reinterpret_cast<TOut *>(c));
   TQ_HIP(h, hipGetLastError());
   return TQ_SUCCESS;
 }
 
+template <class TIn, class TC, class TOut, int TM, int TN, int KPL, int MINW>
+tq_status launch_cfg(tq_handle_t h, const GemmDesc *descs, const RunDesc *runs,
+                     const int32_t *tile_pair, const int32_t *tile_first, int64_t n_tiles,
+                     const void *a, const void *ap, const void *b, const void *bp, void *c) {
+  if (runs)
+    return launch_one<TIn, TC, TOut, TM, TN, KPL, MINW, true>(h, descs, runs, tile_pair,
+                                                              tile_first, n_tiles, a, ap, b,
+                                                              bp, c);
+  return launch_one<TIn, TC, TOut, TM, TN, KPL, MINW, false>(h, descs, runs, tile_pair,
+                                                             tile_first, n_tiles, a, ap, b, bp,
+                                                             c);
+}
+
 template <class TIn, class TC, class TOut>
-tq_status dispatch(tq_handle_t h, int cfg, const GemmDesc *descs, const int32_t *tile_pair,
-                   const int32_t *tile_first, int64_t n_tiles, const void *a, const void *ap,
-                   const void *b, const void *bp, void *c) {
-#define TQ_WCFG(I, TM, TN, KPL, MINW)                                                    \
-  case I:                                                                                \
-    return launch_one<TIn, TC, TOut, TM, TN, KPL, MINW>(h, descs, tile_pair, tile_first, \
-                                                        n_tiles, a, ap, b, bp, c);
+tq_status dispatch(tq_handle_t h, int cfg, const GemmDesc *descs, const RunDesc *runs,
+                   const int32_t *tile_pair, const int32_t *tile_first, int64_t n_tiles,
+                   const void *a, const void *ap, const void *b, const void *bp, void *c) {
+#define TQ_WCFG(I, TM, TN, KPL, MINW)                                                     \
+  case I:                                                                                 \
+    return launch_cfg<TIn, TC, TOut, TM, TN, KPL, MINW>(h, descs, runs, tile_pair,        \
+                                                        tile_first, n_tiles, a, ap, b, bp, \
+                                                        c);
   switch (cfg) {
     TQ_WCFG(0, 1, 1, 4, 1)
     TQ_WCFG(1, 2, 1, 4, 1)
@@ -315,17 +433,17 @@ GemmConfig tq_wave_config(int cfg) {
 }
 
 tq_status tq_launch_wave_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec, int cfg,
-                              const GemmDesc *descs, const int32_t *tile_pair,
-                              const int32_t *tile_first, int64_t n_tiles, const void *a,
-                              const void *a_packed, const void *b, const void *b_packed,
-                              void *c) {
+                              const GemmDesc *descs, const RunDesc *runs,
+                              const int32_t *tile_pair, const int32_t *tile_first,
+                              int64_t n_tiles, const void *a, const void *a_packed,
+                              const void *b, const void *b_packed, void *c) {
   if (n_tiles <= 0) return TQ_SUCCESS;
   if (dt == TQ_F32)
-    return dispatch<float, float, float>(h, cfg, descs, tile_pair, tile_first, n_tiles, a,
-                                         a_packed, b, b_packed, c);
+    return dispatch<float, float, float>(h, cfg, descs, runs, tile_pair, tile_first, n_tiles,
+                                         a, a_packed, b, b_packed, c);
   if (prec == TQ_PREC_F32)
-    return dispatch<double, float, double>(h, cfg, descs, tile_pair, tile_first, n_tiles, a,
-                                           a_packed, b, b_packed, c);
-  return dispatch<double, double, double>(h, cfg, descs, tile_pair, tile_first, n_tiles, a,
-                                          a_packed, b, b_packed, c);
+    return dispatch<double, float, double>(h, cfg, descs, runs, tile_pair, tile_first, n_tiles,
+                                           a, a_packed, b, b_packed, c);
+  return dispatch<double, double, double>(h, cfg, descs, runs, tile_pair, tile_first, n_tiles,
+                                          a, a_packed, b, b_packed, c);
 }
