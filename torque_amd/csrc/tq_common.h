@@ -80,6 +80,7 @@ __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) {
 // One strided-gather job: dst[dst_off + e] = src[src_off + sum_l d_l * s_l]
 // where e is the column-major linear index over dims (leg 0 fastest).
 struct GemmDesc;
+struct RunDesc;
 
 // For the split-BF16 pack (kdim > 0) the first nk legs are the contracted
 // ones (they multiply to kdim) and the rest the rows (n_elem / kdim of
@@ -120,7 +121,7 @@ tq_status tq_launch_split_jobs(tq_handle_t h, const CopyJob *jobs,
                                int64_t total_wg, const double *src,
                                uint16_t *dst, int64_t plane);
 tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, const GemmDesc *descs,
-                                const int32_t *tile_pair,
+                                const RunDesc *runs, const int32_t *tile_pair,
                                 const int32_t *tile_first, int64_t n_tiles,
                                 const uint16_t *a_split, const uint16_t *b_split,
                                 int64_t a_plane, int64_t b_plane, double *c);

@@ -94,6 +94,7 @@ tq_status launch_class(tq_plan_t pl, int cls, const void *a, const void *b, void
   if (tc.n_tiles == 0) return TQ_SUCCESS;
   if (pl->prec == TQ_PREC_BF16X3)
     return tq_launch_bf16x3_gemm(h, tc.tiles_cfg, pl->desc.as<GemmDesc>(),
+                                 tc.runs ? pl->run_desc.as<RunDesc>() : nullptr,
                                  tc.tile_pair.as<int32_t>(), tc.tile_first.as<int32_t>(),
                                  tc.n_tiles, pl->pack_a.panel.as<uint16_t>(),
                                  pl->pack_b.panel.as<uint16_t>(), pl->pack_a.panel_elems,

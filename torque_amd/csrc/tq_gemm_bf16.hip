@@ -79,23 +79,61 @@ struct StepLoader {
   }
 };
 
+// A tile: the GEMM it belongs to (a pair, or a run of pairs) and its corner.
+// bp, brows, brow0 locate this thread's B row (output column) for the
+// loaders: the B vectors of the thread are read from panel bp of brows rows
+// at row brow0 + r, r = the thread's row within the tile.
 struct TileRef {
   GemmDesc d;
   int m0, n0;
+  const uint16_t *bp;
+  int brows, brow0;
 };
 
-template <int BM, int BN>
+template <int BM, int BN, int THREADS>
 __device__ inline TileRef fetch(int64_t t, const GemmDesc *__restrict__ descs,
+                                const RunDesc *__restrict__ runs,
                                 const int32_t *__restrict__ tile_pair,
-                                const int32_t *__restrict__ tile_first) {
+                                const int32_t *__restrict__ tile_first,
+                                const uint16_t *__restrict__ b) {
   TileRef r;
-  const int p = tile_pair[t];
-  r.d = descs[p];
+  const int u = tile_pair[t];
+  RunDesc run{};
+  if (runs) {
+    run = runs[u];
+    r.d = GemmDesc{run.a_off, 0, run.c_off, run.m, run.n, run.k, run.lda, run.lda, run.flags};
+  } else {
+    r.d = descs[u];
+  }
   const tqg::TilePos tp =
-      tqg::tile_position((int)(t - tile_first[p]), (r.d.m + BM - 1) / BM, (r.d.n + BN - 1) / BN,
+      tqg::tile_position((int)(t - tile_first[u]), (r.d.m + BM - 1) / BM, (r.d.n + BN - 1) / BN,
                          tqg::tile_group<BM, BN>(gridDim.x));
   r.m0 = tp.mt * BM;
   r.n0 = tp.nt * BN;
+  if (!runs) {
+    r.bp = b + r.d.b_off;
+    r.brows = r.d.n;
+    r.brow0 = r.n0;
+    return r;
+  }
+  // the pair of the run holding this thread's column (columns past the run
+  // read its last one); the split panels of a run share K, hence the chunk
+  // count
+  const int rt = BN >= THREADS ? (int)threadIdx.x : (int)threadIdx.x % BN;
+  const int col = min(r.n0 + rt, r.d.n - 1);
+  r.bp = b;
+  r.brows = 1;
+  r.brow0 = 0;
+  int start = 0;
+  for (int q = 0; q < run.np; ++q) {
+    const GemmDesc &pd = descs[run.p0 + q];
+    if (col >= start && col < start + pd.n) {
+      r.bp = b + pd.b_off;
+      r.brows = pd.n;
+      r.brow0 = col - start - rt;
+    }
+    start += pd.n;
+  }
   return r;
 }
 
@@ -146,11 +184,12 @@ struct Smem {
 // the 128 x 128 tile fit two workgroups per CU at the price of a few spills).
 template <int BM, int BN, int WAVES_M, int MINW>
 __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
-    const GemmDesc *__restrict__ descs, const int32_t *__restrict__ tile_pair,
-    const int32_t *__restrict__ tile_first, int64_t n_tiles,
-    const uint16_t *__restrict__ a, const uint16_t *__restrict__ b,
+    const GemmDesc *__restrict__ descs, const RunDesc *__restrict__ runs,
+    const int32_t *__restrict__ tile_pair, const int32_t *__restrict__ tile_first,
+    int64_t n_tiles, const uint16_t *__restrict__ a, const uint16_t *__restrict__ b,
     int64_t a_plane, int64_t b_plane, double *__restrict__ c) {
   constexpr int WAVES_N = 4 / WAVES_M;
+  constexpr int kT = kThreads;
   constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
   constexpr int TM = WM / 16, TN = WN / 16;
   static_assert(TM >= 1 && TN >= 1, "tile shape");
@@ -165,11 +204,11 @@ __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
 
   int64_t t = tqg::xcd_virtual_id(blockIdx.x, gridDim.x);
   if (t >= n_tiles) return;
-  TileRef cur = fetch<BM, BN>(t, descs, tile_pair, tile_first);
+  TileRef cur = fetch<BM, BN, kT>(t, descs, runs, tile_pair, tile_first, b);
   StepLoader<BM> la;
   StepLoader<BN> lb;
   la.load(a + cur.d.a_off, a_plane, cur.d.m, cur.m0, cur.d.lda >> 3, 0);
-  lb.load(b + cur.d.b_off, b_plane, cur.d.n, cur.n0, cur.d.ldb >> 3, 0);
+  lb.load(cur.bp, b_plane, cur.brows, cur.brow0, cur.d.ldb >> 3, 0);
   while (true) {
     const GemmDesc &d = cur.d;
     // panels store kpad = lda (A) / ldb (B) values per row: kpad / 8 chunks
@@ -191,13 +230,13 @@ __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
       __syncthreads();
       if (ch0 + kChunks < n_chunks) {
         la.load(a + d.a_off, a_plane, d.m, cur.m0, n_chunks, ch0 + kChunks);
-        lb.load(b + d.b_off, b_plane, d.n, cur.n0, n_chunks_b, ch0 + kChunks);
+        lb.load(cur.bp, b_plane, cur.brows, cur.brow0, n_chunks_b, ch0 + kChunks);
       } else if (t_next < n_tiles) {
         // last step: the next tile's first step loads under these MFMAs and
         // the epilogue
-        nxt = fetch<BM, BN>(t_next, descs, tile_pair, tile_first);
+        nxt = fetch<BM, BN, kT>(t_next, descs, runs, tile_pair, tile_first, b);
         la.load(a + nxt.d.a_off, a_plane, nxt.d.m, nxt.m0, nxt.d.lda >> 3, 0);
-        lb.load(b + nxt.d.b_off, b_plane, nxt.d.n, nxt.n0, nxt.d.ldb >> 3, 0);
+        lb.load(nxt.bp, b_plane, nxt.brows, nxt.brow0, nxt.d.ldb >> 3, 0);
       }
       step_products<BM, BN, TM, TN>(As, Bs, wm0 + lrow, wn0 + lrow, lchunk, acc);
     }
@@ -222,11 +261,12 @@ __device__ inline int tile_steps(const GemmDesc &d) {
 // the buffer the last step read.
 template <int BM, int BN, int WAVES_M>
 __global__ __launch_bounds__(kThreads, 1) void bf16x3_db_kernel(
-    const GemmDesc *__restrict__ descs, const int32_t *__restrict__ tile_pair,
-    const int32_t *__restrict__ tile_first, int64_t n_tiles,
-    const uint16_t *__restrict__ a, const uint16_t *__restrict__ b,
+    const GemmDesc *__restrict__ descs, const RunDesc *__restrict__ runs,
+    const int32_t *__restrict__ tile_pair, const int32_t *__restrict__ tile_first,
+    int64_t n_tiles, const uint16_t *__restrict__ a, const uint16_t *__restrict__ b,
     int64_t a_plane, int64_t b_plane, double *__restrict__ c) {
   constexpr int WAVES_N = 4 / WAVES_M;
+  constexpr int kT = kThreads;
   constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
   constexpr int TM = WM / 16, TN = WN / 16;
   constexpr int kBuf = 3 * kChunks * (BM + BN);  // 16-byte vectors per buffer
@@ -241,7 +281,7 @@ __global__ __launch_bounds__(kThreads, 1) void bf16x3_db_kernel(
 
   int64_t t = tqg::xcd_virtual_id(blockIdx.x, gridDim.x);
   if (t >= n_tiles) return;
-  TileRef cur = fetch<BM, BN>(t, descs, tile_pair, tile_first);
+  TileRef cur = fetch<BM, BN, kT>(t, descs, runs, tile_pair, tile_first, b);
 
   // load cursor: tile lt (lref), k step ls
   TileRef lref = cur;
@@ -251,13 +291,13 @@ __global__ __launch_bounds__(kThreads, 1) void bf16x3_db_kernel(
   StepLoader<BN> lb;
   auto load = [&]() {
     la.load(a + lref.d.a_off, a_plane, lref.d.m, lref.m0, lref.d.lda >> 3, ls * kChunks);
-    lb.load(b + lref.d.b_off, b_plane, lref.d.n, lref.n0, lref.d.ldb >> 3, ls * kChunks);
+    lb.load(lref.bp, b_plane, lref.brows, lref.brow0, lref.d.ldb >> 3, ls * kChunks);
   };
   auto advance = [&]() -> bool {
     if (++ls < tile_steps(lref.d)) return true;
     lt += gridDim.x;
     if (lt >= n_tiles) return false;
-    lref = fetch<BM, BN>(lt, descs, tile_pair, tile_first);
+    lref = fetch<BM, BN, kT>(lt, descs, runs, tile_pair, tile_first, b);
     ls = 0;
     return true;
   };
@@ -295,7 +335,7 @@ __global__ __launch_bounds__(kThreads, 1) void bf16x3_db_kernel(
         acc, c, cur.d, cur.m0, cur.n0, reinterpret_cast<double *>(smem + (buf ^ 1) * kBuf));
     t += gridDim.x;
     if (t >= n_tiles) break;
-    cur = fetch<BM, BN>(t, descs, tile_pair, tile_first);
+    cur = fetch<BM, BN, kT>(t, descs, runs, tile_pair, tile_first, b);
   }
 }
 
@@ -309,11 +349,12 @@ __global__ __launch_bounds__(kThreads, 1) void bf16x3_db_kernel(
 // tiles and two wavefronts per SIMD at one workgroup per CU.
 template <int BM, int BN, int WAVES_M, int NW>
 __global__ __launch_bounds__(NW * 64, 1) void bf16x3_db2_kernel(
-    const GemmDesc *__restrict__ descs, const int32_t *__restrict__ tile_pair,
-    const int32_t *__restrict__ tile_first, int64_t n_tiles,
-    const uint16_t *__restrict__ a, const uint16_t *__restrict__ b,
+    const GemmDesc *__restrict__ descs, const RunDesc *__restrict__ runs,
+    const int32_t *__restrict__ tile_pair, const int32_t *__restrict__ tile_first,
+    int64_t n_tiles, const uint16_t *__restrict__ a, const uint16_t *__restrict__ b,
     int64_t a_plane, int64_t b_plane, double *__restrict__ c) {
   constexpr int WAVES_N = NW / WAVES_M;
+  constexpr int kT = NW * 64;
   constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
   constexpr int TM = WM / 16, TN = WN / 16;
   constexpr int kBuf = 3 * kChunks * (BM + BN);  // 16-byte vectors per buffer
@@ -328,7 +369,7 @@ __global__ __launch_bounds__(NW * 64, 1) void bf16x3_db2_kernel(
 
   int64_t t = tqg::xcd_virtual_id(blockIdx.x, gridDim.x);
   if (t >= n_tiles) return;
-  TileRef cur = fetch<BM, BN>(t, descs, tile_pair, tile_first);
+  TileRef cur = fetch<BM, BN, kT>(t, descs, runs, tile_pair, tile_first, b);
 
   // load cursor: the next step to load is step ls of tile lt (lref)
   TileRef lref = cur;
@@ -340,11 +381,11 @@ __global__ __launch_bounds__(NW * 64, 1) void bf16x3_db2_kernel(
   auto load_next = [&](StepLoader<BM, NW * 64> &la, StepLoader<BN, NW * 64> &lb) -> bool {
     if (!lvalid) return false;
     la.load(a + lref.d.a_off, a_plane, lref.d.m, lref.m0, lref.d.lda >> 3, ls * kChunks);
-    lb.load(b + lref.d.b_off, b_plane, lref.d.n, lref.n0, lref.d.ldb >> 3, ls * kChunks);
+    lb.load(lref.bp, b_plane, lref.brows, lref.brow0, lref.d.ldb >> 3, ls * kChunks);
     if (++ls >= tile_steps(lref.d)) {
       lt += gridDim.x;
       lvalid = lt < n_tiles;
-      if (lvalid) lref = fetch<BM, BN>(lt, descs, tile_pair, tile_first);
+      if (lvalid) lref = fetch<BM, BN, kT>(lt, descs, runs, tile_pair, tile_first, b);
       ls = 0;
     }
     return true;
@@ -385,7 +426,7 @@ __global__ __launch_bounds__(NW * 64, 1) void bf16x3_db2_kernel(
         acc, c, cur.d, cur.m0, cur.n0, reinterpret_cast<double *>(smem + (buf ^ 1) * kBuf));
     t += gridDim.x;
     if (t >= n_tiles) return false;
-    cur = fetch<BM, BN>(t, descs, tile_pair, tile_first);
+    cur = fetch<BM, BN, kT>(t, descs, runs, tile_pair, tile_first, b);
     s = 0;
     steps = tile_steps(cur.d);
 #pragma unroll
@@ -409,13 +450,14 @@ constexpr Cfg kCfgs[] = {{64, 64, 1},   {128, 64, 1},  {64, 128, 1},  {128, 128,
                          {128, 256, 2}};
 constexpr int kNCfg = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
-typedef void (*KernelFn)(const GemmDesc *, const int32_t *, const int32_t *, int64_t,
-                         const uint16_t *, const uint16_t *, int64_t, int64_t, double *);
+typedef void (*KernelFn)(const GemmDesc *, const RunDesc *, const int32_t *, const int32_t *,
+                         int64_t, const uint16_t *, const uint16_t *, int64_t, int64_t,
+                         double *);
 
 template <int BM, int BN, int WAVES_M, int MINW, int DB>
-tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *tile_pair,
-                 const int32_t *tile_first, int64_t n_tiles, const uint16_t *a,
-                 const uint16_t *b, int64_t ap, int64_t bp, double *c) {
+tq_status launch(tq_handle_t h, const GemmDesc *descs, const RunDesc *runs,
+                 const int32_t *tile_pair, const int32_t *tile_first, int64_t n_tiles,
+                 const uint16_t *a, const uint16_t *b, int64_t ap, int64_t bp, double *c) {
   KernelFn kernel;
   const int threads = DB == 2 ? 512 : kThreads;
   if constexpr (DB == 2)
@@ -437,7 +479,7 @@ tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *tile_pair,
   int64_t grid = std::min<int64_t>(n_tiles, (int64_t)h->n_cu * per_cu);
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(threads), 0, h->stream, descs,
+  hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(threads), 0, h->stream, descs, runs,
                      tile_pair, tile_first, n_tiles, a, b, ap, bp, c);
   TQ_HIP(h, hipGetLastError());
   return TQ_SUCCESS;
@@ -453,16 +495,16 @@ GemmConfig tq_bf16x3_config(int cfg) {
 }
 
 tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, const GemmDesc *descs,
-                                const int32_t *tile_pair, const int32_t *tile_first,
-                                int64_t n_tiles, const uint16_t *a_split,
-                                const uint16_t *b_split, int64_t a_plane,
-                                int64_t b_plane, double *c) {
+                                const RunDesc *runs, const int32_t *tile_pair,
+                                const int32_t *tile_first, int64_t n_tiles,
+                                const uint16_t *a_split, const uint16_t *b_split,
+                                int64_t a_plane, int64_t b_plane, double *c) {
   if (n_tiles <= 0) return TQ_SUCCESS;
   switch (cfg) {
 #define TQ_CFG(I, BM, BN, WM, MINW, DB)                                                  \
   case I:                                                                                \
-    return launch<BM, BN, WM, MINW, DB>(h, descs, tile_pair, tile_first, n_tiles, a_split, \
-                                        b_split, a_plane, b_plane, c);
+    return launch<BM, BN, WM, MINW, DB>(h, descs, runs, tile_pair, tile_first, n_tiles,    \
+                                        a_split, b_split, a_plane, b_plane, c);
     TQ_CFG(0, 64, 64, 2, 1, 0)
     TQ_CFG(1, 128, 64, 2, 1, 0)
     TQ_CFG(2, 64, 128, 2, 1, 0)

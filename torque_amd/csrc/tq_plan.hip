@@ -715,10 +715,11 @@ tq_status plan_build_tiles(tq_plan_t pl) {
     const int cfg = tc.current();
     if (!(tc.ready && tc.tiles_cfg == cfg)) {
       const GemmConfig g = tq_gemm_config(pl->prec, cfg);
-      // a wave-tile configuration tiles runs of pairs (FP64 and FP32 tiers)
-      // when runs join pairs of this class (otherwise pair tiles are the same
-      // work with cheaper B addressing)
-      if (g.wave_tile && pl->prec != TQ_PREC_BF16X3) {
+      // the wave-tile configurations (FP64 and FP32 tiers) and every
+      // split-BF16 configuration tile runs of pairs when runs join pairs of
+      // this class (otherwise pair tiles are the same work with cheaper B
+      // addressing)
+      if (g.wave_tile || pl->prec == TQ_PREC_BF16X3) {
         if (!pl->runs_built) TQ_CHECK(build_runs(pl));
         int64_t n_runs = 0, n_cls = 0;
         for (const RunDesc &r : pl->h_runs)
