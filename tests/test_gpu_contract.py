@@ -369,3 +369,33 @@ def test_bf16x3_many_tiles_per_workgroup(cuda):
         if first is None:
             first = got
         assert np.array_equal(got, first), cfg
+
+
+@pytest.mark.parametrize("permuted", [False, True])
+def test_runs_match_pair_tiles(cuda, permuted):
+    # U(1) pairs that share an A block and write adjacent output blocks are
+    # tiled as one GEMM by the wave-tile and split-BF16 kernels (B columns
+    # found per lane in the run's pairs); the workgroup-tile kernel tiles
+    # every pair alone. The two must give the same bits, with B in place or
+    # packed (blocks stored in random leg orders).
+    rng = np.random.default_rng(57)
+    a, b, legs = W.batched(1, seed=23, dim_lo=24, dim_hi=72)[0]
+    A = O.BS(a.dims, a.begin, a.end, permuted_strides(a.begin, a.end, rng) if permuted else None)
+    B = O.BS(b.dims, b.begin, b.end, permuted_strides(b.begin, b.end, rng) if permuted else None)
+    da, db = a.data, b.data
+    ref_c, ref = O.contract(A, da, B, db, legs)
+    configs = gemm_configs(with_kind=True)
+    wave = [q for q, c in enumerate(configs) if c[3] == 1]
+    plan, pair_tiles = gpu_contract(A, da, B, db, legs, cuda, cfg=0)
+    check_maps(plan, ref)
+    assert rel_fro(pair_tiles, ref_c) < FP64_TOL
+    for cfg in wave:
+        _, got = gpu_contract(A, da, B, db, legs, cuda, cfg=cfg)
+        assert np.array_equal(got, pair_tiles), cfg
+    first = None
+    for cfg in range(len(gemm_configs("bf16x3"))):
+        _, got = gpu_contract(A, da, B, db, legs, cuda, precision="bf16x3", cfg=cfg)
+        assert rel_fro(got, ref_c) < 1e-6, cfg
+        if first is None:
+            first = got
+        assert np.array_equal(got, first), cfg
