@@ -176,13 +176,13 @@ def _block_diagonal(rng, n_lo, n_hi, s_lo, s_hi):
 def training_contractions(n, seed):
     """Yields n contractions for the autotuner to time: U(1) rank-4 x rank-4 pairs
     with leg dimensions spanning 8..160, and (every fourth) block-diagonal
-    fused-leg products with 2..12 sectors of 16..2048."""
+    fused-leg products with 2..12 sectors of 16..4096."""
     from . import workloads as W
     rng = np.random.default_rng(seed)
     ranges = [(8, 40), (16, 64), (24, 80), (40, 96), (60, 120), (12, 160)]
     for s in range(n):  # generated one at a time: the large ones hold 100s of MB
         if s % 4 == 3:
-            lo = int(rng.choice([16, 64, 256, 512, 1024]))
+            lo = int(rng.choice([16, 64, 256, 512, 1024, 2048]))
             yield _block_diagonal(rng, 2, 12, lo, 2 * lo)
         else:
             lo, hi = ranges[s % len(ranges)]
