@@ -57,6 +57,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-permute", action="store_true",
                    help="skip the standalone 200M-element rank-6 permutation")
+    p.add_argument("--no-batched", action="store_true",
+                   help="skip the 512-contraction batched workload in the default line")
     p.add_argument("--no-large", action="store_true",
                    help="skip the D=10, chi=256 enlarged-corner product")
     p.add_argument("--workload", default="peps",
@@ -206,7 +208,7 @@ def bench_batched(args):
                      "kernel": "graph replay (grouped_gemm_kernel dominant)"},
         "cpu_baseline": cpu_baseline, "tiers": tiers,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 PERMUTE_DIMS = (10, 20, 25, 40, 50, 20)  # 200,000,000 elements
@@ -407,7 +409,8 @@ def main():
               "(see the sharding design); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batched":
-        return bench_batched(args)
+        print(json.dumps(bench_batched(args)), flush=True)
+        return
     if args.workload in ("permute", "large"):
         import torch
         dev = torch.device("cuda:0")
@@ -580,6 +583,14 @@ def main():
 
     permute = None if args.no_permute else bench_permute(dev)
     large = None if args.no_large else bench_large(dev)
+    batched = None
+    if not args.no_batched:
+        bl = bench_batched(args)
+        batched = {"workload": bl["config"]["workload"], "pairs": bl["config"]["pairs"],
+                   "gflop": bl["config"]["gflop_per_step"], "ms": bl["ms_per_step"],
+                   "gflops": bl["value"], "frac_fp64_mfma_peak": bl["roofline"]["frac"],
+                   "gemm_config": bl["config"]["gemm_config"], "tiers": bl["tiers"],
+                   "cpu_baseline": bl["cpu_baseline"]}
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "GFLOP/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 4),
@@ -598,6 +609,7 @@ def main():
         "fp64_rel_err_vs_oracle": err64,
         "permute_rank6_200M": permute,
         "enlarged_corner_D10_chi256": large,
+        "batched_512_u1_rank4x4": batched,
     }
     print(json.dumps(line), flush=True)
     if args.copy_calibration:
