@@ -542,22 +542,33 @@ __device__ inline void split_kfast_tile(const CopyJob &job, int64_t rows, int64_
   }
 }
 
+// Workgroup index for a grid-stride loop in which each XCD walks a
+// contiguous stretch of the work items: workgroups are dealt to the 8 XCDs
+// round-robin, so neighbouring items (which share cache lines) would
+// otherwise land in different L2s.
+__device__ inline int32_t xcd_local_id() {
+  const uint32_t g = gridDim.x, b = blockIdx.x;
+  if (g < 8 || g % 8 != 0) return (int32_t)b;
+  return (int32_t)((b % 8) * (g / 8) + b / 8);
+}
+
 // Split-BF16 pack. The work items of a job are its (row, 8-value chunk)
 // cells, numbered rows fastest (cell = chunk * rows + row), and a workgroup
 // takes the next kSplitCells of them: one per thread, which reads the cell's
 // 8 values (lanes run along the rows, so the reads coalesce when a row leg
 // is fastest in memory), splits them and writes one 16-byte vector to each
 // plane (a wavefront writes 1 KiB of contiguous output per plane). When the
-// first contracted leg is contiguous a workgroup takes a tile of kSplitRows
-// rows x kSplitK padded k instead (split_kfast_tile). Values past kdim are
-// zeros. Numbering the cells across the
-// whole job keeps every lane busy for jobs of few rows or few chunks.
+// first contracted leg is contiguous each wavefront takes a unit of
+// kSplitRows rows x one chunk instead (split_kfast_tile), units numbered
+// chunks fastest, so that the chunks of a row band share workgroups and
+// XCDs. Values past kdim are zeros. Numbering the work across the whole job
+// keeps every lane busy for jobs of few rows or few chunks.
 __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
     const CopyJob *__restrict__ jobs, const int32_t *__restrict__ job_wg_start,
     int64_t n_jobs, const double *__restrict__ src, uint16_t *__restrict__ dst,
     int64_t plane) {
   const int32_t total = job_wg_start[n_jobs];
-  for (int32_t c = blockIdx.x; c < total; c += gridDim.x) {
+  for (int32_t c = xcd_local_id(); c < total; c += gridDim.x) {
     const int64_t lo = find_job(job_wg_start, n_jobs, c);
     const CopyJob &job = jobs[lo];
     const int nk = job.nk;
@@ -568,9 +579,13 @@ __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
     const uint32_t cells = rows * (uint32_t)(job.kpad >> 3);
     const uint32_t tile = (uint32_t)(c - job_wg_start[lo]);
     if (split_kfast(job)) {
-      const uint32_t k_tiles = (uint32_t)ceil_div(job.kpad, kSplitK);
-      split_kfast_tile(job, rows, (tile / k_tiles) * kSplitRows,
-                       (tile % k_tiles) * (kSplitK / 8) + (threadIdx.x >> 6), src, dst, plane);
+      // unit = (row band, chunk), chunks fastest, one per wavefront
+      const uint32_t chunks = (uint32_t)(job.kpad >> 3);
+      const uint32_t unit = tile * (kSplitK / 8) + (threadIdx.x >> 6);
+      const uint32_t band = unit / chunks;
+      if ((int64_t)band * kSplitRows < rows)
+        split_kfast_tile(job, rows, (int64_t)band * kSplitRows, unit - band * chunks, src, dst,
+                         plane);
       continue;
     }
     const Div32 rdiv(rows);
