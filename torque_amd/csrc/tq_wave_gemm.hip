@@ -226,6 +226,38 @@ __device__ inline void tail_select(TC (&r)[T][KPL], int kvalid) {
     for (int i = 0; i < T; ++i) r[i][s] = 4 * s < kvalid ? r[i][s] : (TC)0;
 }
 
+// Two output elements side by side in memory, stored as one vector (the
+// address is aligned to the element only).
+template <class T>
+struct Pair {
+  typedef T type __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+};
+
+// Accumulator registers 2h and 2h + 1 (x, y) of a 16 x 16 MFMA block as two
+// vertically adjacent elements (v0 above v1) of this lane's column, and the
+// row of v0 within the block. The FP32-accumulating MFMAs hold rows
+// 4g + r (g = lane / 16), already adjacent. The FP64 MFMA holds rows
+// g + 4r, so lanes swap halves across 16-lane rows (permlane16 swap: rows
+// 1 and 3 of x with rows 0 and 2 of y): afterwards group g holds rows
+// b, b + 1 with b = 8h + {0, 4, 2, 6}[g]. Every lane runs the swaps.
+template <bool F64, class TOut, class TC>
+__device__ inline void row_pair(TC x, TC y, int lane, int h, TOut &v0, TOut &v1, int &row) {
+  const int g = lane >> 4;
+  if constexpr (F64) {
+    typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+    const u2 xb = __builtin_bit_cast(u2, (double)x), yb = __builtin_bit_cast(u2, (double)y);
+    const auto lo = __builtin_amdgcn_permlane16_swap(xb.x, yb.x, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(xb.y, yb.y, false, false);
+    v0 = (TOut)__builtin_bit_cast(double, u2{lo[0], hi[0]});
+    v1 = (TOut)__builtin_bit_cast(double, u2{lo[1], hi[1]});
+    row = 8 * h + (((g & 1) << 2) | (g & 2));
+  } else {
+    v0 = (TOut)x;
+    v1 = (TOut)y;
+    row = 4 * g + 2 * h;
+  }
+}
+
 // MINW: wavefronts per SIMD the register allocation must allow; 2 keeps a
 // second wavefront resident to cover the first one's operand waits. RUNS:
 // the tile list indexes runs (RunDesc) rather than pairs.
@@ -329,7 +361,8 @@ __global__ __launch_bounds__(256, MINW) void wave_gemm_kernel(
 #pragma unroll
           for (int j = 0; j < TN; ++j) acc[i][j] = mfma(ra[i][s], rb[j][s], acc[i][j]);
     }
-    // column-major store of the tile's valid rows and columns
+    // column-major store of the tile's valid rows and columns, two adjacent
+    // rows of a column per store
     TOut *cb = c + cur.c_off;
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
@@ -337,9 +370,18 @@ __global__ __launch_bounds__(256, MINW) void wave_gemm_kernel(
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = cur.m0 + 16 * i + tqg::acc_row<kF64>(lane, r);
-          if (row < cur.m && col < cur.n) cb[row + (int64_t)cur.m * col] = (TOut)acc[i][j][r];
+        for (int h = 0; h < 2; ++h) {
+          TOut v0, v1;
+          int row;
+          row_pair<kF64, TOut>(acc[i][j][2 * h], acc[i][j][2 * h + 1], lane, h, v0, v1, row);
+          row += cur.m0 + 16 * i;
+          if (col < cur.n) {
+            TOut *p = cb + row + (int64_t)cur.m * col;
+            if (row + 1 < cur.m)
+              *reinterpret_cast<typename Pair<TOut>::type *>(p) = typename Pair<TOut>::type{v0, v1};
+            else if (row < cur.m)
+              *p = v0;
+          }
         }
     }
     if (!more) break;
