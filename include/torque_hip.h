@@ -100,7 +100,7 @@ typedef struct tq_plan_info {
                            2 = packed by the permute kernel                 */
   int32_t b_layout;     /* same for B (n-fast / k-fast / packed)             */
   int64_t n_tiles;      /* MFMA tiles of the current configuration           */
-  int64_t n_segments;   /* 64-row segments of the streaming small-K kernel   */
+  int64_t n_segments;   /* 64 x 64 segments of the streaming small-K kernel  */
 } tq_plan_info;
 
 /* ---- runtime --------------------------------------------------------- */

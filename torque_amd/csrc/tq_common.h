@@ -179,8 +179,9 @@ struct RunDesc {
 };
 
 // Pairs with K <= kStreamMaxK are written by the streaming small-K kernel,
-// one work item per kStreamRows-row segment of the output block, instead of
-// MFMA tiles.
+// one work item per segment of kStreamRows rows x kStreamCols columns of the
+// output block, instead of MFMA tiles (the column cut keeps the items of a
+// wide block small enough to balance across wavefronts).
 constexpr int kStreamMaxK = 8;
 
 // Work-list sort key of a block pair (larger first). A pair streamed by the
@@ -194,6 +195,7 @@ __host__ __device__ inline int64_t pair_order_key(int64_t m, int64_t n, int64_t 
   return k <= kStreamMaxK ? m * n * (k + 4) : k + 4;
 }
 constexpr int kStreamRows = 64;
+constexpr int kStreamCols = 64;
 
 // For the split-BF16 tier the operands are the split panels (a_packed /
 // b_packed, planes a_plane / b_plane elements apart).

@@ -180,7 +180,7 @@ tq_status tq_bs_contract_multiply(tq_plan_t pl, const void *a, const void *b,
   if (pl->ro > 0) {
     TQ_CHECK(plan_build_tiles(pl));
     if (pl->n_chunks > 0)
-      TQ_CHECK(tq_launch_small_k(h, pl->dtype, pl->prec, pl->desc.as<GemmDesc>(),
+      TQ_CHECK(tq_launch_small_k(h, pl->dtype, pl->prec, pl->stream_desc.as<GemmDesc>(),
                                  pl->chunk_pair.as<int32_t>(),
                                  pl->chunk_first.as<int32_t>(), pl->n_chunks, a,
                                  pl->pack_a.panel.p, b, pl->pack_b.panel.p, c,

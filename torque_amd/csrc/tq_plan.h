@@ -113,11 +113,12 @@ struct tq_plan_s {
   DevBuf order;      // int64 work list (pair ids, largest cost first)
   PackSide pack_a, pack_b;
 
-  // Pairs with K <= stream_max_k go to the streaming small-K kernel (chunk
-  // list); the others to MFMA tiles in two classes, each with its own tile
+  // Pairs with K <= stream_max_k go to the streaming small-K kernel (segment
+  // list over column slices of the pairs); the others to MFMA tiles in two classes, each with its own tile
   // configuration: class 1 when min(M, N) >= large_min, class 0 otherwise.
   int stream_max_k = 0;
   DevBuf chunk_pair, chunk_first;
+  DevBuf stream_desc;  // GemmDesc per column slice of a streamed pair
   int64_t n_chunks = 0;
   bool chunks_ready = false;
   int large_min = kLargeMin;

@@ -517,12 +517,7 @@ __global__ void tiles_per_position_kernel(int64_t n, const i64 *__restrict__ ord
   if (q >= n) return;
   const GemmDesc d = desc[order[q]];
   const int c = d.k <= max_k ? -1 : (d.m >= large_min && d.n >= large_min) ? 1 : 0;
-  if (c != cls)
-    tiles[q] = 0;
-  else if (cls < 0)
-    tiles[q] = ceil_div(d.m, bm);
-  else
-    tiles[q] = ceil_div(d.m, bm) * ceil_div(d.n, bn);
+  tiles[q] = c != cls ? 0 : ceil_div(d.m, bm) * ceil_div(d.n, bn);
 }
 
 __global__ void expand_tiles_kernel(int64_t n, const i64 *__restrict__ order,
@@ -592,8 +587,8 @@ tq_status tq_sort_by_cost(tq_handle_t h, const i64 *cost, int64_t n, i64 *order)
 
 // ---- plan object ----------------------------------------------------------
 
-// Expands the work list into per-item pair indices (tile_pair / chunk_pair)
-// and each pair's first item; `total` is known on the host from the shapes.
+// Expands the work list into per-tile pair indices (tile_pair) and each
+// pair's first tile; `total` is known on the host from the shapes.
 static tq_status expand_work(tq_plan_t pl, int32_t bm, int32_t bn, int32_t cls,
                              int64_t total, DevBuf &item_pair, DevBuf &item_first) {
   tq_handle_t h = pl->h;
@@ -694,21 +689,66 @@ static tq_status build_run_tiles(tq_plan_t pl, int cls, int32_t bm, int32_t bn,
   return TQ_SUCCESS;
 }
 
+// Work items of the streaming small-K kernel, in work-list order: each
+// streamed pair is cut into column slices of at most kStreamCols columns (a
+// slice is a pair of its own for the kernel: B and C offsets moved to its
+// first column), and each slice into kStreamRows-row segments. Slicing wide
+// blocks keeps the items small enough to balance over the wavefronts, which
+// take them round-robin.
+static tq_status build_stream_segments(tq_plan_t pl) {
+  tq_handle_t h = pl->h;
+  const int64_t np = pl->n_pairs;
+  pl->n_chunks = 0;
+  pl->chunks_ready = true;
+  if (np == 0) return TQ_SUCCESS;
+  if ((int64_t)pl->h_desc.size() != np)
+    return tq_fail(h, TQ_ERROR_INVALID_VALUE, "contract: host descriptors missing");
+  std::vector<i64> order(np);
+  TQ_HIP(h, hipMemcpyAsync(order.data(), pl->order.p, sizeof(i64) * np, hipMemcpyDeviceToHost,
+                           h->stream));
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  const bool split = pl->prec == TQ_PREC_BF16X3;
+  std::vector<GemmDesc> slices;
+  std::vector<int32_t> seg_slice, slice_first;
+  for (int64_t q = 0; q < np; ++q) {
+    const GemmDesc &d = pl->h_desc[order[q]];
+    if (pair_class(*pl, d.m, d.n, d.k) >= 0) continue;
+    // element distance between consecutive B rows (output columns): a split
+    // panel with one chunk holds 8 values per row
+    const i64 b_row = split ? 8 : (d.flags & kBKFast) ? d.ldb : 1;
+    const int64_t row_segs = ceil_div(d.m, kStreamRows);
+    for (int32_t n0 = 0; n0 < d.n; n0 += kStreamCols) {
+      GemmDesc s = d;
+      s.b_off += n0 * b_row;
+      s.c_off += (i64)d.m * n0;
+      s.n = std::min<int32_t>(kStreamCols, d.n - n0);
+      if ((int64_t)seg_slice.size() + row_segs > 0x7fffffff)
+        return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many output segments");
+      slice_first.push_back((int32_t)seg_slice.size());
+      seg_slice.insert(seg_slice.end(), (size_t)row_segs, (int32_t)slices.size());
+      slices.push_back(s);
+    }
+  }
+  if (seg_slice.empty()) return TQ_SUCCESS;
+  TQ_HIP(h, pl->stream_desc.alloc(sizeof(GemmDesc) * slices.size()));
+  TQ_HIP(h, pl->chunk_pair.alloc(sizeof(int32_t) * seg_slice.size()));
+  TQ_HIP(h, pl->chunk_first.alloc(sizeof(int32_t) * slice_first.size()));
+  TQ_HIP(h, hipMemcpyAsync(pl->stream_desc.p, slices.data(), sizeof(GemmDesc) * slices.size(),
+                           hipMemcpyHostToDevice, h->stream));
+  TQ_HIP(h, hipMemcpyAsync(pl->chunk_pair.p, seg_slice.data(),
+                           sizeof(int32_t) * seg_slice.size(), hipMemcpyHostToDevice, h->stream));
+  TQ_HIP(h, hipMemcpyAsync(pl->chunk_first.p, slice_first.data(),
+                           sizeof(int32_t) * slice_first.size(), hipMemcpyHostToDevice,
+                           h->stream));
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  pl->n_chunks = (int64_t)seg_slice.size();
+  return TQ_SUCCESS;
+}
+
 tq_status plan_build_tiles(tq_plan_t pl) {
   tq_handle_t h = pl->h;
   const int64_t np = pl->n_pairs;
-  if (!pl->chunks_ready) {
-    int64_t total = 0;
-    for (int64_t p = 0; p < np; ++p)
-      if (pair_class(*pl, pl->h_m[p], pl->h_n[p], pl->h_k[p]) < 0)
-        total += ceil_div(pl->h_m[p], kStreamRows);
-    if (total > 0x7fffffff)
-      return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many output segments");
-    if (total > 0)
-      TQ_CHECK(expand_work(pl, kStreamRows, 0, -1, total, pl->chunk_pair, pl->chunk_first));
-    pl->n_chunks = total;
-    pl->chunks_ready = true;
-  }
+  if (!pl->chunks_ready) TQ_CHECK(build_stream_segments(pl));
   pl->n_tiles = 0;
   for (int c = 0; c < 2; ++c) {
     TileClass &tc = pl->cls[c];
