@@ -100,8 +100,8 @@ struct CopyJob {
   i64 strides[TQ_MAX_RANK];
 };
 
-// Workgroups of the pack kernels per job: kGatherPerWG elements each for a
-// plain gather. A split-BF16 job is cut into its (row, 8-value chunk) cells,
+// Work items of the pack kernels per job: chunks of kGatherPerWG elements for
+// a plain gather (one wavefront each). A split-BF16 job is cut into its (row, 8-value chunk) cells,
 // kSplitCells per workgroup, numbered rows fastest; when its first
 // contracted leg is contiguous in memory (k fastest) into units of
 // kSplitRows rows x one chunk instead, numbered chunks fastest, kSplitK / 8
@@ -133,7 +133,8 @@ tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, const GemmDesc *descs,
 tq_status tq_launch_gather_jobs(tq_handle_t h, tq_dtype dt, const CopyJob *jobs,
                                 const int32_t *job_wg_start, int64_t n_jobs,
                                 int64_t total_wg, const void *src, void *dst);
-// Elements each workgroup of the gather kernel covers.
+// Elements of one work item of the gather kernels (a wavefront's chunk of a
+// job list; a workgroup's part of a single permutation).
 constexpr int kGatherPerWG = 2048;
 
 __host__ __device__ inline int64_t pack_job_wgs(const CopyJob &j) {

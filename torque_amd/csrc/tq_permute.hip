@@ -2,7 +2,7 @@
 // reference's reshape_kernel gather/scatter, src/gpu/block_sparse.cu:9-64).
 //
 // Two kernels, both pure copies of the raw bits (bit-exact):
-//  * gather: each workgroup covers kGatherPerWG output elements of one job;
+//  * gather: each wavefront covers kGatherPerWG output elements of one job;
 //    output writes are coalesced, each source address comes from a
 //    column-major decode of the output index. Used for blocks and for
 //    permutations that keep the fastest leg in place.
@@ -24,33 +24,39 @@ namespace {
 constexpr int kGatherThreads = 256;
 constexpr int kPerThread = kGatherPerWG / kGatherThreads;  // 8
 
+// Elements e0 + q * stride + t, q < 8, of a job (t = the thread's index in a
+// group of `stride` threads): all source offsets, then all loads in flight
+// together, then the stores. Elements past the job skip the index decode (a
+// small block leaves most of a chunk, and whole wavefronts, with nothing to
+// do).
 template <class W>
-__device__ inline void gather_range(const CopyJob &job, int64_t e0,
+__device__ inline void gather_range(const CopyJob &job, int64_t e0, int stride, int t,
                                     const W *__restrict__ src,
                                     W *__restrict__ dst) {
-  // elements e0 + q*256 + tid, q < 8: all source offsets, then all loads in
-  // flight together, then the stores
   const int r = job.rank;
   i64 off[kPerThread];
 #pragma unroll
   for (int q = 0; q < kPerThread; ++q) {
-    const int64_t e = e0 + (int64_t)q * kGatherThreads + threadIdx.x;
-    uint32_t rem = (uint32_t)e;
-    i64 o = job.src_off;
-    for (int l = 0; l < r; ++l) {
-      const uint32_t d = (uint32_t)job.dims[l];
-      const uint32_t qd = rem / d;
-      o += (i64)(rem - qd * d) * job.strides[l];
-      rem = qd;
+    const int64_t e = e0 + (int64_t)q * stride + t;
+    i64 o = -1;
+    if (e < job.n_elem) {
+      uint32_t rem = (uint32_t)e;
+      o = job.src_off;
+      for (int l = 0; l < r; ++l) {
+        const uint32_t d = (uint32_t)job.dims[l];
+        const uint32_t qd = rem / d;
+        o += (i64)(rem - qd * d) * job.strides[l];
+        rem = qd;
+      }
     }
-    off[q] = e < job.n_elem ? o : -1;
+    off[q] = o;
   }
   W v[kPerThread];
 #pragma unroll
   for (int q = 0; q < kPerThread; ++q) v[q] = off[q] >= 0 ? src[off[q]] : W(0);
 #pragma unroll
   for (int q = 0; q < kPerThread; ++q) {
-    const int64_t e = e0 + (int64_t)q * kGatherThreads + threadIdx.x;
+    const int64_t e = e0 + (int64_t)q * stride + t;
     if (off[q] >= 0) dst[job.dst_off + e] = v[q];
   }
 }
@@ -74,19 +80,29 @@ __device__ inline int64_t find_job(const int32_t *__restrict__ start, int64_t n,
   return lo;
 }
 
-// Persistent gather over a job list: chunk c of the whole list belongs to job
-// j with job_wg_start[j] <= c < job_wg_start[j+1]; the chunk count is read
-// from job_wg_start[n_jobs] so no host round trip is needed.
+// Persistent gather over a job list: chunk c (kGatherPerWG elements) of the
+// whole list belongs to job j with job_wg_start[j] <= c < job_wg_start[j+1];
+// the chunk count is read from job_wg_start[n_jobs] so no host round trip is
+// needed. Each wavefront takes chunks of its own, in quarters of 8 x 64
+// elements, stopping at the job's end: a workgroup then covers four small
+// blocks at once instead of one.
 template <class W>
 __global__ __launch_bounds__(kGatherThreads) void gather_jobs_kernel(
     const CopyJob *__restrict__ jobs, const int32_t *__restrict__ job_wg_start,
     int64_t n_jobs, const W *__restrict__ src, W *__restrict__ dst) {
+  constexpr int kWaves = kGatherThreads / 64;
+  constexpr int kQuarter = kPerThread * 64;
   const int32_t total = job_wg_start[n_jobs];
-  for (int32_t c = blockIdx.x; c < total; c += gridDim.x) {
+  const int lane = threadIdx.x & 63;
+  const int32_t wave = __builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
+  for (int32_t c = (int32_t)blockIdx.x * kWaves + wave; c < total;
+       c += (int32_t)gridDim.x * kWaves) {
     const int64_t lo = find_job(job_wg_start, n_jobs, c);
     const CopyJob &job = jobs[lo];
     const int64_t e0 = (int64_t)(c - job_wg_start[lo]) * kGatherPerWG;
-    gather_range(job, e0, src, dst);
+    for (int part = 0; part < kGatherPerWG / kQuarter && e0 + part * kQuarter < job.n_elem;
+         ++part)
+      gather_range(job, e0 + part * kQuarter, 64, lane, src, dst);
   }
 }
 
@@ -94,7 +110,7 @@ template <class W>
 __global__ __launch_bounds__(kGatherThreads) void gather_single_kernel(
     CopyJob job, const W *__restrict__ src, W *__restrict__ dst) {
   const int64_t e0 = (int64_t)blockIdx.x * kGatherPerWG;
-  gather_range(job, e0, src, dst);
+  gather_range(job, e0, kGatherThreads, (int)threadIdx.x, src, dst);
 }
 
 // ---- tiled transpose -----------------------------------------------------
@@ -670,7 +686,9 @@ tq_status tq_launch_gather_jobs(tq_handle_t h, tq_dtype dt, const CopyJob *jobs,
                                 const int32_t *job_wg_start, int64_t n_jobs,
                                 int64_t total_wg, const void *src, void *dst) {
   if (n_jobs <= 0) return TQ_SUCCESS;
-  i64 grid = std::max<i64>(1, std::min<i64>(total_wg, (i64)h->n_cu * 8));
+  // one chunk per wavefront, four wavefronts per workgroup
+  i64 grid = std::max<i64>(1, std::min<i64>(ceil_div(total_wg, kGatherThreads / 64),
+                                            (i64)h->n_cu * 8));
   if (dt == TQ_F64)
     hipLaunchKernelGGL(gather_jobs_kernel<uint64_t>, dim3((unsigned)grid),
                        dim3(kGatherThreads), 0, h->stream, jobs, job_wg_start,
