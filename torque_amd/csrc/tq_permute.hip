@@ -34,22 +34,28 @@ __device__ inline void gather_range(const CopyJob &job, int64_t e0, int stride, 
                                     const W *__restrict__ src,
                                     W *__restrict__ dst) {
   const int r = job.rank;
-  i64 off[kPerThread];
-#pragma unroll
-  for (int q = 0; q < kPerThread; ++q) {
-    const int64_t e = e0 + (int64_t)q * stride + t;
-    i64 o = -1;
-    if (e < job.n_elem) {
-      uint32_t rem = (uint32_t)e;
-      o = job.src_off;
-      for (int l = 0; l < r; ++l) {
-        const uint32_t d = (uint32_t)job.dims[l];
-        const uint32_t qd = rem / d;
-        o += (i64)(rem - qd * d) * job.strides[l];
-        rem = qd;
-      }
+  auto decode = [&](int64_t e) {
+    uint32_t rem = (uint32_t)e;
+    i64 o = job.src_off;
+    for (int l = 0; l < r; ++l) {
+      const uint32_t d = (uint32_t)job.dims[l];
+      const uint32_t qd = rem / d;
+      o += (i64)(rem - qd * d) * job.strides[l];
+      rem = qd;
     }
-    off[q] = o;
+    return o;
+  };
+  i64 off[kPerThread];
+  if (e0 + (int64_t)kPerThread * stride <= job.n_elem) {
+    // a whole chunk: no per-element test between the decodes
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) off[q] = decode(e0 + (int64_t)q * stride + t);
+  } else {
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+      const int64_t e = e0 + (int64_t)q * stride + t;
+      off[q] = e < job.n_elem ? decode(e) : -1;
+    }
   }
   W v[kPerThread];
 #pragma unroll
