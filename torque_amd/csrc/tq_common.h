@@ -100,17 +100,16 @@ struct CopyJob {
   i64 strides[TQ_MAX_RANK];
 };
 
-// Work items of the pack kernels per job: chunks of kGatherPerWG elements for
-// a plain gather (one wavefront each). A split-BF16 job is cut into its (row, 8-value chunk) cells,
-// kSplitCells per workgroup, numbered rows fastest; when its first
-// contracted leg is contiguous in memory (k fastest) into units of
-// kSplitRows rows x one chunk instead, numbered chunks fastest, kSplitK / 8
-// of them per workgroup (one per wavefront), so that lanes read whole cache
-// lines of a row and the chunks of one row band stay in neighbouring
-// workgroups.
-constexpr int kSplitCells = 256;
+// Work items of the pack kernels per job, one wavefront each: chunks of
+// kGatherPerWG elements for a plain gather. A split-BF16 job is cut into its
+// (row, 8-value chunk) cells, numbered rows fastest, kSplitCells per item;
+// when its first contracted leg is contiguous in memory (k fastest) into
+// units of kSplitRows rows x one chunk instead, numbered chunks fastest, so
+// that lanes read whole cache lines of a row and the chunks of one row band
+// stay in neighbouring items. A workgroup thus covers up to four small
+// blocks at once.
+constexpr int kSplitCells = 64;
 constexpr int kSplitRows = 64;
-constexpr int kSplitK = 32;
 __host__ __device__ inline bool split_kfast(const CopyJob &j) {
   return j.nk > 0 && j.strides[0] == 1 && j.dims[0] > 1;
 }
@@ -141,7 +140,7 @@ __host__ __device__ inline int64_t pack_job_wgs(const CopyJob &j) {
   if (j.n_elem <= 0) return 0;
   if (j.kdim <= 0) return ceil_div(j.n_elem, kGatherPerWG);
   const int64_t rows = j.n_elem / j.kdim;
-  if (split_kfast(j)) return ceil_div(ceil_div(rows, kSplitRows) * (j.kpad / 8), kSplitK / 8);
+  if (split_kfast(j)) return ceil_div(rows, kSplitRows) * (j.kpad / 8);
   return ceil_div(rows * (j.kpad / 8), kSplitCells);
 }
 

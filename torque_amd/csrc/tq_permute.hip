@@ -574,23 +574,27 @@ __device__ inline int32_t xcd_local_id() {
   return (int32_t)((b % 8) * (g / 8) + b / 8);
 }
 
-// Split-BF16 pack. The work items of a job are its (row, 8-value chunk)
-// cells, numbered rows fastest (cell = chunk * rows + row), and a workgroup
-// takes the next kSplitCells of them: one per thread, which reads the cell's
-// 8 values (lanes run along the rows, so the reads coalesce when a row leg
-// is fastest in memory), splits them and writes one 16-byte vector to each
-// plane (a wavefront writes 1 KiB of contiguous output per plane). When the
-// first contracted leg is contiguous each wavefront takes a unit of
-// kSplitRows rows x one chunk instead (split_kfast_tile), units numbered
-// chunks fastest, so that the chunks of a row band share workgroups and
-// XCDs. Values past kdim are zeros. Numbering the work across the whole job
-// keeps every lane busy for jobs of few rows or few chunks.
+// Split-BF16 pack, one work item per wavefront. The items of a job are its
+// (row, 8-value chunk) cells, numbered rows fastest (cell = chunk * rows +
+// row), kSplitCells per item: one per lane, which reads the cell's 8 values
+// (lanes run along the rows, so the reads coalesce when a row leg is fastest
+// in memory), splits them and writes one 16-byte vector to each plane (a
+// wavefront writes 1 KiB of contiguous output per plane). When the first
+// contracted leg is contiguous an item is a unit of kSplitRows rows x one
+// chunk instead (split_kfast_tile), units numbered chunks fastest, so that
+// the chunks of a row band share workgroups and XCDs. Values past kdim are
+// zeros. Numbering the work across the whole job keeps every lane busy for
+// jobs of few rows or few chunks.
 __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
     const CopyJob *__restrict__ jobs, const int32_t *__restrict__ job_wg_start,
     int64_t n_jobs, const double *__restrict__ src, uint16_t *__restrict__ dst,
     int64_t plane) {
+  constexpr int kWaves = kGatherThreads / 64;
   const int32_t total = job_wg_start[n_jobs];
-  for (int32_t c = xcd_local_id(); c < total; c += gridDim.x) {
+  const int lane = threadIdx.x & 63;
+  const int32_t wave = __builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
+  for (int32_t c = xcd_local_id() * kWaves + wave; c < total;
+       c += (int32_t)gridDim.x * kWaves) {
     const int64_t lo = find_job(job_wg_start, n_jobs, c);
     const CopyJob &job = jobs[lo];
     const int nk = job.nk;
@@ -601,17 +605,15 @@ __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
     const uint32_t cells = rows * (uint32_t)(job.kpad >> 3);
     const uint32_t tile = (uint32_t)(c - job_wg_start[lo]);
     if (split_kfast(job)) {
-      // unit = (row band, chunk), chunks fastest, one per wavefront
+      // unit = (row band, chunk), chunks fastest
       const uint32_t chunks = (uint32_t)(job.kpad >> 3);
-      const uint32_t unit = tile * (kSplitK / 8) + (threadIdx.x >> 6);
-      const uint32_t band = unit / chunks;
-      if ((int64_t)band * kSplitRows < rows)
-        split_kfast_tile(job, rows, (int64_t)band * kSplitRows, unit - band * chunks, src, dst,
-                         plane);
+      const uint32_t band = tile / chunks;
+      split_kfast_tile(job, rows, (int64_t)band * kSplitRows, tile - band * chunks, src, dst,
+                       plane);
       continue;
     }
     const Div32 rdiv(rows);
-    const uint32_t cell = tile * kSplitCells + threadIdx.x;
+    const uint32_t cell = tile * kSplitCells + lane;
     if (cell >= cells) continue;  // no barrier in this loop
     const uint32_t chunk = rdiv.div(cell), row = cell - chunk * rows;
     // source offset of (row, k = 8 * chunk): rows decode over legs nk.., k
@@ -681,7 +683,9 @@ tq_status tq_launch_split_jobs(tq_handle_t h, const CopyJob *jobs,
                                int64_t total_wg, const double *src,
                                uint16_t *dst, int64_t plane) {
   if (n_jobs <= 0) return TQ_SUCCESS;
-  i64 grid = std::max<i64>(1, std::min<i64>(total_wg, (i64)h->n_cu * 8));
+  // one work item per wavefront, four wavefronts per workgroup
+  i64 grid = std::max<i64>(1, std::min<i64>(ceil_div(total_wg, kGatherThreads / 64),
+                                            (i64)h->n_cu * 8));
   hipLaunchKernelGGL(split_jobs_kernel, dim3((unsigned)grid), dim3(kGatherThreads), 0,
                      h->stream, jobs, job_wg_start, n_jobs, src, dst, plane);
   TQ_HIP(h, hipGetLastError());
