@@ -132,9 +132,8 @@ tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, const GemmDesc *descs,
 tq_status tq_launch_gather_jobs(tq_handle_t h, tq_dtype dt, const CopyJob *jobs,
                                 const int32_t *job_wg_start, int64_t n_jobs,
                                 int64_t total_wg, const void *src, void *dst);
-// Elements of one work item of the gather kernels (a wavefront's chunk of a
-// job list; a workgroup's part of a single permutation).
-constexpr int kGatherPerWG = 2048;
+// Elements of one work item (a wavefront's chunk) of the job-list gather.
+constexpr int kGatherPerWG = 512;
 
 __host__ __device__ inline int64_t pack_job_wgs(const CopyJob &j) {
   if (j.n_elem <= 0) return 0;

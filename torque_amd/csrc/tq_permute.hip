@@ -2,7 +2,7 @@
 // reference's reshape_kernel gather/scatter, src/gpu/block_sparse.cu:9-64).
 //
 // Two kernels, both pure copies of the raw bits (bit-exact):
-//  * gather: each wavefront covers kGatherPerWG output elements of one job;
+//  * gather: each work item covers kGatherPerWG output elements of one job;
 //    output writes are coalesced, each source address comes from a
 //    column-major decode of the output index. Used for blocks and for
 //    permutations that keep the fastest leg in place.
@@ -22,7 +22,9 @@
 namespace {
 
 constexpr int kGatherThreads = 256;
-constexpr int kPerThread = kGatherPerWG / kGatherThreads;  // 8
+constexpr int kPerThread = kGatherPerWG / 64;  // 8 elements per lane
+// elements of one workgroup of the single-job gather
+constexpr int kSingleChunk = kGatherThreads * kPerThread;
 
 // Elements e0 + q * stride + t, q < 8, of a job (t = the thread's index in a
 // group of `stride` threads): all source offsets, then all loads in flight
@@ -86,18 +88,16 @@ __device__ inline int64_t find_job(const int32_t *__restrict__ start, int64_t n,
   return lo;
 }
 
-// Persistent gather over a job list: chunk c (kGatherPerWG elements) of the
-// whole list belongs to job j with job_wg_start[j] <= c < job_wg_start[j+1];
-// the chunk count is read from job_wg_start[n_jobs] so no host round trip is
-// needed. Each wavefront takes chunks of its own, in quarters of 8 x 64
-// elements, stopping at the job's end: a workgroup then covers four small
-// blocks at once instead of one.
+// Persistent gather over a job list: chunk c (kGatherPerWG elements, one
+// wavefront's) of the whole list belongs to job j with job_wg_start[j] <= c <
+// job_wg_start[j+1]; the chunk count is read from job_wg_start[n_jobs] so no
+// host round trip is needed. A workgroup covers up to four small blocks at
+// once, and a large block spreads over many wavefronts.
 template <class W>
 __global__ __launch_bounds__(kGatherThreads) void gather_jobs_kernel(
     const CopyJob *__restrict__ jobs, const int32_t *__restrict__ job_wg_start,
     int64_t n_jobs, const W *__restrict__ src, W *__restrict__ dst) {
   constexpr int kWaves = kGatherThreads / 64;
-  constexpr int kQuarter = kPerThread * 64;
   const int32_t total = job_wg_start[n_jobs];
   const int lane = threadIdx.x & 63;
   const int32_t wave = __builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
@@ -106,16 +106,14 @@ __global__ __launch_bounds__(kGatherThreads) void gather_jobs_kernel(
     const int64_t lo = find_job(job_wg_start, n_jobs, c);
     const CopyJob &job = jobs[lo];
     const int64_t e0 = (int64_t)(c - job_wg_start[lo]) * kGatherPerWG;
-    for (int part = 0; part < kGatherPerWG / kQuarter && e0 + part * kQuarter < job.n_elem;
-         ++part)
-      gather_range(job, e0 + part * kQuarter, 64, lane, src, dst);
+    gather_range(job, e0, 64, lane, src, dst);
   }
 }
 
 template <class W>
 __global__ __launch_bounds__(kGatherThreads) void gather_single_kernel(
     CopyJob job, const W *__restrict__ src, W *__restrict__ dst) {
-  const int64_t e0 = (int64_t)blockIdx.x * kGatherPerWG;
+  const int64_t e0 = (int64_t)blockIdx.x * kSingleChunk;
   gather_range(job, e0, kGatherThreads, (int)threadIdx.x, src, dst);
 }
 
@@ -246,7 +244,7 @@ __global__ __launch_bounds__(256) void tiled_permute_kernel(
 template <class W>
 tq_status launch_gather_single(tq_handle_t h, const CopyJob &job,
                                const void *src, void *dst) {
-  const int64_t nwg = ceil_div(job.n_elem, kGatherPerWG);
+  const int64_t nwg = ceil_div(job.n_elem, kSingleChunk);
   if (nwg == 0) return TQ_SUCCESS;
   hipLaunchKernelGGL(gather_single_kernel<W>, dim3((unsigned)nwg),
                      dim3(kGatherThreads), 0, h->stream, job,
