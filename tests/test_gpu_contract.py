@@ -175,6 +175,48 @@ def test_small_k_stream_and_mfma_agree(cuda, small_k):
         plan.set_small_k(9)
 
 
+@pytest.mark.parametrize("layout", ["k_fast", "n_fast", "mixed"])
+def test_small_k_wide_blocks_are_sliced(cuda, layout):
+    # streamed pairs (K <= 8) wider than 64 columns are cut into 64-column
+    # slices, each with its own B and C offsets; every tier must match the
+    # oracle whatever the B layout (k fastest, n fastest, or per block)
+    rng = np.random.default_rng(17)
+    nb = 12
+    r = rng.integers(40, 151, nb)
+    k = rng.integers(1, 9, nb)
+    n = rng.integers(65, 201, nb)
+
+    def diag(rows, cols):
+        b0 = np.concatenate([[0], np.cumsum(rows)[:-1]])
+        b1 = np.concatenate([[0], np.cumsum(cols)[:-1]])
+        begin = np.stack([b0, b1], 1).astype(np.int64)
+        end = np.stack([b0 + rows - 1, b1 + cols - 1], 1).astype(np.int64)
+        return [int(rows.sum()), int(cols.sum())], begin, end
+
+    da_, ba, ea = diag(r, k)
+    a, dta = tensor(da_, ba, ea, rng)
+    if layout == "n_fast":
+        db_, bb, eb = diag(n, k)
+        legs = [[1, 1]]
+        b, dtb = tensor(db_, bb, eb, rng)
+    else:
+        db_, bb, eb = diag(k, n)
+        legs = [[1, 0]]
+        strides = permuted_strides(bb, eb, rng) if layout == "mixed" else None
+        b, dtb = tensor(db_, bb, eb, rng, strides)
+    ref_c, ref = O.contract(a, dta, b, dtb, legs)
+    plan, got = gpu_contract(a, dta, b, dtb, legs, cuda)
+    check_maps(plan, ref)
+    assert rel_fro(got, ref_c) < FP64_TOL
+    n_tiles, n_segments = plan.work()
+    assert n_tiles == 0
+    assert n_segments == int((np.ceil(r / 64) * np.ceil(n / 64)).sum())
+    _, got32 = gpu_contract(a, dta, b, dtb, legs, cuda, precision="fp32")
+    assert rel_fro(got32, ref_c) < 1e-5
+    _, got3 = gpu_contract(a, dta, b, dtb, legs, cuda, precision="bf16x3")
+    assert rel_fro(got3, ref_c) < 1e-6
+
+
 def test_peps_absorption_d6_chi64(cuda):
     E, A, Ac, l1, l2 = W.peps(6, 64, seed=1)
     TA, dA = from_workload(A)

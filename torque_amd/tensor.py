@@ -361,7 +361,7 @@ class ContractionPlan:
         self.device = a.data.device
 
     def work(self):
-        """(MFMA tiles, streamed 64-row segments) of the current configuration."""
+        """(MFMA tiles, streamed 64 x 64 segments) of the current configuration."""
         info = PlanInfo()
         self.h.check(lib().tq_plan_get_info(self.ptr, ctypes.byref(info)))
         return info.n_tiles, info.n_segments
