@@ -5,7 +5,9 @@
 // MFMA tiles. One wavefront takes a 64-row segment of the pair's column-major
 // output: each lane holds its row of A (K values) in registers and walks the
 // N columns; B[n, :] is the same for the whole wavefront (scalar loads), and
-// each column is one contiguous 64-element store.
+// each column is one contiguous 64-element store. The planner hands wide
+// pairs over as slices of at most kStreamCols columns, each a pair of its own
+// here, so that the segments stay small enough to balance.
 //
 // The split-BF16 tier streams the same pairs from its split panels: a pair
 // with K <= 8 has one 8-value chunk per row, so a row is one 16-byte vector
