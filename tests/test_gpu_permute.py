@@ -99,6 +99,24 @@ def test_block_sparse_permute_bit_exact(cuda, seed):
     assert np.array_equal(out.block_offsets, O.nest_sum(t.n_elem))
 
 
+def test_block_sparse_permute_chunk_edges(cuda):
+    # blocks whose sizes sit on and around the gather's 512-element work items
+    # (one wavefront each), mixed with one-element blocks, so that items end
+    # inside, at and just past block ends; rank 3 with permuted storage
+    rng = np.random.default_rng(5)
+    shapes = [(1, 1, 511), (8, 64, 1), (3, 9, 19), (1, 1, 1), (31, 33, 1), (2, 16, 32),
+              (5, 5, 41), (1, 1, 1), (64, 32, 1), (7, 11, 27), (1, 2, 1), (9, 7, 48)]
+    shapes = np.array(shapes, np.int64)
+    b0 = np.cumsum(np.vstack([np.zeros((1, 3), np.int64), shapes[:-1]]), axis=0)
+    begin, end = b0, b0 + shapes - 1
+    dims = list(end.max(axis=0) + 1)
+    t, d = tensor(dims, begin, end, rng, permuted_strides(begin, end, rng))
+    for perm in ([2, 0, 1], [1, 2, 0], [0, 2, 1]):
+        out = torch_tensor(t, d, cuda).hard_transpose(perm)
+        ref = O.bs_permute(t, d, perm)
+        assert np.array_equal(out.data.cpu().numpy()[: ref.size], ref), perm
+
+
 def test_block_sparse_permute_large_blocks(cuda):
     from torque_amd import workloads as W
     E, A, Ac, _, _ = W.peps(8, 128)
