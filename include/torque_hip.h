@@ -253,7 +253,11 @@ tq_status tq_batch_create(tq_handle_t handle, int32_t n, const tq_plan_t *plans,
 
 /* Captures tq_bs_contract_execute(plan, a, b, c) into a HIP graph owned by
  * the plan (the pointers are baked in); tq_plan_graph_launch replays it on
- * the handle's stream with no host work per replay. */
+ * the handle's stream with no host work per replay. tq_plan_set_config,
+ * tq_plan_set_class_config, tq_plan_set_small_k and tq_plan_autotune
+ * release the captured graph (they may rebuild the work lists it reads):
+ * tq_plan_graph_launch then returns TQ_ERROR_NOT_INITIALIZED until the plan
+ * is captured again. */
 tq_status tq_plan_capture(tq_plan_t plan, const void *a, const void *b, void *c);
 tq_status tq_plan_graph_launch(tq_plan_t plan);
 
@@ -331,6 +335,14 @@ tq_status tq_nrm2_f64(tq_handle_t handle, int64_t n, const double *x,
                       double *result);
 /* x <- x * alpha on the GPU (normalisation of environment tensors). */
 tq_status tq_scale_f64(tq_handle_t handle, int64_t n, double *x, double alpha);
+/* Normalisation of an environment tensor between CTMRG steps, entirely on
+ * the device: *norm (a device pointer) receives sqrt(sum x[i]^2), reduced in
+ * FP64 in the fixed order of tq_nrm2_f64, and x <- x / *norm with the norm
+ * read on the device (x is unchanged when the norm is 0). No reference
+ * counterpart: the reference's PEPS tests normalise on the host
+ * (test/gpu/peps_test.cu:798-842). Allocates its partial sums and
+ * synchronises the handle's stream before returning. */
+tq_status tq_normalize_f64(tq_handle_t handle, int64_t n, double *x, double *norm);
 
 /* ---- learned tile-configuration cost model (no reference counterpart) -- */
 

@@ -92,3 +92,30 @@ def test_graph_replay_equals_eager(cuda):
         p.graph_launch()
         torch.cuda.synchronize()
         assert torch.equal(out, ref)
+
+
+def test_setters_release_captured_graph(cuda):
+    # a captured graph holds the tile lists' addresses: changing the
+    # configuration (which may rebuild them) releases it, and a replay is
+    # refused until the plan is captured again
+    from torque_amd import TorqueError
+    ga, gb, plans, refs, a_all, b_all, c_total, c_off, batch = make_batch(cuda, 6, 11)
+    eager = torch.empty(c_total, dtype=torch.float64, device=cuda)
+    batch.execute(a_all, b_all, eager)
+    c = torch.zeros(c_total, dtype=torch.float64, device=cuda)
+    for change in (lambda: batch.set_config(0), lambda: batch.set_class_config(1, 3),
+                   lambda: batch.set_small_k(4),
+                   lambda: batch.autotune(a_all, b_all, c, reps=1)):
+        batch.capture(a_all, b_all, c)
+        change()
+        with pytest.raises(TorqueError, match="not captured"):
+            batch.graph_launch()
+        now = torch.empty_like(eager)
+        batch.execute(a_all, b_all, now)
+        torch.cuda.synchronize()
+        assert torch.allclose(now, eager, rtol=1e-12, atol=1e-12)
+        batch.capture(a_all, b_all, c)
+        c.zero_()
+        batch.graph_launch()
+        torch.cuda.synchronize()
+        assert torch.equal(c, now)

@@ -34,7 +34,7 @@ def test_rank3_append_and_scalar():
 
 
 def test_metadata_derivation():
-    t = BlockSparseTensor([5, 6], np.zeros(20), [[0, 0], [1, 2]], [[1, 2], [4, 5]], device="cpu")
+    t = BlockSparseTensor([5, 6], np.zeros(22), [[0, 0], [1, 2]], [[1, 2], [4, 5]], device="cpu")
     assert t.block_n_elem.tolist() == [6, 16]
     assert t.block_offsets.tolist() == [0, 6]
     assert t.index_tables.tolist() == [[1, 2], [1, 4]]
@@ -67,3 +67,14 @@ def test_dense_host_access():
     d = DenseTensor([4, 3], device="cpu")
     d.initialize(torch.arange(1, 13, dtype=torch.float64))
     assert d.query([1, 2]) == 10
+
+
+def test_short_source_data_is_rejected():
+    # the kernels read every element of every block: data shorter than the
+    # blocks need is an error, in the constructor and in append_block
+    with pytest.raises(TorqueError, match="the blocks need 22"):
+        BlockSparseTensor([5, 6], np.zeros(20), [[0, 0], [1, 2]], [[1, 2], [4, 5]],
+                          device="cpu")
+    t = BlockSparseTensor([4, 3], [1, 2, 3, 4], [[1, 1]], [[2, 2]], device="cpu")
+    with pytest.raises(TorqueError, match="the block needs 4"):
+        t.append_block([1, 2, 3], [0, 0], [1, 1], [1, 2])

@@ -170,14 +170,7 @@ tq_status tq_plan_capture(tq_plan_t pl, const void *a, const void *b, void *c) {
   // everything the execute needs is allocated before capture starts
   TQ_CHECK(plan_build_tiles(pl));
   TQ_HIP(h, hipStreamSynchronize(h->stream));
-  if (pl->graph_exec) {
-    (void)hipGraphExecDestroy(pl->graph_exec);
-    pl->graph_exec = nullptr;
-  }
-  if (pl->graph) {
-    (void)hipGraphDestroy(pl->graph);
-    pl->graph = nullptr;
-  }
+  pl->drop_graph();
   if (!pl->capture_stream) TQ_HIP(h, hipStreamCreateWithFlags(&pl->capture_stream,
                                                               hipStreamNonBlocking));
   hipStream_t user = h->stream;

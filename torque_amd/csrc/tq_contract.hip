@@ -124,6 +124,7 @@ tq_status check_config(tq_plan_t pl, int32_t config) {
 tq_status tq_plan_set_config(tq_plan_t pl, int32_t config) {
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
   TQ_CHECK(check_config(pl, config));
+  pl->drop_graph();
   pl->cls[0].cfg = pl->cls[1].cfg = config;
   return TQ_SUCCESS;
 }
@@ -132,6 +133,7 @@ tq_status tq_plan_set_class_config(tq_plan_t pl, int32_t cls, int32_t config) {
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
   if (cls < 0 || cls > 1) return TQ_ERROR_INVALID_VALUE;
   TQ_CHECK(check_config(pl, config));
+  pl->drop_graph();
   pl->cls[cls].cfg = config;
   return TQ_SUCCESS;
 }
@@ -222,6 +224,7 @@ tq_status tq_plan_set_small_k(tq_plan_t pl, int32_t max_k) {
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
   if (max_k < 0 || max_k > kStreamMaxK)
     return tq_fail(pl->h, TQ_ERROR_INVALID_VALUE, "small-K threshold above 8");
+  pl->drop_graph();
   pl->stream_max_k = max_k;
   pl->chunks_ready = false;
   pl->runs_built = false;  // runs hold pairs of one class
@@ -245,6 +248,7 @@ tq_status tq_plan_autotune(tq_plan_t pl, const void *a, const void *b, void *c,
       for (int k = 0; k < 2; ++k) best[k] = pl->cls[k].current();
     return TQ_SUCCESS;
   }
+  pl->drop_graph();
   TQ_CHECK(tq_bs_contract_pack(pl, a, b));
   TQ_CHECK(plan_build_tiles(pl));
   hipEvent_t e0, e1;

@@ -771,6 +771,18 @@ tq_status tq_bs_permute(tq_handle_t h, tq_dtype dtype, const tq_bs_desc *d,
                              hipMemcpyDeviceToDevice, h->stream));
     return TQ_SUCCESS;
   }
+  // sizes are checked before anything is uploaded or launched; the work-item
+  // total also bounds the grid and must fit the gather's int32 prefix
+  i64 total_elem = 0;
+  for (i64 b = 0; b < nb; ++b) {
+    i64 n = 1;
+    for (int l = 0; l < r; ++l) n *= d->end[b * r + l] - d->begin[b * r + l] + 1;
+    if (n >= (i64(1) << 32))
+      return tq_fail(h, TQ_ERROR_UNSUPPORTED, "permute: block above 2^32 elements");
+    total_elem += ceil_div(n, kGatherPerWG);
+  }
+  if (total_elem > 0x7fffffff)
+    return tq_fail(h, TQ_ERROR_UNSUPPORTED, "permute: too many work items");
   // one upload: begin, end, strides (nb*r each), offsets (nb), perm (r)
   const size_t meta = (size_t)nb * r * 3 + nb;
   const size_t bytes = meta * 8 + 64;
@@ -807,15 +819,6 @@ tq_status tq_bs_permute(tq_handle_t h, tq_dtype dtype, const tq_bs_desc *d,
                      dim3(256), 0, h->stream, dwg.as<i64>(), dwg32.as<int32_t>(),
                      nb + 1);
   TQ_HIP(h, hipGetLastError());
-  // total chunk count on the host only bounds the grid size
-  i64 total_elem = 0;
-  for (i64 b = 0; b < nb; ++b) {
-    i64 n = 1;
-    for (int l = 0; l < r; ++l) n *= d->end[b * r + l] - d->begin[b * r + l] + 1;
-    if (n >= (i64(1) << 32))
-      return tq_fail(h, TQ_ERROR_UNSUPPORTED, "permute: block above 2^32 elements");
-    total_elem += ceil_div(n, kGatherPerWG);
-  }
   TQ_CHECK(tq_launch_gather_jobs(h, dtype, djobs.as<CopyJob>(),
                                  dwg32.as<int32_t>(), nb, total_elem, in, out));
   // the temporaries are freed on return: wait for the kernels using them

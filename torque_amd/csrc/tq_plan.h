@@ -136,9 +136,17 @@ struct tq_plan_s {
   hipStream_t capture_stream = nullptr;
   bool is_batch = false;
 
-  ~tq_plan_s() {
+  // Releases a captured execute. Every call that may rebuild the tile or
+  // segment lists (configuration setters, autotune) drops the graph first:
+  // it holds the old buffers' addresses and tile counts.
+  void drop_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+  }
+  ~tq_plan_s() {
+    drop_graph();
     if (capture_stream) (void)hipStreamDestroy(capture_stream);
   }
 

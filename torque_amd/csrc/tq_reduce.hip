@@ -56,6 +56,17 @@ __global__ void scale_kernel(double *__restrict__ x, int64_t n, double alpha) {
     x[i] *= alpha;
 }
 
+// x[i] <- x[i] / norm[0], the norm read on the device; x is left unchanged
+// when the norm is zero.
+__global__ void inv_scale_kernel(double *__restrict__ x, int64_t n,
+                                 const double *__restrict__ norm) {
+  const double s = norm[0];
+  if (s == 0.0) return;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    x[i] = x[i] / s;
+}
+
 int64_t n_blocks_for(int64_t n) {
   return std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, ceil_div(n, 8 * kThreads)));
 }
@@ -73,6 +84,18 @@ tq_status reduce(tq_handle_t h, const double *x, const double *y, int64_t n,
                        h->stream, x, nullptr, n, per, part);
   hipLaunchKernelGGL(final_kernel, dim3(1), dim3(kThreads), 0, h->stream, part, nb,
                      out_dev, take_sqrt ? 1 : 0);
+  TQ_HIP(h, hipGetLastError());
+  return TQ_SUCCESS;
+}
+
+// norm[0] = ||x|| (device), then x <- x / ||x||: two reduction launches and
+// one scaling launch on the handle's stream, partial sums in tmp.
+tq_status normalize(tq_handle_t h, int64_t n, double *x, double *norm, void *tmp) {
+  TQ_CHECK(reduce(h, x, x, n, norm, tmp, true));
+  if (n == 0) return TQ_SUCCESS;
+  const int64_t g = std::min<int64_t>(ceil_div(n, 256), (int64_t)h->n_cu * 16);
+  hipLaunchKernelGGL(inv_scale_kernel, dim3((unsigned)g), dim3(256), 0, h->stream, x, n,
+                     norm);
   TQ_HIP(h, hipGetLastError());
   return TQ_SUCCESS;
 }
@@ -109,6 +132,17 @@ tq_status tq_nrm2_f64(tq_handle_t h, int64_t n, const double *x, double *result)
   double *out = tmp.as<double>() + n_blocks_for(n) + 1;
   TQ_CHECK(reduce(h, x, x, n, out, tmp.p, true));
   TQ_HIP(h, hipMemcpyAsync(result, out, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  return TQ_SUCCESS;
+}
+
+tq_status tq_normalize_f64(tq_handle_t h, int64_t n, double *x, double *norm) {
+  if (!h) return TQ_ERROR_NOT_INITIALIZED;
+  if (n < 0 || !norm || (n > 0 && !x)) return TQ_ERROR_INVALID_VALUE;
+  DevBuf tmp;
+  TQ_HIP(h, tmp.alloc(tq_sum_tmp_bytes(n)));
+  TQ_CHECK(normalize(h, n, x, norm, tmp.p));
+  // the partial sums are released on return
   TQ_HIP(h, hipStreamSynchronize(h->stream));
   return TQ_SUCCESS;
 }

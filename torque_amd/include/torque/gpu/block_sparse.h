@@ -248,7 +248,13 @@ class BlockSparseTensor {
     if (permutation.n_elem() != rank)
       throw Error("The number of permutation does not match the rank of tensor");
     std::vector<int32_t> perm(rank);
-    for (uword k = 0; k < rank; ++k) perm[k] = (int32_t)permutation[k];
+    std::vector<bool> seen(rank, false);
+    for (uword k = 0; k < rank; ++k) {
+      if (permutation[k] >= rank || seen[permutation[k]])
+        throw Error("The number of permutation does not match the rank of tensor");
+      seen[permutation[k]] = true;
+      perm[k] = (int32_t)permutation[k];
+    }
     BlockSparseTensor out;
     out.rank = rank;
     out.dimension = uvec(rank);

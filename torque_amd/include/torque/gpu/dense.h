@@ -90,7 +90,14 @@ class DenseTensor {
       throw Error("The number of permutation does not match the rank of tensor");
     std::vector<int64_t> dims(dimension.vec().begin(), dimension.vec().end());
     std::vector<int32_t> perm(rank);
+    std::vector<bool> seen(rank, false);
     uvec nd(rank);
+    // every leg exactly once, checked before any index or allocation uses it
+    for (uword k = 0; k < rank; ++k) {
+      if (permutation[k] >= rank || seen[permutation[k]])
+        throw Error("The number of permutation does not match the rank of tensor");
+      seen[permutation[k]] = true;
+    }
     for (uword k = 0; k < rank; ++k) {
       perm[k] = (int32_t)permutation[k];
       nd[k] = dimension[permutation[k]];

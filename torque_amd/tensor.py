@@ -96,6 +96,17 @@ def default_handle(device=None):
     return h
 
 
+def _extent(t):
+    """Elements a kernel may read from the tensor's data buffer: one past
+    the highest element address of any block."""
+    if t.rank == 0:
+        return 1
+    if t.n_blocks == 0:
+        return 0
+    last = t.block_offsets + np.sum((t.end_points - t.begin_points) * t.index_tables, axis=1)
+    return int(last.max()) + 1
+
+
 def _require_gpu(t):
     if not t.is_cuda:
         raise TorqueError(_lib.TQ_ERROR_UNSUPPORTED,
@@ -136,6 +147,12 @@ class BlockSparseTensor:
         if source_data is None:
             raise TorqueError(_lib.TQ_ERROR_INVALID_VALUE, "Source data not allocated!")
         src = torch.as_tensor(source_data).reshape(-1)
+        if src.numel() < n:
+            # the kernels read every block element: a short buffer would be
+            # read past its end
+            raise TorqueError(_lib.TQ_ERROR_INVALID_VALUE,
+                              "source data holds %d elements, the blocks need %d"
+                              % (src.numel(), n))
         self.data = src[:max(n, 1)].to(device=self.device, dtype=dtype).clone()
         self._meta_changed()
 
@@ -228,7 +245,12 @@ class BlockSparseTensor:
         n = int(np.prod(e - b + 1))
         had_blocks = self.n_blocks > 0
         old_n = int(self.block_n_elem.sum())
-        src = torch.as_tensor(source_data).reshape(-1)[:n].to(self.device, self.dtype)
+        src = torch.as_tensor(source_data).reshape(-1)
+        if src.numel() < n:
+            raise TorqueError(_lib.TQ_ERROR_INVALID_VALUE,
+                              "source data holds %d elements, the block needs %d"
+                              % (src.numel(), n))
+        src = src[:n].to(self.device, self.dtype)
         self.begin_points = np.concatenate([self.begin_points, b])
         self.end_points = np.concatenate([self.end_points, e])
         self.index_tables = np.concatenate([self.index_tables, t])
@@ -359,6 +381,20 @@ class ContractionPlan:
             p, _i64p(self.out_dims), _i64p(self.out_begin), _i64p(self.out_end),
             _i64p(self.out_strides), _i64p(self.out_offsets)))
         self.device = a.data.device
+        self.a_extent = _extent(a)
+        self.b_extent = _extent(b)
+
+    def _check_sizes(self, a_data, b_data, out=None):
+        # the kernels take raw pointers: buffers too short for the block
+        # structure would be read (or written) past their end
+        if a_data.numel() < self.a_extent or b_data.numel() < self.b_extent:
+            raise TorqueError(_lib.TQ_ERROR_INVALID_VALUE,
+                              "operand data shorter than the planned block structure")
+        if out is not None and out.numel() < self.out_n_elem:
+            raise TorqueError(_lib.TQ_ERROR_INVALID_VALUE,
+                              "output buffer shorter than the planned output")
+        for t in (a_data, b_data) + ((out,) if out is not None else ()):
+            _require_gpu(t)
 
     def work(self):
         """(MFMA tiles, streamed 64 x 64 segments) of the current configuration."""
@@ -382,6 +418,7 @@ class ContractionPlan:
         """Time every tile configuration of each MFMA tile class
         (tq_plan_autotune); returns ([class 0, class 1] chosen configuration,
         [class 0 mean ms per configuration, class 1 ...])."""
+        self._check_sizes(a_data, b_data, out)
         self.h.bind_stream()
         n = lib().tq_gemm_config_count(_lib.PRECISIONS[self.precision])
         times = (ctypes.c_float * max(2 * n, 1))()
@@ -407,17 +444,20 @@ class ContractionPlan:
         self.h.check(lib().tq_plan_set_config(self.ptr, int(cfg)))
 
     def execute(self, a_data, b_data, out):
+        self._check_sizes(a_data, b_data, out)
         self.h.bind_stream()
         self.h.check(lib().tq_bs_contract_execute(
             self.ptr, ctypes.c_void_p(a_data.data_ptr()),
             ctypes.c_void_p(b_data.data_ptr()), ctypes.c_void_p(out.data_ptr())))
 
     def pack(self, a_data, b_data):
+        self._check_sizes(a_data, b_data)
         self.h.bind_stream()
         self.h.check(lib().tq_bs_contract_pack(
             self.ptr, ctypes.c_void_p(a_data.data_ptr()), ctypes.c_void_p(b_data.data_ptr())))
 
     def multiply(self, a_data, b_data, out):
+        self._check_sizes(a_data, b_data, out)
         self.h.bind_stream()
         self.h.check(lib().tq_bs_contract_multiply(
             self.ptr, ctypes.c_void_p(a_data.data_ptr()),
@@ -441,6 +481,7 @@ class ContractionPlan:
 
     def capture(self, a_data, b_data, out):
         """Capture execute(a, b, out) into a HIP graph (tq_plan_capture)."""
+        self._check_sizes(a_data, b_data, out)
         self.h.bind_stream()
         self.h.check(lib().tq_plan_capture(
             self.ptr, ctypes.c_void_p(a_data.data_ptr()), ctypes.c_void_p(b_data.data_ptr()),
@@ -491,6 +532,8 @@ class BatchPlan(ContractionPlan):
         self.flops = info.flops
         self.bytes = info.bytes
         self.out_rank = info.out_rank
+        self.a_extent = max([int(o) + p.a_extent for o, p in zip(ao, plans)] or [0])
+        self.b_extent = max([int(o) + p.b_extent for o, p in zip(bo, plans)] or [0])
 
 
 def gemm_configs(precision="native", with_kind=False):
