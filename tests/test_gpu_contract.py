@@ -12,7 +12,7 @@ import torch
 from helpers import O, W, dense_of, from_workload, permuted_strides, random_blocks, rel_fro, \
     tensor, torch_tensor
 from torque_amd import BlockSparseTensor, DenseTensor, TorqueError, dot_f64, gemm_configs, \
-    nrm2_f64, scale_f64
+    normalize_f64, nrm2_f64, scale_f64
 
 pytestmark = pytest.mark.gpu
 
@@ -349,6 +349,21 @@ def test_reductions(cuda):
     assert abs(nrm2_f64(gx) - 1.0) < 1e-14
     # repeated reductions agree bitwise
     assert dot_f64(gx, gy) == dot_f64(gx, gy)
+    # device-side normalisation: the norm stays on the device and equals
+    # tq_nrm2_f64 bitwise (same reduction order); x / ||x|| has norm 1
+    gx = torch.tensor(x * 3.5, device=cuda)
+    want = nrm2_f64(gx)
+    nrm = normalize_f64(gx)
+    torch.cuda.synchronize()
+    assert nrm.is_cuda and float(nrm[0]) == want
+    assert abs(float(nrm[0]) - 3.5 * np.linalg.norm(x)) < 1e-12 * 3.5 * np.linalg.norm(x)
+    assert np.array_equal(gx.cpu().numpy(), torch.tensor(x * 3.5).numpy() / want)
+    assert abs(nrm2_f64(gx) - 1.0) < 1e-14
+    # a zero vector keeps its values and reports norm 0
+    z = torch.zeros(1000, dtype=torch.float64, device=cuda)
+    assert float(normalize_f64(z)[0]) == 0.0 and not z.any()
+    e = torch.zeros(1, dtype=torch.float64, device=cuda)
+    assert float(normalize_f64(e[:0])[0]) == 0.0
 
 
 @pytest.mark.parametrize("maker", ["c1", "batched", "overlap"])

@@ -852,3 +852,15 @@ def scale_f64(x, alpha):
     h = default_handle(x.device.index)
     h.check(lib().tq_scale_f64(h.ptr, x.numel(), ctypes.c_void_p(x.data_ptr()),
                                ctypes.c_double(alpha)))
+
+
+def normalize_f64(x, norm=None):
+    """x <- x / ||x|| on the device (tq_normalize_f64); the FP64 norm is
+    written to the one-element device tensor `norm` (allocated when None),
+    which is returned."""
+    h = default_handle(x.device.index)
+    if norm is None:
+        norm = torch.empty(1, dtype=torch.float64, device=x.device)
+    h.check(lib().tq_normalize_f64(h.ptr, x.numel(), ctypes.c_void_p(x.data_ptr()),
+                                   ctypes.c_void_p(norm.data_ptr())))
+    return norm
