@@ -343,6 +343,15 @@ tq_status tq_scale_f64(tq_handle_t handle, int64_t n, double *x, double alpha);
  * (test/gpu/peps_test.cu:798-842). Allocates its partial sums and
  * synchronises the handle's stream before returning. */
 tq_status tq_normalize_f64(tq_handle_t handle, int64_t n, double *x, double *norm);
+/* tq_normalize_f64 without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops). No reference
+ * counterpart. The partial sums live in a static device array of 1024 FP64
+ * values per GPU (8 KiB, held until the process ends; every n fits, each
+ * partial covering a 1/1024 share of x). Stream-ordered: it returns without
+ * synchronising. All calls on one GPU share that array, so calls that
+ * overlap on one GPU (other streams or threads) race. Results are bitwise
+ * those of tq_normalize_f64. */
+tq_status tq_normalize_f64_noalloc(tq_handle_t handle, int64_t n, double *x, double *norm);
 
 /* ---- learned tile-configuration cost model (no reference counterpart) -- */
 

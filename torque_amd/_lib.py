@@ -132,6 +132,7 @@ def _proto(lib):
         "tq_nrm2_f64": (S, [H, ctypes.c_int64, V, ctypes.POINTER(ctypes.c_double)]),
         "tq_scale_f64": (S, [H, ctypes.c_int64, V, ctypes.c_double]),
         "tq_normalize_f64": (S, [H, ctypes.c_int64, V, V]),
+        "tq_normalize_f64_noalloc": (S, [H, ctypes.c_int64, V, V]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

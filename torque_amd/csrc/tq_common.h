@@ -242,6 +242,10 @@ tq_status tq_scan_i64(tq_handle_t h, const i64 *in, i64 *out, int64_t n,
 
 // ---- reductions (tq_reduce.hip) ----------------------------------------
 
+// Per-device set-up of the reductions' static scratch (called by tq_create
+// with the handle's device current).
+tq_status tq_reduce_setup_device(tq_handle_t h);
+
 // Deterministic FP64 sum of n values into *out (device pointer).
 tq_status tq_sum_f64_dev(tq_handle_t h, const double *x, int64_t n,
                          double *out, void *tmp);

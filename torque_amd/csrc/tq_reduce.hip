@@ -5,6 +5,8 @@
 // summation order (two stages, fixed grid), so repeated runs agree bitwise.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <vector>
 
 #include "tq_common.h"
 
@@ -102,6 +104,31 @@ tq_status normalize(tq_handle_t h, int64_t n, double *x, double *norm, void *tmp
 
 }  // namespace
 
+// Scratch of tq_normalize_f64_noalloc: the first stage's partial sums (at
+// most kMaxBlocks of them, whatever n is). Each device the library runs on
+// holds one copy of the array until the process ends; host code reaches a
+// device's copy only through the address hipGetSymbolAddress returned for it
+// with that device current (scratch_dev), never through the array's name.
+static __device__ __align__(256) double scratch[kMaxBlocks];
+
+namespace {
+std::mutex scratch_mu;
+std::vector<double *> scratch_dev;  // device index -> address of its copy
+}  // namespace
+
+tq_status tq_reduce_setup_device(tq_handle_t h) {
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if ((int)scratch_dev.size() <= h->device) scratch_dev.resize(h->device + 1, nullptr);
+  if (scratch_dev[h->device]) return TQ_SUCCESS;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return tq_fail(h, TQ_ERROR_HIP,
+                   std::string("scratch address: ") + hipGetErrorString(e));
+  scratch_dev[h->device] = reinterpret_cast<double *>(p);
+  return TQ_SUCCESS;
+}
+
 size_t tq_sum_tmp_bytes(int64_t n) { return sizeof(double) * (n_blocks_for(n) + 1); }
 
 tq_status tq_sum_f64_dev(tq_handle_t h, const double *x, int64_t n, double *out,
@@ -145,6 +172,20 @@ tq_status tq_normalize_f64(tq_handle_t h, int64_t n, double *x, double *norm) {
   // the partial sums are released on return
   TQ_HIP(h, hipStreamSynchronize(h->stream));
   return TQ_SUCCESS;
+}
+
+tq_status tq_normalize_f64_noalloc(tq_handle_t h, int64_t n, double *x, double *norm) {
+  if (!h) return TQ_ERROR_NOT_INITIALIZED;
+  if (n < 0 || !norm || (n > 0 && !x)) return TQ_ERROR_INVALID_VALUE;
+  double *tmp = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (h->device < (int)scratch_dev.size()) tmp = scratch_dev[h->device];
+  }
+  if (!tmp)
+    return tq_fail(h, TQ_ERROR_NOT_INITIALIZED, "normalize: device scratch not set up");
+  // every partial is written before it is read: nothing needs clearing
+  return normalize(h, n, x, norm, tmp);
 }
 
 tq_status tq_scale_f64(tq_handle_t h, int64_t n, double *x, double alpha) {

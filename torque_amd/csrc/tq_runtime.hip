@@ -46,6 +46,11 @@ tq_status tq_create(tq_handle_t *handle, int device) {
   if (hipGetDeviceProperties(&prop, device) == hipSuccess)
     h->n_cu = prop.multiProcessorCount;
   h->stream = nullptr;
+  const tq_status st = tq_reduce_setup_device(h);
+  if (st != TQ_SUCCESS) {
+    delete h;
+    return st;
+  }
   *handle = h;
   return TQ_SUCCESS;
 }
