@@ -226,7 +226,9 @@ tq_status tq_plan_set_small_k(tq_plan_t plan, int32_t max_k);
  * per call, src/gpu/block_sparse.cuh:836-863). Packs A and B once, then, for
  * each tile class, times `reps` grouped-GEMM launches of every tile
  * configuration with HIP events on the handle's stream and keeps the fastest
- * for that class. times_ms (host, 2 * tq_gemm_config_count(precision)
+ * for that class. A split-BF16 plan in the automatic operand form (see
+ * tq_plan_set_split_operands) is swept in both forms and keeps the form whose
+ * whole execute (pack and multiply) is faster. times_ms (host, 2 * tq_gemm_config_count(precision)
  * entries, class 0 first, may be NULL) receives the mean time of each
  * (class, configuration), 0 for a class without tiles and -1 for a
  * configuration the plan cannot run (wave-tile configurations need leading
@@ -235,6 +237,18 @@ tq_status tq_plan_set_small_k(tq_plan_t plan, int32_t max_k);
 tq_status tq_plan_autotune(tq_plan_t plan, const void *a, const void *b,
                            void *c, int32_t reps, int32_t *best,
                            float *times_ms);
+
+/* Operand form of a split-BF16 plan whose operands are both plain matrices
+ * (m or k fastest with one leading dimension): it can read them in place,
+ * each FP64 value split in registers as the GEMM stages it (no pack pass,
+ * but a value is split once per tile that reads it), or pack them once into
+ * split panels. form: 0 = automatic (in place until tq_plan_autotune times a
+ * configuration sweep in both forms and keeps the faster whole execute),
+ * 1 = split panels, 2 = in place (TQ_ERROR_UNSUPPORTED when an operand is
+ * not plain). Both forms give the same bits. tq_plan_info's a_layout and
+ * b_layout read 2 for panels. Releases a captured graph. No reference
+ * counterpart. */
+tq_status tq_plan_set_split_operands(tq_plan_t plan, int32_t form);
 
 /* ---- batches and graphs (no reference counterpart) ------------------- */
 

@@ -456,3 +456,85 @@ def test_runs_match_pair_tiles(cuda, permuted):
         if first is None:
             first = got
         assert np.array_equal(got, first), cfg
+
+
+def restride(bs, data, strides):
+    """The same logical blocks stored under other index tables."""
+    out = np.empty_like(data)
+    for b in range(bs.begin.shape[0]):
+        dims = bs.end[b] - bs.begin[b] + 1
+        n = int(np.prod(dims))
+        idx = np.unravel_index(np.arange(n), tuple(dims), order="F")
+        pos = sum(i * int(s) for i, s in zip(idx, strides[b]))
+        out[bs.offsets[b] + pos] = data[bs.offsets[b]:bs.offsets[b] + n]
+    return out
+
+
+def test_bf16x3_in_place_equals_split_panels(cuda):
+    # plain operands are read in place by the split-BF16 GEMM and split on the
+    # way to LDS (no pack pass); the same logical tensors stored in random leg
+    # orders go through the split pack and panels. Both stage the same BF16
+    # pieces and add the products in one order: the same bits in every
+    # configuration, with small-K pairs (streamed) and runs of pairs included.
+    rng = np.random.default_rng(58)
+    a, b, legs = W.batched(1, seed=29, dim_lo=8, dim_hi=80)[0]
+    A, da = from_workload(a)
+    B, db = from_workload(b)
+    ref_c, ref = O.contract(A, da, B, db, legs)
+    sa = permuted_strides(a.begin, a.end, rng)
+    sb = permuted_strides(b.begin, b.end, rng)
+    Ap = O.BS(a.dims, a.begin, a.end, sa)
+    Bp = O.BS(b.dims, b.begin, b.end, sb)
+    dap, dbp = restride(A, da, sa), restride(B, db, sb)
+    p = torch_tensor(A, da, cuda).plan_contract(torch_tensor(B, db, cuda), legs, "bf16x3")
+    assert (p.a_layout, p.b_layout) != (2, 2)  # read in place
+    assert p.pairs()["k"].min() <= 8 < p.pairs()["k"].max()
+    for cfg in range(len(gemm_configs("bf16x3"))):
+        plan, direct = gpu_contract(A, da, B, db, legs, cuda, precision="bf16x3", cfg=cfg)
+        check_maps(plan, ref)
+        assert rel_fro(direct, ref_c) < 1e-6, cfg
+        plan_p, packed = gpu_contract(Ap, dap, Bp, dbp, legs, cuda, precision="bf16x3", cfg=cfg)
+        assert plan_p.a_layout == 2 and plan_p.b_layout == 2
+        assert np.array_equal(direct, packed), cfg
+
+
+def test_split_operand_forms(cuda):
+    # a split-BF16 plan with plain operands reads them in place or packs them
+    # into split panels, switchable in either direction with the same bits;
+    # the autotuner keeps one of the two forms; a plan whose operands are not
+    # plain refuses the in-place form
+    a, b, legs = W.batched(1, seed=31, dim_lo=16, dim_hi=90)[0]
+    A, da = from_workload(a)
+    B, db = from_workload(b)
+    ref_c, _ = O.contract(A, da, B, db, legs)
+    ga, gb = torch_tensor(A, da, cuda), torch_tensor(B, db, cuda)
+    plan = ga.plan_contract(gb, legs, "bf16x3")
+    out = torch.empty(plan.out_n_elem, dtype=torch.float64, device=cuda)
+    outs = {}
+    for form in ("in_place", "panels", "in_place", "panels"):
+        plan.set_split_operands(form)
+        assert (plan.a_layout == 2) == (form == "panels")
+        out.fill_(float("nan"))
+        plan.execute(ga.data, gb.data, out)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        assert rel_fro(got, ref_c) < 1e-6, form
+        if form in outs:
+            assert np.array_equal(got, outs[form])
+        outs[form] = got
+    assert np.array_equal(outs["in_place"], outs["panels"])
+    plan.set_split_operands("auto")
+    plan.autotune(ga.data, gb.data, out, reps=1)
+    assert plan.a_layout in (0, 1, 2)
+    plan.execute(ga.data, gb.data, out)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), outs["in_place"])
+    rng = np.random.default_rng(4)
+    Ap = O.BS(a.dims, a.begin, a.end, permuted_strides(a.begin, a.end, rng))
+    pp = torch_tensor(Ap, restride(A, da, Ap.strides), cuda).plan_contract(gb, legs, "bf16x3")
+    assert pp.a_layout == 2
+    with pytest.raises(TorqueError, match="plain"):
+        pp.set_split_operands("in_place")
+    pp.set_split_operands("panels")
+    with pytest.raises(TorqueError, match="split-BF16"):
+        ga.plan_contract(gb, legs).set_split_operands("panels")

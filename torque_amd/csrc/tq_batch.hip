@@ -107,6 +107,10 @@ tq_status tq_batch_create(tq_handle_t h, int32_t n, const tq_plan_t *plans,
       return tq_fail(h, TQ_ERROR_INVALID_VALUE,
                      "batch: plans must share dtype, precision and small-K threshold, "
                      "and have a tensor (not scalar) result");
+    if (plans[i]->split_direct != plans[0]->split_direct)
+      return tq_fail(h, TQ_ERROR_INVALID_VALUE,
+                     "batch: split-BF16 plans must all read their operands in place "
+                     "(plain matrices) or all from split panels");
   }
   std::unique_ptr<tq_plan_s> pl(new tq_plan_s());
   pl->h = h;
@@ -115,6 +119,7 @@ tq_status tq_batch_create(tq_handle_t h, int32_t n, const tq_plan_t *plans,
   pl->prec = plans[0]->prec;
   pl->ro = plans[0]->ro;
   pl->stream_max_k = plans[0]->stream_max_k;
+  pl->split_direct = plans[0]->split_direct;
   std::vector<int64_t> pa_base, pb_base;
   TQ_CHECK(merge_side(h, *pl, false, n, plans, a_offset, pa_base));
   TQ_CHECK(merge_side(h, *pl, true, n, plans, b_offset, pb_base));

@@ -121,11 +121,14 @@ tq_status tq_launch_split_jobs(tq_handle_t h, const CopyJob *jobs,
                                const int32_t *job_wg_start, int64_t n_jobs,
                                int64_t total_wg, const double *src,
                                uint16_t *dst, int64_t plane);
-tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, const GemmDesc *descs,
+// Split-BF16 grouped GEMM. direct = false: a and b are the split panels
+// (planes a_plane / b_plane elements apart); direct = true: a and b are the
+// FP64 operands read in place (the descriptors' offsets, leading dimensions
+// and k-fast flags address them) and split on the way to LDS.
+tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, bool direct, const GemmDesc *descs,
                                 const RunDesc *runs, const int32_t *tile_pair,
-                                const int32_t *tile_first, int64_t n_tiles,
-                                const uint16_t *a_split, const uint16_t *b_split,
-                                int64_t a_plane, int64_t b_plane, double *c);
+                                const int32_t *tile_first, int64_t n_tiles, const void *a,
+                                const void *b, int64_t a_plane, int64_t b_plane, double *c);
 
 // Runs a list of gather jobs that already sit in device memory. job_wg_start
 // is the exclusive prefix sum of workgroups per job (n_jobs + 1 entries).
@@ -197,12 +200,14 @@ constexpr int kStreamRows = 64;
 constexpr int kStreamCols = 64;
 
 // For the split-BF16 tier the operands are the split panels (a_packed /
-// b_packed, planes a_plane / b_plane elements apart).
+// b_packed, planes a_plane / b_plane elements apart), or with split_direct
+// the FP64 operands in place, each value rounded to its split (hi + mid + lo).
 tq_status tq_launch_small_k(tq_handle_t h, tq_dtype dt, tq_precision prec,
-                            const GemmDesc *descs, const int32_t *chunk_pair,
-                            const int32_t *chunk_first, int64_t n_chunks, const void *a,
-                            const void *a_packed, const void *b, const void *b_packed,
-                            void *c, int64_t a_plane, int64_t b_plane);
+                            bool split_direct, const GemmDesc *descs,
+                            const int32_t *chunk_pair, const int32_t *chunk_first,
+                            int64_t n_chunks, const void *a, const void *a_packed,
+                            const void *b, const void *b_packed, void *c, int64_t a_plane,
+                            int64_t b_plane);
 
 // The wave-tile kernel addresses operands with 32-bit byte offsets from a
 // per-tile base; it runs only when every leading dimension is below this.

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <vector>
 
 #include "tq_common.h"
 #include "tq_plan.h"
@@ -93,12 +94,13 @@ tq_status launch_class(tq_plan_t pl, int cls, const void *a, const void *b, void
   const TileClass &tc = pl->cls[cls];
   if (tc.n_tiles == 0) return TQ_SUCCESS;
   if (pl->prec == TQ_PREC_BF16X3)
-    return tq_launch_bf16x3_gemm(h, tc.tiles_cfg, pl->desc.as<GemmDesc>(),
+    return tq_launch_bf16x3_gemm(h, tc.tiles_cfg, pl->split_direct, pl->desc.as<GemmDesc>(),
                                  tc.runs ? pl->run_desc.as<RunDesc>() : nullptr,
                                  tc.tile_pair.as<int32_t>(), tc.tile_first.as<int32_t>(),
-                                 tc.n_tiles, pl->pack_a.panel.as<uint16_t>(),
-                                 pl->pack_b.panel.as<uint16_t>(), pl->pack_a.panel_elems,
-                                 pl->pack_b.panel_elems, reinterpret_cast<double *>(c));
+                                 tc.n_tiles, pl->split_direct ? a : pl->pack_a.panel.p,
+                                 pl->split_direct ? b : pl->pack_b.panel.p,
+                                 pl->pack_a.panel_elems, pl->pack_b.panel_elems,
+                                 reinterpret_cast<double *>(c));
   return tq_launch_grouped_gemm(h, pl->dtype, pl->prec, tc.tiles_cfg, pl->desc.as<GemmDesc>(),
                                 tc.runs ? pl->run_desc.as<RunDesc>() : nullptr,
                                 tc.tile_pair.as<int32_t>(), tc.tile_first.as<int32_t>(),
@@ -182,7 +184,8 @@ tq_status tq_bs_contract_multiply(tq_plan_t pl, const void *a, const void *b,
   if (pl->ro > 0) {
     TQ_CHECK(plan_build_tiles(pl));
     if (pl->n_chunks > 0)
-      TQ_CHECK(tq_launch_small_k(h, pl->dtype, pl->prec, pl->stream_desc.as<GemmDesc>(),
+      TQ_CHECK(tq_launch_small_k(h, pl->dtype, pl->prec, pl->split_direct,
+                                 pl->stream_desc.as<GemmDesc>(),
                                  pl->chunk_pair.as<int32_t>(),
                                  pl->chunk_first.as<int32_t>(), pl->n_chunks, a,
                                  pl->pack_a.panel.p, b, pl->pack_b.panel.p, c,
@@ -235,6 +238,73 @@ tq_status tq_plan_set_small_k(tq_plan_t pl, int32_t max_k) {
   return TQ_SUCCESS;
 }
 
+namespace {
+
+// One configuration sweep of both tile classes in the plan's current operand
+// form: every configuration of a class is timed over `reps` launches (after a
+// warm-up) and the fastest kept. times_ms as in tq_plan_autotune.
+tq_status sweep_configs(tq_plan_t pl, const void *a, const void *b, void *c, int32_t reps,
+                        hipEvent_t e0, hipEvent_t e1, float *times_ms) {
+  tq_handle_t h = pl->h;
+  const int nc = tq_gemm_n_configs(pl->prec);
+  TQ_CHECK(tq_bs_contract_pack(pl, a, b));
+  TQ_CHECK(plan_build_tiles(pl));
+  // each class is timed alone: its launches do not depend on the other's
+  for (int k = 0; k < 2; ++k) {
+    TileClass &tc = pl->cls[k];
+    if (tc.n_tiles == 0) continue;
+    int best_cfg = tc.current();
+    float best_t = 3.4e38f;
+    for (int q = 0; q < nc; ++q) {
+      if (tq_gemm_config(pl->prec, q).wave_tile && !pl->wave_ok()) {
+        if (times_ms) times_ms[k * nc + q] = -1.f;  // not applicable
+        continue;
+      }
+      tc.cfg = q;
+      TQ_CHECK(plan_build_tiles(pl));
+      TQ_CHECK(launch_class(pl, k, a, b, c));  // warm-up
+      TQ_HIP(h, hipEventRecord(e0, h->stream));
+      for (int r = 0; r < reps; ++r) TQ_CHECK(launch_class(pl, k, a, b, c));
+      TQ_HIP(h, hipEventRecord(e1, h->stream));
+      TQ_HIP(h, hipEventSynchronize(e1));
+      float ms = 0.f;
+      TQ_HIP(h, hipEventElapsedTime(&ms, e0, e1));
+      ms /= reps;
+      if (times_ms) times_ms[k * nc + q] = ms;
+      if (ms < best_t) {
+        best_t = ms;
+        best_cfg = q;
+      }
+    }
+    tc.cfg = best_cfg;
+  }
+  return plan_build_tiles(pl);
+}
+
+// Mean time of `reps` whole executes (pack and multiply) after a warm-up.
+tq_status time_execute(tq_plan_t pl, const void *a, const void *b, void *c, int32_t reps,
+                       hipEvent_t e0, hipEvent_t e1, float *ms) {
+  tq_handle_t h = pl->h;
+  TQ_CHECK(tq_bs_contract_execute(pl, a, b, c));
+  TQ_HIP(h, hipEventRecord(e0, h->stream));
+  for (int r = 0; r < reps; ++r) TQ_CHECK(tq_bs_contract_execute(pl, a, b, c));
+  TQ_HIP(h, hipEventRecord(e1, h->stream));
+  TQ_HIP(h, hipEventSynchronize(e1));
+  TQ_HIP(h, hipEventElapsedTime(ms, e0, e1));
+  *ms /= reps;
+  return TQ_SUCCESS;
+}
+
+struct Events {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ~Events() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+};
+
+}  // namespace
+
 tq_status tq_plan_autotune(tq_plan_t pl, const void *a, const void *b, void *c,
                            int32_t reps, int32_t *best, float *times_ms) {
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
@@ -249,48 +319,36 @@ tq_status tq_plan_autotune(tq_plan_t pl, const void *a, const void *b, void *c,
     return TQ_SUCCESS;
   }
   pl->drop_graph();
-  TQ_CHECK(tq_bs_contract_pack(pl, a, b));
-  TQ_CHECK(plan_build_tiles(pl));
-  hipEvent_t e0, e1;
-  TQ_HIP(h, hipEventCreate(&e0));
-  TQ_HIP(h, hipEventCreate(&e1));
-  tq_status st = TQ_SUCCESS;
-  // each class is timed alone: its launches do not depend on the other's
-  for (int k = 0; k < 2 && st == TQ_SUCCESS; ++k) {
-    TileClass &tc = pl->cls[k];
-    if (tc.n_tiles == 0) continue;
-    int best_cfg = tc.current();
-    float best_t = 3.4e38f;
-    for (int q = 0; q < nc && st == TQ_SUCCESS; ++q) {
-      if (tq_gemm_config(pl->prec, q).wave_tile && !pl->wave_ok()) {
-        if (times_ms) times_ms[k * nc + q] = -1.f;  // not applicable
-        continue;
-      }
-      tc.cfg = q;
-      st = plan_build_tiles(pl);
-      if (st == TQ_SUCCESS) st = launch_class(pl, k, a, b, c);  // warm-up
-      if (st != TQ_SUCCESS) break;
-      (void)hipEventRecord(e0, h->stream);
-      for (int r = 0; r < reps && st == TQ_SUCCESS; ++r) st = launch_class(pl, k, a, b, c);
-      (void)hipEventRecord(e1, h->stream);
-      (void)hipEventSynchronize(e1);
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      ms /= reps;
-      if (times_ms) times_ms[k * nc + q] = ms;
-      if (ms < best_t) {
-        best_t = ms;
-        best_cfg = q;
-      }
+  Events ev;
+  TQ_HIP(h, hipEventCreate(&ev.e0));
+  TQ_HIP(h, hipEventCreate(&ev.e1));
+  if (pl->can_split_direct && pl->split_form == 0) {
+    // split-BF16 with plain operands: sweep both operand forms and keep the
+    // one whose whole execute is faster (reading in place skips the pack
+    // pass but splits each value once per tile that reads it)
+    std::vector<float> t_in(2 * nc, 0.f), t_pan(2 * nc, 0.f);
+    float ms_in = 0.f, ms_pan = 0.f;
+    int cfg_in[2];
+    TQ_CHECK(plan_set_split_form(pl, true));
+    TQ_CHECK(sweep_configs(pl, a, b, c, reps, ev.e0, ev.e1, t_in.data()));
+    TQ_CHECK(time_execute(pl, a, b, c, reps, ev.e0, ev.e1, &ms_in));
+    for (int k = 0; k < 2; ++k) cfg_in[k] = pl->cls[k].cfg;
+    TQ_CHECK(plan_set_split_form(pl, false));
+    TQ_CHECK(sweep_configs(pl, a, b, c, reps, ev.e0, ev.e1, t_pan.data()));
+    TQ_CHECK(time_execute(pl, a, b, c, reps, ev.e0, ev.e1, &ms_pan));
+    const bool in_place = ms_in <= ms_pan;
+    if (in_place) {
+      TQ_CHECK(plan_set_split_form(pl, true));
+      for (int k = 0; k < 2; ++k) pl->cls[k].cfg = cfg_in[k];
+      TQ_CHECK(plan_build_tiles(pl));
     }
-    tc.cfg = best_cfg;
+    if (times_ms) std::copy_n((in_place ? t_in : t_pan).data(), 2 * nc, times_ms);
+  } else {
+    TQ_CHECK(sweep_configs(pl, a, b, c, reps, ev.e0, ev.e1, times_ms));
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (st != TQ_SUCCESS) return st;
   if (best)
     for (int k = 0; k < 2; ++k) best[k] = pl->cls[k].current();
-  return plan_build_tiles(pl);
+  return TQ_SUCCESS;
 }
 
 tq_status tq_bs_contract_execute(tq_plan_t pl, const void *a, const void *b,

@@ -1,59 +1,88 @@
 // tq_gemm_bf16.hip -- the split-BF16 tier of the grouped GEMM.
 //
-// Each FP64 operand value x is stored as three BF16 pieces hi + mid + lo in
-// three planes written by the split pack (tq_permute.hip). A panel of R rows
-// is k-chunked: value (row, k) of a plane sits at (k / 8) * R * 8 + row * 8 +
-// k % 8, so the 8 values of one row in one chunk are one 16-byte vector and
-// consecutive rows are consecutive vectors. The product keeps the six piece
-// products of order >= 2^-16 relative,
+// Each FP64 operand value x is taken as three BF16 pieces hi + mid + lo
+// (tq_split.h). The product keeps the six piece products of order >= 2^-16
+// relative,
 //   hi*lo + mid*mid + hi*mid + lo*hi + mid*hi + hi*hi   (A piece * B piece,
 // in this order per k step: grouped by B plane, smaller products first),
 // on v_mfma_f32_16x16x32_bf16 with FP32 accumulation; the dropped terms
 // (mid*lo, lo*mid, lo*lo) are below 2^-23 of |a||b| and the pieces represent
-// each operand to 2^-24 relative. Results are written as FP64.
+// each operand to 2^-26 relative. Results are written as FP64.
 //
-// Schedule: persistent workgroups of four wavefronts walk the same tile list
-// as the FP64 kernel (tq_gemm.hip). A k step covers 32 values (four chunks):
-// the workgroup stages the step's A and B vectors of all three planes in LDS
+// Operands come in one of two forms, fixed per plan:
+//  * packed: three BF16 planes written by the split pack (tq_permute.hip). A
+//    panel of R rows is k-chunked: value (row, k) of a plane sits at
+//    (k / 8) * R * 8 + row * 8 + k % 8, so the 8 values of one row in one
+//    chunk are one 16-byte vector and consecutive rows are consecutive
+//    vectors.
+//  * in place: the FP64 operand as the contraction's data holds it (a plain
+//    matrix, m or k fastest). The loaders read FP64 and the split happens in
+//    registers as the step is written to LDS, so no pack pass runs at all.
+// Both forms stage the same BF16 vectors in LDS and multiply them in the same
+// order: a contraction gives the same bits either way.
+//
+// Schedule: persistent workgroups walk the same tile list as the FP64 kernel
+// (tq_gemm.hip). A k step covers 32 values (four chunks): the workgroup
+// stages the step's A and B vectors of all three planes in LDS
 // ([plane][chunk][row] vectors, so the 16 lanes reading one MFMA operand read
-// 256 contiguous bytes), and the vectors of the next step (or of the next
+// 256 contiguous bytes), and the values of the next step (or of the next
 // tile's first step) are loaded into registers while the current step's
 // MFMAs run. The six piece products are six passes over the wavefront's
 // register tile, so consecutive MFMAs never depend on each other; the three
 // A planes stay in registers and the B planes are read one at a time.
+#include <type_traits>
+
 #include "tq_gemm_common.h"
+#include "tq_split.h"
 
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 using tqg::kThreads;
 constexpr int kBK = 32;     // k per step
 constexpr int kChunks = 4;  // 8-value chunks per step
 
-// One operand's vectors of one k step: 3 planes x 4 chunks x ROWS rows of
-// 16 bytes; vector v of the step is (row v % ROWS, chunk (v / ROWS) % 4,
-// plane v / (4 ROWS)), so consecutive threads read consecutive rows.
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+// One operand as a thread's loader reads it: p is the operand's first
+// element (plane 0 of a split panel, or the FP64 value in place); the tile
+// starts at row row0 of its `rows` rows (rows of A, or output columns for
+// B); ld is the panel's padded K (packed) or the leading dimension (in
+// place); kfast says whether k is the fast index in place.
+struct OpRef {
+  const void *p;
+  int64_t ld;
+  int rows, row0;
+  int kfast;
+};
 
-template <int ROWS, int THREADS = kThreads>
-struct StepLoader {
+template <int ROWS, int THREADS, bool DIRECT>
+struct StepLoader;
+
+// Packed operand: one k step is 3 planes x 4 chunks x ROWS rows of 16-byte
+// vectors; vector v of the step is (row v % ROWS, chunk (v / ROWS) % 4,
+// plane v / (4 ROWS)), so consecutive threads read consecutive rows.
+template <int ROWS, int THREADS>
+struct StepLoader<ROWS, THREADS, false> {
   static constexpr int kVec = 3 * kChunks * ROWS;
   static constexpr int kN = (kVec + THREADS - 1) / THREADS;
   static_assert(kN <= 32, "one mask bit per vector");
   u32x4 v[kN];
   uint32_t ok;  // bit q: v[q] is in range (otherwise it is stored as zeros)
 
-  // base: plane 0 of the panel; rows R of the panel; rows past R are read
-  // from row R - 1 (their products only reach unstored output); chunks past
-  // the padded K are zeros. Branch-free and without waiting on the loads:
-  // out-of-range vectors load a clamped in-range vector and become zeros
-  // when stored. (Native vector types: the loaded values stay in registers.)
-  __device__ inline void load(const uint16_t *__restrict__ base, int64_t plane, int rows,
-                              int row0, int n_chunks, int chunk0) {
+  // Rows past R are read from row R - 1 (their products only reach
+  // unstored output); chunks past the padded K are zeros. Branch-free and
+  // without waiting on the loads: out-of-range vectors load a clamped
+  // in-range vector and become zeros when stored. (Native vector types: the
+  // loaded values stay in registers.)
+  __device__ inline void load(const OpRef &o, int64_t plane, int /*K*/, int step) {
     ok = 0;
+    const int n_chunks = (int)(o.ld >> 3);
     if (n_chunks <= 0) return;  // no k values: the panel holds nothing to read
+    const uint16_t *base = static_cast<const uint16_t *>(o.p);
+    const int chunk0 = step * kChunks;
     const int last = n_chunks - 1;
 #pragma unroll
     for (int q = 0; q < kN; ++q) {
@@ -61,11 +90,11 @@ struct StepLoader {
       const int r = e % ROWS;
       const int c = (e / ROWS) % kChunks;
       const int p = min(e / (ROWS * kChunks), 2);
-      const int row = min(row0 + r, rows - 1);
+      const int row = min(o.row0 + r, o.rows - 1);
       const int chunk = chunk0 + c;
       ok |= (uint32_t)(chunk < n_chunks && (kVec % THREADS == 0 || e < kVec)) << q;
       v[q] = *reinterpret_cast<const u32x4 *>(
-          base + p * plane + ((int64_t)min(chunk, last) * rows + row) * 8);
+          base + p * plane + ((int64_t)min(chunk, last) * o.rows + row) * 8);
     }
   }
 
@@ -79,23 +108,104 @@ struct StepLoader {
   }
 };
 
-// A tile: the GEMM it belongs to (a pair, or a run of pairs) and its corner.
-// bp, brows, brow0 locate this thread's B row (output column) for the
-// loaders: the B vectors of the thread are read from panel bp of brows rows
-// at row brow0 + r, r = the thread's row within the tile.
+// FP64 operand in place: one k step is ROWS x 4 cells of 8 values (a row's
+// chunk); cell e is (row e % ROWS, chunk e / ROWS), so consecutive threads
+// take consecutive rows (coalesced when m is the fast index; k-fast rows are
+// read as 16-byte pairs where aligned). Rows past R read row R - 1 and k past
+// K read k = K - 1 (in bounds); the split on the way to LDS zeroes the
+// values at k >= K.
+template <int ROWS, int THREADS>
+struct StepLoader<ROWS, THREADS, true> {
+  static constexpr int kCells = kChunks * ROWS;
+  static constexpr int kN = (kCells + THREADS - 1) / THREADS;
+  double v[kN][8];
+  int kleft;  // K minus the step's first k
+
+  __device__ inline void load(const OpRef &o, int64_t /*plane*/, int K, int step) {
+    const double *base = static_cast<const double *>(o.p);
+    const int k0 = step * kBK;
+    kleft = K - k0;
+    if (kleft <= 0) return;  // stored as zeros
+#pragma unroll
+    for (int q = 0; q < kN; ++q) {
+      const int e = threadIdx.x + q * THREADS;
+      const int r = e % ROWS;
+      const int c = min(e / ROWS, kChunks - 1);  // lanes past the cells load, never store
+      const int64_t row = min(max(o.row0 + r, 0), o.rows - 1);
+      const int kb = k0 + 8 * c;
+      const bool full = kb + 8 <= K;  // the whole chunk lies below K
+      if (o.kfast) {
+        const double *p = base + row * o.ld + kb;
+        if (full && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const f64x2 t = *reinterpret_cast<const f64x2 *>(p + 2 * j);
+            v[q][2 * j] = t.x;
+            v[q][2 * j + 1] = t.y;
+          }
+        } else if (full) {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) v[q][j] = p[j];
+        } else {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) v[q][j] = base[row * o.ld + min(kb + j, K - 1)];
+        }
+      } else {
+        const double *p = base + row + o.ld * kb;
+        if (full) {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) v[q][j] = p[j * o.ld];
+        } else {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) v[q][j] = base[row + o.ld * min(kb + j, K - 1)];
+        }
+      }
+    }
+  }
+
+  __device__ inline void store(uint4 *lds) const {
+#pragma unroll
+    for (int q = 0; q < kN; ++q) {
+      const int e = threadIdx.x + q * THREADS;
+      if (kCells % THREADS != 0 && e >= kCells) continue;
+      const int r = e % ROWS, c = e / ROWS;
+      const int cnt = kleft - 8 * c;  // values of the cell with k < K
+      u32x4 w[3];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        uint32_t h, m, l;
+        tqs::split3x2(2 * j < cnt ? v[q][2 * j] : 0.0, 2 * j + 1 < cnt ? v[q][2 * j + 1] : 0.0,
+                      h, m, l);
+        w[0][j] = h;
+        w[1][j] = m;
+        w[2][j] = l;
+      }
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+        reinterpret_cast<u32x4 *>(lds)[(p * kChunks + c) * ROWS + r] = w[p];
+    }
+  }
+};
+
+// A tile: the GEMM it belongs to (a pair, or a run of pairs), its corner,
+// and this thread's view of the A and B operands for the loaders. For a run
+// the B rows of the thread (its output column) come from the run's pair
+// holding that column.
 struct TileRef {
   GemmDesc d;
   int m0, n0;
-  const uint16_t *bp;
-  int brows, brow0;
+  OpRef a, b;
 };
 
-template <int BM, int BN, int THREADS>
+template <int BM, int BN, int THREADS, bool DIRECT>
 __device__ inline TileRef fetch(int64_t t, const GemmDesc *__restrict__ descs,
                                 const RunDesc *__restrict__ runs,
                                 const int32_t *__restrict__ tile_pair,
                                 const int32_t *__restrict__ tile_first,
-                                const uint16_t *__restrict__ b) {
+                                const void *__restrict__ a, const void *__restrict__ b) {
+  using Elem = typename std::conditional<DIRECT, double, uint16_t>::type;
+  const Elem *pa = static_cast<const Elem *>(a);
+  const Elem *pb = static_cast<const Elem *>(b);
   TileRef r;
   const int u = tile_pair[t];
   RunDesc run{};
@@ -110,32 +220,28 @@ __device__ inline TileRef fetch(int64_t t, const GemmDesc *__restrict__ descs,
                          tqg::tile_group<BM, BN>(gridDim.x));
   r.m0 = tp.mt * BM;
   r.n0 = tp.nt * BN;
+  r.a = OpRef{pa + r.d.a_off, r.d.lda, r.d.m, r.m0, (r.d.flags & kAKFast) != 0};
   if (!runs) {
-    r.bp = b + r.d.b_off;
-    r.brows = r.d.n;
-    r.brow0 = r.n0;
+    r.b = OpRef{pb + r.d.b_off, r.d.ldb, r.d.n, r.n0, (r.d.flags & kBKFast) != 0};
     return r;
   }
   // the pair of the run holding this thread's column (columns past the run
-  // read its last one); the split panels of a run share K, hence the chunk
-  // count
+  // read its last one); the pairs of a run share K
   const int rt = BN >= THREADS ? (int)threadIdx.x : (int)threadIdx.x % BN;
   const int col = min(r.n0 + rt, r.d.n - 1);
-  r.bp = b;
-  r.brows = 1;
-  r.brow0 = 0;
+  r.b = OpRef{pb, 8, 1, 0, 1};
   int start = 0;
   for (int q = 0; q < run.np; ++q) {
     const GemmDesc &pd = descs[run.p0 + q];
-    if (col >= start && col < start + pd.n) {
-      r.bp = b + pd.b_off;
-      r.brows = pd.n;
-      r.brow0 = col - start - rt;
-    }
+    if (col >= start && col < start + pd.n)
+      r.b = OpRef{pb + pd.b_off, pd.ldb, pd.n, col - start - rt, (pd.flags & kBKFast) != 0};
     start += pd.n;
   }
   return r;
 }
+
+// k steps of a tile: a tile with no k values still runs one step over zeros.
+__device__ inline int tile_steps(const GemmDesc &d) { return max(1, (d.k + kBK - 1) / kBK); }
 
 // The six piece products of one k step for a wavefront's TM x TN blocks:
 // all three A planes stay in registers; B planes are read one at a time
@@ -182,11 +288,11 @@ struct Smem {
 
 // MINW: wavefronts per SIMD the register allocation must allow (2 makes
 // the 128 x 128 tile fit two workgroups per CU at the price of a few spills).
-template <int BM, int BN, int WAVES_M, int MINW>
+template <int BM, int BN, int WAVES_M, int MINW, bool DIRECT>
 __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
     const GemmDesc *__restrict__ descs, const RunDesc *__restrict__ runs,
     const int32_t *__restrict__ tile_pair, const int32_t *__restrict__ tile_first,
-    int64_t n_tiles, const uint16_t *__restrict__ a, const uint16_t *__restrict__ b,
+    int64_t n_tiles, const void *__restrict__ a, const void *__restrict__ b,
     int64_t a_plane, int64_t b_plane, double *__restrict__ c) {
   constexpr int WAVES_N = 4 / WAVES_M;
   constexpr int kT = kThreads;
@@ -204,15 +310,14 @@ __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
 
   int64_t t = tqg::xcd_virtual_id(blockIdx.x, gridDim.x);
   if (t >= n_tiles) return;
-  TileRef cur = fetch<BM, BN, kT>(t, descs, runs, tile_pair, tile_first, b);
-  StepLoader<BM> la;
-  StepLoader<BN> lb;
-  la.load(a + cur.d.a_off, a_plane, cur.d.m, cur.m0, cur.d.lda >> 3, 0);
-  lb.load(cur.bp, b_plane, cur.brows, cur.brow0, cur.d.ldb >> 3, 0);
+  TileRef cur = fetch<BM, BN, kT, DIRECT>(t, descs, runs, tile_pair, tile_first, a, b);
+  StepLoader<BM, kT, DIRECT> la;
+  StepLoader<BN, kT, DIRECT> lb;
+  la.load(cur.a, a_plane, cur.d.k, 0);
+  lb.load(cur.b, b_plane, cur.d.k, 0);
   while (true) {
     const GemmDesc &d = cur.d;
-    // panels store kpad = lda (A) / ldb (B) values per row: kpad / 8 chunks
-    const int n_chunks = d.lda >> 3, n_chunks_b = d.ldb >> 3;
+    const int steps = tile_steps(d);
     // every MFMA block runs (blocks past M or N compute on clamped rows and
     // are not stored), so the MFMA stream has no branches
     f4 acc[TM][TN];
@@ -223,20 +328,20 @@ __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
 
     const int64_t t_next = t + gridDim.x;
     TileRef nxt = cur;
-    for (int ch0 = 0; ch0 < n_chunks; ch0 += kChunks) {
+    for (int s = 0; s < steps; ++s) {
       __syncthreads();  // the previous step (or epilogue staging) is read
       la.store(As);
       lb.store(Bs);
       __syncthreads();
-      if (ch0 + kChunks < n_chunks) {
-        la.load(a + d.a_off, a_plane, d.m, cur.m0, n_chunks, ch0 + kChunks);
-        lb.load(cur.bp, b_plane, cur.brows, cur.brow0, n_chunks_b, ch0 + kChunks);
+      if (s + 1 < steps) {
+        la.load(cur.a, a_plane, d.k, s + 1);
+        lb.load(cur.b, b_plane, d.k, s + 1);
       } else if (t_next < n_tiles) {
         // last step: the next tile's first step loads under these MFMAs and
         // the epilogue
-        nxt = fetch<BM, BN, kT>(t_next, descs, runs, tile_pair, tile_first, b);
-        la.load(a + nxt.d.a_off, a_plane, nxt.d.m, nxt.m0, nxt.d.lda >> 3, 0);
-        lb.load(nxt.bp, b_plane, nxt.brows, nxt.brow0, nxt.d.ldb >> 3, 0);
+        nxt = fetch<BM, BN, kT, DIRECT>(t_next, descs, runs, tile_pair, tile_first, a, b);
+        la.load(nxt.a, a_plane, nxt.d.k, 0);
+        lb.load(nxt.b, b_plane, nxt.d.k, 0);
       }
       step_products<BM, BN, TM, TN>(As, Bs, wm0 + lrow, wn0 + lrow, lchunk, acc);
     }
@@ -247,26 +352,22 @@ __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
   }
 }
 
-// k steps of a tile: a tile with no k values still runs one step over zeros.
-__device__ inline int tile_steps(const GemmDesc &d) {
-  return max(1, ((d.lda >> 3) + kChunks - 1) / kChunks);
-}
-
 // Double-buffered variant for large tiles at one workgroup per CU: LDS holds
 // two k steps, so each step needs one barrier, and that barrier orders LDS
 // only (the registers' loads of the step after next stay in flight across
 // it). A load cursor runs one step ahead of the MFMAs through the flattened
 // sequence of (tile, k step) of this workgroup, so the last step of a tile
 // already has the next tile's first step in LDS. The epilogue stages through
-// the buffer the last step read.
-template <int BM, int BN, int WAVES_M>
-__global__ __launch_bounds__(kThreads, 1) void bf16x3_db_kernel(
+// the buffer the last step read. NW wavefronts (four, or eight for two per
+// SIMD).
+template <int BM, int BN, int WAVES_M, int NW, bool DIRECT>
+__global__ __launch_bounds__(NW * 64, 1) void bf16x3_db_kernel(
     const GemmDesc *__restrict__ descs, const RunDesc *__restrict__ runs,
     const int32_t *__restrict__ tile_pair, const int32_t *__restrict__ tile_first,
-    int64_t n_tiles, const uint16_t *__restrict__ a, const uint16_t *__restrict__ b,
+    int64_t n_tiles, const void *__restrict__ a, const void *__restrict__ b,
     int64_t a_plane, int64_t b_plane, double *__restrict__ c) {
-  constexpr int WAVES_N = 4 / WAVES_M;
-  constexpr int kT = kThreads;
+  constexpr int WAVES_N = NW / WAVES_M;
+  constexpr int kT = NW * 64;
   constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
   constexpr int TM = WM / 16, TN = WN / 16;
   constexpr int kBuf = 3 * kChunks * (BM + BN);  // 16-byte vectors per buffer
@@ -281,23 +382,23 @@ __global__ __launch_bounds__(kThreads, 1) void bf16x3_db_kernel(
 
   int64_t t = tqg::xcd_virtual_id(blockIdx.x, gridDim.x);
   if (t >= n_tiles) return;
-  TileRef cur = fetch<BM, BN, kT>(t, descs, runs, tile_pair, tile_first, b);
+  TileRef cur = fetch<BM, BN, kT, DIRECT>(t, descs, runs, tile_pair, tile_first, a, b);
 
   // load cursor: tile lt (lref), k step ls
   TileRef lref = cur;
   int64_t lt = t;
   int ls = 0;
-  StepLoader<BM> la;
-  StepLoader<BN> lb;
+  StepLoader<BM, kT, DIRECT> la;
+  StepLoader<BN, kT, DIRECT> lb;
   auto load = [&]() {
-    la.load(a + lref.d.a_off, a_plane, lref.d.m, lref.m0, lref.d.lda >> 3, ls * kChunks);
-    lb.load(lref.bp, b_plane, lref.brows, lref.brow0, lref.d.ldb >> 3, ls * kChunks);
+    la.load(lref.a, a_plane, lref.d.k, ls);
+    lb.load(lref.b, b_plane, lref.d.k, ls);
   };
   auto advance = [&]() -> bool {
     if (++ls < tile_steps(lref.d)) return true;
     lt += gridDim.x;
     if (lt >= n_tiles) return false;
-    lref = fetch<BM, BN, kT>(lt, descs, runs, tile_pair, tile_first, b);
+    lref = fetch<BM, BN, kT, DIRECT>(lt, descs, runs, tile_pair, tile_first, a, b);
     ls = 0;
     return true;
   };
@@ -331,11 +432,11 @@ __global__ __launch_bounds__(kThreads, 1) void bf16x3_db_kernel(
       buf ^= 1;
     }
     // buf now holds the next tile's first step; buf ^ 1 is free for staging
-    tqg::store_tile<double, false, BM, BN, WAVES_M, TM, TN>(
+    tqg::store_tile<double, false, BM, BN, WAVES_M, TM, TN, NW>(
         acc, c, cur.d, cur.m0, cur.n0, reinterpret_cast<double *>(smem + (buf ^ 1) * kBuf));
     t += gridDim.x;
     if (t >= n_tiles) break;
-    cur = fetch<BM, BN, kT>(t, descs, runs, tile_pair, tile_first, b);
+    cur = fetch<BM, BN, kT, DIRECT>(t, descs, runs, tile_pair, tile_first, a, b);
   }
 }
 
@@ -346,12 +447,14 @@ __global__ __launch_bounds__(kThreads, 1) void bf16x3_db_kernel(
 // of k steps over its tiles, walked two steps per loop turn (one per
 // register set); a tile ends with its epilogue inside the step that finishes
 // it. NW wavefronts: with eight, a 256 x 128 tile keeps 64 x 64 wavefront
-// tiles and two wavefronts per SIMD at one workgroup per CU.
+// tiles and two wavefronts per SIMD at one workgroup per CU. Packed operands
+// only: two sets of in-place FP64 values do not fit two wavefronts per SIMD
+// (those configurations run bf16x3_db_kernel with eight wavefronts instead).
 template <int BM, int BN, int WAVES_M, int NW>
 __global__ __launch_bounds__(NW * 64, 1) void bf16x3_db2_kernel(
     const GemmDesc *__restrict__ descs, const RunDesc *__restrict__ runs,
     const int32_t *__restrict__ tile_pair, const int32_t *__restrict__ tile_first,
-    int64_t n_tiles, const uint16_t *__restrict__ a, const uint16_t *__restrict__ b,
+    int64_t n_tiles, const void *__restrict__ a, const void *__restrict__ b,
     int64_t a_plane, int64_t b_plane, double *__restrict__ c) {
   constexpr int WAVES_N = NW / WAVES_M;
   constexpr int kT = NW * 64;
@@ -361,6 +464,8 @@ __global__ __launch_bounds__(NW * 64, 1) void bf16x3_db2_kernel(
   static_assert(TM >= 1 && TN >= 1, "tile shape");
   static_assert(sizeof(double) * 16 * (BM + 1) <= sizeof(uint4) * kBuf,
                 "epilogue staging fits one buffer");
+  using LA = StepLoader<BM, kT, false>;
+  using LB = StepLoader<BN, kT, false>;
   __shared__ uint4 smem[2 * kBuf];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -369,7 +474,7 @@ __global__ __launch_bounds__(NW * 64, 1) void bf16x3_db2_kernel(
 
   int64_t t = tqg::xcd_virtual_id(blockIdx.x, gridDim.x);
   if (t >= n_tiles) return;
-  TileRef cur = fetch<BM, BN, kT>(t, descs, runs, tile_pair, tile_first, b);
+  TileRef cur = fetch<BM, BN, kT, false>(t, descs, runs, tile_pair, tile_first, a, b);
 
   // load cursor: the next step to load is step ls of tile lt (lref)
   TileRef lref = cur;
@@ -378,20 +483,20 @@ __global__ __launch_bounds__(NW * 64, 1) void bf16x3_db2_kernel(
   bool lvalid = true;
   // loads the step at the cursor into (la, lb) and advances the cursor;
   // false once the workgroup's steps are all loaded
-  auto load_next = [&](StepLoader<BM, NW * 64> &la, StepLoader<BN, NW * 64> &lb) -> bool {
+  auto load_next = [&](LA &la, LB &lb) -> bool {
     if (!lvalid) return false;
-    la.load(a + lref.d.a_off, a_plane, lref.d.m, lref.m0, lref.d.lda >> 3, ls * kChunks);
-    lb.load(lref.bp, b_plane, lref.brows, lref.brow0, lref.d.ldb >> 3, ls * kChunks);
+    la.load(lref.a, a_plane, lref.d.k, ls);
+    lb.load(lref.b, b_plane, lref.d.k, ls);
     if (++ls >= tile_steps(lref.d)) {
       lt += gridDim.x;
       lvalid = lt < n_tiles;
-      if (lvalid) lref = fetch<BM, BN, kT>(lt, descs, runs, tile_pair, tile_first, b);
+      if (lvalid) lref = fetch<BM, BN, kT, false>(lt, descs, runs, tile_pair, tile_first, a, b);
       ls = 0;
     }
     return true;
   };
-  StepLoader<BM, NW * 64> xa, ya;
-  StepLoader<BN, NW * 64> xb, yb;
+  LA xa, ya;
+  LB xb, yb;
   bool hx = load_next(xa, xb);  // step 0
   bool hy = load_next(ya, yb);  // step 1
   xa.store(smem);
@@ -408,7 +513,7 @@ __global__ __launch_bounds__(NW * 64, 1) void bf16x3_db2_kernel(
   // one k step: the register set (la, lb) holds the step after this one; it
   // goes to the other buffer (last read before the previous barrier) and is
   // refilled with the step three ahead. False once the workgroup is done.
-  auto step = [&](StepLoader<BM, NW * 64> &la, StepLoader<BN, NW * 64> &lb, bool &h) -> bool {
+  auto step = [&](LA &la, LB &lb, bool &h) -> bool {
     if (h) {
       uint4 *An = smem + (buf ^ 1) * kBuf;
       la.store(An);
@@ -426,7 +531,7 @@ __global__ __launch_bounds__(NW * 64, 1) void bf16x3_db2_kernel(
         acc, c, cur.d, cur.m0, cur.n0, reinterpret_cast<double *>(smem + (buf ^ 1) * kBuf));
     t += gridDim.x;
     if (t >= n_tiles) return false;
-    cur = fetch<BM, BN, kT>(t, descs, runs, tile_pair, tile_first, b);
+    cur = fetch<BM, BN, kT, false>(t, descs, runs, tile_pair, tile_first, a, b);
     s = 0;
     steps = tile_steps(cur.d);
 #pragma unroll
@@ -444,6 +549,7 @@ struct Cfg {
 };
 // keep in step with the switch in tq_launch_bf16x3_gemm(); configurations 9
 // and 10 are the double-buffered kernel, 11 and 12 its two-step-ahead form
+// (eight wavefronts)
 constexpr Cfg kCfgs[] = {{64, 64, 1},   {128, 64, 1},  {64, 128, 1},  {128, 128, 1},
                          {256, 32, 1},  {32, 32, 1},   {128, 16, 1},  {256, 16, 1},
                          {128, 128, 2}, {256, 128, 1}, {128, 256, 1}, {256, 128, 2},
@@ -451,21 +557,22 @@ constexpr Cfg kCfgs[] = {{64, 64, 1},   {128, 64, 1},  {64, 128, 1},  {128, 128,
 constexpr int kNCfg = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
 typedef void (*KernelFn)(const GemmDesc *, const RunDesc *, const int32_t *, const int32_t *,
-                         int64_t, const uint16_t *, const uint16_t *, int64_t, int64_t,
-                         double *);
+                         int64_t, const void *, const void *, int64_t, int64_t, double *);
 
-template <int BM, int BN, int WAVES_M, int MINW, int DB>
+template <int BM, int BN, int WAVES_M, int MINW, int DB, bool DIRECT>
 tq_status launch(tq_handle_t h, const GemmDesc *descs, const RunDesc *runs,
                  const int32_t *tile_pair, const int32_t *tile_first, int64_t n_tiles,
-                 const uint16_t *a, const uint16_t *b, int64_t ap, int64_t bp, double *c) {
+                 const void *a, const void *b, int64_t ap, int64_t bp, double *c) {
   KernelFn kernel;
   const int threads = DB == 2 ? 512 : kThreads;
-  if constexpr (DB == 2)
+  if constexpr (DB == 2 && DIRECT)
+    kernel = bf16x3_db_kernel<BM, BN, WAVES_M, 8, true>;
+  else if constexpr (DB == 2)
     kernel = bf16x3_db2_kernel<BM, BN, WAVES_M, 8>;
   else if constexpr (DB == 1)
-    kernel = bf16x3_db_kernel<BM, BN, WAVES_M>;
+    kernel = bf16x3_db_kernel<BM, BN, WAVES_M, 4, DIRECT>;
   else
-    kernel = bf16x3_gemm_kernel<BM, BN, WAVES_M, MINW>;
+    kernel = bf16x3_gemm_kernel<BM, BN, WAVES_M, MINW, DIRECT>;
   // persistent grid: exactly the workgroups that can be resident at once
   // (occupancy from registers and LDS), a multiple of the 8 XCDs
   static int per_cu = 0;
@@ -485,26 +592,15 @@ tq_status launch(tq_handle_t h, const GemmDesc *descs, const RunDesc *runs,
   return TQ_SUCCESS;
 }
 
-}  // namespace
-
-int tq_bf16x3_n_configs() { return kNCfg; }
-
-GemmConfig tq_bf16x3_config(int cfg) {
-  if (cfg < 0 || cfg >= kNCfg) cfg = 0;
-  return GemmConfig{kCfgs[cfg].bm, kCfgs[cfg].bn, kBK, 0, kCfgs[cfg].min_waves};
-}
-
-tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, const GemmDesc *descs,
-                                const RunDesc *runs, const int32_t *tile_pair,
-                                const int32_t *tile_first, int64_t n_tiles,
-                                const uint16_t *a_split, const uint16_t *b_split,
-                                int64_t a_plane, int64_t b_plane, double *c) {
-  if (n_tiles <= 0) return TQ_SUCCESS;
+template <bool DIRECT>
+tq_status dispatch(tq_handle_t h, int cfg, const GemmDesc *descs, const RunDesc *runs,
+                   const int32_t *tile_pair, const int32_t *tile_first, int64_t n_tiles,
+                   const void *a, const void *b, int64_t a_plane, int64_t b_plane, double *c) {
   switch (cfg) {
-#define TQ_CFG(I, BM, BN, WM, MINW, DB)                                                  \
-  case I:                                                                                \
-    return launch<BM, BN, WM, MINW, DB>(h, descs, runs, tile_pair, tile_first, n_tiles,    \
-                                        a_split, b_split, a_plane, b_plane, c);
+#define TQ_CFG(I, BM, BN, WM, MINW, DB)                                                       \
+  case I:                                                                                     \
+    return launch<BM, BN, WM, MINW, DB, DIRECT>(h, descs, runs, tile_pair, tile_first, n_tiles, \
+                                                a, b, a_plane, b_plane, c);
     TQ_CFG(0, 64, 64, 2, 1, 0)
     TQ_CFG(1, 128, 64, 2, 1, 0)
     TQ_CFG(2, 64, 128, 2, 1, 0)
@@ -521,4 +617,25 @@ tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, const GemmDesc *descs,
 #undef TQ_CFG
   }
   return tq_fail(h, TQ_ERROR_INVALID_VALUE, "gemm: unknown tile configuration");
+}
+
+}  // namespace
+
+int tq_bf16x3_n_configs() { return kNCfg; }
+
+GemmConfig tq_bf16x3_config(int cfg) {
+  if (cfg < 0 || cfg >= kNCfg) cfg = 0;
+  return GemmConfig{kCfgs[cfg].bm, kCfgs[cfg].bn, kBK, 0, kCfgs[cfg].min_waves};
+}
+
+tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, bool direct, const GemmDesc *descs,
+                                const RunDesc *runs, const int32_t *tile_pair,
+                                const int32_t *tile_first, int64_t n_tiles, const void *a,
+                                const void *b, int64_t a_plane, int64_t b_plane, double *c) {
+  if (n_tiles <= 0) return TQ_SUCCESS;
+  if (direct)
+    return dispatch<true>(h, cfg, descs, runs, tile_pair, tile_first, n_tiles, a, b, a_plane,
+                          b_plane, c);
+  return dispatch<false>(h, cfg, descs, runs, tile_pair, tile_first, n_tiles, a, b, a_plane,
+                         b_plane, c);
 }

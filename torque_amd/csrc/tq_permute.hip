@@ -18,6 +18,7 @@
 #include <numeric>
 
 #include "tq_common.h"
+#include "tq_split.h"
 
 namespace {
 
@@ -434,19 +435,6 @@ __global__ void i64_to_i32_kernel(const i64 *__restrict__ in,
 
 namespace {
 
-// Three-way BF16 split of an FP64 value: hi = bf16(x), mid = bf16(x - hi),
-// lo = bf16(x - hi - mid), each rounded to nearest even from FP32. The
-// residuals are exact in FP64 (each piece is within a factor two of what it
-// approximates), so x - (hi + mid + lo) is below 2^-24 |x|.
-__device__ inline uint16_t bf16_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ inline double bf16_value(uint16_t h) {
-  return (double)__uint_as_float((uint32_t)h << 16);
-}
-
 // e / t for any 32-bit e, from m = ceil(2^32 / t): umulhi(e, m) exceeds the
 // quotient by at most one.
 struct Div32 {
@@ -460,14 +448,6 @@ struct Div32 {
   }
 };
 
-// Splits a value into its three BF16 pieces.
-__device__ inline void split3(double x, uint16_t &h, uint16_t &m, uint16_t &w) {
-  h = bf16_rne((float)x);
-  const double r1 = x - bf16_value(h);
-  m = bf16_rne((float)r1);
-  const double r2 = r1 - bf16_value(m);
-  w = bf16_rne((float)r2);
-}
 
 // Source offset of row `row` over the row legs (nk..rank-1) of a job.
 __device__ inline i64 split_row_offset(const CopyJob &job, uint32_t row) {
@@ -542,19 +522,8 @@ __device__ inline void split_kfast_tile(const CopyJob &job, int64_t rows, int64_
   for (int rg = 0; rg < kG; ++rg) {
     const int64_t row = row0 + rg * 16 + (lane >> 2);
     if (row >= rows) continue;
-    uint32_t wh = 0, wm = 0, wl = 0;
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const double x = xv[rg][e];
-      const uint16_t h = bf16_rne((float)x);
-      const double r1 = x - bf16_value(h);
-      const uint16_t m = bf16_rne((float)r1);
-      const double r2 = r1 - bf16_value(m);
-      const uint16_t w = bf16_rne((float)r2);
-      wh |= (uint32_t)h << (16 * e);
-      wm |= (uint32_t)m << (16 * e);
-      wl |= (uint32_t)w << (16 * e);
-    }
+    uint32_t wh, wm, wl;
+    tqs::split3x2(xv[rg][0], xv[rg][1], wh, wm, wl);
     const int64_t o = job.dst_off + chunk * rows * 8 + row * 8 + 2 * (lane & 3);
     *reinterpret_cast<uint32_t *>(dst + o) = wh;
     *reinterpret_cast<uint32_t *>(dst + plane + o) = wm;
@@ -651,23 +620,12 @@ __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
     double xv[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) xv[e] = ((int64_t)chunk * 8 + e < kd) ? src[offs[e]] : 0.0;
-    uint16_t h[8], m[8], w[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) split3(xv[e], h[e], m[e], w[e]);
     const int64_t o = job.dst_off + ((int64_t)chunk * rows + row) * 8;
     uint4 vh, vm, vw;
-    vh.x = h[0] | ((uint32_t)h[1] << 16);
-    vh.y = h[2] | ((uint32_t)h[3] << 16);
-    vh.z = h[4] | ((uint32_t)h[5] << 16);
-    vh.w = h[6] | ((uint32_t)h[7] << 16);
-    vm.x = m[0] | ((uint32_t)m[1] << 16);
-    vm.y = m[2] | ((uint32_t)m[3] << 16);
-    vm.z = m[4] | ((uint32_t)m[5] << 16);
-    vm.w = m[6] | ((uint32_t)m[7] << 16);
-    vw.x = w[0] | ((uint32_t)w[1] << 16);
-    vw.y = w[2] | ((uint32_t)w[3] << 16);
-    vw.z = w[4] | ((uint32_t)w[5] << 16);
-    vw.w = w[6] | ((uint32_t)w[7] << 16);
+    tqs::split3x2(xv[0], xv[1], vh.x, vm.x, vw.x);
+    tqs::split3x2(xv[2], xv[3], vh.y, vm.y, vw.y);
+    tqs::split3x2(xv[4], xv[5], vh.z, vm.z, vw.z);
+    tqs::split3x2(xv[6], xv[7], vh.w, vm.w, vw.w);
     *reinterpret_cast<uint4 *>(dst + o) = vh;
     *reinterpret_cast<uint4 *>(dst + plane + o) = vm;
     *reinterpret_cast<uint4 *>(dst + 2 * plane + o) = vw;

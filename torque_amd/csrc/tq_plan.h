@@ -61,7 +61,8 @@ struct PlanUpload {
 };
 
 struct PackSide {
-  int mode = kModeDirect;
+  int mode = kModeDirect;        // active: kModeDirect skips the pack
+  int built_mode = kModeDirect;  // what the jobs below pack
   bool split = false;  // panels hold three BF16 planes
   DevBuf panel;      // packed operand
   DevBuf panel_off;  // per block or per pair
@@ -101,6 +102,19 @@ struct tq_plan_s {
   int64_t out_n_elem = 1;
   double flops = 0.0, bytes = 0.0;
   int a_layout = 0, b_layout = 0;
+  // split-BF16 tier with both operands plain matrices in place: the GEMM
+  // splits FP64 values as it stages them, and no pack runs
+  bool split_direct = false;
+  // split-BF16 operand form: the plan may read plain operands in place or
+  // pack them into split panels (built on first use); split_form is the
+  // caller's request (0 automatic: the autotuner times both, in place until
+  // then; 1 panels; 2 in place)
+  bool can_split_direct = false;
+  int split_form = 0;
+  bool panels_built = false;
+  int64_t geom_flags = 0;
+  DevBuf desc_inplace, desc_panels;
+  std::vector<GemmDesc> h_desc_inplace, h_desc_panels;
 
   DevBuf meta;       // uploaded A/B metadata
   PlanUpload up;     // device pointers into meta
@@ -165,6 +179,10 @@ struct tq_plan_s {
 // Builds the streaming segment list and both tile classes' lists for their
 // current configurations (no work when already built).
 tq_status plan_build_tiles(tq_plan_t pl);
+// Switches a split-BF16 plan with plain operands between reading them in
+// place (in_place) and packing them into split panels; drops the work lists
+// and any captured graph when the form changes.
+tq_status plan_set_split_form(tq_plan_t pl, bool in_place);
 // Analytic configuration choice for one tile class.
 int tq_plan_pick_config(tq_plan_t pl, int cls);
 // 0, 1: MFMA tile class of a pair; -1: streaming.

@@ -707,7 +707,7 @@ static tq_status build_stream_segments(tq_plan_t pl) {
   TQ_HIP(h, hipMemcpyAsync(order.data(), pl->order.p, sizeof(i64) * np, hipMemcpyDeviceToHost,
                            h->stream));
   TQ_HIP(h, hipStreamSynchronize(h->stream));
-  const bool split = pl->prec == TQ_PREC_BF16X3;
+  const bool split = pl->prec == TQ_PREC_BF16X3 && !pl->split_direct;
   std::vector<GemmDesc> slices;
   std::vector<int32_t> seg_slice, slice_first;
   for (int64_t q = 0; q < np; ++q) {
@@ -799,6 +799,7 @@ static tq_status build_pack(tq_plan_t pl, bool is_b, int mode, bool split,
   const int64_t np = pl->n_pairs;
   PackSide &side = is_b ? pl->pack_b : pl->pack_a;
   side.mode = mode;
+  side.built_mode = mode;
   side.split = split;
   if (mode == kModeDirect) return TQ_SUCCESS;
   const int self_rank = is_b ? pl->rb : pl->ra;
@@ -914,7 +915,94 @@ static tq_status build_pack(tq_plan_t pl, bool is_b, int mode, bool split,
   return TQ_SUCCESS;
 }
 
+// Pack jobs of both operands for the geometry flags of the plan: operands
+// that are not plain matrices (or every operand, for split panels) are
+// permuted into block panels, or per-pair panels where blocks are used only
+// in part and a block panel would not serve.
+static tq_status build_operand_packs(tq_plan_t pl, int64_t flags, bool split) {
+  // split panels are read with 16-byte loads: a pair that starts inside a
+  // block panel row would be misaligned, so such pairs get their own panels
+  if (split && (flags & kNotFullA)) flags |= kNeedPairPanelA;
+  if (split && (flags & kNotFullB)) flags |= kNeedPairPanelB;
+  const bool pack_a = split || (flags & kNeedPackA);
+  const bool pack_b = split || (flags & kNeedPackB);
+  const int mode_a = !pack_a ? kModeDirect
+                     : (flags & kNeedPairPanelA) ? kModePairPanel
+                                                 : kModeBlockPanel;
+  const int mode_b = !pack_b ? kModeDirect
+                     : (flags & kNeedPairPanelB) ? kModePairPanel
+                                                 : kModeBlockPanel;
+  TQ_CHECK(build_pack(pl, false, mode_a, split, pl->up));
+  return build_pack(pl, true, mode_b, split, pl->up);
+}
+
+static void set_layouts(tq_plan_t pl) {
+  pl->a_layout = pl->pack_a.mode == kModeDirect ? 0 : 2;
+  pl->b_layout = pl->pack_b.mode == kModeDirect ? 0 : 2;
+  if (pl->n_pairs > 0 && !pl->h_desc.empty()) {
+    if (pl->pack_a.mode == kModeDirect && (pl->h_desc[0].flags & kAKFast)) pl->a_layout = 1;
+    if (pl->pack_b.mode == kModeDirect && (pl->h_desc[0].flags & kBKFast)) pl->b_layout = 1;
+  }
+}
+
+tq_status plan_set_split_form(tq_plan_t pl, bool in_place) {
+  tq_handle_t h = pl->h;
+  if (!pl->can_split_direct || in_place == pl->split_direct) return TQ_SUCCESS;
+  const int64_t np = pl->n_pairs;
+  const size_t bytes = sizeof(GemmDesc) * np;
+  pl->drop_graph();
+  if (in_place) {
+    TQ_HIP(h, hipMemcpyAsync(pl->desc.p, pl->desc_inplace.p, bytes, hipMemcpyDeviceToDevice,
+                             h->stream));
+    pl->pack_a.mode = pl->pack_b.mode = kModeDirect;
+    pl->h_desc = pl->h_desc_inplace;
+  } else if (!pl->panels_built) {
+    // desc holds the in-place descriptors the panel jobs are built from
+    TQ_CHECK(build_operand_packs(pl, pl->geom_flags, true));
+    TQ_HIP(h, pl->desc_panels.alloc(bytes));
+    TQ_HIP(h, hipMemcpyAsync(pl->desc_panels.p, pl->desc.p, bytes, hipMemcpyDeviceToDevice,
+                             h->stream));
+    pl->h_desc_panels.resize(np);
+    TQ_HIP(h, hipMemcpyAsync(pl->h_desc_panels.data(), pl->desc.p, bytes,
+                             hipMemcpyDeviceToHost, h->stream));
+    pl->panels_built = true;
+  } else {
+    TQ_HIP(h, hipMemcpyAsync(pl->desc.p, pl->desc_panels.p, bytes, hipMemcpyDeviceToDevice,
+                             h->stream));
+    pl->pack_a.mode = pl->pack_a.built_mode;
+    pl->pack_b.mode = pl->pack_b.built_mode;
+  }
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  if (!in_place) pl->h_desc = pl->h_desc_panels;
+  pl->split_direct = in_place;
+  pl->max_ld = 0;
+  for (const GemmDesc &d : pl->h_desc)
+    pl->max_ld = std::max<int64_t>(pl->max_ld, std::max(d.lda, d.ldb));
+  // runs, segments and tiles address the operands through the descriptors
+  pl->runs_built = false;
+  pl->chunks_ready = false;
+  for (int c = 0; c < 2; ++c) {
+    pl->cls[c].ready = false;
+    pl->cls[c].auto_cfg = tq_plan_pick_config(pl, c);
+  }
+  set_layouts(pl);
+  return TQ_SUCCESS;
+}
+
 extern "C" {
+
+tq_status tq_plan_set_split_operands(tq_plan_t pl, int32_t form) {
+  if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  if (form < 0 || form > 2) return TQ_ERROR_INVALID_VALUE;
+  if (pl->prec != TQ_PREC_BF16X3)
+    return tq_fail(pl->h, TQ_ERROR_UNSUPPORTED, "split operands: not a split-BF16 plan");
+  if (form == 2 && !pl->can_split_direct)
+    return tq_fail(pl->h, TQ_ERROR_UNSUPPORTED,
+                   "split operands: in place needs both operands to be plain matrices");
+  pl->split_form = form;
+  if (form == 0) return TQ_SUCCESS;
+  return plan_set_split_form(pl, form == 2);
+}
 
 tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
                               const tq_bs_desc *b, int32_t n_legs,
@@ -1170,24 +1258,22 @@ tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
 
   // ---- 4. operand layouts and pack jobs ----
   if (np > 0) {
-    // the split-BF16 tier always reads split panels
-    const bool split = pl->prec == TQ_PREC_BF16X3;
-    // split panels are read with 16-byte loads: a pair that starts inside a
-    // block panel row would be misaligned, so such pairs get their own panels
-    if (split && (flags_h & kNotFullA)) flags_h |= kNeedPairPanelA;
-    if (split && (flags_h & kNotFullB)) flags_h |= kNeedPairPanelB;
-    const int mode_a = (flags_h & kNeedPairPanelA) ? kModePairPanel
-                       : (split || (flags_h & kNeedPackA)) ? kModeBlockPanel
-                                                           : kModeDirect;
-    const int mode_b = (flags_h & kNeedPairPanelB) ? kModePairPanel
-                       : (split || (flags_h & kNeedPackB)) ? kModeBlockPanel
-                                                           : kModeDirect;
-    TQ_CHECK(build_pack(pl.get(), false, (split || (flags_h & kNeedPackA)) ? mode_a
-                                                                          : kModeDirect,
-                        split, up));
-    TQ_CHECK(build_pack(pl.get(), true, (split || (flags_h & kNeedPackB)) ? mode_b
-                                                                         : kModeDirect,
-                        split, up));
+    // The split-BF16 tier reads split panels, or, when both operands are
+    // plain matrices, may read them in place and split FP64 values in
+    // registers (no pack pass); it starts in place and plan_set_split_form
+    // builds the panels when asked.
+    pl->geom_flags = flags_h;
+    pl->can_split_direct =
+        pl->prec == TQ_PREC_BF16X3 && !(flags_h & (kNeedPackA | kNeedPackB));
+    pl->split_direct = pl->can_split_direct;
+    const bool split = pl->prec == TQ_PREC_BF16X3 && !pl->split_direct;
+    if (pl->can_split_direct) {
+      TQ_HIP(h, pl->desc_inplace.alloc(sizeof(GemmDesc) * np));
+      TQ_HIP(h, hipMemcpyAsync(pl->desc_inplace.p, pl->desc.p, sizeof(GemmDesc) * np,
+                               hipMemcpyDeviceToDevice, h->stream));
+      pl->h_desc_inplace = hd;
+    }
+    TQ_CHECK(build_operand_packs(pl.get(), flags_h, split));
 
     // packing rewrote descriptors: the largest leading dimension gates the
     // wave-tile kernel's 32-bit offsets
@@ -1202,10 +1288,7 @@ tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
     TQ_HIP(h, pl->order.alloc(sizeof(i64) * np));
     TQ_CHECK(tq_sort_by_cost(h, cost.as<i64>(), np, pl->order.as<i64>()));
   }
-  pl->a_layout = pl->pack_a.mode == kModeDirect ? 0 : 2;
-  pl->b_layout = pl->pack_b.mode == kModeDirect ? 0 : 2;
-  if (np > 0 && pl->pack_a.mode == kModeDirect && (hd[0].flags & kAKFast)) pl->a_layout = 1;
-  if (np > 0 && pl->pack_b.mode == kModeDirect && (hd[0].flags & kBKFast)) pl->b_layout = 1;
+  set_layouts(pl.get());
 
   // ---- output metadata on the host (the drop-in tensor keeps it there) ----
   pl->h_out_begin.resize((size_t)np * ro);

@@ -16,13 +16,16 @@
 // FMAs -- BF16 MFMA tiles of 32 k would leave three quarters of each step
 // empty on a pair whose cost is its output.
 #include "tq_common.h"
+#include "tq_split.h"
 
 namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 
-template <class TIn, class TC, class TOut>
+// kRound3: each operand value enters as its split-BF16 representation
+// hi + mid + lo (the split-BF16 tier reading plain operands in place).
+template <class TIn, class TC, class TOut, bool kRound3 = false>
 __global__ __launch_bounds__(kThreads) void small_k_kernel(
     const GemmDesc *__restrict__ descs, const int32_t *__restrict__ seg_pair,
     const int32_t *__restrict__ seg_first, int64_t n_segs,
@@ -46,17 +49,23 @@ __global__ __launch_bounds__(kThreads) void small_k_kernel(
     const int64_t sak = (d.flags & kAKFast) ? 1 : d.lda;
     const int64_t sbn = (d.flags & kBKFast) ? d.ldb : 1;
     const int64_t sbk = (d.flags & kBKFast) ? 1 : d.ldb;
+    auto in = [](TIn x) -> TC {
+      if constexpr (kRound3)
+        return (TC)tqs::round3((double)x);
+      else
+        return (TC)x;
+    };
     TC av[kStreamMaxK];
 #pragma unroll
     for (int k = 0; k < kStreamMaxK; ++k)
-      av[k] = (live && k < d.k) ? (TC)pa[m * sam + k * sak] : (TC)0;
+      av[k] = (live && k < d.k) ? in(pa[m * sam + k * sak]) : (TC)0;
     TOut *pc = c + d.c_off + m;
     for (int n = 0; n < d.n; ++n) {
       const TIn *rb = pb + n * sbn;
-      TC acc = av[0] * (TC)rb[0];
+      TC acc = av[0] * in(rb[0]);
 #pragma unroll
       for (int k = 1; k < kStreamMaxK; ++k)
-        if (k < d.k) acc = fma(av[k], (TC)rb[k * sbk], acc);
+        if (k < d.k) acc = fma(av[k], in(rb[k * sbk]), acc);
       if (live) pc[(int64_t)d.m * n] = (TOut)acc;
     }
   }
@@ -131,7 +140,7 @@ __global__ __launch_bounds__(kThreads) void small_k_split_kernel(
   }
 }
 
-template <class TIn, class TC, class TOut>
+template <class TIn, class TC, class TOut, bool kRound3 = false>
 tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *chunk_pair,
                  const int32_t *chunk_first, int64_t n_chunks, const void *a,
                  const void *ap, const void *b, const void *bp, void *c) {
@@ -139,13 +148,15 @@ tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *chunk_pair
   if (per_cu == 0) {
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &occ, small_k_kernel<TIn, TC, TOut>, kThreads, 0) != hipSuccess || occ < 1)
+            &occ, small_k_kernel<TIn, TC, TOut, kRound3>, kThreads, 0) != hipSuccess ||
+        occ < 1)
       occ = 1;
     per_cu = occ;
   }
   const int64_t grid = std::max<int64_t>(
       1, std::min<int64_t>(ceil_div(n_chunks, kWaves), (int64_t)h->n_cu * per_cu));
-  hipLaunchKernelGGL((small_k_kernel<TIn, TC, TOut>), dim3((unsigned)grid), dim3(kThreads),
+  hipLaunchKernelGGL((small_k_kernel<TIn, TC, TOut, kRound3>), dim3((unsigned)grid),
+                     dim3(kThreads),
                      0, h->stream, descs, chunk_pair, chunk_first, n_chunks,
                      reinterpret_cast<const TIn *>(a), reinterpret_cast<const TIn *>(ap),
                      reinterpret_cast<const TIn *>(b), reinterpret_cast<const TIn *>(bp),
@@ -157,11 +168,15 @@ tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *chunk_pair
 }  // namespace
 
 tq_status tq_launch_small_k(tq_handle_t h, tq_dtype dt, tq_precision prec,
-                            const GemmDesc *descs, const int32_t *chunk_pair,
-                            const int32_t *chunk_first, int64_t n_chunks, const void *a,
-                            const void *a_packed, const void *b, const void *b_packed,
-                            void *c, int64_t a_plane, int64_t b_plane) {
+                            bool split_direct, const GemmDesc *descs,
+                            const int32_t *chunk_pair, const int32_t *chunk_first,
+                            int64_t n_chunks, const void *a, const void *a_packed,
+                            const void *b, const void *b_packed, void *c, int64_t a_plane,
+                            int64_t b_plane) {
   if (n_chunks <= 0) return TQ_SUCCESS;
+  if (prec == TQ_PREC_BF16X3 && split_direct)
+    return launch<double, double, double, true>(h, descs, chunk_pair, chunk_first, n_chunks,
+                                                a, a_packed, b, b_packed, c);
   if (prec == TQ_PREC_BF16X3) {
     static int per_cu = 0;
     if (per_cu == 0) {

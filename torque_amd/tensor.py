@@ -426,6 +426,7 @@ class ContractionPlan:
         self.h.check(lib().tq_plan_autotune(
             self.ptr, ctypes.c_void_p(a_data.data_ptr()), ctypes.c_void_p(b_data.data_ptr()),
             ctypes.c_void_p(out.data_ptr()), int(reps), best, times))
+        self._refresh_layouts()
         return [best[0], best[1]], [[times[c * n + i] for i in range(n)] for c in range(2)]
 
     def config(self):
@@ -442,6 +443,18 @@ class ContractionPlan:
 
     def set_config(self, cfg):
         self.h.check(lib().tq_plan_set_config(self.ptr, int(cfg)))
+
+    def set_split_operands(self, form):
+        """Split-BF16 operand form: "auto", "panels" or "in_place"
+        (tq_plan_set_split_operands); refreshes the reported layouts."""
+        code = {"auto": 0, "panels": 1, "in_place": 2}[form]
+        self.h.check(lib().tq_plan_set_split_operands(self.ptr, code))
+        self._refresh_layouts()
+
+    def _refresh_layouts(self):
+        info = PlanInfo()
+        self.h.check(lib().tq_plan_get_info(self.ptr, ctypes.byref(info)))
+        self.a_layout, self.b_layout = info.a_layout, info.b_layout
 
     def execute(self, a_data, b_data, out):
         self._check_sizes(a_data, b_data, out)
