@@ -92,7 +92,8 @@ typedef struct tq_bs_desc {
 typedef struct tq_plan_info {
   int32_t out_rank;
   int32_t n_legs;
-  int64_t n_pairs;      /* matched (A block, B block) pairs == output blocks  */
+  int64_t n_pairs;      /* output blocks: matched (A block, B block) pairs, or
+                           distinct output blocks of a merged-output plan   */
   int64_t out_n_elem;   /* elements of the output data buffer (>= 1)         */
   double flops;         /* sum over pairs of 2*M*N*K                         */
   double bytes;         /* compulsory bytes: sizeof(T)*(sum nA + nB + nC)    */
@@ -101,6 +102,7 @@ typedef struct tq_plan_info {
   int32_t b_layout;     /* same for B (n-fast / k-fast / packed)             */
   int64_t n_tiles;      /* MFMA tiles of the current configuration           */
   int64_t n_segments;   /* 64 x 64 segments of the streaming small-K kernel  */
+  int64_t n_matched;    /* matched pairs (== n_pairs unless merged output)   */
 } tq_plan_info;
 
 /* ---- runtime --------------------------------------------------------- */
@@ -175,13 +177,27 @@ tq_status tq_bs_contract_plan(tq_handle_t handle, const tq_bs_desc *a,
                               const tq_bs_desc *b, int32_t n_legs,
                               const int32_t *legs, tq_dtype dtype,
                               tq_precision precision, tq_plan_t *plan);
+/* As tq_bs_contract_plan, with the output merged: one output block per
+ * distinct output block (pairs with equal begin and end points) holding the
+ * sum of its pairs' products, instead of the reference's one block per
+ * matched pair (src/tensor/block_sparse.h:548-551,
+ * src/gpu/block_sparse.cuh:972-975). Blocks are listed in the order of their
+ * first pair, column-major over [A free legs, B free legs]; the dense image
+ * equals the unmerged output's. Each output block is one GEMM over the
+ * concatenated contracted ranges of its pairs (operands packed per block).
+ * tq_plan_get_pairs still lists the matched pairs (tq_plan_info.n_matched);
+ * tq_plan_get_output and the worklist list the merged blocks. */
+tq_status tq_bs_contract_plan_merged(tq_handle_t handle, const tq_bs_desc *a,
+                                     const tq_bs_desc *b, int32_t n_legs,
+                                     const int32_t *legs, tq_dtype dtype,
+                                     tq_precision precision, tq_plan_t *plan);
 tq_status tq_plan_get_info(tq_plan_t plan, tq_plan_info *info);
 /* Output metadata, host arrays: dims[out_rank], begin/end/strides
  * [n_pairs*out_rank] (block-major), offsets[n_pairs]. Any may be NULL. */
 tq_status tq_plan_get_output(tq_plan_t plan, int64_t *dims, int64_t *begin,
                              int64_t *end, int64_t *strides,
                              int64_t *offsets);
-/* Integer maps per matched pair, host arrays of n_pairs entries (any may be
+/* Integer maps per matched pair, host arrays of n_matched entries (any may be
  * NULL): A block, B block, A sub-block element offset, B sub-block element
  * offset, GEMM shape M, N, K. These are the quantities the reference computes
  * in src/gpu/block_sparse.cuh:700-732 and src/tensor/block_sparse.h:516-525. */

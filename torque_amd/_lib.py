@@ -49,7 +49,8 @@ class PlanInfo(ctypes.Structure):
                 ("n_pairs", ctypes.c_int64), ("out_n_elem", ctypes.c_int64),
                 ("flops", ctypes.c_double), ("bytes", ctypes.c_double),
                 ("a_layout", ctypes.c_int32), ("b_layout", ctypes.c_int32),
-                ("n_tiles", ctypes.c_int64), ("n_segments", ctypes.c_int64)]
+                ("n_tiles", ctypes.c_int64), ("n_segments", ctypes.c_int64),
+                ("n_matched", ctypes.c_int64)]
 
 
 class TorqueError(RuntimeError):
@@ -84,6 +85,8 @@ def _proto(lib):
         "tq_bs_slice": (S, [H, S, ctypes.POINTER(BsDesc), c_i64_p, V, V, c_i64_p, c_i64_p]),
         "tq_bs_contract_plan": (S, [H, ctypes.POINTER(BsDesc), ctypes.POINTER(BsDesc),
                                     ctypes.c_int32, c_i32_p, S, S, ctypes.POINTER(V)]),
+        "tq_bs_contract_plan_merged": (S, [H, ctypes.POINTER(BsDesc), ctypes.POINTER(BsDesc),
+                                           ctypes.c_int32, c_i32_p, S, S, ctypes.POINTER(V)]),
         "tq_plan_get_info": (S, [V, ctypes.POINTER(PlanInfo)]),
         "tq_plan_get_output": (S, [V, c_i64_p, c_i64_p, c_i64_p, c_i64_p, c_i64_p]),
         "tq_plan_get_pairs": (S, [V, c_i64_p, c_i64_p, c_i64_p, c_i64_p, c_i64_p,

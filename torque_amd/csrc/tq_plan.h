@@ -171,6 +171,12 @@ struct tq_plan_s {
   DevBuf run_desc;
   bool runs_built = false;
 
+  // merged output: GEMM units are groups of pairs sharing an output block;
+  // n_pairs then counts groups and n_matched the matched pairs
+  bool merged = false;
+  int64_t n_matched = 0;
+  std::vector<i64> h_pair_m, h_pair_n, h_pair_k;
+
   // host copies
   std::vector<i64> h_m, h_n, h_k;
   std::vector<i64> h_out_begin, h_out_end, h_c_off;

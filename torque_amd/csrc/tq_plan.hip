@@ -21,6 +21,7 @@
 //               keys, then the tile list of the grouped GEMM.
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <memory>
 
 #include "tq_common.h"
@@ -793,8 +794,13 @@ tq_status plan_build_tiles(tq_plan_t pl) {
   return TQ_SUCCESS;
 }
 
+// pair_dst (per-pair panels only, may be null): the element offset of each
+// pair's panel and, last, the panel's total size (n_pairs + 1 entries)
+// instead of consecutive panels -- the K-concatenated group panels of a
+// merged-output plan.
 static tq_status build_pack(tq_plan_t pl, bool is_b, int mode, bool split,
-                            const PlanUpload &up) {
+                            const PlanUpload &up,
+                            const std::vector<i64> *pair_dst = nullptr) {
   tq_handle_t h = pl->h;
   const int64_t np = pl->n_pairs;
   PackSide &side = is_b ? pl->pack_b : pl->pack_a;
@@ -874,11 +880,16 @@ static tq_status build_pack(tq_plan_t pl, bool is_b, int mode, bool split,
     TQ_HIP(h, side.jobs.alloc(sizeof(CopyJob) * std::max<int64_t>(np, 1)));
     TQ_HIP(h, side.wg.alloc(sizeof(i64) * (np + 1)));
     TQ_HIP(h, side.wg32.alloc(sizeof(int32_t) * (np + 1)));
-    hipLaunchKernelGGL(pair_panel_sizes_kernel, dim3(grid_for(np)), dim3(256), 0,
-                       h->stream, np, (int32_t)is_b, (int32_t)split,
-                       pl->desc.as<GemmDesc>(), sizes.as<i64>());
-    TQ_CHECK(tq_scan_i64(h, sizes.as<i64>(), side.panel_off.as<i64>(), np,
-                         side.panel_off.as<i64>() + np, tmp.p));
+    if (pair_dst) {
+      TQ_HIP(h, hipMemcpyAsync(side.panel_off.p, pair_dst->data(), sizeof(i64) * (np + 1),
+                               hipMemcpyHostToDevice, h->stream));
+    } else {
+      hipLaunchKernelGGL(pair_panel_sizes_kernel, dim3(grid_for(np)), dim3(256), 0,
+                         h->stream, np, (int32_t)is_b, (int32_t)split,
+                         pl->desc.as<GemmDesc>(), sizes.as<i64>());
+      TQ_CHECK(tq_scan_i64(h, sizes.as<i64>(), side.panel_off.as<i64>(), np,
+                           side.panel_off.as<i64>() + np, tmp.p));
+    }
     hipLaunchKernelGGL(pair_panel_jobs_kernel, dim3(grid_for(np)), dim3(256), 0,
                        h->stream, box, pair_self, pair_other, self_str, self_off,
                        side.panel_off.as<i64>(), pl->desc.as<GemmDesc>(),
@@ -989,6 +1000,109 @@ tq_status plan_set_split_form(tq_plan_t pl, bool in_place) {
   return TQ_SUCCESS;
 }
 
+// Merged output. The reference writes one output block per matched pair
+// (src/tensor/block_sparse.h:548-551; src/gpu/block_sparse.cuh:972-975); a
+// merged plan writes one block per distinct output block instead, holding
+// the sum of the products of its pairs. Pairs with equal output begin and
+// end points form a group, listed in the order of their first pair (A block,
+// then B block). Each group is one GEMM over the concatenation of its pairs'
+// contracted ranges: both operands are packed into per-group panels in which
+// pair p's slice starts at K offset kofs_p (split-BF16 slices padded to 8
+// values, the padding zero), so the grouped GEMM and the streaming kernel run
+// unchanged with one unit per group.
+static tq_status merge_groups(tq_plan_t pl, const std::vector<GemmDesc> &hd,
+                              double operand_elems) {
+  tq_handle_t h = pl->h;
+  const int64_t np = pl->n_pairs;
+  const int ro = pl->ro;
+  const bool split = pl->prec == TQ_PREC_BF16X3;
+  std::map<std::vector<i64>, int64_t> key_group;
+  std::vector<int64_t> grp(np), first;
+  std::vector<i64> key(2 * ro);
+  for (int64_t p = 0; p < np; ++p) {
+    std::copy_n(pl->h_out_begin.begin() + p * ro, ro, key.begin());
+    std::copy_n(pl->h_out_end.begin() + p * ro, ro, key.begin() + ro);
+    auto it = key_group.emplace(key, (int64_t)first.size());
+    if (it.second) first.push_back(p);
+    grp[p] = it.first->second;
+  }
+  const int64_t ng = (int64_t)first.size();
+  std::vector<i64> kofs(np), ktot(ng, 0);
+  for (int64_t p = 0; p < np; ++p) {
+    const i64 kp = split ? (hd[p].k + 7) / 8 * 8 : hd[p].k;
+    kofs[p] = ktot[grp[p]];
+    ktot[grp[p]] += kp;
+  }
+  std::vector<i64> a_base(ng), b_base(ng);
+  i64 a_tot = 0, b_tot = 0;
+  for (int64_t g = 0; g < ng; ++g) {
+    const GemmDesc &d = hd[first[g]];
+    if (ktot[g] > 0x7fffffff || d.m * ktot[g] > 0x7fffffff || d.n * ktot[g] > 0x7fffffff)
+      return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: merged block GEMM above 2^31 elements");
+    a_base[g] = a_tot;
+    b_base[g] = b_tot;
+    a_tot += d.m * ktot[g];
+    b_tot += d.n * ktot[g];
+  }
+  std::vector<i64> dst_a(np + 1), dst_b(np + 1);
+  for (int64_t p = 0; p < np; ++p) {
+    dst_a[p] = a_base[grp[p]] + kofs[p] * hd[p].m;
+    dst_b[p] = b_base[grp[p]] + kofs[p] * hd[p].n;
+  }
+  dst_a[np] = a_tot;
+  dst_b[np] = b_tot;
+  TQ_CHECK(build_pack(pl, false, kModePairPanel, split, pl->up, &dst_a));
+  TQ_CHECK(build_pack(pl, true, kModePairPanel, split, pl->up, &dst_b));
+
+  // one GEMM unit per group; the matched pairs keep their own maps
+  pl->merged = true;
+  pl->n_matched = np;
+  pl->h_pair_m.swap(pl->h_m);
+  pl->h_pair_n.swap(pl->h_n);
+  pl->h_pair_k.swap(pl->h_k);
+  std::vector<GemmDesc> gd(ng);
+  std::vector<i64> cost(ng), ob((size_t)ng * ro), oe((size_t)ng * ro), coff(ng + 1);
+  pl->h_m.resize(ng);
+  pl->h_n.resize(ng);
+  pl->h_k.resize(ng);
+  pl->max_ld = 0;
+  i64 c = 0;
+  const int32_t kf = split ? (kAKFast | kBKFast) : 0;
+  for (int64_t g = 0; g < ng; ++g) {
+    const int64_t p0 = first[g];
+    const GemmDesc &d = hd[p0];
+    const int32_t kt = (int32_t)ktot[g];
+    gd[g] = GemmDesc{a_base[g], b_base[g], c, d.m, d.n, kt, split ? kt : d.m,
+                     split ? kt : d.n, kAPacked | kBPacked | kf};
+    std::copy_n(pl->h_out_begin.begin() + p0 * ro, ro, ob.begin() + g * ro);
+    std::copy_n(pl->h_out_end.begin() + p0 * ro, ro, oe.begin() + g * ro);
+    coff[g] = c;
+    c += (i64)d.m * d.n;
+    pl->h_m[g] = d.m;
+    pl->h_n[g] = d.n;
+    pl->h_k[g] = kt;
+    cost[g] = pair_order_key(d.m, d.n, kt);
+    pl->max_ld = std::max<int64_t>(pl->max_ld, std::max(gd[g].lda, gd[g].ldb));
+  }
+  coff[ng] = c;
+  pl->n_pairs = ng;
+  pl->h_desc = gd;
+  pl->h_out_begin.swap(ob);
+  pl->h_out_end.swap(oe);
+  pl->h_c_off.swap(coff);
+  pl->out_n_elem = c;
+  pl->bytes = (pl->dtype == TQ_F64 ? 8.0 : 4.0) * (operand_elems + (double)c);
+  TQ_HIP(h, hipMemcpyAsync(pl->desc.p, gd.data(), sizeof(GemmDesc) * ng, hipMemcpyHostToDevice,
+                           h->stream));
+  DevBuf dcost;
+  TQ_HIP(h, dcost.alloc(sizeof(i64) * ng));
+  TQ_HIP(h, hipMemcpyAsync(dcost.p, cost.data(), sizeof(i64) * ng, hipMemcpyHostToDevice,
+                           h->stream));
+  TQ_HIP(h, pl->order.alloc(sizeof(i64) * ng));
+  TQ_CHECK(tq_sort_by_cost(h, dcost.as<i64>(), ng, pl->order.as<i64>()));
+  return TQ_SUCCESS;
+}
+
 extern "C" {
 
 tq_status tq_plan_set_split_operands(tq_plan_t pl, int32_t form) {
@@ -1004,10 +1118,11 @@ tq_status tq_plan_set_split_operands(tq_plan_t pl, int32_t form) {
   return plan_set_split_form(pl, form == 2);
 }
 
-tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
-                              const tq_bs_desc *b, int32_t n_legs,
-                              const int32_t *legs, tq_dtype dtype,
-                              tq_precision precision, tq_plan_t *plan_out) {
+}  // extern "C"
+
+static tq_status plan_contract(tq_handle_t h, const tq_bs_desc *a, const tq_bs_desc *b,
+                              int32_t n_legs, const int32_t *legs, tq_dtype dtype,
+                              tq_precision precision, bool merged, tq_plan_t *plan_out) {
   if (!h) return TQ_ERROR_NOT_INITIALIZED;
   if (!plan_out) return TQ_ERROR_INVALID_VALUE;
   *plan_out = nullptr;
@@ -1241,6 +1356,10 @@ tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
   }
   pl->flops = flops;
   pl->out_n_elem = ro == 0 ? 1 : out_total;
+  pl->n_matched = np;
+  // a rank-0 result is one scalar whatever the layout
+  merged = merged && ro > 0 && np > 0;
+  double operand_elems = 0;
   {
     double na_elem = 0, nb_elem = 0;
     for (int64_t i = 0; i < na; ++i) {
@@ -1253,11 +1372,12 @@ tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
       for (int l = 0; l < rb; ++l) n *= b->end[j * rb + l] - b->begin[j * rb + l] + 1;
       nb_elem += n;
     }
+    operand_elems = na_elem + nb_elem;
     pl->bytes = (dtype == TQ_F64 ? 8.0 : 4.0) * (na_elem + nb_elem + (double)pl->out_n_elem);
   }
 
   // ---- 4. operand layouts and pack jobs ----
-  if (np > 0) {
+  if (np > 0 && !merged) {
     // The split-BF16 tier reads split panels, or, when both operands are
     // plain matrices, may read them in place and split FP64 values in
     // registers (no pack pass); it starts in place and plan_set_split_form
@@ -1303,6 +1423,10 @@ tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
   TQ_HIP(h, hipMemcpyAsync(pl->h_c_off.data(), pl->c_off.p, sizeof(i64) * (np + 1),
                            hipMemcpyDeviceToHost, h->stream));
   TQ_HIP(h, hipStreamSynchronize(h->stream));
+  if (merged) {
+    TQ_CHECK(merge_groups(pl.get(), hd, operand_elems));
+    set_layouts(pl.get());
+  }
 
   // short contractions are written as streams (the split-BF16 tier streams
   // them from its split panels)
@@ -1311,6 +1435,20 @@ tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a,
   if (np > 0 && ro > 0) TQ_CHECK(plan_build_tiles(pl.get()));
   *plan_out = pl.release();
   return TQ_SUCCESS;
+}
+
+extern "C" {
+
+tq_status tq_bs_contract_plan(tq_handle_t h, const tq_bs_desc *a, const tq_bs_desc *b,
+                              int32_t n_legs, const int32_t *legs, tq_dtype dtype,
+                              tq_precision precision, tq_plan_t *plan_out) {
+  return plan_contract(h, a, b, n_legs, legs, dtype, precision, false, plan_out);
+}
+
+tq_status tq_bs_contract_plan_merged(tq_handle_t h, const tq_bs_desc *a, const tq_bs_desc *b,
+                                     int32_t n_legs, const int32_t *legs, tq_dtype dtype,
+                                     tq_precision precision, tq_plan_t *plan_out) {
+  return plan_contract(h, a, b, n_legs, legs, dtype, precision, true, plan_out);
 }
 
 tq_status tq_plan_get_info(tq_plan_t pl, tq_plan_info *info) {
@@ -1327,6 +1465,7 @@ tq_status tq_plan_get_info(tq_plan_t pl, tq_plan_info *info) {
   if (pl->n_pairs > 0 && pl->ro > 0) TQ_CHECK(plan_build_tiles(pl));
   info->n_tiles = pl->n_tiles;
   info->n_segments = pl->n_chunks;
+  info->n_matched = pl->merged ? pl->n_matched : pl->n_pairs;
   return TQ_SUCCESS;
 }
 
@@ -1361,7 +1500,10 @@ tq_status tq_plan_get_pairs(tq_plan_t pl, int64_t *a_block, int64_t *b_block,
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
   tq_handle_t h = pl->h;
   if (pl->is_batch) return tq_fail(h, TQ_ERROR_UNSUPPORTED, "batch: ask the member plans");
-  const int64_t np = pl->n_pairs;
+  const int64_t np = pl->merged ? pl->n_matched : pl->n_pairs;
+  const std::vector<i64> &hm = pl->merged ? pl->h_pair_m : pl->h_m;
+  const std::vector<i64> &hn = pl->merged ? pl->h_pair_n : pl->h_n;
+  const std::vector<i64> &hk = pl->merged ? pl->h_pair_k : pl->h_k;
   if (np == 0) return TQ_SUCCESS;
   std::vector<int32_t> pa(np), pb(np);
   TQ_HIP(h, hipMemcpyAsync(pa.data(), pl->pair_a.p, sizeof(int32_t) * np,
@@ -1372,9 +1514,9 @@ tq_status tq_plan_get_pairs(tq_plan_t pl, int64_t *a_block, int64_t *b_block,
   for (int64_t p = 0; p < np; ++p) {
     if (a_block) a_block[p] = pa[p];
     if (b_block) b_block[p] = pb[p];
-    if (m) m[p] = pl->h_m[p];
-    if (n) n[p] = pl->h_n[p];
-    if (k) k[p] = pl->h_k[p];
+    if (m) m[p] = hm[p];
+    if (n) n[p] = hn[p];
+    if (k) k[p] = hk[p];
   }
   if (a_offset || b_offset) {
     // sub-block offsets as the reference computes them

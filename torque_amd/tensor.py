@@ -317,15 +317,17 @@ class BlockSparseTensor:
         return BlockSparseTensor(self.dimension, out, nbeg, nend, dtype=self.dtype,
                                  device=self.data.device)
 
-    def plan_contract(self, tensor, contracting_indices, precision="native", handle=None):
-        """Build (or reuse) the device plan of contract(tensor, legs)."""
+    def plan_contract(self, tensor, contracting_indices, precision="native", handle=None,
+                      merged=False):
+        """Build (or reuse) the device plan of contract(tensor, legs); merged
+        plans write one block per distinct output block (tq_bs_contract_plan_merged)."""
         legs = np.asarray(contracting_indices, dtype=np.int32).reshape(-1, 2)
         h = handle or default_handle(self.data.device.index if self.data.is_cuda else None)
         key = (self.structure_key(), tensor.structure_key(), legs.tobytes(),
-               str(self.dtype), precision)
+               str(self.dtype), precision, bool(merged))
         plan = h.plans.get(key)
         if plan is None:
-            plan = ContractionPlan(h, self, tensor, legs, precision)
+            plan = ContractionPlan(h, self, tensor, legs, precision, merged)
             if len(h.plans) > 64:
                 # drop the cache's reference only: a caller may still hold the
                 # plan, which frees itself when the last reference goes
@@ -333,14 +335,16 @@ class BlockSparseTensor:
             h.plans[key] = plan
         return plan
 
-    def contract(self, tensor, contracting_indices, precision="native", handle=None):
+    def contract(self, tensor, contracting_indices, precision="native", handle=None,
+                 merged=False):
         """BlockSparseTensor::contract (src/gpu/block_sparse.cuh:558-1026):
-        one output block per matched pair, ordered by A block then B block."""
+        one output block per matched pair, ordered by A block then B block
+        (merged=True: one block per distinct output block)."""
         if tensor.dtype != self.dtype:
             raise TorqueError(_lib.TQ_ERROR_UNSUPPORTED, "operands must share a dtype")
         _require_gpu(self.data)
         _require_gpu(tensor.data)
-        plan = self.plan_contract(tensor, contracting_indices, precision, handle)
+        plan = self.plan_contract(tensor, contracting_indices, precision, handle, merged)
         out = torch.empty(max(plan.out_n_elem, 1), dtype=self.dtype,
                           device=self.data.device)
         plan.execute(self.data, tensor.data, out)
@@ -351,14 +355,16 @@ class ContractionPlan:
     """Owner of a tq_plan_t: device matching, index maps, pack jobs and the
     grouped-GEMM work list of one contraction structure."""
 
-    def __init__(self, handle, a, b, legs, precision="native"):
+    def __init__(self, handle, a, b, legs, precision="native", merged=False):
         self.h = handle
         legs = np.ascontiguousarray(np.asarray(legs, dtype=np.int32).reshape(-1, 2))
         self.legs = legs
         self.dtype = a.dtype
         self.precision = precision
+        self.merged = bool(merged)
         p = ctypes.c_void_p()
-        handle.check(lib().tq_bs_contract_plan(
+        entry = lib().tq_bs_contract_plan_merged if merged else lib().tq_bs_contract_plan
+        handle.check(entry(
             handle.ptr, ctypes.byref(a.desc()), ctypes.byref(b.desc()),
             legs.shape[0], _i32p(legs), _DT[a.dtype], _lib.PRECISIONS[precision],
             ctypes.byref(p)))
@@ -367,6 +373,7 @@ class ContractionPlan:
         handle.check(lib().tq_plan_get_info(p, ctypes.byref(info)))
         self.out_rank = info.out_rank
         self.n_pairs = info.n_pairs
+        self.n_matched = info.n_matched
         self.out_n_elem = info.out_n_elem
         self.flops = info.flops
         self.bytes = info.bytes
@@ -403,7 +410,7 @@ class ContractionPlan:
         return info.n_tiles, info.n_segments
 
     def pairs(self):
-        n = self.n_pairs
+        n = self.n_matched
         arrs = [np.zeros(n, np.int64) for _ in range(7)]
         self.h.check(lib().tq_plan_get_pairs(self.ptr, *[_i64p(x) for x in arrs]))
         keys = ["a_block", "b_block", "a_offset", "b_offset", "m", "n", "k"]
@@ -541,6 +548,7 @@ class BatchPlan(ContractionPlan):
         info = PlanInfo()
         handle.check(lib().tq_plan_get_info(ptr, ctypes.byref(info)))
         self.n_pairs = info.n_pairs
+        self.n_matched = info.n_matched
         self.out_n_elem = info.out_n_elem
         self.flops = info.flops
         self.bytes = info.bytes
