@@ -43,6 +43,35 @@ def committed_traffic(name):
     return None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    """Threads for the all-core CPU leg: every core the process may use
+    (OMP_NUM_THREADS caps it where the host sets a share)."""
+    n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(env))) if env.isdigit() else n
+
+
+def median_time(fn, runs=5):
+    """Median wall time of `runs` calls after one warm-up call."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -93,14 +122,17 @@ def bench_batched(args):
     b_off = np.cumsum([0] + [b.data.size for _, b, _ in pairs])[:-1]
     gen_s = time.perf_counter() - t0
 
-    def build(precision):
+    def build(precision, split_form=None):
         plans = []
         for i, (a, b, legs) in enumerate(pairs):
             A = BlockSparseTensor(a.dims, a_all[a_off[i]:a_off[i] + a.data.size], a.begin,
                                   a.end, device=dev)
             B = BlockSparseTensor(b.dims, b_all[b_off[i]:b_off[i] + b.data.size], b.begin,
                                   b.end, device=dev)
-            plans.append(A.plan_contract(B, legs, precision))
+            p = A.plan_contract(B, legs, precision)
+            if split_form:
+                p.set_split_operands(split_form)
+            plans.append(p)
         c_off = np.cumsum([0] + [p.out_n_elem for p in plans])[:-1]
         batch = BatchPlan(default_handle(), plans, a_off, b_off, c_off)
         return plans, c_off, batch
@@ -162,21 +194,37 @@ def bench_batched(args):
     tiers = {"fp64": {"gflops": round(value, 1), "rel_err": err64, "max_contraction_rel_err": worst64,
                       "frac_fp64_peak": round(value / (FP64_PEAK_TFLOPS * 1e3), 4)}}
     for tier in [t for t in args.tiers.split(",") if t]:
-        try:
-            _, _, bt = build(tier)
-            ct = torch.empty_like(c)
-            bt.autotune(a_all, b_all, ct, reps=2)
-            bt.capture(a_all, b_all, ct)
-        except TorqueError as e:
-            tiers[tier] = {"error": str(e)}
-            continue
-        tdt = time_graph(bt)
-        e, w = errors(ct)
-        tiers[tier] = {"gflops": round(bt.flops / tdt / 1e9, 1), "ms": round(tdt * 1e3, 4),
-                       "rel_err": e, "max_contraction_rel_err": w,
-                       "speedup_vs_fp64": round(dt / tdt, 3)}
-        print("# %s graph replay %.3f ms, %.1f GFLOP/s, speedup %.2fx, rel err %.2e"
-              % (tier, tdt * 1e3, bt.flops / tdt / 1e9, dt / tdt, e), flush=True)
+        # split-BF16: the batch reads its plain operands in place or packs them
+        # into split panels; both forms are timed and the faster reported
+        forms = ["in_place", "panels"] if tier == "bf16x3" else [None]
+        for form in forms:
+            try:
+                _, _, bt = build(tier, form)
+                ct = torch.empty_like(c)
+                bt.autotune(a_all, b_all, ct, reps=2)
+                bt.capture(a_all, b_all, ct)
+            except TorqueError as e:
+                tiers[tier] = {"error": str(e)}
+                continue
+            tdt = time_graph(bt)
+            e, w = errors(ct)
+            entry = {"gflops": round(bt.flops / tdt / 1e9, 1), "ms": round(tdt * 1e3, 4),
+                     "rel_err": e, "max_contraction_rel_err": w,
+                     "speedup_vs_fp64": round(dt / tdt, 3)}
+            if form:
+                entry["operands"] = form
+            print("# %s%s graph replay %.3f ms, %.1f GFLOP/s, speedup %.2fx, rel err %.2e"
+                  % (tier, " (%s)" % form if form else "", tdt * 1e3, bt.flops / tdt / 1e9,
+                     dt / tdt, e), flush=True)
+            prev = tiers.get(tier)
+            if prev is None or "error" in prev or entry["ms"] < prev["ms"]:
+                if prev is not None and "ms" in prev:
+                    entry["other_form_ms"] = prev["ms"]
+                tiers[tier] = entry
+            else:
+                prev["other_form_ms"] = entry["ms"]
+            bt.destroy()
+            del ct
 
     cpu_baseline = None
     if not args.no_cpu:
@@ -283,21 +331,22 @@ def bench_permute(dev, reps=10, dims=PERMUTE_DIMS, perms=PERMUTE_PERMS):
 
 
 def bench_large(dev, steps=5, warmup=2, tiers=("fp32", "bf16x3"), oracle_max_block=1368,
-                seed=0):
-    """The corner product of a D=10, chi=256 CTMRG step in the fused-leg view
-    (enlarged corners as block-diagonal matrices over the fused (chi, D, D)
-    leg: 19 GEMMs of 4..3600, 392 GFLOP): FP64 and the tiers, each autotuned
-    by a full sweep, timed with HIP events over whole executes (pack +
-    multiply) and over the multiply alone; relative error against the oracle
-    on every block up to oracle_max_block; and the committed cost model's
-    choice against the sweep (regret, time to schedule)."""
+                seed=0, D=10, chi=256):
+    """The corner product of a CTMRG step in the fused-leg view (enlarged
+    corners as block-diagonal matrices over the fused (chi, D, D) leg; at
+    D=10, chi=256: 19 GEMMs of 4..3600, 392 GFLOP; at D=8, chi=128: 13 GEMMs
+    of 8..1568, 23.9 GFLOP): FP64 and the tiers, each autotuned by a full
+    sweep, timed with HIP events over whole executes (pack + multiply) and
+    over the multiply alone; relative error against the oracle on every
+    block up to oracle_max_block; and the committed cost model's choice
+    against the sweep (regret, time to schedule)."""
     import torch
 
     from oracle import oracle as O
     from torque_amd import BlockSparseTensor, costmodel
     from torque_amd import workloads as W
 
-    c1, c2, legs = W.enlarged_corners(10, 256, seed=seed)
+    c1, c2, legs = W.enlarged_corners(D, chi, seed=seed)
     A = BlockSparseTensor(c1.dims, c1.data, c1.begin, c1.end, device=dev)
     B = BlockSparseTensor(c2.dims, c2.data, c2.begin, c2.end, device=dev)
     stream = torch.cuda.current_stream(dev)
@@ -346,11 +395,20 @@ def bench_large(dev, steps=5, warmup=2, tiers=("fp32", "bf16x3"), oracle_max_blo
         got = np.concatenate([host[p.out_offsets[i]:p.out_offsets[i] + sizes[i] ** 2]
                               for i in sel])
         err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        worst, off = 0.0, 0
+        for i in sel:
+            n2 = int(sizes[i]) ** 2
+            r = ref[off:off + n2]
+            worst = max(worst, float(np.linalg.norm(got[off:off + n2] - r) / np.linalg.norm(r)))
+            off += n2
         name = "fp64" if prec == "native" else prec
         e = {"ms": round(ms, 4), "gflops": round(p.flops / ms / 1e6, 1),
              "gemm_ms": round(gemm_ms, 4), "gemm_gflops": round(p.flops / gemm_ms / 1e6, 1),
-             "rel_err_vs_oracle": err, "gemm_config": best,
+             "rel_err_vs_oracle": err, "max_block_rel_err": worst, "gemm_config": best,
              "autotune_sweep_s": round(sweep_s, 3)}
+        if prec == "bf16x3":
+            e["operands"] = "split panels + pack" if p.a_layout == 2 else "in place"
+            e["executed_bf16_mfma_tflops"] = round(6 * p.flops / gemm_ms / 1e9, 1)
         if prec == "native":
             fp64 = (ms, gemm_ms)
             e["frac_fp64_mfma_peak"] = round(p.flops / ms / 1e9 / FP64_PEAK_TFLOPS, 4)
@@ -380,10 +438,10 @@ def bench_large(dev, steps=5, warmup=2, tiers=("fp32", "bf16x3"), oracle_max_blo
             e["speedup_vs_fp64"] = round(fp64[0] / ms, 3)
             e["gemm_speedup_vs_fp64"] = round(fp64[1] / gemm_ms, 3)
         res[name] = e
-        print("# corner product %s: %.3f ms (%.1f GFLOP/s), multiply alone %.3f ms "
-              "(%.1f GFLOP/s), rel err %.2e, configs %s"
-              % (name, ms, p.flops / ms / 1e6, gemm_ms, p.flops / gemm_ms / 1e6, err, best),
-              flush=True)
+        print("# corner product D=%d chi=%d %s: %.3f ms (%.1f GFLOP/s), multiply alone %.3f ms "
+              "(%.1f GFLOP/s), rel err %.2e (block max %.2e), configs %s"
+              % (D, chi, name, ms, p.flops / ms / 1e6, gemm_ms, p.flops / gemm_ms / 1e6, err,
+                 worst, best), flush=True)
         p.destroy()
         del out
     if "cost_model" in res.get("fp64", {}):
@@ -393,7 +451,7 @@ def bench_large(dev, steps=5, warmup=2, tiers=("fp32", "bf16x3"), oracle_max_blo
                  res["fp64"]["cost_model"]["sweep_time_to_schedule_s"]), flush=True)
     del A, B
     torch.cuda.empty_cache()
-    return {"workload": "enlarged_corner_product_D10_chi256_fused",
+    return {"workload": "enlarged_corner_product_D%d_chi%d_fused" % (D, chi),
             "blocks": sizes.tolist(), "gflop": round(float(2 * (sizes.astype(np.float64) ** 3).sum())
                                                      / 1e9, 3),
             "tiers": res,
@@ -503,22 +561,35 @@ def main():
     TEs = O.BS(E.dims, E.begin[:s_blocks], E.end[:s_blocks])
     Ta = O.BS(ga.dimension, ga.begin_points, ga.end_points)
     aa_host = aa.cpu().numpy()
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()
     cpu_baseline = None
-    t0 = time.perf_counter()
     ref_c, ref_p = O.contract(TEs, E.data[:n_sub], Ta, aa_host, l2, threads=threads)
-    cpu_s = time.perf_counter() - t0
     sample_flops = float((2 * ref_p["m"] * ref_p["n"] * ref_p["k"]).sum())
     if not args.no_cpu:
-        # time again (warm) and report
-        t0 = time.perf_counter()
-        O.contract(TEs, E.data[:n_sub], Ta, aa_host, l2, threads=threads)
-        cpu_s = time.perf_counter() - t0
+        # the oracle (the reference's per-pair pack + GEMM + append, restated)
+        # on this host: all threads over the whole E.a, one thread over its
+        # first tenth of the E blocks; median of 5 runs after a warm-up each
+        cpu_s = median_time(lambda: O.contract(TEs, E.data[:n_sub], Ta, aa_host, l2,
+                                               threads=threads))
+        s1 = max(1, s_blocks // 10)
+        n1 = int(np.prod(E.end[:s1] - E.begin[:s1] + 1, axis=1).sum())
+        TE1 = O.BS(E.dims, E.begin[:s1], E.end[:s1])
+        _, p1s = O.contract(TE1, E.data[:n1], Ta, aa_host, l2, threads=threads)
+        f1 = float((2 * p1s["m"] * p1s["n"] * p1s["k"]).sum())
+        cpu1_s = median_time(lambda: O.contract(TE1, E.data[:n1], Ta, aa_host, l2, threads=1))
         cpu_baseline = {"value": round(sample_flops / cpu_s / 1e9, 3), "unit": "GFLOP/s",
                         "cores": threads, "kind": "port",
                         "sample": "E.a restricted to the first %d E blocks (%d pairs, %.3f "
-                                  "GFLOP), oracle per-pair pack + GEMM, %.2f s"
-                                  % (s_blocks, ref_p["m"].size, sample_flops / 1e9, cpu_s)}
+                                  "GFLOP), oracle per-pair pack + GEMM, median %.3f s of 5"
+                                  % (s_blocks, ref_p["m"].size, sample_flops / 1e9, cpu_s),
+                        "single_thread": {"value": round(f1 / cpu1_s / 1e9, 3),
+                                          "sample": "first %d E blocks (%d pairs, %.3f GFLOP), "
+                                                    "median %.3f s of 5"
+                                                    % (s1, p1s["m"].size, f1 / 1e9, cpu1_s)},
+                        "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+        print("# CPU oracle (%s, nproc %d): %d threads %.2f GFLOP/s, 1 thread %.2f GFLOP/s"
+              % (cpu_model(), os.cpu_count() or 0, threads, sample_flops / cpu_s / 1e9,
+                 f1 / cpu1_s / 1e9), flush=True)
     n_ref = ref_c.size
 
     def rel(x):
@@ -581,8 +652,74 @@ def main():
         print("# %s step %.3f ms, %.1f GFLOP/s, rel err %.2e" %
               (tier, tdt * 1e3, flops / tdt / 1e9, rel(got)), flush=True)
 
+    # merged output: E.a writes one block per distinct output block (the sum
+    # of its pairs) instead of one per matched pair; same dense result
+    pm = gE.plan_contract(ga, l2, merged=True)
+    outm = torch.empty(max(pm.out_n_elem, 1), dtype=torch.float64, device=dev)
+    bestm, _ = pm.autotune(gE.data, aa, outm)
+
+    def mstep():
+        p1.execute(gA.data, gAc.data, aa)
+        pm.execute(gE.data, aa, outm)
+
+    for _ in range(args.warmup):
+        mstep()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mstep()
+    torch.cuda.synchronize()
+    mdt = (time.perf_counter() - t0) / args.steps
+    merged = {"ms_per_step": round(mdt * 1e3, 4), "gflops": round(flops / mdt / 1e9, 1),
+              "out_blocks": int(pm.n_pairs), "matched_pairs": int(pm.n_matched),
+              "out_elements": int(pm.out_n_elem), "speedup_vs_unmerged": round(dt / mdt, 3),
+              "gemm_config": bestm}
+    print("# merged output: E.a %d blocks for %d pairs, %.1f M elements; step %.3f ms, "
+          "%.1f GFLOP/s (%.2fx the unmerged step)"
+          % (pm.n_pairs, pm.n_matched, pm.out_n_elem / 1e6, mdt * 1e3, flops / mdt / 1e9,
+             dt / mdt), flush=True)
+    del outm
+    pm.destroy()
+
+    # per-call cost of the drop-in contract(): planning included (cold, a
+    # fresh plan), a repeated call on the cached plan (warm), and the execute
+    # alone on the same plan
+    from torque_amd import default_handle
+    default_handle().plans.clear()
+    del out
+    torch.cuda.empty_cache()
+    t0 = time.perf_counter()
+    cold = gE.contract(ga, l2)
+    torch.cuda.synchronize()
+    cold_s = time.perf_counter() - t0
+    del cold
+
+    def warm_call():
+        c = gE.contract(ga, l2)
+        torch.cuda.synchronize()
+        return c
+
+    warm_s = median_time(warm_call)
+    pw = gE.plan_contract(ga, l2)
+    outw = torch.empty(max(pw.out_n_elem, 1), dtype=torch.float64, device=dev)
+
+    def exec_call():
+        pw.execute(gE.data, ga.data, outw)
+        torch.cuda.synchronize()
+
+    exec_s = median_time(exec_call)
+    per_call = {"workload": "E.a of the D=%d chi=%d step" % (args.D, args.chi),
+                "cold_contract_ms": round(cold_s * 1e3, 3),
+                "warm_contract_ms": round(warm_s * 1e3, 3),
+                "execute_ms": round(exec_s * 1e3, 3),
+                "warm_over_execute": round(warm_s / exec_s, 3)}
+    print("# contract() per call: cold %.2f ms (plan + execute), warm %.3f ms, execute %.3f ms"
+          % (cold_s * 1e3, warm_s * 1e3, exec_s * 1e3), flush=True)
+    del outw
+
     permute = None if args.no_permute else bench_permute(dev)
     large = None if args.no_large else bench_large(dev)
+    corner128 = None if args.no_large else bench_large(dev, D=8, chi=128, oracle_max_block=4096)
     batched = None
     if not args.no_batched:
         bl = bench_batched(args)
@@ -607,6 +744,9 @@ def main():
         "cpu_baseline": cpu_baseline,
         "tiers": tiers,
         "fp64_rel_err_vs_oracle": err64,
+        "merged_output": merged,
+        "per_call": per_call,
+        "enlarged_corner_D8_chi128": corner128,
         "permute_rank6_200M": permute,
         "enlarged_corner_D10_chi256": large,
         "batched_512_u1_rank4x4": batched,

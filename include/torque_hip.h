@@ -191,6 +191,19 @@ tq_status tq_bs_contract_plan_merged(tq_handle_t handle, const tq_bs_desc *a,
                                      const tq_bs_desc *b, int32_t n_legs,
                                      const int32_t *legs, tq_dtype dtype,
                                      tq_precision precision, tq_plan_t *plan);
+/* tq_bs_contract_plan (merged = 0) or tq_bs_contract_plan_merged (merged =
+ * 1) through a cache kept by the handle: a call whose inputs (dtype,
+ * precision, legs and both tensors' metadata, compared byte for byte) match
+ * a cached plan returns that plan without planning. The handle owns the plan:
+ * the caller must not destroy it; tq_destroy releases the cache, which keeps
+ * the 32 most recently used plans. No reference counterpart: the reference
+ * re-derives every map on each contract call
+ * (src/gpu/block_sparse.cuh:570-732). */
+tq_status tq_bs_contract_plan_cached(tq_handle_t handle, const tq_bs_desc *a,
+                                     const tq_bs_desc *b, int32_t n_legs,
+                                     const int32_t *legs, tq_dtype dtype,
+                                     tq_precision precision, int32_t merged,
+                                     tq_plan_t *plan);
 tq_status tq_plan_get_info(tq_plan_t plan, tq_plan_info *info);
 /* Output metadata, host arrays: dims[out_rank], begin/end/strides
  * [n_pairs*out_rank] (block-major), offsets[n_pairs]. Any may be NULL. */

@@ -310,6 +310,43 @@ static void sparse_sections(tq_handle_t handle) {
   }
 }
 
+// Not a reference section: contractions repeated through the handle's plan
+// cache give the same values, and contract_merged writes one block per
+// distinct output block with the same dense image.
+static void cache_and_merged_section(tq_handle_t handle) {
+  const std::vector<float> d1{0, 1, 2, 3}, d2{4, 5, 6, 7}, v{1, 2};
+  BST t(uvec{2, 2, 2});
+  t.append_block(d1.data(), {0, 0, 0}, {1, 1, 0}, {1, 2, 4});
+  t.append_block(d2.data(), {0, 0, 1}, {1, 1, 1}, {1, 2, 4});
+  BST vt(uvec{2});
+  vt.append_block(v.data(), {0}, {1}, {1});
+  for (int rep = 0; rep < 3; ++rep) {
+    const auto c = t.contract(handle, vt, umat{{2, 0}});
+    CHECK(c.n_blocks() == 2);
+    CHECK(c.query({1, 0}) == 11);
+    CHECK(c.query({0, 1}) == 2 * 1 + 6 * 2);
+  }
+  const auto m = t.contract_merged(handle, vt, umat{{2, 0}});
+  CHECK(m.n_blocks() == 1);
+  CHECK(m.query({0, 0}) == 8);
+  CHECK(m.query({1, 0}) == 11);
+  CHECK(m.query({0, 1}) == 14);
+  CHECK(m.query({1, 1}) == 17);
+  // a cached plan belongs to the handle: destroying it is refused
+  std::vector<int64_t> dims{2, 2, 2}, beg{0, 0, 0}, end{1, 1, 1}, str{1, 2, 4}, off{0};
+  std::vector<int64_t> vd{2}, vb{0}, ve{1}, vs{1};
+  tq_bs_desc da{3, 1, dims.data(), beg.data(), end.data(), str.data(), off.data()};
+  tq_bs_desc db{1, 1, vd.data(), vb.data(), ve.data(), vs.data(), off.data()};
+  const int32_t legs[2] = {2, 0};
+  tq_plan_t p1 = nullptr, p2 = nullptr;
+  CHECK(tq_bs_contract_plan_cached(handle, &da, &db, 1, legs, TQ_F32, TQ_PREC_NATIVE, 0, &p1) ==
+        TQ_SUCCESS);
+  CHECK(tq_bs_contract_plan_cached(handle, &da, &db, 1, legs, TQ_F32, TQ_PREC_NATIVE, 0, &p2) ==
+        TQ_SUCCESS);
+  CHECK(p1 != nullptr && p1 == p2);
+  CHECK(tq_plan_destroy(p1) == TQ_ERROR_INVALID_VALUE);
+}
+
 int main() {
   tq_handle_t handle = nullptr;
   if (tq_create(&handle, 0) != TQ_SUCCESS) {
@@ -321,6 +358,7 @@ int main() {
     dense_sections();
     sparse_sections(handle);
     slice_section();
+    cache_and_merged_section(handle);
   } catch (const std::exception &e) {
     std::printf("FAILED with exception: %s\n", e.what());
     ++g_fail;

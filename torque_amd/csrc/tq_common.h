@@ -13,6 +13,14 @@
 using i64 = int64_t;
 using u64 = uint64_t;
 
+// A plan kept by the handle for tq_bs_contract_plan_cached: the key holds
+// every input of the plan (flags, legs and both tensors' metadata).
+struct PlanCacheEntry {
+  std::string key;
+  tq_plan_t plan = nullptr;
+  uint64_t last_use = 0;
+};
+
 struct tq_handle_s {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -21,6 +29,9 @@ struct tq_handle_s {
   // pinned staging buffer for small host<->device transfers of metadata
   void *pinned = nullptr;
   size_t pinned_bytes = 0;
+  // plans of tq_bs_contract_plan_cached, released by tq_destroy
+  std::vector<PlanCacheEntry> plan_cache;
+  uint64_t plan_clock = 0;
 };
 
 // Records an error message on the handle and returns the status.

@@ -149,6 +149,7 @@ struct tq_plan_s {
   hipGraphExec_t graph_exec = nullptr;
   hipStream_t capture_stream = nullptr;
   bool is_batch = false;
+  bool cached = false;  // owned by its handle's plan cache
 
   // Releases a captured execute. Every call that may rebuild the tile or
   // segment lists (configuration setters, autotune) drops the graph first:

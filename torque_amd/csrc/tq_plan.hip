@@ -1451,6 +1451,62 @@ tq_status tq_bs_contract_plan_merged(tq_handle_t h, const tq_bs_desc *a, const t
   return plan_contract(h, a, b, n_legs, legs, dtype, precision, true, plan_out);
 }
 
+tq_status tq_bs_contract_plan_cached(tq_handle_t h, const tq_bs_desc *a, const tq_bs_desc *b,
+                                     int32_t n_legs, const int32_t *legs, tq_dtype dtype,
+                                     tq_precision precision, int32_t merged,
+                                     tq_plan_t *plan_out) {
+  if (!h) return TQ_ERROR_NOT_INITIALIZED;
+  if (!plan_out) return TQ_ERROR_INVALID_VALUE;
+  *plan_out = nullptr;
+  if (!valid_desc(a) || !valid_desc(b) || n_legs < 0 || n_legs > TQ_MAX_RANK ||
+      (n_legs > 0 && !legs))
+    return tq_fail(h, TQ_ERROR_INVALID_VALUE, "contract: invalid tensor metadata");
+  // the key: every input the plan depends on, byte for byte
+  std::string key;
+  auto put = [&key](const void *p, size_t n) {
+    key.append(reinterpret_cast<const char *>(p), n);
+  };
+  const int32_t head[4] = {(int32_t)dtype, (int32_t)precision, merged ? 1 : 0, n_legs};
+  key.reserve(64 + 8 * (size_t)(a->n_blocks * (3 * a->rank + 1) +
+                                b->n_blocks * (3 * b->rank + 1) + a->rank + b->rank));
+  put(head, sizeof(head));
+  put(legs, sizeof(int32_t) * 2 * n_legs);
+  for (const tq_bs_desc *d : {a, b}) {
+    const int64_t r = d->rank, nb = d->n_blocks;
+    put(&r, 8);
+    put(&nb, 8);
+    put(d->dims, 8 * r);
+    if (r > 0 && nb > 0) {
+      put(d->begin, 8 * r * nb);
+      put(d->end, 8 * r * nb);
+      put(d->strides, 8 * r * nb);
+      put(d->offsets, 8 * nb);
+    }
+  }
+  for (PlanCacheEntry &e : h->plan_cache)
+    if (e.key == key) {
+      e.last_use = ++h->plan_clock;
+      *plan_out = e.plan;
+      return TQ_SUCCESS;
+    }
+  tq_plan_t pl = nullptr;
+  TQ_CHECK(plan_contract(h, a, b, n_legs, legs, dtype, precision, merged != 0, &pl));
+  constexpr size_t kCachePlans = 32;
+  if (h->plan_cache.size() >= kCachePlans) {
+    auto lru = std::min_element(h->plan_cache.begin(), h->plan_cache.end(),
+                                [](const PlanCacheEntry &x, const PlanCacheEntry &y) {
+                                  return x.last_use < y.last_use;
+                                });
+    lru->plan->cached = false;
+    (void)tq_plan_destroy(lru->plan);
+    h->plan_cache.erase(lru);
+  }
+  pl->cached = true;
+  h->plan_cache.push_back(PlanCacheEntry{std::move(key), pl, ++h->plan_clock});
+  *plan_out = pl;
+  return TQ_SUCCESS;
+}
+
 tq_status tq_plan_get_info(tq_plan_t pl, tq_plan_info *info) {
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
   if (!info) return TQ_ERROR_INVALID_VALUE;
@@ -1545,6 +1601,9 @@ tq_status tq_plan_get_worklist(tq_plan_t pl, int64_t *order) {
 
 tq_status tq_plan_destroy(tq_plan_t pl) {
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  if (pl->cached)
+    return tq_fail(pl->h, TQ_ERROR_INVALID_VALUE,
+                   "plan destroy: the plan belongs to its handle's cache");
   (void)hipStreamSynchronize(pl->h->stream);
   delete pl;
   return TQ_SUCCESS;

@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "tq_common.h"
+#include "tq_plan.h"
 
 tq_status tq_fail(tq_handle_t h, tq_status s, const std::string &msg) {
   if (h) h->last_error = msg;
@@ -58,6 +59,10 @@ tq_status tq_create(tq_handle_t *handle, int device) {
 tq_status tq_destroy(tq_handle_t h) {
   if (!h) return TQ_ERROR_NOT_INITIALIZED;
   (void)hipStreamSynchronize(h->stream);
+  for (PlanCacheEntry &e : h->plan_cache) {
+    e.plan->cached = false;
+    (void)tq_plan_destroy(e.plan);
+  }
   if (h->pinned) (void)hipHostFree(h->pinned);
   delete h;
   return TQ_SUCCESS;

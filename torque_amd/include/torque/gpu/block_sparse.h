@@ -191,10 +191,28 @@ class BlockSparseTensor {
   }
 
   // src/gpu/block_sparse.cuh:558-1026 (device matching, permute-to-GEMM
-  // layout, one grouped GEMM launch; the output blocks follow the reference)
+  // layout, one grouped GEMM launch; the output blocks follow the reference).
+  // Plans come from the handle's cache, so a repeated contraction of the
+  // same block structure (a CTMRG step) skips planning.
   BlockSparseTensor contract(tq_handle_t handle, const BlockSparseTensor &tensor,
                              const umat &contracting_indices,
                              tq_precision precision = TQ_PREC_NATIVE) const {
+    return contract_impl(handle, tensor, contracting_indices, precision, false);
+  }
+
+  // As contract, with one output block per distinct output block (the sum
+  // of the pairs writing it) instead of one per matched pair
+  // (tq_bs_contract_plan_merged); the dense image is the same.
+  BlockSparseTensor contract_merged(tq_handle_t handle, const BlockSparseTensor &tensor,
+                                    const umat &contracting_indices,
+                                    tq_precision precision = TQ_PREC_NATIVE) const {
+    return contract_impl(handle, tensor, contracting_indices, precision, true);
+  }
+
+ private:
+  BlockSparseTensor contract_impl(tq_handle_t handle, const BlockSparseTensor &tensor,
+                                  const umat &contracting_indices, tq_precision precision,
+                                  bool merged) const {
     const std::size_t n_legs = contracting_indices.n_rows();
     std::vector<int32_t> legs(2 * n_legs);
     for (std::size_t k = 0; k < n_legs; ++k) {
@@ -206,11 +224,10 @@ class BlockSparseTensor {
           dimension[legs[2 * k]] != tensor.dimension[legs[2 * k + 1]])
         throw Error("The dimensions from two tensors to be contracted do not match");
     Desc da(*this), db(tensor);
-    tq_plan_t plan = nullptr;
-    check(tq_bs_contract_plan(handle, &da.d, &db.d, (int32_t)n_legs, legs.data(),
-                              dtype_of<T>(), precision, &plan),
+    tq_plan_t plan = nullptr;  // owned by the handle's cache
+    check(tq_bs_contract_plan_cached(handle, &da.d, &db.d, (int32_t)n_legs, legs.data(),
+                                     dtype_of<T>(), precision, merged ? 1 : 0, &plan),
           handle);
-    PlanGuard guard{plan};
     tq_plan_info info;
     check(tq_plan_get_info(plan, &info), handle);
     uvec new_dim(info.out_rank);
@@ -243,6 +260,7 @@ class BlockSparseTensor {
     return out;
   }
 
+ public:
   // src/gpu/block_sparse.cuh:1031-1122
   BlockSparseTensor hard_transpose(const uvec &permutation) const {
     if (permutation.n_elem() != rank)
@@ -315,13 +333,6 @@ class BlockSparseTensor {
   T *data = nullptr;
 
   BlockSparseTensor() = default;
-
-  struct PlanGuard {
-    tq_plan_t p;
-    ~PlanGuard() {
-      if (p) (void)tq_plan_destroy(p);
-    }
-  };
 
   // Block-major metadata of this tensor as the C-ABI descriptor.
   struct Desc {

@@ -43,12 +43,12 @@ def index_table(dims):
 
 def nest_sum(v, initial=0):
     """Exclusive prefix sum, util::nest_sum (src/util/space.h:76-88)."""
-    v = np.asarray(v, dtype=np.int64)
+    v = np.asarray(v, dtype=np.int64).reshape(-1)
     out = np.empty_like(v)
-    acc = initial
-    for i, x in enumerate(v):
-        out[i] = acc
-        acc += int(x)
+    if v.size:
+        out[0] = initial
+        np.cumsum(v[:-1], out=out[1:])
+        out[1:] += initial
     return out
 
 
