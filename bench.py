@@ -460,6 +460,68 @@ def bench_large(dev, steps=5, warmup=2, tiers=("fp32", "bf16x3"), oracle_max_blo
                                      % (oracle_max_block, sample_flops / 1e9, cpu_s)}}
 
 
+def bench_ctmrg(dev, D, chi, steps, warmup):
+    """One CTMRG edge update (double layer, absorption, two projector
+    contractions, all merged-output plans, then the FP64 normalisation on the
+    device) run eagerly and as one captured HIP graph; both timed over whole
+    steps."""
+    import torch
+
+    from torque_amd import BlockSparseTensor, default_handle, normalize_f64_noalloc
+    from torque_amd import workloads as W
+
+    E, A, Ac, P, Q, legs = W.ctmrg_step(D, chi)
+    t = [BlockSparseTensor(x.dims, x.data, x.begin, x.end, device=dev) for x in (E, A, Ac, P, Q)]
+    gE, gA, gAc, gP, gQ = t
+    plans, bufs = [], []
+    prev = None
+    for i, (lhs, rhs) in enumerate([(gA, gAc), (gE, None), (None, gP), (None, gQ)]):
+        a = lhs if lhs is not None else prev
+        b = rhs if rhs is not None else prev
+        p = a.plan_contract(b, legs[i], merged=True)
+        buf = torch.empty(max(p.out_n_elem, 1), dtype=torch.float64, device=dev)
+        p.autotune(a.data, b.data, buf)
+        plans.append((p, a, b))
+        bufs.append(buf)
+        prev = p.output_tensor(buf)
+    norm = torch.empty(1, dtype=torch.float64, device=dev)
+    g = bufs[-1][:plans[-1][0].out_n_elem]
+
+    def step():
+        for (p, a, b), buf in zip(plans, bufs):
+            p.execute(a.data, b.data, buf)
+        normalize_f64_noalloc(g, norm)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    eager_s = timed(step)
+    h = default_handle(dev.index)
+    with h.capture() as cap:
+        step()
+    graph_s = timed(cap.graph.launch)
+    flops = sum(p.flops for p, _, _ in plans)
+    res = {"workload": "ctmrg_edge_update_D%d_chi%d" % (D, chi),
+           "blocks_out": [int(p.n_pairs) for p, _, _ in plans],
+           "pairs": [int(p.n_matched) for p, _, _ in plans],
+           "gflop": round(flops / 1e9, 4),
+           "eager_ms": round(eager_s * 1e3, 4), "graph_ms": round(graph_s * 1e3, 4),
+           "graph_gflops": round(flops / graph_s / 1e9, 1), "norm": float(norm[0])}
+    print("# CTMRG edge update D=%d chi=%d (4 merged contractions + FP64 normalisation, %.3f "
+          "GFLOP): eager %.3f ms, one graph %.3f ms" % (D, chi, flops / 1e9, eager_s * 1e3,
+                                                       graph_s * 1e3), flush=True)
+    del bufs
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -717,6 +779,7 @@ def main():
           % (cold_s * 1e3, warm_s * 1e3, exec_s * 1e3), flush=True)
     del outw
 
+    ctmrg = bench_ctmrg(dev, args.D, args.chi, args.steps, args.warmup)
     permute = None if args.no_permute else bench_permute(dev)
     large = None if args.no_large else bench_large(dev)
     corner128 = None if args.no_large else bench_large(dev, D=8, chi=128, oracle_max_block=4096)
@@ -745,6 +808,7 @@ def main():
         "tiers": tiers,
         "fp64_rel_err_vs_oracle": err64,
         "merged_output": merged,
+        "ctmrg_step_graph": ctmrg,
         "per_call": per_call,
         "enlarged_corner_D8_chi128": corner128,
         "permute_rank6_200M": permute,

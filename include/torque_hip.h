@@ -304,6 +304,23 @@ tq_status tq_batch_create(tq_handle_t handle, int32_t n, const tq_plan_t *plans,
 tq_status tq_plan_capture(tq_plan_t plan, const void *a, const void *b, void *c);
 tq_status tq_plan_graph_launch(tq_plan_t plan);
 
+/* Records a sequence of calls into one HIP graph: between tq_capture_begin
+ * and tq_capture_end the handle's work is captured on a stream of the
+ * handle's own instead of running, and tq_capture_end returns the graph
+ * (the pointers each call received are baked in). Only calls that neither
+ * allocate nor synchronise may run in between: tq_bs_contract_execute /
+ * _pack / _multiply of plans executed at least once before (their tile
+ * lists exist), tq_scale_f64, tq_normalize_f64_noalloc; tq_set_stream is
+ * refused. A call that cannot be captured makes tq_capture_end fail.
+ * tq_graph_launch replays the graph on the handle's stream. Used for a whole
+ * CTMRG step (absorb, projector contractions, FP64 normalisation). No
+ * reference counterpart. */
+typedef struct tq_graph_s *tq_graph_t;
+tq_status tq_capture_begin(tq_handle_t handle);
+tq_status tq_capture_end(tq_handle_t handle, tq_graph_t *graph);
+tq_status tq_graph_launch(tq_graph_t graph, tq_handle_t handle);
+tq_status tq_graph_destroy(tq_graph_t graph);
+
 /* Number of grouped-GEMM tile configurations and a description of one:
  * the tile's M and N extents, its k step, wave_tile = 1 when one
  * wavefront (not a workgroup) computes a tile, and min_waves, the

@@ -9,9 +9,9 @@ from ._lib import TorqueError, lib, LIB_PATH  # noqa: F401
 from .tensor import (BatchPlan, BlockSparseTensor, ContractionPlan, DenseTensor, Handle,  # noqa: F401
                      SparseTensor,
                      default_handle, dot_f64, gemm_configs, index_table,
-                     nest_sum, normalize_f64, nrm2_f64, scale_f64)
+                     nest_sum, normalize_f64, normalize_f64_noalloc, nrm2_f64, scale_f64)
 
 __all__ = ["BatchPlan", "BlockSparseTensor", "ContractionPlan", "DenseTensor", "Handle",
            "SparseTensor",
            "TorqueError", "default_handle", "dot_f64", "gemm_configs",
-           "index_table", "nest_sum", "normalize_f64", "nrm2_f64", "scale_f64", "lib"]
+           "index_table", "nest_sum", "normalize_f64", "normalize_f64_noalloc", "nrm2_f64", "scale_f64", "lib"]

@@ -117,6 +117,10 @@ def _proto(lib):
                                 c_i64_p, ctypes.POINTER(V)]),
         "tq_plan_capture": (S, [V, V, V, V]),
         "tq_plan_graph_launch": (S, [V]),
+        "tq_capture_begin": (S, [H]),
+        "tq_capture_end": (S, [H, ctypes.POINTER(V)]),
+        "tq_graph_launch": (S, [V, H]),
+        "tq_graph_destroy": (S, [V]),
         "tq_plan_set_small_k": (S, [V, ctypes.c_int32]),
         "tq_plan_set_split_operands": (S, [V, ctypes.c_int32]),
         "tq_plan_autotune": (S, [V, V, V, V, ctypes.c_int32, c_i32_p,

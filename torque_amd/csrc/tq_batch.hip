@@ -197,6 +197,62 @@ tq_status tq_plan_capture(tq_plan_t pl, const void *a, const void *b, void *c) {
   return TQ_SUCCESS;
 }
 
+struct tq_graph_s {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+tq_status tq_capture_begin(tq_handle_t h) {
+  if (!h) return TQ_ERROR_NOT_INITIALIZED;
+  if (h->capturing) return tq_fail(h, TQ_ERROR_INVALID_VALUE, "capture: already capturing");
+  if (!h->capture_stream)
+    TQ_HIP(h, hipStreamCreateWithFlags(&h->capture_stream, hipStreamNonBlocking));
+  TQ_HIP(h, hipStreamBeginCapture(h->capture_stream, hipStreamCaptureModeThreadLocal));
+  h->saved_stream = h->stream;
+  h->stream = h->capture_stream;
+  h->capturing = true;
+  return TQ_SUCCESS;
+}
+
+tq_status tq_capture_end(tq_handle_t h, tq_graph_t *graph) {
+  if (!h) return TQ_ERROR_NOT_INITIALIZED;
+  if (!h->capturing) return tq_fail(h, TQ_ERROR_INVALID_VALUE, "capture: not capturing");
+  h->stream = h->saved_stream;
+  h->capturing = false;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->capture_stream, &g);
+  if (e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return tq_fail(h, TQ_ERROR_HIP,
+                   std::string("capture: ") + hipGetErrorString(e) +
+                       " (an entry point that allocates or synchronises ran while capturing)");
+  }
+  if (!graph) {
+    (void)hipGraphDestroy(g);
+    return TQ_ERROR_INVALID_VALUE;
+  }
+  std::unique_ptr<tq_graph_s> out(new tq_graph_s());
+  out->graph = g;
+  TQ_HIP(h, hipGraphInstantiate(&out->exec, g, nullptr, nullptr, 0));
+  *graph = out.release();
+  return TQ_SUCCESS;
+}
+
+tq_status tq_graph_launch(tq_graph_t graph, tq_handle_t h) {
+  if (!graph || !h) return TQ_ERROR_NOT_INITIALIZED;
+  if (h->capturing) return tq_fail(h, TQ_ERROR_INVALID_VALUE, "graph launch: capturing");
+  TQ_HIP(h, hipGraphLaunch(graph->exec, h->stream));
+  return TQ_SUCCESS;
+}
+
+tq_status tq_graph_destroy(tq_graph_t graph) {
+  if (!graph) return TQ_ERROR_NOT_INITIALIZED;
+  if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+  if (graph->graph) (void)hipGraphDestroy(graph->graph);
+  delete graph;
+  return TQ_SUCCESS;
+}
+
 tq_status tq_plan_graph_launch(tq_plan_t pl) {
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
   if (!pl->graph_exec)

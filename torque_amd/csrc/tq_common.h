@@ -32,6 +32,11 @@ struct tq_handle_s {
   // plans of tq_bs_contract_plan_cached, released by tq_destroy
   std::vector<PlanCacheEntry> plan_cache;
   uint64_t plan_clock = 0;
+  // tq_capture_begin: work is recorded on capture_stream; the caller's
+  // stream comes back at tq_capture_end
+  hipStream_t capture_stream = nullptr;
+  hipStream_t saved_stream = nullptr;
+  bool capturing = false;
 };
 
 // Records an error message on the handle and returns the status.

@@ -63,6 +63,7 @@ tq_status tq_destroy(tq_handle_t h) {
     e.plan->cached = false;
     (void)tq_plan_destroy(e.plan);
   }
+  if (h->capture_stream) (void)hipStreamDestroy(h->capture_stream);
   if (h->pinned) (void)hipHostFree(h->pinned);
   delete h;
   return TQ_SUCCESS;
@@ -70,6 +71,8 @@ tq_status tq_destroy(tq_handle_t h) {
 
 tq_status tq_set_stream(tq_handle_t h, void *stream) {
   if (!h) return TQ_ERROR_NOT_INITIALIZED;
+  if (h->capturing)
+    return tq_fail(h, TQ_ERROR_INVALID_VALUE, "set stream: the handle is capturing");
   h->stream = reinterpret_cast<hipStream_t>(stream);
   return TQ_SUCCESS;
 }

@@ -64,8 +64,16 @@ class Handle:
         self.plans = {}
 
     def bind_stream(self):
+        if getattr(self, "capturing", False):
+            return  # the handle records on its own capture stream
         s = torch.cuda.current_stream(self.device).cuda_stream
         check(lib().tq_set_stream(self.ptr, ctypes.c_void_p(s)), self.ptr)
+
+    def capture(self):
+        """Context manager recording the handle's work into one HIP graph
+        (tq_capture_begin / tq_capture_end); the Graph is set on the returned
+        holder's `graph` attribute when the block exits."""
+        return _Capture(self)
 
     def check(self, status):
         check(status, self.ptr)
@@ -81,6 +89,50 @@ class Handle:
                 lib().tq_destroy(self.ptr)
         except Exception:
             pass
+
+
+class Graph:
+    """A recorded sequence of library calls (tq_graph_t)."""
+
+    def __init__(self, handle, ptr):
+        self.h = handle
+        self.ptr = ptr
+
+    def launch(self):
+        self.h.bind_stream()
+        self.h.check(lib().tq_graph_launch(self.ptr, self.h.ptr))
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().tq_graph_destroy(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class _Capture:
+    def __init__(self, handle):
+        self.h = handle
+        self.graph = None
+
+    def __enter__(self):
+        self.h.bind_stream()
+        torch.cuda.synchronize(self.h.device)
+        self.h.check(lib().tq_capture_begin(self.h.ptr))
+        self.h.capturing = True
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        self.h.capturing = False
+        g = ctypes.c_void_p()
+        st = lib().tq_capture_end(self.h.ptr, ctypes.byref(g))
+        if exc_type is None:
+            self.h.check(st)
+            self.graph = Graph(self.h, g)
+        elif st == 0:
+            lib().tq_graph_destroy(g)
+        return False
 
 
 _handles = {}
@@ -885,3 +937,12 @@ def normalize_f64(x, norm=None):
     h.check(lib().tq_normalize_f64(h.ptr, x.numel(), ctypes.c_void_p(x.data_ptr()),
                                    ctypes.c_void_p(norm.data_ptr())))
     return norm
+
+
+def normalize_f64_noalloc(x, norm):
+    """x <- x / ||x|| with the norm written to the one-element device tensor
+    `norm`, allocating nothing (tq_normalize_f64_noalloc): usable while a
+    handle captures a graph."""
+    h = default_handle(x.device.index)
+    h.check(lib().tq_normalize_f64_noalloc(h.ptr, x.numel(), ctypes.c_void_p(x.data_ptr()),
+                                           ctypes.c_void_p(norm.data_ptr())))

@@ -233,3 +233,24 @@ def enlarged_corners(D=10, chi=256, seed=0):
     c1 = Tensor([dim, dim], begin, end, random_data(begin, end, rng))
     c2 = Tensor([dim, dim], begin.copy(), end.copy(), random_data(begin, end, rng))
     return c1, c2, [[1, 0]]
+
+
+def ctmrg_step(D, chi, seed=0):
+    """Tensors of one CTMRG edge update: absorb the double-layer site into
+    the edge, then renormalise the enlarged edge with two projectors.
+
+    Returns (E, A, Ac, P, Q, legs) with legs a list of the four
+    contractions' leg pairs:
+      a  = A . conj(A) over p                 -> a(u,l,d,r,u',l',d',r')
+      E1 = E . a over (l, l')                 -> E1(chi,chi',u,d,r,u',d',r')
+      F  = E1 . P over (chi, u, u')           -> F(chi',d,r,d',r',x)
+      G  = F . Q over (chi', d, d')           -> G(r,r',x,y)
+    and G is then normalised by its FP64 norm. P(chi, u, u', x) and
+    Q(chi', d, d', y) are U(1) projectors onto a new leg of dimension chi."""
+    rng = np.random.default_rng(seed + 1000)
+    E, A, Ac, l_aa, l_ea = peps(D, chi, seed=seed)
+    lchi, lD, _ = peps_legs(D, chi)
+    P = u1_tensor([lchi, lD, lD, lchi], [1, 1, -1, -1], rng)
+    Q = u1_tensor([lchi, lD, lD, lchi], [1, 1, -1, -1], rng)
+    legs = [l_aa, l_ea, [[0, 0], [2, 1], [5, 2]], [[0, 0], [1, 1], [3, 2]]]
+    return E, A, Ac, P, Q, legs
