@@ -460,6 +460,57 @@ def bench_large(dev, steps=5, warmup=2, tiers=("fp32", "bf16x3"), oracle_max_blo
                                      % (oracle_max_block, sample_flops / 1e9, cpu_s)}}
 
 
+def bench_split_k(dev, reps=20):
+    """Split-K on a contraction with few output tiles and long contracted
+    ranges (block-diagonal, 8 blocks of 64 x 16384 times 16384 x 64, FP64):
+    the multiply with whole pairs against the autotuned split."""
+    import torch
+
+    from torque_amd import BlockSparseTensor
+
+    n_blk, m, k = 8, 64, 16384
+    off_m = np.arange(n_blk, dtype=np.int64) * m
+    off_k = np.arange(n_blk, dtype=np.int64) * k
+    rng = np.random.default_rng(3)
+    ba = np.stack([off_m, off_k], 1)
+    ea = ba + np.array([m - 1, k - 1])
+    bb = np.stack([off_k, off_m], 1)
+    eb = bb + np.array([k - 1, m - 1])
+    A = BlockSparseTensor([n_blk * m, n_blk * k], rng.standard_normal(n_blk * m * k), ba, ea,
+                          device=dev)
+    B = BlockSparseTensor([n_blk * k, n_blk * m], rng.standard_normal(n_blk * m * k), bb, eb,
+                          device=dev)
+    p = A.plan_contract(B, [[1, 0]])
+    out = torch.empty(p.out_n_elem, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def timed():
+        p.execute(A.data, B.data, out)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            p.execute(A.data, B.data, out)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    best, _ = p.autotune(A.data, B.data, out, reps=3)
+    split = p.split()
+    ms_split = timed()
+    p.set_class_split(0, 1)
+    p.set_class_split(1, 1)
+    ms_whole = timed()
+    res = {"workload": "block_diagonal_8x_64x16384x64", "gflop": round(p.flops / 1e9, 3),
+           "whole_pairs_ms": round(ms_whole, 4), "split_ms": round(ms_split, 4),
+           "split": split, "gemm_config": best, "speedup": round(ms_whole / ms_split, 2),
+           "split_gflops": round(p.flops / ms_split / 1e6, 1)}
+    print("# split-K (8 blocks 64 x 16384 x 64): whole pairs %.3f ms, split %s %.3f ms (%.2fx)"
+          % (ms_whole, split, ms_split, ms_whole / ms_split), flush=True)
+    p.destroy()
+    return res
+
+
 def bench_ctmrg(dev, D, chi, steps, warmup):
     """One CTMRG edge update (double layer, absorption, two projector
     contractions, all merged-output plans, then the FP64 normalisation on the
@@ -780,6 +831,7 @@ def main():
     del outw
 
     ctmrg = bench_ctmrg(dev, args.D, args.chi, args.steps, args.warmup)
+    split_k = bench_split_k(dev)
     permute = None if args.no_permute else bench_permute(dev)
     large = None if args.no_large else bench_large(dev)
     corner128 = None if args.no_large else bench_large(dev, D=8, chi=128, oracle_max_block=4096)
@@ -809,6 +861,7 @@ def main():
         "fp64_rel_err_vs_oracle": err64,
         "merged_output": merged,
         "ctmrg_step_graph": ctmrg,
+        "split_k": split_k,
         "per_call": per_call,
         "enlarged_corner_D8_chi128": corner128,
         "permute_rank6_200M": permute,

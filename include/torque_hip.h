@@ -242,6 +242,17 @@ tq_status tq_bs_contract_multiply(tq_plan_t plan, const void *a, const void *b,
 tq_status tq_plan_set_config(tq_plan_t plan, int32_t config);
 tq_status tq_plan_set_class_config(tq_plan_t plan, int32_t cls, int32_t config);
 tq_status tq_plan_get_config(tq_plan_t plan, int32_t *config /* [2] */);
+/* Split-K of one MFMA tile class: each pair's contracted range is cut into
+ * up to `split` slices (1..64; ceil(K / split) rounded up to 32 values, so
+ * pairs with K <= 32 stay whole) that run as GEMMs of their own into a
+ * plan-owned workspace; a reduction adds each pair's slices into C in slice
+ * order, so results repeat bit for bit (and differ from split = 1 in the
+ * last bits). tq_plan_autotune times splits 1, 2, 4, 8 and 16 on each
+ * class's chosen configuration and keeps the fastest; tq_plan_choose_config
+ * predicts them. get_split writes both classes' splits. No reference
+ * counterpart (src/gpu/block_sparse.cuh:894-930 runs K whole). */
+tq_status tq_plan_set_class_split(tq_plan_t plan, int32_t cls, int32_t split);
+tq_status tq_plan_get_split(tq_plan_t plan, int32_t *split /* [2] */);
 tq_status tq_plan_destroy(tq_plan_t plan);
 
 /* Pairs whose contracted extent K is at most max_k (0..8) are written by the
@@ -468,8 +479,10 @@ tq_status tq_costmodel_fit(tq_costmodel_t model, const float *x, const float *y,
  * output for the given inputs), raw = 0 applies the fit's. */
 tq_status tq_costmodel_predict(tq_costmodel_t model, const float *x, int64_t n, int32_t raw,
                                float *y);
-/* Sets each tile class of the plan to the configuration with the least
- * predicted time; best (may be NULL) receives the two choices. */
+/* Sets each tile class of the plan to the configuration and split-K with
+ * the least predicted time (a split class is predicted as the grouped GEMM
+ * of its slices plus the reduction's traffic); best (may be NULL) receives
+ * the two configurations. */
 tq_status tq_plan_choose_config(tq_plan_t plan, tq_costmodel_t model, int32_t *best);
 
 #ifdef __cplusplus

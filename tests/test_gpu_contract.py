@@ -538,3 +538,76 @@ def test_split_operand_forms(cuda):
     pp.set_split_operands("panels")
     with pytest.raises(TorqueError, match="split-BF16"):
         ga.plan_contract(gb, legs).set_split_operands("panels")
+
+
+def thin_pairs(rng, shapes):
+    """Block-diagonal A (sum m x sum k) and B (sum k x sum n) whose i-th
+    blocks are m_i x k_i and k_i x n_i."""
+    r, k, n = (np.array(x, np.int64) for x in zip(*shapes))
+
+    def mat(rows, cols):
+        b0 = np.concatenate([[0], np.cumsum(rows)[:-1]])
+        b1 = np.concatenate([[0], np.cumsum(cols)[:-1]])
+        begin = np.stack([b0, b1], 1).astype(np.int64)
+        end = np.stack([b0 + rows - 1, b1 + cols - 1], 1).astype(np.int64)
+        return W.Tensor([rows.sum(), cols.sum()], begin, end,
+                        rng.standard_normal(int((rows * cols).sum())))
+
+    return mat(r, k), mat(k, n), [[1, 0]]
+
+
+@pytest.mark.parametrize("precision", ["native", "fp32", "bf16x3"])
+def test_split_k(cuda, precision):
+    # few output tiles, long contracted ranges: each pair's K is cut into
+    # slices whose partial products are added in slice order; every split
+    # matches the oracle, repeats bit for bit, and splits of both tile
+    # classes combine with every configuration class
+    rng = np.random.default_rng(71)
+    shapes = [(48, 3001, 40), (16, 2048, 64), (130, 1500, 140), (64, 33, 64), (8, 700, 5)]
+    a, b, legs = thin_pairs(rng, shapes)
+    (A, da), (B, db) = from_workload(a), from_workload(b)
+    ref_c, ref = O.contract(A, da, B, db, legs)
+    tol = {"native": FP64_TOL, "fp32": 1e-5, "bf16x3": 1e-6}[precision]
+    ga, gb = torch_tensor(A, da, cuda), torch_tensor(B, db, cuda)
+    plan = ga.plan_contract(gb, legs, precision)
+    out = torch.empty(plan.out_n_elem, dtype=torch.float64, device=cuda)
+    for s0, s1 in [(1, 1), (2, 2), (3, 5), (8, 1), (64, 64)]:
+        plan.set_class_split(0, s0)
+        plan.set_class_split(1, s1)
+        assert plan.split() == [s0, s1]
+        runs = []
+        for _ in range(2):
+            out.fill_(float("nan"))
+            plan.execute(ga.data, gb.data, out)
+            torch.cuda.synchronize()
+            runs.append(out.cpu().numpy())
+        assert np.array_equal(runs[0], runs[1]), (s0, s1)
+        assert rel_fro(runs[0], ref_c) <= tol, (s0, s1)
+    best, _ = plan.autotune(ga.data, gb.data, out, reps=1)
+    assert all(s in (1, 2, 4, 8, 16) for s in plan.split())
+    plan.execute(ga.data, gb.data, out)
+    torch.cuda.synchronize()
+    assert rel_fro(out.cpu().numpy(), ref_c) <= tol
+    with pytest.raises(TorqueError, match="split"):
+        plan.set_class_split(0, 65)
+
+
+def test_split_k_cost_model_choice(cuda):
+    # the committed cost model predicts split classes from the shapes of
+    # their slices; whatever it picks must compute the same contraction
+    from torque_amd import costmodel
+    model = costmodel.load_default()
+    if model is None:
+        pytest.skip("no committed cost model")
+    rng = np.random.default_rng(72)
+    a, b, legs = thin_pairs(rng, [(64, 8192, 64), (32, 4096, 48)])
+    (A, da), (B, db) = from_workload(a), from_workload(b)
+    ref_c, _ = O.contract(A, da, B, db, legs)
+    ga, gb = torch_tensor(A, da, cuda), torch_tensor(B, db, cuda)
+    plan = ga.plan_contract(gb, legs)
+    model.choose(plan)
+    assert plan.split()[0] in (1, 2, 4, 8, 16)
+    out = torch.empty(plan.out_n_elem, dtype=torch.float64, device=cuda)
+    plan.execute(ga.data, gb.data, out)
+    torch.cuda.synchronize()
+    assert rel_fro(out.cpu().numpy(), ref_c) <= FP64_TOL

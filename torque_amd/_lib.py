@@ -113,6 +113,8 @@ def _proto(lib):
         "tq_plan_set_config": (S, [V, ctypes.c_int32]),
         "tq_plan_set_class_config": (S, [V, ctypes.c_int32, ctypes.c_int32]),
         "tq_plan_get_config": (S, [V, c_i32_p]),
+        "tq_plan_set_class_split": (S, [V, ctypes.c_int32, ctypes.c_int32]),
+        "tq_plan_get_split": (S, [V, c_i32_p]),
         "tq_batch_create": (S, [H, ctypes.c_int32, ctypes.POINTER(V), c_i64_p, c_i64_p,
                                 c_i64_p, ctypes.POINTER(V)]),
         "tq_plan_capture": (S, [V, V, V, V]),

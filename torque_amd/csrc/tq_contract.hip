@@ -45,7 +45,35 @@ __global__ void store_scalar_kernel(const double *__restrict__ v, T *__restrict_
   if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = (T)v[0];
 }
 
+// Split-K reduction: one workgroup per job; every element adds its slices
+// in slice order, so the result does not depend on the schedule.
+template <class T>
+__global__ __launch_bounds__(256) void split_reduce_kernel(const SplitReduceJob *__restrict__ jobs,
+                                                           const T *__restrict__ ws,
+                                                           T *__restrict__ c) {
+  const SplitReduceJob j = jobs[blockIdx.x];
+  for (int e = threadIdx.x; e < j.n; e += 256) {
+    T s = ws[j.ws_off + e];
+    for (int q = 1; q < j.slices; ++q) s += ws[j.ws_off + (i64)q * j.stride + e];
+    c[j.c_off + e] = s;
+  }
+}
+
 }  // namespace
+
+tq_status tq_launch_split_reduce(tq_handle_t h, bool f32, const SplitReduceJob *jobs,
+                                 int64_t n_jobs, const void *ws, void *c) {
+  if (n_jobs <= 0) return TQ_SUCCESS;
+  if (n_jobs > 0x7fffffff) return tq_fail(h, TQ_ERROR_UNSUPPORTED, "split-K: too many jobs");
+  if (f32)
+    hipLaunchKernelGGL(split_reduce_kernel<float>, dim3((unsigned)n_jobs), dim3(256), 0,
+                       h->stream, jobs, (const float *)ws, (float *)c);
+  else
+    hipLaunchKernelGGL(split_reduce_kernel<double>, dim3((unsigned)n_jobs), dim3(256), 0,
+                       h->stream, jobs, (const double *)ws, (double *)c);
+  TQ_HIP(h, hipGetLastError());
+  return TQ_SUCCESS;
+}
 
 // Analytic choice of the tile configuration for one MFMA tile class: models
 // each tile as MFMA work on the padded tile plus its operand and output
@@ -88,23 +116,31 @@ int tq_plan_pick_config(tq_plan_t pl, int cls) {
 
 namespace {
 
-// One grouped-GEMM launch over the tiles of one MFMA tile class.
+// One grouped-GEMM launch over the tiles of one MFMA tile class (for a
+// split-K class: the slices' GEMMs into the workspace, then the reduction).
 tq_status launch_class(tq_plan_t pl, int cls, const void *a, const void *b, void *c) {
   tq_handle_t h = pl->h;
   const TileClass &tc = pl->cls[cls];
   if (tc.n_tiles == 0) return TQ_SUCCESS;
+  const bool split = tc.tiles_split > 1;
+  const GemmDesc *descs = split ? tc.vdesc.as<GemmDesc>() : pl->desc.as<GemmDesc>();
+  void *out = split ? tc.ws.p : c;
+  const RunDesc *runs = tc.runs ? pl->run_desc.as<RunDesc>() : nullptr;
   if (pl->prec == TQ_PREC_BF16X3)
-    return tq_launch_bf16x3_gemm(h, tc.tiles_cfg, pl->split_direct, pl->desc.as<GemmDesc>(),
-                                 tc.runs ? pl->run_desc.as<RunDesc>() : nullptr,
-                                 tc.tile_pair.as<int32_t>(), tc.tile_first.as<int32_t>(),
-                                 tc.n_tiles, pl->split_direct ? a : pl->pack_a.panel.p,
-                                 pl->split_direct ? b : pl->pack_b.panel.p,
-                                 pl->pack_a.panel_elems, pl->pack_b.panel_elems,
-                                 reinterpret_cast<double *>(c));
-  return tq_launch_grouped_gemm(h, pl->dtype, pl->prec, tc.tiles_cfg, pl->desc.as<GemmDesc>(),
-                                tc.runs ? pl->run_desc.as<RunDesc>() : nullptr,
-                                tc.tile_pair.as<int32_t>(), tc.tile_first.as<int32_t>(),
-                                tc.n_tiles, a, pl->pack_a.panel.p, b, pl->pack_b.panel.p, c);
+    TQ_CHECK(tq_launch_bf16x3_gemm(h, tc.tiles_cfg, pl->split_direct, descs, runs,
+                                   tc.tile_pair.as<int32_t>(), tc.tile_first.as<int32_t>(),
+                                   tc.n_tiles, pl->split_direct ? a : pl->pack_a.panel.p,
+                                   pl->split_direct ? b : pl->pack_b.panel.p,
+                                   pl->pack_a.panel_elems, pl->pack_b.panel_elems,
+                                   reinterpret_cast<double *>(out)));
+  else
+    TQ_CHECK(tq_launch_grouped_gemm(h, pl->dtype, pl->prec, tc.tiles_cfg, descs, runs,
+                                    tc.tile_pair.as<int32_t>(), tc.tile_first.as<int32_t>(),
+                                    tc.n_tiles, a, pl->pack_a.panel.p, b, pl->pack_b.panel.p,
+                                    out));
+  if (!split) return TQ_SUCCESS;
+  return tq_launch_split_reduce(h, pl->dtype == TQ_F32, tc.red_jobs.as<SplitReduceJob>(),
+                                tc.n_red_jobs, tc.ws.p, c);
 }
 
 }  // namespace
@@ -137,6 +173,22 @@ tq_status tq_plan_set_class_config(tq_plan_t pl, int32_t cls, int32_t config) {
   TQ_CHECK(check_config(pl, config));
   pl->drop_graph();
   pl->cls[cls].cfg = config;
+  return TQ_SUCCESS;
+}
+
+tq_status tq_plan_set_class_split(tq_plan_t pl, int32_t cls, int32_t split) {
+  if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  if (cls < 0 || cls > 1 || split < 1 || split > kMaxSplit)
+    return tq_fail(pl->h, TQ_ERROR_INVALID_VALUE, "split-K: class 0 or 1, split 1..64");
+  pl->drop_graph();
+  pl->cls[cls].split = split;
+  return TQ_SUCCESS;
+}
+
+tq_status tq_plan_get_split(tq_plan_t pl, int32_t *split) {
+  if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  if (!split) return TQ_ERROR_INVALID_VALUE;
+  for (int c = 0; c < 2; ++c) split[c] = pl->cls[c].split;
   return TQ_SUCCESS;
 }
 
@@ -253,6 +305,7 @@ tq_status sweep_configs(tq_plan_t pl, const void *a, const void *b, void *c, int
   for (int k = 0; k < 2; ++k) {
     TileClass &tc = pl->cls[k];
     if (tc.n_tiles == 0) continue;
+    tc.split = 1;  // configurations are compared on whole pairs
     int best_cfg = tc.current();
     float best_t = 3.4e38f;
     for (int q = 0; q < nc; ++q) {
@@ -277,6 +330,27 @@ tq_status sweep_configs(tq_plan_t pl, const void *a, const void *b, void *c, int
       }
     }
     tc.cfg = best_cfg;
+    // split-K over the chosen configuration: the contracted range cut into
+    // 2..16 slices (partials plus a reduction), kept when faster
+    int best_split = 1;
+    float best_split_t = 3.4e38f;
+    for (int sp : {1, 2, 4, 8, 16}) {
+      tc.split = sp;
+      TQ_CHECK(plan_build_tiles(pl));
+      TQ_CHECK(launch_class(pl, k, a, b, c));  // warm-up
+      TQ_HIP(h, hipEventRecord(e0, h->stream));
+      for (int r = 0; r < reps; ++r) TQ_CHECK(launch_class(pl, k, a, b, c));
+      TQ_HIP(h, hipEventRecord(e1, h->stream));
+      TQ_HIP(h, hipEventSynchronize(e1));
+      float ms = 0.f;
+      TQ_HIP(h, hipEventElapsedTime(&ms, e0, e1));
+      ms /= reps;
+      if (ms < best_split_t) {
+        best_split_t = ms;
+        best_split = sp;
+      }
+    }
+    tc.split = best_split;
   }
   return plan_build_tiles(pl);
 }

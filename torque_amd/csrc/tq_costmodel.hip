@@ -479,16 +479,54 @@ tq_status tq_costmodel_predict(tq_costmodel_t md, const float *x, int64_t n, int
 tq_status tq_plan_choose_config(tq_plan_t pl, tq_costmodel_t md, int32_t *best) {
   if (!pl || !md) return TQ_ERROR_NOT_INITIALIZED;
   const int nc = tq_gemm_n_configs(pl->prec);
+  const int n_cu = pl->h ? pl->h->n_cu : 256;
   for (int c = 0; c < 2; ++c) {
-    std::vector<float> feat((size_t)nc * kF), pred(nc);
-    for (int q = 0; q < nc; ++q) TQ_CHECK(tq_plan_features(pl, c, q, &feat[(size_t)q * kF]));
-    TQ_CHECK(tq_costmodel_predict(md, feat.data(), nc, 0, pred.data()));
-    int arg = -1;
-    for (int q = 0; q < nc; ++q) {
-      if (tq_gemm_config(pl->prec, q).wave_tile && !pl->wave_ok()) continue;
-      if (arg < 0 || pred[q] < pred[arg]) arg = q;
+    std::vector<int64_t> m, n, k;
+    for (int64_t p = 0; p < pl->n_pairs; ++p)
+      if (pair_class(*pl, pl->h_m[p], pl->h_n[p], pl->h_k[p]) == c) {
+        m.push_back(pl->h_m[p]);
+        n.push_back(pl->h_n[p]);
+        k.push_back(pl->h_k[p]);
+      }
+    int arg = -1, arg_split = 1;
+    double arg_t = 0;
+    // split-K enters through the shapes it produces: a split class is the
+    // grouped GEMM of its (pair, slice) GEMMs, predicted by the same model,
+    // plus the reduction's traffic (its slices read, the output written)
+    for (int sp : {1, 2, 4, 8, 16}) {
+      std::vector<int64_t> vm, vn, vk;
+      double out_elems = 0, slices = 0;
+      for (size_t p = 0; p < m.size(); ++p) {
+        const int64_t ks = split_slice_k(k[p], sp);
+        const int64_t ns = ceil_div(k[p], ks);
+        for (int64_t q = 0; q < ns; ++q) {
+          vm.push_back(m[p]);
+          vn.push_back(n[p]);
+          vk.push_back(std::min<int64_t>(ks, k[p] - q * ks));
+        }
+        out_elems += (double)m[p] * n[p];
+        slices += (double)ns * m[p] * n[p];
+      }
+      if (sp > 1 && vm.size() == m.size()) continue;  // no pair would split
+      std::vector<float> feat((size_t)nc * kF), pred(nc);
+      for (int q = 0; q < nc; ++q)
+        tq_cost_features_host(vm.data(), vn.data(), vk.data(), (int64_t)vm.size(),
+                              tq_gemm_config(pl->prec, q), n_cu, &feat[(size_t)q * kF]);
+      TQ_CHECK(tq_costmodel_predict(md, feat.data(), nc, 0, pred.data()));
+      const double reduce_ms = sp > 1 ? 8.0 * (slices + out_elems) / 5e9 : 0.0;
+      for (int q = 0; q < nc; ++q) {
+        if (tq_gemm_config(pl->prec, q).wave_tile && !pl->wave_ok()) continue;
+        const double t = std::exp((double)pred[q]) + reduce_ms;
+        if (arg < 0 || t < arg_t) {
+          arg = q;
+          arg_t = t;
+          arg_split = sp;
+        }
+      }
     }
+    pl->drop_graph();
     pl->cls[c].cfg = arg < 0 ? -1 : arg;
+    pl->cls[c].split = arg_split;
     if (best) best[c] = pl->cls[c].current();
   }
   return TQ_SUCCESS;

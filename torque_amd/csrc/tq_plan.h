@@ -87,7 +87,35 @@ struct TileClass {
   int cfg = -1;  // user-selected configuration (-1 = automatic)
   int auto_cfg = 0;
   int current() const { return cfg >= 0 ? cfg : auto_cfg; }
+  // split-K: each pair's contracted range cut into up to `split` slices;
+  // the slices' partial products go to ws (one output block per slice) and a
+  // reduction adds them into C in slice order (deterministic)
+  int split = 1;
+  int tiles_split = 1;  // the split the tile list was built for
+  DevBuf vdesc;         // descriptors of the (pair, slice) GEMMs
+  DevBuf ws;
+  DevBuf red_jobs;
+  int64_t n_red_jobs = 0;
 };
+
+// k extent of one split-K slice: ceil(K / split) rounded up to whole 32-value
+// k steps (pairs with K <= 32 stay whole).
+inline int64_t split_slice_k(int64_t k, int split) {
+  if (split <= 1) return k;
+  const int64_t ks = ceil_div(ceil_div(k, split), 32) * 32;
+  return ks < k ? ks : k;
+}
+
+// One reduction job of split-K: C[c_off + e] = sum over slices q of
+// ws[ws_off + q * stride + e] for e < n, slices added in order.
+struct SplitReduceJob {
+  i64 c_off, ws_off;
+  int32_t n, stride, slices, pad;
+};
+constexpr int kSplitReduceElems = 4096;
+constexpr int kMaxSplit = 64;
+tq_status tq_launch_split_reduce(tq_handle_t h, bool f32, const SplitReduceJob *jobs,
+                                 int64_t n_jobs, const void *ws, void *c);
 
 struct tq_plan_s {
   tq_handle_t h = nullptr;

@@ -746,6 +746,84 @@ static tq_status build_stream_segments(tq_plan_t pl) {
   return TQ_SUCCESS;
 }
 
+// Tile list of a split-K class: every pair of the class (in work-list
+// order) becomes up to `split` GEMMs over slices of its contracted range,
+// each writing its own block of the workspace; the reduction jobs then add a
+// pair's slices into its output block.
+static tq_status build_split_tiles(tq_plan_t pl, int cls, const GemmConfig &g,
+                                   TileClass &tc) {
+  tq_handle_t h = pl->h;
+  const int64_t np = pl->n_pairs;
+  std::vector<i64> order(np);
+  TQ_HIP(h, hipMemcpyAsync(order.data(), pl->order.p, sizeof(i64) * np, hipMemcpyDeviceToHost,
+                           h->stream));
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  // split-BF16 panels are k-chunked: a slice starting at k0 (a multiple of
+  // 32) begins k0 * rows values into the panel and holds round8(slice) k
+  const bool chunked = pl->prec == TQ_PREC_BF16X3 && !pl->split_direct;
+  std::vector<GemmDesc> vd;
+  std::vector<SplitReduceJob> jobs;
+  std::vector<int32_t> tiles, first;
+  i64 ws_off = 0;
+  for (int64_t q = 0; q < np; ++q) {
+    const GemmDesc &d = pl->h_desc[order[q]];
+    if (pair_class(*pl, d.m, d.n, d.k) != cls) continue;
+    const int64_t ks = split_slice_k(d.k, tc.split);
+    const int64_t ns = ceil_div(d.k, ks);
+    const i64 mn = (i64)d.m * d.n;
+    for (int64_t sl = 0; sl < ns; ++sl) {
+      const i64 k0 = sl * ks;
+      const int32_t kl = (int32_t)std::min<i64>(ks, d.k - k0);
+      GemmDesc v = d;
+      if (chunked && (d.flags & kAPacked)) {
+        v.a_off += k0 * d.m;
+        v.lda = (kl + 7) / 8 * 8;
+      } else {
+        v.a_off += (d.flags & kAKFast) ? k0 : k0 * d.lda;
+      }
+      if (chunked && (d.flags & kBPacked)) {
+        v.b_off += k0 * d.n;
+        v.ldb = (kl + 7) / 8 * 8;
+      } else {
+        v.b_off += (d.flags & kBKFast) ? k0 : k0 * d.ldb;
+      }
+      v.k = kl;
+      v.c_off = ws_off + sl * mn;
+      const int64_t nt = ceil_div(d.m, g.bm) * ceil_div(d.n, g.bn);
+      if ((int64_t)tiles.size() + nt > 0x7fffffff || vd.size() >= 0x7fffffff)
+        return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many split-K tiles");
+      first.push_back((int32_t)tiles.size());
+      tiles.insert(tiles.end(), (size_t)nt, (int32_t)vd.size());
+      vd.push_back(v);
+    }
+    for (i64 e0 = 0; e0 < mn; e0 += kSplitReduceElems)
+      jobs.push_back(SplitReduceJob{d.c_off + e0, ws_off + e0,
+                                    (int32_t)std::min<i64>(kSplitReduceElems, mn - e0),
+                                    (int32_t)mn, (int32_t)ns, 0});
+    ws_off += ns * mn;
+  }
+  const size_t es = pl->dtype == TQ_F32 ? 4 : 8;
+  TQ_HIP(h, tc.vdesc.alloc(sizeof(GemmDesc) * std::max<size_t>(vd.size(), 1)));
+  TQ_HIP(h, tc.tile_pair.alloc(sizeof(int32_t) * std::max<size_t>(tiles.size(), 1)));
+  TQ_HIP(h, tc.tile_first.alloc(sizeof(int32_t) * std::max<size_t>(first.size(), 1)));
+  TQ_HIP(h, tc.red_jobs.alloc(sizeof(SplitReduceJob) * std::max<size_t>(jobs.size(), 1)));
+  TQ_HIP(h, tc.ws.alloc(es * std::max<i64>(ws_off, 1)));
+  if (!vd.empty()) {
+    TQ_HIP(h, hipMemcpyAsync(tc.vdesc.p, vd.data(), sizeof(GemmDesc) * vd.size(),
+                             hipMemcpyHostToDevice, h->stream));
+    TQ_HIP(h, hipMemcpyAsync(tc.tile_pair.p, tiles.data(), sizeof(int32_t) * tiles.size(),
+                             hipMemcpyHostToDevice, h->stream));
+    TQ_HIP(h, hipMemcpyAsync(tc.tile_first.p, first.data(), sizeof(int32_t) * first.size(),
+                             hipMemcpyHostToDevice, h->stream));
+    TQ_HIP(h, hipMemcpyAsync(tc.red_jobs.p, jobs.data(), sizeof(SplitReduceJob) * jobs.size(),
+                             hipMemcpyHostToDevice, h->stream));
+  }
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  tc.n_tiles = (int64_t)tiles.size();
+  tc.n_red_jobs = (int64_t)jobs.size();
+  return TQ_SUCCESS;
+}
+
 tq_status plan_build_tiles(tq_plan_t pl) {
   tq_handle_t h = pl->h;
   const int64_t np = pl->n_pairs;
@@ -754,8 +832,18 @@ tq_status plan_build_tiles(tq_plan_t pl) {
   for (int c = 0; c < 2; ++c) {
     TileClass &tc = pl->cls[c];
     const int cfg = tc.current();
-    if (!(tc.ready && tc.tiles_cfg == cfg)) {
+    if (!(tc.ready && tc.tiles_cfg == cfg && tc.tiles_split == tc.split)) {
       const GemmConfig g = tq_gemm_config(pl->prec, cfg);
+      if (tc.split > 1) {
+        TQ_CHECK(build_split_tiles(pl, c, g, tc));
+        tc.runs = false;
+        tc.tiles_cfg = cfg;
+        tc.tiles_split = tc.split;
+        tc.ready = true;
+        pl->n_tiles += tc.n_tiles;
+        continue;
+      }
+      tc.tiles_split = 1;
       // the wave-tile configurations (FP64 and FP32 tiers) and every
       // split-BF16 configuration tile runs of pairs when runs join pairs of
       // this class (otherwise pair tiles are the same work with cheaper B

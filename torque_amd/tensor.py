@@ -497,6 +497,15 @@ class ContractionPlan:
     def set_class_config(self, cls, cfg):
         self.h.check(lib().tq_plan_set_class_config(self.ptr, int(cls), int(cfg)))
 
+    def split(self):
+        """Split-K of [class 0, class 1] (tq_plan_get_split)."""
+        sp = (ctypes.c_int32 * 2)()
+        self.h.check(lib().tq_plan_get_split(self.ptr, sp))
+        return [sp[0], sp[1]]
+
+    def set_class_split(self, cls, split):
+        self.h.check(lib().tq_plan_set_class_split(self.ptr, int(cls), int(split)))
+
     def set_small_k(self, max_k):
         self.h.check(lib().tq_plan_set_small_k(self.ptr, int(max_k)))
 
