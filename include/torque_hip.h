@@ -54,9 +54,15 @@ typedef enum tq_status {
   TQ_ERROR_NOT_INITIALIZED = 8     /* null handle or plan                     */
 } tq_status;
 
+/* Element types. TQ_F16 is IEEE binary16 (the reference's `half`
+ * instantiation, src/gpu/block_sparse.cu:94-148): its contractions run on
+ * FP16 MFMA with FP32 accumulation and write FP16 (the reference's
+ * cublasHgemmStridedBatched, src/gpu/block_sparse.cuh:918-929); its rank-0
+ * results reduce in FP64 (:991-1003). */
 typedef enum tq_dtype {
   TQ_F32 = 0,
-  TQ_F64 = 1
+  TQ_F64 = 1,
+  TQ_F16 = 2
 } tq_dtype;
 
 /* Arithmetic used by the grouped GEMM of a contraction plan.

@@ -26,6 +26,9 @@ using torque::uword;
 using BST = torque::gpu::BlockSparseTensor<float>;
 using DT = torque::gpu::DenseTensor<float>;
 using ST = torque::gpu::SparseTensor<float>;
+using torque::half;
+using torque::gpu::BlockSparseTensor;
+using torque::gpu::load_block_sparse;
 
 static void block_sparse_sections(tq_handle_t handle) {
   {  // block sparse matrix initialization
@@ -347,7 +350,59 @@ static void cache_and_merged_section(tq_handle_t handle) {
   CHECK(tq_plan_destroy(p1) == TQ_ERROR_INVALID_VALUE);
 }
 
-int main() {
+// Not a reference section: the reference's `half` instantiation as
+// torque::half (FP16 MFMA, FP32 accumulation); small integers keep every
+// product and sum exact.
+static void half_section(tq_handle_t handle) {
+  using HT = BlockSparseTensor<half>;
+  const std::vector<half> d1{0, 1, 2, 3}, d2{4, 5, 6, 7}, v{1, 2};
+  HT t(uvec{2, 2, 2});
+  t.append_block(d1.data(), {0, 0, 0}, {1, 1, 0}, {1, 2, 4});
+  t.append_block(d2.data(), {0, 0, 1}, {1, 1, 1}, {1, 2, 4});
+  HT vt(uvec{2});
+  vt.append_block(v.data(), {0}, {1}, {1});
+  const auto c1 = t.contract(handle, vt, umat{{0, 0}});
+  CHECK(float(c1.query({0, 0})) == 2);
+  CHECK(float(c1.query({1, 0})) == 8);
+  CHECK(float(c1.query({0, 1})) == 14);
+  CHECK(float(c1.query({1, 1})) == 20);
+  CHECK(float(t.contract(handle, vt, umat{{2, 0}}).query({1, 0})) == 11);
+  CHECK(float(t.contract_merged(handle, vt, umat{{2, 0}}).query({1, 1})) == 17);
+  const auto tt = t.hard_transpose({2, 0, 1});
+  CHECK(float(tt.query({1, 0, 1})) == 6);
+  CHECK(float(t.contract(handle, t, umat{{0, 0}, {1, 1}, {2, 2}}).to_number()) == 140);
+  // one 48 x 40 x 72 GEMM block through the MFMA tiles (K > 8)
+  const uword m = 48, k = 40, n = 72;
+  std::vector<half> a(m * k), b(k * n);
+  for (uword i = 0; i < m * k; ++i) a[i] = half((int)(i * 7 % 7) - 3 + (int)(i % 5));
+  for (uword i = 0; i < k * n; ++i) b[i] = half((int)(i * 3 % 5) - 2);
+  HT A(a.data(), uvec{0, 0}, uvec{m - 1, k - 1}, uvec{m, k});
+  HT B(b.data(), uvec{0, 0}, uvec{k - 1, n - 1}, uvec{k, n});
+  const auto C = A.contract(handle, B, umat{{1, 0}});
+  bool exact = true;
+  for (uword i = 0; i < m; ++i)
+    for (uword j = 0; j < n; ++j) {
+      float ref = 0;
+      for (uword q = 0; q < k; ++q) ref += float(a[i + m * q]) * float(b[q + k * j]);
+      exact = exact && float(C.query({i, j})) == ref;
+    }
+  CHECK(exact);
+}
+
+// Not a reference section: the binary block-tensor file. Reads dir/in.tqbs
+// (written by the Python test), contracts it with itself over legs (2, 0),
+// (3, 1) and writes the result to dir/out.tqbs for the Python test to check.
+static void file_section(tq_handle_t handle, const std::string &dir) {
+  if (dir.empty()) return;
+  const auto t = load_block_sparse<double>(dir + "/in.tqbs");
+  const auto c = t.contract(handle, t, umat({{2, 0}, {3, 1}}));
+  torque::gpu::save(c, dir + "/out.tqbs");
+  const auto back = load_block_sparse<double>(dir + "/out.tqbs");
+  CHECK(back.n_blocks() == c.n_blocks());
+  CHECK(back.query({1, 2, 3, 4}) == c.query({1, 2, 3, 4}));
+}
+
+int main(int argc, char **argv) {
   tq_handle_t handle = nullptr;
   if (tq_create(&handle, 0) != TQ_SUCCESS) {
     std::printf("FAILED: no device\n");
@@ -359,6 +414,8 @@ int main() {
     sparse_sections(handle);
     slice_section();
     cache_and_merged_section(handle);
+    half_section(handle);
+    file_section(handle, argc > 1 ? argv[1] : "");
   } catch (const std::exception &e) {
     std::printf("FAILED with exception: %s\n", e.what());
     ++g_fail;

@@ -78,3 +78,25 @@ def test_short_source_data_is_rejected():
     t = BlockSparseTensor([4, 3], [1, 2, 3, 4], [[1, 1]], [[2, 2]], device="cpu")
     with pytest.raises(TorqueError, match="the block needs 4"):
         t.append_block([1, 2, 3], [0, 0], [1, 1], [1, 2])
+
+
+def test_block_tensor_file_round_trip(tmp_path):
+    # the binary block-tensor file keeps metadata and values bit for bit, in
+    # every dtype, including a tensor without blocks
+    rng = np.random.default_rng(3)
+    for dtype in (torch.float64, torch.float32, torch.float16):
+        t = BlockSparseTensor([5, 6], rng.standard_normal(22), [[0, 0], [1, 2]],
+                              [[1, 2], [4, 5]], index_tables=[[3, 1], [1, 4]], dtype=dtype,
+                              device="cpu")
+        t.save(tmp_path / "t.tqbs")
+        u = BlockSparseTensor.load(tmp_path / "t.tqbs", device="cpu")
+        assert u.dtype == dtype
+        for k in ("dimension", "begin_points", "end_points", "index_tables", "block_offsets"):
+            assert np.array_equal(getattr(u, k), getattr(t, k)), k
+        assert torch.equal(u.data, t.data)
+    e = BlockSparseTensor([3, 4], device="cpu")
+    e.save(tmp_path / "e.tqbs")
+    assert BlockSparseTensor.load(tmp_path / "e.tqbs", device="cpu").n_blocks == 0
+    (tmp_path / "bad.tqbs").write_bytes(b"XXXX")
+    with pytest.raises(TorqueError, match="block-tensor"):
+        BlockSparseTensor.load(tmp_path / "bad.tqbs", device="cpu")

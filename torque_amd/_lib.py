@@ -19,6 +19,7 @@ TQ_ERROR_NOT_INITIALIZED = 8
 
 TQ_F32 = 0
 TQ_F64 = 1
+TQ_F16 = 2
 
 TQ_PREC_NATIVE = 0
 TQ_PREC_F32 = 1

@@ -83,7 +83,7 @@ tq_status merge_side(tq_handle_t h, tq_plan_s &out, bool is_b, int32_t n,
     TQ_HIP(h, dst.panel.alloc(bytes));
     TQ_HIP(h, hipMemsetAsync(dst.panel.p, 0, bytes, h->stream));
   } else {
-    TQ_HIP(h, dst.panel.alloc((out.dtype == TQ_F64 ? 8 : 4) * std::max<int64_t>(panel, 1)));
+    TQ_HIP(h, dst.panel.alloc(tq_elem_size(out.dtype) * std::max<int64_t>(panel, 1)));
   }
   return TQ_SUCCESS;
 }

@@ -39,6 +39,14 @@ struct tq_handle_s {
   bool capturing = false;
 };
 
+// Bytes of one element of a data type.
+inline size_t tq_elem_size(tq_dtype dt) {
+  return dt == TQ_F64 ? 8 : dt == TQ_F16 ? 2 : 4;
+}
+inline bool tq_valid_dtype(tq_dtype dt) {
+  return dt == TQ_F32 || dt == TQ_F64 || dt == TQ_F16;
+}
+
 // Records an error message on the handle and returns the status.
 tq_status tq_fail(tq_handle_t h, tq_status s, const std::string &msg);
 

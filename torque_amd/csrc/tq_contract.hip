@@ -47,30 +47,33 @@ __global__ void store_scalar_kernel(const double *__restrict__ v, T *__restrict_
 
 // Split-K reduction: one workgroup per job; every element adds its slices
 // in slice order, so the result does not depend on the schedule.
-template <class T>
+template <class T, class TS>
 __global__ __launch_bounds__(256) void split_reduce_kernel(const SplitReduceJob *__restrict__ jobs,
                                                            const T *__restrict__ ws,
                                                            T *__restrict__ c) {
   const SplitReduceJob j = jobs[blockIdx.x];
   for (int e = threadIdx.x; e < j.n; e += 256) {
-    T s = ws[j.ws_off + e];
-    for (int q = 1; q < j.slices; ++q) s += ws[j.ws_off + (i64)q * j.stride + e];
-    c[j.c_off + e] = s;
+    TS s = (TS)ws[j.ws_off + e];
+    for (int q = 1; q < j.slices; ++q) s += (TS)ws[j.ws_off + (i64)q * j.stride + e];
+    c[j.c_off + e] = (T)s;
   }
 }
 
 }  // namespace
 
-tq_status tq_launch_split_reduce(tq_handle_t h, bool f32, const SplitReduceJob *jobs,
+tq_status tq_launch_split_reduce(tq_handle_t h, tq_dtype out, const SplitReduceJob *jobs,
                                  int64_t n_jobs, const void *ws, void *c) {
   if (n_jobs <= 0) return TQ_SUCCESS;
   if (n_jobs > 0x7fffffff) return tq_fail(h, TQ_ERROR_UNSUPPORTED, "split-K: too many jobs");
-  if (f32)
-    hipLaunchKernelGGL(split_reduce_kernel<float>, dim3((unsigned)n_jobs), dim3(256), 0,
-                       h->stream, jobs, (const float *)ws, (float *)c);
+  if (out == TQ_F32)
+    hipLaunchKernelGGL((split_reduce_kernel<float, float>), dim3((unsigned)n_jobs), dim3(256),
+                       0, h->stream, jobs, (const float *)ws, (float *)c);
+  else if (out == TQ_F16)
+    hipLaunchKernelGGL((split_reduce_kernel<_Float16, float>), dim3((unsigned)n_jobs),
+                       dim3(256), 0, h->stream, jobs, (const _Float16 *)ws, (_Float16 *)c);
   else
-    hipLaunchKernelGGL(split_reduce_kernel<double>, dim3((unsigned)n_jobs), dim3(256), 0,
-                       h->stream, jobs, (const double *)ws, (double *)c);
+    hipLaunchKernelGGL((split_reduce_kernel<double, double>), dim3((unsigned)n_jobs),
+                       dim3(256), 0, h->stream, jobs, (const double *)ws, (double *)c);
   TQ_HIP(h, hipGetLastError());
   return TQ_SUCCESS;
 }
@@ -81,8 +84,9 @@ tq_status tq_launch_split_reduce(tq_handle_t h, bool f32, const SplitReduceJob *
 // autotuner measures the candidates instead and overrides this.
 int tq_plan_pick_config(tq_plan_t pl, int cls) {
   const int nc = tq_gemm_n_configs(pl->prec);
-  const double es = pl->dtype == TQ_F64 ? 8.0 : 4.0;
-  const double peak = pl->dtype == TQ_F32 || pl->prec == TQ_PREC_F32 ? 157e12
+  const double es = (double)tq_elem_size(pl->dtype);
+  const double peak = pl->dtype == TQ_F16                            ? 2.5e15
+                      : pl->dtype == TQ_F32 || pl->prec == TQ_PREC_F32 ? 157e12
                       : pl->prec == TQ_PREC_BF16X3               ? 2.5e15 / 6
                                                                    : 78.6e12;
   const double bw = 8e12;
@@ -91,7 +95,7 @@ int tq_plan_pick_config(tq_plan_t pl, int cls) {
   double best_t = 1e300;
   for (int c = 0; c < nc; ++c) {
     const GemmConfig g = tq_gemm_config(pl->prec, c);
-    if (g.wave_tile && !pl->wave_ok()) continue;
+    if (!plan_config_ok(*pl, c)) continue;
     double flop = 0, bytes = 0, tiles = 0;
     for (int64_t p = 0; p < pl->n_pairs; ++p) {
       if (pair_class(*pl, pl->h_m[p], pl->h_n[p], pl->h_k[p]) != cls) continue;
@@ -139,8 +143,8 @@ tq_status launch_class(tq_plan_t pl, int cls, const void *a, const void *b, void
                                     tc.n_tiles, a, pl->pack_a.panel.p, b, pl->pack_b.panel.p,
                                     out));
   if (!split) return TQ_SUCCESS;
-  return tq_launch_split_reduce(h, pl->dtype == TQ_F32, tc.red_jobs.as<SplitReduceJob>(),
-                                tc.n_red_jobs, tc.ws.p, c);
+  return tq_launch_split_reduce(h, pl->dtype, tc.red_jobs.as<SplitReduceJob>(), tc.n_red_jobs,
+                                tc.ws.p, c);
 }
 
 }  // namespace
@@ -151,7 +155,10 @@ namespace {
 
 tq_status check_config(tq_plan_t pl, int32_t config) {
   if (config < -1 || config >= tq_gemm_n_configs(pl->prec)) return TQ_ERROR_INVALID_VALUE;
-  if (config >= 0 && tq_gemm_config(pl->prec, config).wave_tile && !pl->wave_ok())
+  if (config >= 0 && pl->dtype == TQ_F16 && !plan_config_ok(*pl, config))
+    return tq_fail(pl->h, TQ_ERROR_UNSUPPORTED,
+                   "gemm: FP16 runs the workgroup-tile configurations with 16-value k steps");
+  if (config >= 0 && !plan_config_ok(*pl, config))
     return tq_fail(pl->h, TQ_ERROR_UNSUPPORTED,
                    "gemm: wave-tile configurations need leading dimensions below 2^22");
   return TQ_SUCCESS;
@@ -227,7 +234,7 @@ tq_status tq_bs_contract_multiply(tq_plan_t pl, const void *a, const void *b,
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
   tq_handle_t h = pl->h;
   if (!c) return TQ_ERROR_INVALID_VALUE;
-  const size_t es = pl->dtype == TQ_F64 ? 8 : 4;
+  const size_t es = tq_elem_size(pl->dtype);
   if (pl->n_pairs == 0) {
     if (pl->ro == 0) TQ_HIP(h, hipMemsetAsync(c, 0, es, h->stream));
     return TQ_SUCCESS;
@@ -256,6 +263,11 @@ tq_status tq_bs_contract_multiply(tq_plan_t pl, const void *a, const void *b,
                        pl->desc.as<GemmDesc>(), np, (const double *)a,
                        (const double *)pl->pack_a.panel.p, (const double *)b,
                        (const double *)pl->pack_b.panel.p, pl->partial.as<double>());
+  } else if (pl->dtype == TQ_F16) {
+    hipLaunchKernelGGL(pair_dot_kernel<_Float16>, dim3(g), dim3(256), 0, h->stream,
+                       pl->desc.as<GemmDesc>(), np, (const _Float16 *)a,
+                       (const _Float16 *)pl->pack_a.panel.p, (const _Float16 *)b,
+                       (const _Float16 *)pl->pack_b.panel.p, pl->partial.as<double>());
   } else {
     hipLaunchKernelGGL(pair_dot_kernel<float>, dim3(g), dim3(256), 0, h->stream,
                        pl->desc.as<GemmDesc>(), np, (const float *)a,
@@ -268,6 +280,9 @@ tq_status tq_bs_contract_multiply(tq_plan_t pl, const void *a, const void *b,
   if (pl->dtype == TQ_F64)
     hipLaunchKernelGGL(store_scalar_kernel<double>, dim3(1), dim3(64), 0, h->stream,
                        sum, (double *)c);
+  else if (pl->dtype == TQ_F16)
+    hipLaunchKernelGGL(store_scalar_kernel<_Float16>, dim3(1), dim3(64), 0, h->stream,
+                       sum, (_Float16 *)c);
   else
     hipLaunchKernelGGL(store_scalar_kernel<float>, dim3(1), dim3(64), 0, h->stream,
                        sum, (float *)c);
@@ -309,7 +324,7 @@ tq_status sweep_configs(tq_plan_t pl, const void *a, const void *b, void *c, int
     int best_cfg = tc.current();
     float best_t = 3.4e38f;
     for (int q = 0; q < nc; ++q) {
-      if (tq_gemm_config(pl->prec, q).wave_tile && !pl->wave_ok()) {
+      if (!plan_config_ok(*pl, q)) {
         if (times_ms) times_ms[k * nc + q] = -1.f;  // not applicable
         continue;
       }

@@ -515,7 +515,7 @@ tq_status tq_plan_choose_config(tq_plan_t pl, tq_costmodel_t md, int32_t *best) 
       TQ_CHECK(tq_costmodel_predict(md, feat.data(), nc, 0, pred.data()));
       const double reduce_ms = sp > 1 ? 8.0 * (slices + out_elems) / 5e9 : 0.0;
       for (int q = 0; q < nc; ++q) {
-        if (tq_gemm_config(pl->prec, q).wave_tile && !pl->wave_ok()) continue;
+        if (!plan_config_ok(*pl, q)) continue;
         const double t = std::exp((double)pred[q]) + reduce_ms;
         if (arg < 0 || t < arg_t) {
           arg = q;

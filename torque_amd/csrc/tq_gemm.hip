@@ -17,9 +17,12 @@
 // The wave-tile variant without LDS is in tq_wave_gemm.hip.
 //
 // Arithmetic: FP64 on v_mfma_f64_16x16x4_f64; FP32 on v_mfma_f32_16x16x4_f32
-// (for FP32 data, or FP64 data at the FP32 tier); split-BF16 (three BF16
+// (for FP32 data, or FP64 data at the FP32 tier); FP16 data on
+// v_mfma_f32_16x16x16_f16 with FP32 accumulation; split-BF16 (three BF16
 // pieces per operand, six piece products on v_mfma_f32_16x16x32_bf16 with
 // FP32 accumulation) in tq_gemm_bf16.hip.
+#include <type_traits>
+
 #include "tq_gemm_common.h"
 
 namespace {
@@ -36,6 +39,14 @@ __device__ inline f4 mfma(float a, float b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+
+// FP16: v_mfma_f32_16x16x16_f16, lane l supplying row (or column) l % 16 and
+// k = 4 (l / 16) .. 4 (l / 16) + 3 of each operand, FP32 accumulation.
+__device__ inline f4 mfma(h4 a, h4 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
 template <class TC>
 struct Acc;
 template <>
@@ -44,6 +55,10 @@ struct Acc<double> {
 };
 template <>
 struct Acc<float> {
+  typedef f4 type;
+};
+template <>
+struct Acc<_Float16> {
   typedef f4 type;
 };
 
@@ -184,21 +199,45 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
         la.load(nxt.pa, nxt.a_kfast, nxt.d.lda, nxt.d.m, nxt.m0, nxt.d.k, 0);
         lb.load(nxt.pb, nxt.b_kfast, nxt.d.ldb, nxt.d.n, nxt.n0, nxt.d.k, 0);
       }
-      const int ksteps = min(BK / 4, (d.k - k0 + 3) / 4);
-      for (int ks = 0; ks < ksteps; ++ks) {
-        const int kr = ks * 4 + (lane >> 4);
-        TC af[TM], bf[TN];
+      if constexpr (std::is_same<TC, _Float16>::value) {
+        // 16 k per MFMA; k past K reads the loaders' zeros
+        const int ksteps = min(BK / 16, (d.k - k0 + 15) / 16);
+        for (int ks = 0; ks < ksteps; ++ks) {
+          const int kr = ks * 16 + 4 * (lane >> 4);
+          h4 af[TM], bf[TN];
 #pragma unroll
-        for (int i = 0; i < TM; ++i)
-          af[i] = As[kr * (BM + 1) + wm0 + i * 16 + (lane & 15)];
+          for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j)
-          bf[j] = Bs[kr * (BN + 1) + wn0 + j * 16 + (lane & 15)];
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
+            for (int q = 0; q < 4; ++q)
+              af[i][q] = As[(kr + q) * (BM + 1) + wm0 + i * 16 + (lane & 15)];
 #pragma unroll
           for (int j = 0; j < TN; ++j)
-            if (i < tm_live && j < tn_live) acc[i][j] = mfma(af[i], bf[j], acc[i][j]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              bf[j][q] = Bs[(kr + q) * (BN + 1) + wn0 + j * 16 + (lane & 15)];
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+              if (i < tm_live && j < tn_live) acc[i][j] = mfma(af[i], bf[j], acc[i][j]);
+        }
+      } else {
+        const int ksteps = min(BK / 4, (d.k - k0 + 3) / 4);
+        for (int ks = 0; ks < ksteps; ++ks) {
+          const int kr = ks * 4 + (lane >> 4);
+          TC af[TM], bf[TN];
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+            af[i] = As[kr * (BM + 1) + wm0 + i * 16 + (lane & 15)];
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            bf[j] = Bs[kr * (BN + 1) + wn0 + j * 16 + (lane & 15)];
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+              if (i < tm_live && j < tn_live) acc[i][j] = mfma(af[i], bf[j], acc[i][j]);
+        }
       }
     }
     tqg::store_tile<TOut, sizeof(TC) == 8, BM, BN, WAVES_M, TM, TN>(acc, c, d, cur.m0,
@@ -259,6 +298,21 @@ tq_status dispatch(tq_handle_t h, int cfg, const GemmDesc *descs,
     return launch_one<TIn, TC, TOut, BM, BN, BK, WM>(h, descs, tile_pair,        \
                                                      tile_first, n_tiles, a, ap, \
                                                      b, bp, c);
+  if (std::is_same<TC, _Float16>::value && cfg >= 0 && cfg < kNCfg && kCfgs[cfg].bk % 16 != 0)
+    return tq_fail(h, TQ_ERROR_UNSUPPORTED, "gemm: FP16 needs a 16-value k step");
+  if constexpr (std::is_same<TC, _Float16>::value) {
+    switch (cfg) {
+      TQ_CFG(0, 64, 64, 16, 2)
+      TQ_CFG(1, 128, 64, 16, 2)
+      TQ_CFG(2, 64, 128, 16, 2)
+      TQ_CFG(3, 128, 128, 16, 2)
+      TQ_CFG(4, 256, 32, 16, 2)
+      TQ_CFG(5, 32, 256, 16, 2)
+      TQ_CFG(6, 32, 32, 16, 2)
+      TQ_CFG(7, 128, 16, 16, 4)
+    }
+    return tq_fail(h, TQ_ERROR_INVALID_VALUE, "gemm: unknown tile configuration");
+  }
   switch (cfg) {
     TQ_CFG(0, 64, 64, 16, 2)
     TQ_CFG(1, 128, 64, 16, 2)
@@ -314,11 +368,17 @@ tq_status tq_launch_grouped_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec,
   if (n_tiles <= 0) return TQ_SUCCESS;
   if (prec == TQ_PREC_BF16X3)
     return tq_fail(h, TQ_ERROR_INVALID_VALUE, "split-BF16 tier runs on split panels");
-  if (cfg >= kNCfg)
+  if (cfg >= kNCfg) {
+    if (dt == TQ_F16)
+      return tq_fail(h, TQ_ERROR_UNSUPPORTED, "gemm: FP16 has no wave-tile configuration");
     return tq_launch_wave_gemm(h, dt, prec, cfg - kNCfg, descs, runs, tile_pair, tile_first,
                                n_tiles, a, a_packed, b, b_packed, c);
+  }
   if (runs)
     return tq_fail(h, TQ_ERROR_INVALID_VALUE, "gemm: runs need a wave-tile configuration");
+  if (dt == TQ_F16)
+    return dispatch<_Float16, _Float16, _Float16>(h, cfg, descs, tile_pair, tile_first,
+                                                  n_tiles, a, a_packed, b, b_packed, c);
   if (dt == TQ_F32)
     return dispatch<float, float, float>(h, cfg, descs, tile_pair, tile_first,
                                          n_tiles, a, a_packed, b, b_packed, c);

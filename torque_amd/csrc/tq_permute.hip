@@ -255,7 +255,6 @@ tq_status launch_gather_single(tq_handle_t h, const CopyJob &job,
   return TQ_SUCCESS;
 }
 
-size_t elem_size(tq_dtype dt) { return dt == TQ_F64 ? 8 : 4; }
 
 // Effective legs after dropping unit dims and fusing output-adjacent legs
 // that are also adjacent (same order, contiguous) in the input.
@@ -660,6 +659,11 @@ tq_status tq_launch_gather_jobs(tq_handle_t h, tq_dtype dt, const CopyJob *jobs,
                        dim3(kGatherThreads), 0, h->stream, jobs, job_wg_start,
                        n_jobs, reinterpret_cast<const uint64_t *>(src),
                        reinterpret_cast<uint64_t *>(dst));
+  else if (dt == TQ_F16)
+    hipLaunchKernelGGL(gather_jobs_kernel<uint16_t>, dim3((unsigned)grid),
+                       dim3(kGatherThreads), 0, h->stream, jobs, job_wg_start,
+                       n_jobs, reinterpret_cast<const uint16_t *>(src),
+                       reinterpret_cast<uint16_t *>(dst));
   else
     hipLaunchKernelGGL(gather_jobs_kernel<uint32_t>, dim3((unsigned)grid),
                        dim3(kGatherThreads), 0, h->stream, jobs, job_wg_start,
@@ -677,7 +681,7 @@ tq_status tq_permute(tq_handle_t h, tq_dtype dtype, int32_t rank,
   if (!h) return TQ_ERROR_NOT_INITIALIZED;
   if (rank < 0 || rank > TQ_MAX_RANK)
     return tq_fail(h, TQ_ERROR_UNSUPPORTED, "permute: rank above TQ_MAX_RANK");
-  if (dtype != TQ_F32 && dtype != TQ_F64) return TQ_ERROR_UNSUPPORTED;
+  if (!tq_valid_dtype(dtype)) return TQ_ERROR_UNSUPPORTED;
   if ((rank > 0 && (!dims || !perm)) || !in || !out)
     return TQ_ERROR_INVALID_VALUE;
   if (!valid_perm(rank, perm))
@@ -693,6 +697,7 @@ tq_status tq_permute(tq_handle_t h, tq_dtype dtype, int32_t rank,
   const i64 *s = in_strides ? in_strides : table;
   Legs L = simplify(rank, dims, s, perm);
   if (dtype == TQ_F64) return dense_permute_t<uint64_t>(h, L, n_total, in, out);
+  if (dtype == TQ_F16) return dense_permute_t<uint16_t>(h, L, n_total, in, out);
   return dense_permute_t<uint32_t>(h, L, n_total, in, out);
 }
 
@@ -703,7 +708,7 @@ tq_status tq_bs_permute(tq_handle_t h, tq_dtype dtype, const tq_bs_desc *d,
   if (!d) return TQ_ERROR_INVALID_VALUE;
   const int r = d->rank;
   if (r < 0 || r > TQ_MAX_RANK) return TQ_ERROR_UNSUPPORTED;
-  if (dtype != TQ_F32 && dtype != TQ_F64) return TQ_ERROR_UNSUPPORTED;
+  if (!tq_valid_dtype(dtype)) return TQ_ERROR_UNSUPPORTED;
   if (r > 0 && !perm) return TQ_ERROR_INVALID_VALUE;
   if (!valid_perm(r, perm))
     return tq_fail(h, TQ_ERROR_RANK_MISMATCH,
@@ -725,7 +730,7 @@ tq_status tq_bs_permute(tq_handle_t h, tq_dtype dtype, const tq_bs_desc *d,
     return TQ_ERROR_INVALID_VALUE;
   if (r == 0) {
     // rank-0 blocks hold one element each
-    TQ_HIP(h, hipMemcpyAsync(out, in, (size_t)nb * (dtype == TQ_F64 ? 8 : 4),
+    TQ_HIP(h, hipMemcpyAsync(out, in, (size_t)nb * tq_elem_size(dtype),
                              hipMemcpyDeviceToDevice, h->stream));
     return TQ_SUCCESS;
   }
@@ -791,7 +796,7 @@ tq_status tq_bs_slice(tq_handle_t h, tq_dtype dtype, const tq_bs_desc *d,
   if (!d || !divisor) return TQ_ERROR_INVALID_VALUE;
   const int r = d->rank;
   if (r < 1 || r > TQ_MAX_RANK) return TQ_ERROR_UNSUPPORTED;
-  if (dtype != TQ_F32 && dtype != TQ_F64) return TQ_ERROR_UNSUPPORTED;
+  if (!tq_valid_dtype(dtype)) return TQ_ERROR_UNSUPPORTED;
   const i64 nb = d->n_blocks;
   if (nb > 0 && (!d->begin || !d->end || !d->strides || !d->offsets)) return TQ_ERROR_INVALID_VALUE;
   i64 n_sub = 1;

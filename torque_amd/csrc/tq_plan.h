@@ -114,7 +114,7 @@ struct SplitReduceJob {
 };
 constexpr int kSplitReduceElems = 4096;
 constexpr int kMaxSplit = 64;
-tq_status tq_launch_split_reduce(tq_handle_t h, bool f32, const SplitReduceJob *jobs,
+tq_status tq_launch_split_reduce(tq_handle_t h, tq_dtype out, const SplitReduceJob *jobs,
                                  int64_t n_jobs, const void *ws, void *c);
 
 struct tq_plan_s {
@@ -220,6 +220,14 @@ tq_status plan_build_tiles(tq_plan_t pl);
 tq_status plan_set_split_form(tq_plan_t pl, bool in_place);
 // Analytic configuration choice for one tile class.
 int tq_plan_pick_config(tq_plan_t pl, int cls);
+// Whether configuration cfg can run this plan: wave tiles address operands
+// with 32-bit offsets; FP16 data runs the workgroup-tile kernel with
+// 16-value k steps only.
+inline bool plan_config_ok(const tq_plan_s &pl, int cfg) {
+  const GemmConfig g = tq_gemm_config(pl.prec, cfg);
+  if (pl.dtype == TQ_F16) return !g.wave_tile && g.bk % 16 == 0;
+  return !(g.wave_tile && !pl.wave_ok());
+}
 // 0, 1: MFMA tile class of a pair; -1: streaming.
 inline int pair_class(const tq_plan_s &pl, int64_t m, int64_t n, int64_t k) {
   if (k <= pl.stream_max_k) return -1;

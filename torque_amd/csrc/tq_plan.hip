@@ -802,7 +802,7 @@ static tq_status build_split_tiles(tq_plan_t pl, int cls, const GemmConfig &g,
                                     (int32_t)mn, (int32_t)ns, 0});
     ws_off += ns * mn;
   }
-  const size_t es = pl->dtype == TQ_F32 ? 4 : 8;
+  const size_t es = tq_elem_size(pl->dtype);
   TQ_HIP(h, tc.vdesc.alloc(sizeof(GemmDesc) * std::max<size_t>(vd.size(), 1)));
   TQ_HIP(h, tc.tile_pair.alloc(sizeof(int32_t) * std::max<size_t>(tiles.size(), 1)));
   TQ_HIP(h, tc.tile_first.alloc(sizeof(int32_t) * std::max<size_t>(first.size(), 1)));
@@ -1008,8 +1008,7 @@ static tq_status build_pack(tq_plan_t pl, bool is_b, int mode, bool split,
     TQ_HIP(h, hipMemsetAsync(side.panel.p, 0, bytes, h->stream));
     TQ_HIP(h, hipStreamSynchronize(h->stream));
   } else {
-    const size_t es = pl->dtype == TQ_F64 ? 8 : 4;
-    TQ_HIP(h, side.panel.alloc(es * std::max<int64_t>(sizes_h[0], 1)));
+    TQ_HIP(h, side.panel.alloc(tq_elem_size(pl->dtype) * std::max<int64_t>(sizes_h[0], 1)));
   }
   return TQ_SUCCESS;
 }
@@ -1179,7 +1178,7 @@ static tq_status merge_groups(tq_plan_t pl, const std::vector<GemmDesc> &hd,
   pl->h_out_end.swap(oe);
   pl->h_c_off.swap(coff);
   pl->out_n_elem = c;
-  pl->bytes = (pl->dtype == TQ_F64 ? 8.0 : 4.0) * (operand_elems + (double)c);
+  pl->bytes = (double)tq_elem_size(pl->dtype) * (operand_elems + (double)c);
   TQ_HIP(h, hipMemcpyAsync(pl->desc.p, gd.data(), sizeof(GemmDesc) * ng, hipMemcpyHostToDevice,
                            h->stream));
   DevBuf dcost;
@@ -1216,7 +1215,7 @@ static tq_status plan_contract(tq_handle_t h, const tq_bs_desc *a, const tq_bs_d
   *plan_out = nullptr;
   if (!valid_desc(a) || !valid_desc(b))
     return tq_fail(h, TQ_ERROR_INVALID_VALUE, "contract: invalid tensor metadata");
-  if (dtype != TQ_F32 && dtype != TQ_F64) return TQ_ERROR_UNSUPPORTED;
+  if (!tq_valid_dtype(dtype)) return TQ_ERROR_UNSUPPORTED;
   if (precision != TQ_PREC_NATIVE && dtype != TQ_F64)
     return tq_fail(h, TQ_ERROR_UNSUPPORTED,
                    "contract: reduced-precision tiers take FP64 data");
@@ -1461,7 +1460,7 @@ static tq_status plan_contract(tq_handle_t h, const tq_bs_desc *a, const tq_bs_d
       nb_elem += n;
     }
     operand_elems = na_elem + nb_elem;
-    pl->bytes = (dtype == TQ_F64 ? 8.0 : 4.0) * (na_elem + nb_elem + (double)pl->out_n_elem);
+    pl->bytes = (double)tq_elem_size(dtype) * (na_elem + nb_elem + (double)pl->out_n_elem);
   }
 
   // ---- 4. operand layouts and pack jobs ----

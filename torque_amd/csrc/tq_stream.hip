@@ -200,6 +200,9 @@ tq_status tq_launch_small_k(tq_handle_t h, tq_dtype dt, tq_precision prec,
   if (dt == TQ_F32)
     return launch<float, float, float>(h, descs, chunk_pair, chunk_first, n_chunks, a,
                                        a_packed, b, b_packed, c);
+  if (dt == TQ_F16)
+    return launch<_Float16, float, _Float16>(h, descs, chunk_pair, chunk_first, n_chunks, a,
+                                             a_packed, b, b_packed, c);
   if (prec == TQ_PREC_F32)
     return launch<double, float, double>(h, descs, chunk_pair, chunk_first, n_chunks, a,
                                          a_packed, b, b_packed, c);

@@ -8,15 +8,20 @@
 // pair, A block major, B block ascending, column-major [A free, B free]).
 // Differences a caller sees: contract takes a tq_handle_t where the
 // reference takes a cublasHandle_t; device failures throw Error instead of
-// exiting; T is float or double (the reference also instantiates half).
+// exiting; T is float, double or torque::half (IEEE binary16 bits, the
+// reference's half).
 #pragma once
 
 #include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
 #include <type_traits>
 #include <utility>
 #include <vector>
 
 #include "torque/error.h"
+#include "torque/half.h"
 #include "torque/index.h"
 
 namespace torque {
@@ -26,9 +31,12 @@ enum MemcpyKind { HostToDevice = 0, DeviceToHost = 1, DeviceToDevice = 2 };
 
 template <typename T>
 constexpr tq_dtype dtype_of() {
-  static_assert(std::is_same<T, float>::value || std::is_same<T, double>::value,
-                "torque::gpu tensors hold float or double");
-  return std::is_same<T, double>::value ? TQ_F64 : TQ_F32;
+  static_assert(std::is_same<T, float>::value || std::is_same<T, double>::value ||
+                    std::is_same<T, half>::value,
+                "torque::gpu tensors hold float, double or torque::half");
+  return std::is_same<T, double>::value ? TQ_F64
+         : std::is_same<T, half>::value ? TQ_F16
+                                        : TQ_F32;
 }
 
 template <typename T>
@@ -417,6 +425,63 @@ class BlockSparseTensor {
     return a;
   }
 };
+
+static_assert(sizeof(uword) == 8, "block-tensor files hold 64-bit metadata");
+
+// Binary block-tensor file, the format torque_amd.BlockSparseTensor.save
+// writes: "TQBS", little-endian u32 version (1), u32 dtype (tq_dtype), u32
+// rank, u64 n_blocks, u64 dims[rank], u64 begin, end and index tables
+// [n_blocks][rank] (block-major), u64 n_elem, then the n_elem values.
+template <typename T>
+void save(const BlockSparseTensor<T> &t, const std::string &path) {
+  uword n = 0;
+  for (uword j = 0; j < t.n_blocks(); ++j) n += t.block_n_elem[j];
+  n = std::max<uword>(n, 1);
+  std::vector<T> host(n);
+  check(tq_memcpy(default_handle(), host.data(), t.device_data(), sizeof(T) * n, DeviceToHost),
+        default_handle());
+  std::FILE *f = std::fopen(path.c_str(), "wb");
+  if (!f) throw Error("cannot write " + path);
+  const uint32_t head[3] = {1, (uint32_t)dtype_of<T>(), (uint32_t)t.rank};
+  const uint64_t nb = t.n_blocks(), ne = n;
+  bool ok = std::fwrite("TQBS", 1, 4, f) == 4 && std::fwrite(head, 4, 3, f) == 3 &&
+            std::fwrite(&nb, 8, 1, f) == 1 &&
+            std::fwrite(t.dimension.data(), 8, t.rank, f) == t.rank;
+  for (const umat *m : {&t.begin_points, &t.end_points, &t.index_tables})
+    ok = ok && std::fwrite(m->data(), 8, nb * t.rank, f) == nb * t.rank;
+  ok = ok && std::fwrite(&ne, 8, 1, f) == 1 && std::fwrite(host.data(), sizeof(T), n, f) == n;
+  std::fclose(f);
+  if (!ok) throw Error("cannot write " + path);
+}
+
+template <typename T>
+BlockSparseTensor<T> load_block_sparse(const std::string &path) {
+  std::FILE *f = std::fopen(path.c_str(), "rb");
+  if (!f) throw Error("cannot read " + path);
+  char magic[4] = {0, 0, 0, 0};
+  uint32_t head[3] = {0, 0, 0};
+  uint64_t nb = 0, ne = 0;
+  bool ok = std::fread(magic, 1, 4, f) == 4 && std::memcmp(magic, "TQBS", 4) == 0 &&
+            std::fread(head, 4, 3, f) == 3 && head[0] == 1 &&
+            head[1] == (uint32_t)dtype_of<T>() && std::fread(&nb, 8, 1, f) == 1;
+  const uword rank = ok ? head[2] : 0;
+  uvec dims(rank);
+  umat b(rank, nb), e(rank, nb), tb(rank, nb);
+  ok = ok && std::fread(dims.data(), 8, rank, f) == rank;
+  for (umat *m : {&b, &e, &tb})
+    ok = ok && std::fread(m->data(), 8, nb * rank, f) == nb * rank;
+  ok = ok && std::fread(&ne, 8, 1, f) == 1 && ne >= 1 && ne < (uint64_t(1) << 40);
+  std::vector<T> host(ok ? ne : 0);
+  ok = ok && std::fread(host.data(), sizeof(T), ne, f) == ne;
+  std::fclose(f);
+  if (!ok) throw Error("not a block-tensor file of this type: " + path);
+  if (nb == 0) {
+    BlockSparseTensor<T> t(dims);
+    t.modify(uvec(rank, 0), host[0]);
+    return t;
+  }
+  return BlockSparseTensor<T>(host.data(), b, e, dims, tb, HostToDevice);
+}
 
 }  // namespace gpu
 }  // namespace torque

@@ -81,6 +81,7 @@ class umat {
     ++c_;
   }
   const uword *data() const { return v_.data(); }
+  uword *data() { return v_.data(); }
 
  private:
   std::size_t r_ = 0, c_ = 0;

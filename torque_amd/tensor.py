@@ -20,7 +20,9 @@ import torch
 from . import _lib
 from ._lib import BsDesc, PlanInfo, TorqueError, check, lib
 
-_DT = {torch.float64: _lib.TQ_F64, torch.float32: _lib.TQ_F32}
+_DT = {torch.float64: _lib.TQ_F64, torch.float32: _lib.TQ_F32, torch.float16: _lib.TQ_F16}
+_DT_OF = {v: k for k, v in _DT.items()}
+_DUMP_MAGIC = b"TQBS"
 
 
 def _i64p(a):
@@ -207,6 +209,50 @@ class BlockSparseTensor:
                               % (src.numel(), n))
         self.data = src[:max(n, 1)].to(device=self.device, dtype=dtype).clone()
         self._meta_changed()
+
+    # -- binary block-tensor file (fixtures, cross-run comparison) --
+    def save(self, path):
+        """Writes the tensor as a binary block-tensor file: "TQBS", then
+        little-endian u32 version (1), u32 dtype (0 FP32, 1 FP64, 2 FP16), u32
+        rank, u64 n_blocks, u64 dims[rank], u64 begin, end and index tables
+        [n_blocks][rank] (block-major), u64 n_elem, the n_elem data values.
+        The C++ classes read and write the same format."""
+        data = self.data.detach().to("cpu").contiguous()
+        with open(path, "wb") as f:
+            f.write(_DUMP_MAGIC)
+            np.array([1, _DT[self.dtype], self.rank], "<u4").tofile(f)
+            np.array([self.n_blocks], "<u8").tofile(f)
+            for a in (self.dimension, self.begin_points, self.end_points, self.index_tables):
+                np.ascontiguousarray(a, "<u8").tofile(f)
+            np.array([data.numel()], "<u8").tofile(f)
+            f.write(data.numpy().astype(data.numpy().dtype.newbyteorder("<")).tobytes())
+
+    @classmethod
+    def load(cls, path, device=None):
+        """Reads a binary block-tensor file written by save (or by the C++
+        classes) onto `device`."""
+        with open(path, "rb") as f:
+            if f.read(4) != _DUMP_MAGIC:
+                raise TorqueError(_lib.TQ_ERROR_INVALID_VALUE, "not a block-tensor file")
+            version, code, rank = (int(x) for x in np.fromfile(f, "<u4", 3))
+            if version != 1 or code not in _DT_OF:
+                raise TorqueError(_lib.TQ_ERROR_UNSUPPORTED, "block-tensor file version or dtype")
+            nb = int(np.fromfile(f, "<u8", 1)[0])
+            dims = np.fromfile(f, "<u8", rank).astype(np.int64)
+            meta = [np.fromfile(f, "<u8", nb * rank).astype(np.int64).reshape(nb, rank)
+                    for _ in range(3)]
+            n = int(np.fromfile(f, "<u8", 1)[0])
+            dtype = _DT_OF[code]
+            np_dtype = {torch.float64: "<f8", torch.float32: "<f4", torch.float16: "<f2"}[dtype]
+            data = np.fromfile(f, np_dtype, n)
+            if data.size != n:
+                raise TorqueError(_lib.TQ_ERROR_INVALID_VALUE, "truncated block-tensor file")
+        if nb == 0:
+            t = cls(dims, dtype=dtype, device=device)
+            t.data = torch.from_numpy(data.copy()).to(t.device)
+            return t
+        return cls(dims, torch.from_numpy(data.copy()), meta[0], meta[1], meta[2], dtype=dtype,
+                   device=device)
 
     # -- derived metadata (src/gpu/block_sparse.cuh:199-212) --
     def _meta_changed(self):
