@@ -296,6 +296,16 @@ tq_status tq_plan_autotune(tq_plan_t plan, const void *a, const void *b,
  * counterpart. */
 tq_status tq_plan_set_split_operands(tq_plan_t plan, int32_t form);
 
+/* Input-domain check of the split-BF16 tier: *out_of_range = 1 when an FP64
+ * operand value split by this plan's executes since the last check (or since
+ * planning) lay outside the domain in which hi + mid + lo represents it to
+ * 2^-26: |x| >= 2^127 (hi overflows), 0 < |x| < 2^-100 (lo loses bits),
+ * NaN or infinity (they turn into NaN; the FP64 tier keeps infinities).
+ * The pack and GEMM kernels set a device flag as they split; this call
+ * reads and clears it (synchronises the handle's stream). Always 0 for the
+ * other tiers. No reference counterpart. */
+tq_status tq_plan_split_range_check(tq_plan_t plan, int32_t *out_of_range);
+
 /* ---- batches and graphs (no reference counterpart) ------------------- */
 
 /* Joins n plans into one batch plan: every pair of every plan goes into one

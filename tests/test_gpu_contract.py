@@ -611,3 +611,40 @@ def test_split_k_cost_model_choice(cuda):
     plan.execute(ga.data, gb.data, out)
     torch.cuda.synchronize()
     assert rel_fro(out.cpu().numpy(), ref_c) <= FP64_TOL
+
+
+@pytest.mark.parametrize("form", ["panels", "in_place"])
+def test_split_domain_flag(cuda, form):
+    # values outside the split-BF16 domain (|x| >= 2^127, 0 < |x| < 2^-100,
+    # NaN) raise the plan's flag in the pack, the GEMM's in-place loads and
+    # the small-K stream; the check clears it; zeros and ordinary values do
+    # not raise it; other tiers never do
+    rng = np.random.default_rng(81)
+    a, b, legs = thin_pairs(rng, [(40, 48, 36), (20, 4, 30)])  # an MFMA and a streamed pair
+    (A, da), (B, db) = from_workload(a), from_workload(b)
+    ga, gb = torch_tensor(A, da, cuda), torch_tensor(B, db, cuda)
+    plan = ga.plan_contract(gb, legs, "bf16x3")
+    plan.set_split_operands(form)
+    out = torch.empty(plan.out_n_elem, dtype=torch.float64, device=cuda)
+
+    def run(tensor, index, value):
+        saved = float(tensor.data[index])
+        tensor.data[index] = value
+        plan.execute(ga.data, gb.data, out)
+        torch.cuda.synchronize()
+        tensor.data[index] = saved
+        return plan.split_range_check()
+
+    plan.execute(ga.data, gb.data, out)
+    assert not plan.split_range_check()
+    n_mfma_a = 40 * 48
+    for tensor, index in ((ga, 5), (gb, 40), (ga, n_mfma_a + 3)):  # MFMA A, MFMA B, streamed A
+        for value in (1e300, -2.0 ** 127, 1e-320, 2.0 ** -101, float("nan"), float("inf")):
+            assert run(tensor, index, value), (tensor is ga, index, value)
+            assert not plan.split_range_check()  # cleared
+        for value in (0.0, 2.0 ** 126, 2.0 ** -99, -1e-20):
+            assert not run(tensor, index, value), (index, value)
+    p64 = ga.plan_contract(gb, legs)
+    ga.data[5] = 1e300
+    p64.execute(ga.data, gb.data, out)
+    assert not p64.split_range_check()

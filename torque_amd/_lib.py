@@ -126,6 +126,7 @@ def _proto(lib):
         "tq_graph_destroy": (S, [V]),
         "tq_plan_set_small_k": (S, [V, ctypes.c_int32]),
         "tq_plan_set_split_operands": (S, [V, ctypes.c_int32]),
+        "tq_plan_split_range_check": (S, [V, c_i32_p]),
         "tq_plan_autotune": (S, [V, V, V, V, ctypes.c_int32, c_i32_p,
                                  ctypes.POINTER(ctypes.c_float)]),
         "tq_plan_destroy": (S, [V]),

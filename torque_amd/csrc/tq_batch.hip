@@ -120,6 +120,10 @@ tq_status tq_batch_create(tq_handle_t h, int32_t n, const tq_plan_t *plans,
   pl->ro = plans[0]->ro;
   pl->stream_max_k = plans[0]->stream_max_k;
   pl->split_direct = plans[0]->split_direct;
+  if (pl->prec == TQ_PREC_BF16X3) {
+    TQ_HIP(h, pl->range_flag.alloc(sizeof(int32_t)));
+    TQ_HIP(h, hipMemsetAsync(pl->range_flag.p, 0, sizeof(int32_t), h->stream));
+  }
   std::vector<int64_t> pa_base, pb_base;
   TQ_CHECK(merge_side(h, *pl, false, n, plans, a_offset, pa_base));
   TQ_CHECK(merge_side(h, *pl, true, n, plans, b_offset, pb_base));

@@ -141,10 +141,12 @@ __host__ __device__ inline int64_t pack_job_wgs(const CopyJob &j);
 
 // Split-BF16 pack: like tq_launch_gather_jobs, reading FP64 and writing the
 // three BF16 pieces of each value to planes dst, dst + plane, dst + 2*plane.
+// range_flag (may be null) is or-ed with 1 when a value lies outside the
+// split's domain (tqs::split_out_of_range).
 tq_status tq_launch_split_jobs(tq_handle_t h, const CopyJob *jobs,
                                const int32_t *job_wg_start, int64_t n_jobs,
                                int64_t total_wg, const double *src,
-                               uint16_t *dst, int64_t plane);
+                               uint16_t *dst, int64_t plane, int32_t *range_flag);
 // Split-BF16 grouped GEMM. direct = false: a and b are the split panels
 // (planes a_plane / b_plane elements apart); direct = true: a and b are the
 // FP64 operands read in place (the descriptors' offsets, leading dimensions
@@ -152,7 +154,8 @@ tq_status tq_launch_split_jobs(tq_handle_t h, const CopyJob *jobs,
 tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, bool direct, const GemmDesc *descs,
                                 const RunDesc *runs, const int32_t *tile_pair,
                                 const int32_t *tile_first, int64_t n_tiles, const void *a,
-                                const void *b, int64_t a_plane, int64_t b_plane, double *c);
+                                const void *b, int64_t a_plane, int64_t b_plane, double *c,
+                                int32_t *range_flag);
 
 // Runs a list of gather jobs that already sit in device memory. job_wg_start
 // is the exclusive prefix sum of workgroups per job (n_jobs + 1 entries).
@@ -231,7 +234,7 @@ tq_status tq_launch_small_k(tq_handle_t h, tq_dtype dt, tq_precision prec,
                             const int32_t *chunk_pair, const int32_t *chunk_first,
                             int64_t n_chunks, const void *a, const void *a_packed,
                             const void *b, const void *b_packed, void *c, int64_t a_plane,
-                            int64_t b_plane);
+                            int64_t b_plane, int32_t *range_flag);
 
 // The wave-tile kernel addresses operands with 32-bit byte offsets from a
 // per-tile base; it runs only when every leading dimension is below this.

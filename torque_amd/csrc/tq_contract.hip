@@ -136,7 +136,7 @@ tq_status launch_class(tq_plan_t pl, int cls, const void *a, const void *b, void
                                    tc.n_tiles, pl->split_direct ? a : pl->pack_a.panel.p,
                                    pl->split_direct ? b : pl->pack_b.panel.p,
                                    pl->pack_a.panel_elems, pl->pack_b.panel_elems,
-                                   reinterpret_cast<double *>(out)));
+                                   reinterpret_cast<double *>(out), pl->range_flag.as<int32_t>()));
   else
     TQ_CHECK(tq_launch_grouped_gemm(h, pl->dtype, pl->prec, tc.tiles_cfg, descs, runs,
                                     tc.tile_pair.as<int32_t>(), tc.tile_first.as<int32_t>(),
@@ -183,6 +183,21 @@ tq_status tq_plan_set_class_config(tq_plan_t pl, int32_t cls, int32_t config) {
   return TQ_SUCCESS;
 }
 
+tq_status tq_plan_split_range_check(tq_plan_t pl, int32_t *out_of_range) {
+  if (!pl) return TQ_ERROR_NOT_INITIALIZED;
+  if (!out_of_range) return TQ_ERROR_INVALID_VALUE;
+  *out_of_range = 0;
+  if (pl->prec != TQ_PREC_BF16X3 || !pl->range_flag.p) return TQ_SUCCESS;
+  tq_handle_t h = pl->h;
+  int32_t v = 0;
+  TQ_HIP(h, hipMemcpyAsync(&v, pl->range_flag.p, sizeof(int32_t), hipMemcpyDeviceToHost,
+                           h->stream));
+  TQ_HIP(h, hipMemsetAsync(pl->range_flag.p, 0, sizeof(int32_t), h->stream));
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  *out_of_range = v != 0;
+  return TQ_SUCCESS;
+}
+
 tq_status tq_plan_set_class_split(tq_plan_t pl, int32_t cls, int32_t split) {
   if (!pl) return TQ_ERROR_NOT_INITIALIZED;
   if (cls < 0 || cls > 1 || split < 1 || split > kMaxSplit)
@@ -220,7 +235,8 @@ tq_status tq_bs_contract_pack(tq_plan_t pl, const void *a, const void *b) {
       TQ_CHECK(tq_launch_split_jobs(h, sd.jobs.as<CopyJob>(), sd.wg32.as<int32_t>(),
                                     sd.n_jobs, sd.total_wg,
                                     reinterpret_cast<const double *>(src[q]),
-                                    sd.panel.as<uint16_t>(), sd.panel_elems));
+                                    sd.panel.as<uint16_t>(), sd.panel_elems,
+                                    pl->range_flag.as<int32_t>()));
     else
       TQ_CHECK(tq_launch_gather_jobs(h, pl->dtype, sd.jobs.as<CopyJob>(),
                                      sd.wg32.as<int32_t>(), sd.n_jobs, sd.total_wg,
@@ -248,7 +264,8 @@ tq_status tq_bs_contract_multiply(tq_plan_t pl, const void *a, const void *b,
                                  pl->chunk_pair.as<int32_t>(),
                                  pl->chunk_first.as<int32_t>(), pl->n_chunks, a,
                                  pl->pack_a.panel.p, b, pl->pack_b.panel.p, c,
-                                 pl->pack_a.panel_elems, pl->pack_b.panel_elems));
+                                 pl->pack_a.panel_elems, pl->pack_b.panel_elems,
+                                 pl->range_flag.as<int32_t>()));
     // large tiles first: the small-tile launch then fills the tail
     TQ_CHECK(launch_class(pl, 1, a, b, c));
     return launch_class(pl, 0, a, b, c);

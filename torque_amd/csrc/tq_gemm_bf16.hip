@@ -71,6 +71,7 @@ struct StepLoader<ROWS, THREADS, false> {
   static_assert(kN <= 32, "one mask bit per vector");
   u32x4 v[kN];
   uint32_t ok;  // bit q: v[q] is in range (otherwise it is stored as zeros)
+  int32_t *flag = nullptr;  // unused: the split pack checks the values
 
   // Rows past R are read from row R - 1 (their products only reach
   // unstored output); chunks past the padded K are zeros. Branch-free and
@@ -120,6 +121,7 @@ struct StepLoader<ROWS, THREADS, true> {
   static constexpr int kN = (kCells + THREADS - 1) / THREADS;
   double v[kN][8];
   int kleft;  // K minus the step's first k
+  int32_t *flag = nullptr;  // or-ed with 1 when a value is outside the split's domain
 
   __device__ inline void load(const OpRef &o, int64_t /*plane*/, int K, int step) {
     const double *base = static_cast<const double *>(o.p);
@@ -164,6 +166,7 @@ struct StepLoader<ROWS, THREADS, true> {
   }
 
   __device__ inline void store(uint4 *lds) const {
+    bool bad = false;
 #pragma unroll
     for (int q = 0; q < kN; ++q) {
       const int e = threadIdx.x + q * THREADS;
@@ -173,9 +176,11 @@ struct StepLoader<ROWS, THREADS, true> {
       u32x4 w[3];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
+        const double x0 = 2 * j < cnt ? v[q][2 * j] : 0.0;
+        const double x1 = 2 * j + 1 < cnt ? v[q][2 * j + 1] : 0.0;
+        bad = bad || tqs::split_out_of_range(x0) || tqs::split_out_of_range(x1);
         uint32_t h, m, l;
-        tqs::split3x2(2 * j < cnt ? v[q][2 * j] : 0.0, 2 * j + 1 < cnt ? v[q][2 * j + 1] : 0.0,
-                      h, m, l);
+        tqs::split3x2(x0, x1, h, m, l);
         w[0][j] = h;
         w[1][j] = m;
         w[2][j] = l;
@@ -184,6 +189,7 @@ struct StepLoader<ROWS, THREADS, true> {
       for (int p = 0; p < 3; ++p)
         reinterpret_cast<u32x4 *>(lds)[(p * kChunks + c) * ROWS + r] = w[p];
     }
+    if (bad && flag) atomicOr(flag, 1);
   }
 };
 
@@ -293,7 +299,8 @@ __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
     const GemmDesc *__restrict__ descs, const RunDesc *__restrict__ runs,
     const int32_t *__restrict__ tile_pair, const int32_t *__restrict__ tile_first,
     int64_t n_tiles, const void *__restrict__ a, const void *__restrict__ b,
-    int64_t a_plane, int64_t b_plane, double *__restrict__ c) {
+    int64_t a_plane, int64_t b_plane, double *__restrict__ c,
+    int32_t *__restrict__ range_flag) {
   constexpr int WAVES_N = 4 / WAVES_M;
   constexpr int kT = kThreads;
   constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
@@ -313,6 +320,7 @@ __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
   TileRef cur = fetch<BM, BN, kT, DIRECT>(t, descs, runs, tile_pair, tile_first, a, b);
   StepLoader<BM, kT, DIRECT> la;
   StepLoader<BN, kT, DIRECT> lb;
+  la.flag = lb.flag = range_flag;
   la.load(cur.a, a_plane, cur.d.k, 0);
   lb.load(cur.b, b_plane, cur.d.k, 0);
   while (true) {
@@ -365,7 +373,8 @@ __global__ __launch_bounds__(NW * 64, 1) void bf16x3_db_kernel(
     const GemmDesc *__restrict__ descs, const RunDesc *__restrict__ runs,
     const int32_t *__restrict__ tile_pair, const int32_t *__restrict__ tile_first,
     int64_t n_tiles, const void *__restrict__ a, const void *__restrict__ b,
-    int64_t a_plane, int64_t b_plane, double *__restrict__ c) {
+    int64_t a_plane, int64_t b_plane, double *__restrict__ c,
+    int32_t *__restrict__ range_flag) {
   constexpr int WAVES_N = NW / WAVES_M;
   constexpr int kT = NW * 64;
   constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
@@ -390,6 +399,7 @@ __global__ __launch_bounds__(NW * 64, 1) void bf16x3_db_kernel(
   int ls = 0;
   StepLoader<BM, kT, DIRECT> la;
   StepLoader<BN, kT, DIRECT> lb;
+  la.flag = lb.flag = range_flag;
   auto load = [&]() {
     la.load(lref.a, a_plane, lref.d.k, ls);
     lb.load(lref.b, b_plane, lref.d.k, ls);
@@ -455,7 +465,8 @@ __global__ __launch_bounds__(NW * 64, 1) void bf16x3_db2_kernel(
     const GemmDesc *__restrict__ descs, const RunDesc *__restrict__ runs,
     const int32_t *__restrict__ tile_pair, const int32_t *__restrict__ tile_first,
     int64_t n_tiles, const void *__restrict__ a, const void *__restrict__ b,
-    int64_t a_plane, int64_t b_plane, double *__restrict__ c) {
+    int64_t a_plane, int64_t b_plane, double *__restrict__ c,
+    int32_t *__restrict__ range_flag) {
   constexpr int WAVES_N = NW / WAVES_M;
   constexpr int kT = NW * 64;
   constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
@@ -557,12 +568,14 @@ constexpr Cfg kCfgs[] = {{64, 64, 1},   {128, 64, 1},  {64, 128, 1},  {128, 128,
 constexpr int kNCfg = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
 typedef void (*KernelFn)(const GemmDesc *, const RunDesc *, const int32_t *, const int32_t *,
-                         int64_t, const void *, const void *, int64_t, int64_t, double *);
+                         int64_t, const void *, const void *, int64_t, int64_t, double *,
+                         int32_t *);
 
 template <int BM, int BN, int WAVES_M, int MINW, int DB, bool DIRECT>
 tq_status launch(tq_handle_t h, const GemmDesc *descs, const RunDesc *runs,
                  const int32_t *tile_pair, const int32_t *tile_first, int64_t n_tiles,
-                 const void *a, const void *b, int64_t ap, int64_t bp, double *c) {
+                 const void *a, const void *b, int64_t ap, int64_t bp, double *c,
+                 int32_t *range_flag) {
   KernelFn kernel;
   const int threads = DB == 2 ? 512 : kThreads;
   if constexpr (DB == 2 && DIRECT)
@@ -587,7 +600,7 @@ tq_status launch(tq_handle_t h, const GemmDesc *descs, const RunDesc *runs,
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(threads), 0, h->stream, descs, runs,
-                     tile_pair, tile_first, n_tiles, a, b, ap, bp, c);
+                     tile_pair, tile_first, n_tiles, a, b, ap, bp, c, range_flag);
   TQ_HIP(h, hipGetLastError());
   return TQ_SUCCESS;
 }
@@ -595,12 +608,13 @@ tq_status launch(tq_handle_t h, const GemmDesc *descs, const RunDesc *runs,
 template <bool DIRECT>
 tq_status dispatch(tq_handle_t h, int cfg, const GemmDesc *descs, const RunDesc *runs,
                    const int32_t *tile_pair, const int32_t *tile_first, int64_t n_tiles,
-                   const void *a, const void *b, int64_t a_plane, int64_t b_plane, double *c) {
+                   const void *a, const void *b, int64_t a_plane, int64_t b_plane, double *c,
+                   int32_t *range_flag) {
   switch (cfg) {
 #define TQ_CFG(I, BM, BN, WM, MINW, DB)                                                       \
   case I:                                                                                     \
     return launch<BM, BN, WM, MINW, DB, DIRECT>(h, descs, runs, tile_pair, tile_first, n_tiles, \
-                                                a, b, a_plane, b_plane, c);
+                                                a, b, a_plane, b_plane, c, range_flag);
     TQ_CFG(0, 64, 64, 2, 1, 0)
     TQ_CFG(1, 128, 64, 2, 1, 0)
     TQ_CFG(2, 64, 128, 2, 1, 0)
@@ -631,11 +645,12 @@ GemmConfig tq_bf16x3_config(int cfg) {
 tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, bool direct, const GemmDesc *descs,
                                 const RunDesc *runs, const int32_t *tile_pair,
                                 const int32_t *tile_first, int64_t n_tiles, const void *a,
-                                const void *b, int64_t a_plane, int64_t b_plane, double *c) {
+                                const void *b, int64_t a_plane, int64_t b_plane, double *c,
+                                int32_t *range_flag) {
   if (n_tiles <= 0) return TQ_SUCCESS;
   if (direct)
     return dispatch<true>(h, cfg, descs, runs, tile_pair, tile_first, n_tiles, a, b, a_plane,
-                          b_plane, c);
+                          b_plane, c, range_flag);
   return dispatch<false>(h, cfg, descs, runs, tile_pair, tile_first, n_tiles, a, b, a_plane,
-                         b_plane, c);
+                         b_plane, c, range_flag);
 }

@@ -468,7 +468,8 @@ __device__ inline i64 split_row_offset(const CopyJob &job, uint32_t row) {
 // bytes per plane (a 32-bit word of two BF16 pieces per lane).
 __device__ inline void split_kfast_tile(const CopyJob &job, int64_t rows, int64_t row0,
                                         int64_t chunk, const double *__restrict__ src,
-                                        uint16_t *__restrict__ dst, int64_t plane) {
+                                        uint16_t *__restrict__ dst, int64_t plane,
+                                        bool &bad) {
   if (chunk * 8 >= job.kpad) return;
   const int lane = threadIdx.x & 63;
   const int nk = job.nk;
@@ -522,6 +523,7 @@ __device__ inline void split_kfast_tile(const CopyJob &job, int64_t rows, int64_
     const int64_t row = row0 + rg * 16 + (lane >> 2);
     if (row >= rows) continue;
     uint32_t wh, wm, wl;
+    bad = bad || tqs::split_out_of_range(xv[rg][0]) || tqs::split_out_of_range(xv[rg][1]);
     tqs::split3x2(xv[rg][0], xv[rg][1], wh, wm, wl);
     const int64_t o = job.dst_off + chunk * rows * 8 + row * 8 + 2 * (lane & 3);
     *reinterpret_cast<uint32_t *>(dst + o) = wh;
@@ -554,10 +556,11 @@ __device__ inline int32_t xcd_local_id() {
 __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
     const CopyJob *__restrict__ jobs, const int32_t *__restrict__ job_wg_start,
     int64_t n_jobs, const double *__restrict__ src, uint16_t *__restrict__ dst,
-    int64_t plane) {
+    int64_t plane, int32_t *__restrict__ range_flag) {
   constexpr int kWaves = kGatherThreads / 64;
   const int32_t total = job_wg_start[n_jobs];
   const int lane = threadIdx.x & 63;
+  bool bad = false;  // a value outside the split's domain (flagged at the end)
   const int32_t wave = __builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
   for (int32_t c = xcd_local_id() * kWaves + wave; c < total;
        c += (int32_t)gridDim.x * kWaves) {
@@ -575,7 +578,7 @@ __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
       const uint32_t chunks = (uint32_t)(job.kpad >> 3);
       const uint32_t band = tile / chunks;
       split_kfast_tile(job, rows, (int64_t)band * kSplitRows, tile - band * chunks, src, dst,
-                       plane);
+                       plane, bad);
       continue;
     }
     const Div32 rdiv(rows);
@@ -620,6 +623,8 @@ __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
 #pragma unroll
     for (int e = 0; e < 8; ++e) xv[e] = ((int64_t)chunk * 8 + e < kd) ? src[offs[e]] : 0.0;
     const int64_t o = job.dst_off + ((int64_t)chunk * rows + row) * 8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) bad = bad || tqs::split_out_of_range(xv[e]);
     uint4 vh, vm, vw;
     tqs::split3x2(xv[0], xv[1], vh.x, vm.x, vw.x);
     tqs::split3x2(xv[2], xv[3], vh.y, vm.y, vw.y);
@@ -629,6 +634,7 @@ __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
     *reinterpret_cast<uint4 *>(dst + plane + o) = vm;
     *reinterpret_cast<uint4 *>(dst + 2 * plane + o) = vw;
   }
+  if (bad && range_flag) atomicOr(range_flag, 1);
 }
 
 }  // namespace
@@ -636,13 +642,13 @@ __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
 tq_status tq_launch_split_jobs(tq_handle_t h, const CopyJob *jobs,
                                const int32_t *job_wg_start, int64_t n_jobs,
                                int64_t total_wg, const double *src,
-                               uint16_t *dst, int64_t plane) {
+                               uint16_t *dst, int64_t plane, int32_t *range_flag) {
   if (n_jobs <= 0) return TQ_SUCCESS;
   // one work item per wavefront, four wavefronts per workgroup
   i64 grid = std::max<i64>(1, std::min<i64>(ceil_div(total_wg, kGatherThreads / 64),
                                             (i64)h->n_cu * 8));
   hipLaunchKernelGGL(split_jobs_kernel, dim3((unsigned)grid), dim3(kGatherThreads), 0,
-                     h->stream, jobs, job_wg_start, n_jobs, src, dst, plane);
+                     h->stream, jobs, job_wg_start, n_jobs, src, dst, plane, range_flag);
   TQ_HIP(h, hipGetLastError());
   return TQ_SUCCESS;
 }

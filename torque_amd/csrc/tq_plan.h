@@ -133,6 +133,9 @@ struct tq_plan_s {
   // split-BF16 tier with both operands plain matrices in place: the GEMM
   // splits FP64 values as it stages them, and no pack runs
   bool split_direct = false;
+  // split-BF16: a device word or-ed with 1 by the pack and GEMM kernels when
+  // an operand value lies outside the split's domain (zero from plan time)
+  DevBuf range_flag;
   // split-BF16 operand form: the plan may read plain operands in place or
   // pack them into split panels (built on first use); split_form is the
   // caller's request (0 automatic: the autotuner times both, in place until

@@ -1266,6 +1266,10 @@ static tq_status plan_contract(tq_handle_t h, const tq_bs_desc *a, const tq_bs_d
   pl->ro = (int)pl->out_dims.size();
   // a rank-0 result (the network scalar) is always reduced in FP64
   if (pl->ro == 0) pl->prec = TQ_PREC_NATIVE;
+  if (pl->prec == TQ_PREC_BF16X3) {
+    TQ_HIP(h, pl->range_flag.alloc(sizeof(int32_t)));
+    TQ_HIP(h, hipMemsetAsync(pl->range_flag.p, 0, sizeof(int32_t), h->stream));
+  }
   const int ra = a->rank, rb = b->rank, ro = pl->ro, nl = n_legs;
   const int64_t na = a->n_blocks, nb = b->n_blocks;
 

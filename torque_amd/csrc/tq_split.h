@@ -38,6 +38,15 @@ __device__ inline void split3x2(double x0, double x1, uint32_t &h, uint32_t &m, 
   l = pack_bf16(g0 - low_value(m), g1 - high_value(m));
 }
 
+// Whether x lies outside the domain in which the split represents it to
+// 2^-26: |x| >= 2^127 (the hi piece leaves the BF16 range), 0 < |x| < 2^-100
+// (the lo piece falls below the normal range and loses bits), NaN or
+// infinity (they split into NaN).
+__device__ inline bool split_out_of_range(double x) {
+  const double ax = __builtin_fabs(x);
+  return !(ax < 0x1p127) || (ax < 0x1p-100 && ax != 0.0);
+}
+
 // The value the split represents, hi + mid + lo summed in FP64 in that order
 // (the operand rounding of the split-BF16 tier).
 __device__ inline double round3(double x) {

@@ -31,8 +31,9 @@ __global__ __launch_bounds__(kThreads) void small_k_kernel(
     const int32_t *__restrict__ seg_first, int64_t n_segs,
     const TIn *__restrict__ a, const TIn *__restrict__ a_packed,
     const TIn *__restrict__ b, const TIn *__restrict__ b_packed,
-    TOut *__restrict__ c) {
+    TOut *__restrict__ c, int32_t *__restrict__ range_flag) {
   const int lane = threadIdx.x & 63;
+  bool bad = false;  // kRound3: a value outside the split's domain
   // the segment index is the same on every lane of the wavefront: say so, so
   // that the descriptor and B reads become scalar loads
   const int wave_id = __builtin_amdgcn_readfirstlane(
@@ -49,11 +50,13 @@ __global__ __launch_bounds__(kThreads) void small_k_kernel(
     const int64_t sak = (d.flags & kAKFast) ? 1 : d.lda;
     const int64_t sbn = (d.flags & kBKFast) ? d.ldb : 1;
     const int64_t sbk = (d.flags & kBKFast) ? 1 : d.ldb;
-    auto in = [](TIn x) -> TC {
-      if constexpr (kRound3)
+    auto in = [&bad](TIn x) -> TC {
+      if constexpr (kRound3) {
+        bad = bad || tqs::split_out_of_range((double)x);
         return (TC)tqs::round3((double)x);
-      else
+      } else {
         return (TC)x;
+      }
     };
     TC av[kStreamMaxK];
 #pragma unroll
@@ -69,6 +72,7 @@ __global__ __launch_bounds__(kThreads) void small_k_kernel(
       if (live) pc[(int64_t)d.m * n] = (TOut)acc;
     }
   }
+  if (kRound3 && bad && range_flag) atomicOr(range_flag, 1);
 }
 
 __device__ inline double bf16_bits_value(uint32_t h) {
@@ -143,7 +147,8 @@ __global__ __launch_bounds__(kThreads) void small_k_split_kernel(
 template <class TIn, class TC, class TOut, bool kRound3 = false>
 tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *chunk_pair,
                  const int32_t *chunk_first, int64_t n_chunks, const void *a,
-                 const void *ap, const void *b, const void *bp, void *c) {
+                 const void *ap, const void *b, const void *bp, void *c,
+                 int32_t *range_flag = nullptr) {
   static int per_cu = 0;
   if (per_cu == 0) {
     int occ = 0;
@@ -160,7 +165,7 @@ tq_status launch(tq_handle_t h, const GemmDesc *descs, const int32_t *chunk_pair
                      0, h->stream, descs, chunk_pair, chunk_first, n_chunks,
                      reinterpret_cast<const TIn *>(a), reinterpret_cast<const TIn *>(ap),
                      reinterpret_cast<const TIn *>(b), reinterpret_cast<const TIn *>(bp),
-                     reinterpret_cast<TOut *>(c));
+                     reinterpret_cast<TOut *>(c), range_flag);
   TQ_HIP(h, hipGetLastError());
   return TQ_SUCCESS;
 }
@@ -172,11 +177,11 @@ tq_status tq_launch_small_k(tq_handle_t h, tq_dtype dt, tq_precision prec,
                             const int32_t *chunk_pair, const int32_t *chunk_first,
                             int64_t n_chunks, const void *a, const void *a_packed,
                             const void *b, const void *b_packed, void *c, int64_t a_plane,
-                            int64_t b_plane) {
+                            int64_t b_plane, int32_t *range_flag) {
   if (n_chunks <= 0) return TQ_SUCCESS;
   if (prec == TQ_PREC_BF16X3 && split_direct)
     return launch<double, double, double, true>(h, descs, chunk_pair, chunk_first, n_chunks,
-                                                a, a_packed, b, b_packed, c);
+                                                a, a_packed, b, b_packed, c, range_flag);
   if (prec == TQ_PREC_BF16X3) {
     static int per_cu = 0;
     if (per_cu == 0) {

@@ -565,6 +565,13 @@ class ContractionPlan:
         self.h.check(lib().tq_plan_set_split_operands(self.ptr, code))
         self._refresh_layouts()
 
+    def split_range_check(self):
+        """True when a split-BF16 execute since the last check split an
+        operand value outside the tier's domain (tq_plan_split_range_check)."""
+        v = (ctypes.c_int32 * 1)()
+        self.h.check(lib().tq_plan_split_range_check(self.ptr, v))
+        return bool(v[0])
+
     def _refresh_layouts(self):
         info = PlanInfo()
         self.h.check(lib().tq_plan_get_info(self.ptr, ctypes.byref(info)))
