@@ -159,3 +159,57 @@ def test_peps_d8_chi128_merged_sizes(cuda):
         assert overall <= TOL[precision] and worst <= TOL[precision]
         plan.destroy()
         del out
+
+
+def shared_output_blocks(n_members, w, rng, m=37, n=70):
+    """A (m x K) and B (K x n) whose blocks pair up along K only: every pair
+    writes the same output block, so one group of n_members pairs, each with
+    w contracted values (pairs stored in opposite leg orders)."""
+    K = n_members * w
+    ba = np.array([[0, i * w] for i in range(n_members)], np.int64)
+    ea = np.array([[m - 1, i * w + w - 1] for i in range(n_members)], np.int64)
+    bb = np.array([[i * w, 0] for i in range(n_members)], np.int64)
+    eb = np.array([[i * w + w - 1, n - 1] for i in range(n_members)], np.int64)
+    a, da = tensor([m, K], ba, ea, rng)
+    b, db = tensor([K, n], bb, eb, rng)
+    return a, da, b, db
+
+
+@pytest.mark.parametrize("n_members,w,streamed", [(1, 8, True), (2, 5, True), (2, 8, True),
+                                                  (3, 4, False), (2, 9, False)])
+def test_streamed_groups(cuda, n_members, w, streamed):
+    # a merged plan streams a group (no MFMA tiles, no group panels) when
+    # its pairs are short (K <= 8) and at most two; otherwise it runs the
+    # K-concatenated group GEMM; either way the group's block is the sum of
+    # its pairs' products
+    rng = np.random.default_rng(40 + 10 * n_members + w)
+    a, da, b, db = shared_output_blocks(n_members, w, rng)
+    legs = [[1, 0]]
+    ref_c, ref = O.contract(a, da, b, db, legs)
+    for precision in ("native", "fp32"):
+        plan, got = run(a, da, b, db, legs, cuda, precision)
+        assert plan.n_pairs == 1 and plan.n_matched == n_members
+        tiles, segs = plan.work()
+        assert (tiles == 0) == streamed, (tiles, segs)
+        check_merged(plan, got, ref_c, ref, TOL[precision])
+    # the FP32 data path
+    ga = torch_tensor(a, da, cuda, torch.float32)
+    gb = torch_tensor(b, db, cuda, torch.float32)
+    plan = ga.plan_contract(gb, legs, merged=True)
+    out = torch.full((plan.out_n_elem,), float("nan"), dtype=torch.float32, device=cuda)
+    plan.execute(ga.data, gb.data, out)
+    torch.cuda.synchronize()
+    want, _, _, _ = merged_reference(ref_c, ref)
+    assert rel_fro(out.cpu().numpy().astype(np.float64), want) <= 1e-5
+
+
+def test_streamed_groups_refuse_batch(cuda):
+    # a merged plan that streams its groups keeps its own member list: it
+    # does not join a batch
+    from torque_amd import BatchPlan, TorqueError, default_handle
+    rng = np.random.default_rng(77)
+    a, da, b, db = shared_output_blocks(2, 4, rng)
+    ga, gb = torch_tensor(a, da, cuda), torch_tensor(b, db, cuda)
+    plan = ga.plan_contract(gb, [[1, 0]], merged=True)
+    with pytest.raises(TorqueError, match="streams its groups"):
+        BatchPlan(default_handle(), [plan], [0], [0], [0])

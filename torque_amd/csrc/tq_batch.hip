@@ -107,6 +107,10 @@ tq_status tq_batch_create(tq_handle_t h, int32_t n, const tq_plan_t *plans,
       return tq_fail(h, TQ_ERROR_INVALID_VALUE,
                      "batch: plans must share dtype, precision and small-K threshold, "
                      "and have a tensor (not scalar) result");
+    if (plans[i]->member_stream)
+      return tq_fail(h, TQ_ERROR_INVALID_VALUE,
+                     "batch: a merged plan that streams its groups (short pairs) "
+                     "cannot join a batch");
     if (plans[i]->split_direct != plans[0]->split_direct)
       return tq_fail(h, TQ_ERROR_INVALID_VALUE,
                      "batch: split-BF16 plans must all read their operands in place "

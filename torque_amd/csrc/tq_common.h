@@ -236,6 +236,25 @@ tq_status tq_launch_small_k(tq_handle_t h, tq_dtype dt, tq_precision prec,
                             const void *b, const void *b_packed, void *c, int64_t a_plane,
                             int64_t b_plane, int32_t *range_flag);
 
+// Streamed groups of a merged-output plan: an output block that is the sum
+// of up to kGroupMaxMembers matched pairs, each with K <= kStreamMaxK, is
+// streamed like a single pair; each lane keeps its row of every member's A
+// in registers and every output column is one sum over the members'
+// contracted values, written once. A slice is kStreamCols (or fewer)
+// columns of a group from column n0: the members are members[mem0 ..
+// mem0 + nmem) (their own operand descriptors; the group's M and N), the
+// output at c_off with ldc = m.
+constexpr int kGroupMaxMembers = 2;
+struct GroupSlice {
+  i64 c_off;
+  int32_t m, n, n0, mem0, nmem, pad;
+};
+tq_status tq_launch_small_k_groups(tq_handle_t h, tq_dtype dt, tq_precision prec,
+                                   const GroupSlice *slices, const GemmDesc *members,
+                                   const int32_t *seg_slice, const int32_t *slice_first,
+                                   int64_t n_segs, const void *a, const void *a_packed,
+                                   const void *b, const void *b_packed, void *c);
+
 // The wave-tile kernel addresses operands with 32-bit byte offsets from a
 // per-tile base; it runs only when every leading dimension is below this.
 constexpr int64_t kWaveMaxLd = int64_t(1) << 22;

@@ -258,7 +258,13 @@ tq_status tq_bs_contract_multiply(tq_plan_t pl, const void *a, const void *b,
   if (!a || !b) return TQ_ERROR_INVALID_VALUE;
   if (pl->ro > 0) {
     TQ_CHECK(plan_build_tiles(pl));
-    if (pl->n_chunks > 0)
+    if (pl->n_chunks > 0 && pl->member_stream)
+      TQ_CHECK(tq_launch_small_k_groups(
+          h, pl->dtype, pl->prec, pl->group_slices.as<GroupSlice>(),
+          pl->member_desc.as<GemmDesc>(), pl->chunk_pair.as<int32_t>(),
+          pl->chunk_first.as<int32_t>(), pl->n_chunks, a, pl->pack_a.panel.p, b,
+          pl->pack_b.panel.p, c));
+    else if (pl->n_chunks > 0)
       TQ_CHECK(tq_launch_small_k(h, pl->dtype, pl->prec, pl->split_direct,
                                  pl->stream_desc.as<GemmDesc>(),
                                  pl->chunk_pair.as<int32_t>(),

@@ -463,9 +463,10 @@ __device__ inline i64 split_row_offset(const CopyJob &job, uint32_t row) {
 }
 
 // Split tile of a job whose first contracted leg is contiguous in memory
-// (k fastest): four lanes share a row's 8-value chunk, two values each, so
-// a wavefront reads 16 rows x 64 contiguous bytes and writes 256 contiguous
-// bytes per plane (a 32-bit word of two BF16 pieces per lane).
+// (k fastest): two lanes share a row's 8-value chunk, four values each, so
+// a wavefront reads 32 rows x 64 contiguous bytes (16-byte loads when the
+// four values are adjacent) and writes 512 contiguous bytes per plane (an
+// 8-byte word of four BF16 pieces per lane).
 __device__ inline void split_kfast_tile(const CopyJob &job, int64_t rows, int64_t row0,
                                         int64_t chunk, const double *__restrict__ src,
                                         uint16_t *__restrict__ dst, int64_t plane,
@@ -473,11 +474,11 @@ __device__ inline void split_kfast_tile(const CopyJob &job, int64_t rows, int64_
   if (chunk * 8 >= job.kpad) return;
   const int lane = threadIdx.x & 63;
   const int nk = job.nk;
-  const int64_t k0 = chunk * 8 + 2 * (lane & 3);
-  // offsets of k0 and k0 + 1 over the contracted legs (-1: past kdim)
-  i64 koff[2];
+  const int64_t k0 = chunk * 8 + 4 * (lane & 1);
+  // offsets of k0 .. k0 + 3 over the contracted legs (-1: past kdim)
+  i64 koff[4];
 #pragma unroll
-  for (int e = 0; e < 2; ++e) {
+  for (int e = 0; e < 4; ++e) {
     koff[e] = -1;
     if (k0 + e < job.kdim) {
       uint32_t rem = (uint32_t)(k0 + e);
@@ -493,13 +494,13 @@ __device__ inline void split_kfast_tile(const CopyJob &job, int64_t rows, int64_
       koff[e] = o;
     }
   }
-  // the row offsets of the four row groups, then all eight loads in flight
+  // the row offsets of the two row groups, then all loads in flight
   // together, then the splits and stores
-  constexpr int kG = kSplitRows / 16;
+  constexpr int kG = kSplitRows / 32;
   i64 roff[kG];
 #pragma unroll
   for (int rg = 0; rg < kG; ++rg) {
-    const int64_t row = row0 + rg * 16 + (lane >> 2);
+    const int64_t row = row0 + rg * 32 + (lane >> 1);
     i64 o = job.src_off;
     uint32_t rem = (uint32_t)row;
 #pragma unroll
@@ -512,23 +513,44 @@ __device__ inline void split_kfast_tile(const CopyJob &job, int64_t rows, int64_
     }
     roff[rg] = row < rows ? o : -1;
   }
-  double xv[kG][2];
+  double xv[kG][4];
+  if (koff[3] == koff[0] + 3 && koff[1] == koff[0] + 1 && koff[2] == koff[0] + 2) {
+    // adjacent in memory: two 16-byte loads per row (element aligned only,
+    // which the global loads accept)
+    typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
 #pragma unroll
-  for (int rg = 0; rg < kG; ++rg)
+    for (int rg = 0; rg < kG; ++rg) {
+      d2u v0 = d2u{0.0, 0.0}, v1 = d2u{0.0, 0.0};
+      if (roff[rg] >= 0) {
+        const d2u *p = reinterpret_cast<const d2u *>(src + roff[rg] + koff[0]);
+        v0 = p[0];
+        v1 = p[1];
+      }
+      xv[rg][0] = v0.x;
+      xv[rg][1] = v0.y;
+      xv[rg][2] = v1.x;
+      xv[rg][3] = v1.y;
+    }
+  } else {
 #pragma unroll
-    for (int e = 0; e < 2; ++e)
-      xv[rg][e] = (roff[rg] >= 0 && koff[e] >= 0) ? src[roff[rg] + koff[e]] : 0.0;
+    for (int rg = 0; rg < kG; ++rg)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        xv[rg][e] = (roff[rg] >= 0 && koff[e] >= 0) ? src[roff[rg] + koff[e]] : 0.0;
+  }
 #pragma unroll
   for (int rg = 0; rg < kG; ++rg) {
-    const int64_t row = row0 + rg * 16 + (lane >> 2);
+    const int64_t row = row0 + rg * 32 + (lane >> 1);
     if (row >= rows) continue;
-    uint32_t wh, wm, wl;
-    bad = bad || tqs::split_out_of_range(xv[rg][0]) || tqs::split_out_of_range(xv[rg][1]);
-    tqs::split3x2(xv[rg][0], xv[rg][1], wh, wm, wl);
-    const int64_t o = job.dst_off + chunk * rows * 8 + row * 8 + 2 * (lane & 3);
-    *reinterpret_cast<uint32_t *>(dst + o) = wh;
-    *reinterpret_cast<uint32_t *>(dst + plane + o) = wm;
-    *reinterpret_cast<uint32_t *>(dst + 2 * plane + o) = wl;
+    uint2 wh, wm, wl;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) bad = bad || tqs::split_out_of_range(xv[rg][e]);
+    tqs::split3x2(xv[rg][0], xv[rg][1], wh.x, wm.x, wl.x);
+    tqs::split3x2(xv[rg][2], xv[rg][3], wh.y, wm.y, wl.y);
+    const int64_t o = job.dst_off + chunk * rows * 8 + row * 8 + 4 * (lane & 1);
+    *reinterpret_cast<uint2 *>(dst + o) = wh;
+    *reinterpret_cast<uint2 *>(dst + plane + o) = wm;
+    *reinterpret_cast<uint2 *>(dst + 2 * plane + o) = wl;
   }
 }
 

@@ -208,6 +208,16 @@ struct tq_plan_s {
   bool merged = false;
   int64_t n_matched = 0;
   std::vector<i64> h_pair_m, h_pair_n, h_pair_k;
+  // streamed groups (every pair K <= kStreamMaxK, at most kGroupMaxMembers
+  // pairs per group, not the split-BF16 tier): the operands stay packed as
+  // for the unmerged plan and the streaming kernel sums each group's
+  // members (member_desc, in group order, group g from h_member_first[g])
+  // instead of reading K-concatenated group panels; every group streams
+  bool member_stream = false;
+  std::vector<GemmDesc> h_members;
+  std::vector<int64_t> h_member_first;
+  DevBuf member_desc;
+  DevBuf group_slices;  // GroupSlice per column slice of a streamed group
 
   // host copies
   std::vector<i64> h_m, h_n, h_k;
@@ -233,7 +243,7 @@ inline bool plan_config_ok(const tq_plan_s &pl, int cfg) {
 }
 // 0, 1: MFMA tile class of a pair; -1: streaming.
 inline int pair_class(const tq_plan_s &pl, int64_t m, int64_t n, int64_t k) {
-  if (k <= pl.stream_max_k) return -1;
+  if (pl.member_stream || k <= pl.stream_max_k) return -1;
   return (m >= pl.large_min && n >= pl.large_min) ? 1 : 0;
 }
 tq_status tq_sort_by_cost(tq_handle_t h, const i64 *cost, int64_t n, i64 *order);

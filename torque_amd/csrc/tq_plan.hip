@@ -711,7 +711,32 @@ static tq_status build_stream_segments(tq_plan_t pl) {
   const bool split = pl->prec == TQ_PREC_BF16X3 && !pl->split_direct;
   std::vector<GemmDesc> slices;
   std::vector<int32_t> seg_slice, slice_first;
-  for (int64_t q = 0; q < np; ++q) {
+  if (pl->member_stream) {
+    // streamed groups: slices of a group carry its member range and first
+    // column (each member's B is offset by the column in the kernel)
+    std::vector<GroupSlice> gs;
+    for (int64_t q = 0; q < np; ++q) {
+      const int64_t g = order[q];
+      const GemmDesc &d = pl->h_desc[g];
+      const int64_t row_segs = ceil_div(d.m, kStreamRows);
+      for (int32_t n0 = 0; n0 < d.n; n0 += kStreamCols) {
+        if ((int64_t)seg_slice.size() + row_segs > 0x7fffffff)
+          return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many output segments");
+        slice_first.push_back((int32_t)seg_slice.size());
+        seg_slice.insert(seg_slice.end(), (size_t)row_segs, (int32_t)gs.size());
+        gs.push_back(GroupSlice{d.c_off + (i64)d.m * n0, d.m,
+                                std::min<int32_t>(kStreamCols, d.n - n0), n0,
+                                (int32_t)pl->h_member_first[g],
+                                (int32_t)(pl->h_member_first[g + 1] - pl->h_member_first[g]),
+                                0});
+      }
+    }
+    if (seg_slice.empty()) return TQ_SUCCESS;
+    TQ_HIP(h, pl->group_slices.alloc(sizeof(GroupSlice) * gs.size()));
+    TQ_HIP(h, hipMemcpyAsync(pl->group_slices.p, gs.data(), sizeof(GroupSlice) * gs.size(),
+                             hipMemcpyHostToDevice, h->stream));
+  }
+  for (int64_t q = 0; q < np && !pl->member_stream; ++q) {
     const GemmDesc &d = pl->h_desc[order[q]];
     if (pair_class(*pl, d.m, d.n, d.k) >= 0) continue;
     // element distance between consecutive B rows (output columns): a split
@@ -731,11 +756,14 @@ static tq_status build_stream_segments(tq_plan_t pl) {
     }
   }
   if (seg_slice.empty()) return TQ_SUCCESS;
-  TQ_HIP(h, pl->stream_desc.alloc(sizeof(GemmDesc) * slices.size()));
   TQ_HIP(h, pl->chunk_pair.alloc(sizeof(int32_t) * seg_slice.size()));
   TQ_HIP(h, pl->chunk_first.alloc(sizeof(int32_t) * slice_first.size()));
-  TQ_HIP(h, hipMemcpyAsync(pl->stream_desc.p, slices.data(), sizeof(GemmDesc) * slices.size(),
-                           hipMemcpyHostToDevice, h->stream));
+  if (!pl->member_stream) {
+    TQ_HIP(h, pl->stream_desc.alloc(sizeof(GemmDesc) * slices.size()));
+    TQ_HIP(h, hipMemcpyAsync(pl->stream_desc.p, slices.data(),
+                             sizeof(GemmDesc) * slices.size(), hipMemcpyHostToDevice,
+                             h->stream));
+  }
   TQ_HIP(h, hipMemcpyAsync(pl->chunk_pair.p, seg_slice.data(),
                            sizeof(int32_t) * seg_slice.size(), hipMemcpyHostToDevice, h->stream));
   TQ_HIP(h, hipMemcpyAsync(pl->chunk_first.p, slice_first.data(),
@@ -1096,9 +1124,13 @@ tq_status plan_set_split_form(tq_plan_t pl, bool in_place) {
 // contracted ranges: both operands are packed into per-group panels in which
 // pair p's slice starts at K offset kofs_p (split-BF16 slices padded to 8
 // values, the padding zero), so the grouped GEMM and the streaming kernel run
-// unchanged with one unit per group.
+// unchanged with one unit per group. With member_stream the operands keep the
+// unmerged plan's packing (hd holds its descriptors) and the streaming
+// kernel reads each group's pairs in place of the group panels (a group's
+// sum runs over its pairs' contracted values in the same order as over the
+// concatenated range).
 static tq_status merge_groups(tq_plan_t pl, const std::vector<GemmDesc> &hd,
-                              double operand_elems) {
+                              double operand_elems, bool member_stream) {
   tq_handle_t h = pl->h;
   const int64_t np = pl->n_pairs;
   const int ro = pl->ro;
@@ -1122,7 +1154,20 @@ static tq_status merge_groups(tq_plan_t pl, const std::vector<GemmDesc> &hd,
   }
   std::vector<i64> a_base(ng), b_base(ng);
   i64 a_tot = 0, b_tot = 0;
-  for (int64_t g = 0; g < ng; ++g) {
+  if (member_stream) {
+    // each group's pairs in pair order
+    pl->h_member_first.assign(ng + 1, 0);
+    for (int64_t p = 0; p < np; ++p) ++pl->h_member_first[grp[p] + 1];
+    for (int64_t g = 0; g < ng; ++g) pl->h_member_first[g + 1] += pl->h_member_first[g];
+    std::vector<int64_t> fill(pl->h_member_first.begin(), pl->h_member_first.end() - 1);
+    pl->h_members.resize(np);
+    for (int64_t p = 0; p < np; ++p) pl->h_members[fill[grp[p]]++] = hd[p];
+    TQ_HIP(h, pl->member_desc.alloc(sizeof(GemmDesc) * std::max<int64_t>(np, 1)));
+    TQ_HIP(h, hipMemcpyAsync(pl->member_desc.p, pl->h_members.data(), sizeof(GemmDesc) * np,
+                             hipMemcpyHostToDevice, h->stream));
+    pl->member_stream = true;
+  }
+  for (int64_t g = 0; g < ng && !member_stream; ++g) {
     const GemmDesc &d = hd[first[g]];
     if (ktot[g] > 0x7fffffff || d.m * ktot[g] > 0x7fffffff || d.n * ktot[g] > 0x7fffffff)
       return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: merged block GEMM above 2^31 elements");
@@ -1138,8 +1183,10 @@ static tq_status merge_groups(tq_plan_t pl, const std::vector<GemmDesc> &hd,
   }
   dst_a[np] = a_tot;
   dst_b[np] = b_tot;
-  TQ_CHECK(build_pack(pl, false, kModePairPanel, split, pl->up, &dst_a));
-  TQ_CHECK(build_pack(pl, true, kModePairPanel, split, pl->up, &dst_b));
+  if (!member_stream) {
+    TQ_CHECK(build_pack(pl, false, kModePairPanel, split, pl->up, &dst_a));
+    TQ_CHECK(build_pack(pl, true, kModePairPanel, split, pl->up, &dst_b));
+  }
 
   // one GEMM unit per group; the matched pairs keep their own maps
   pl->merged = true;
@@ -1152,7 +1199,9 @@ static tq_status merge_groups(tq_plan_t pl, const std::vector<GemmDesc> &hd,
   pl->h_m.resize(ng);
   pl->h_n.resize(ng);
   pl->h_k.resize(ng);
-  pl->max_ld = 0;
+  // (streamed groups keep the pairs' leading dimensions: their members are
+  // the GEMM operands)
+  if (!member_stream) pl->max_ld = 0;
   i64 c = 0;
   const int32_t kf = split ? (kAKFast | kBKFast) : 0;
   for (int64_t g = 0; g < ng; ++g) {
@@ -1168,7 +1217,8 @@ static tq_status merge_groups(tq_plan_t pl, const std::vector<GemmDesc> &hd,
     pl->h_m[g] = d.m;
     pl->h_n[g] = d.n;
     pl->h_k[g] = kt;
-    cost[g] = pair_order_key(d.m, d.n, kt);
+    // (a streamed group costs its output writes and K-long sums, whatever K)
+    cost[g] = member_stream ? (i64)d.m * d.n * (kt + 4) : pair_order_key(d.m, d.n, kt);
     pl->max_ld = std::max<int64_t>(pl->max_ld, std::max(gd[g].lda, gd[g].ldb));
   }
   coff[ng] = c;
@@ -1467,8 +1517,39 @@ static tq_status plan_contract(tq_handle_t h, const tq_bs_desc *a, const tq_bs_d
     pl->bytes = (double)tq_elem_size(dtype) * (na_elem + nb_elem + (double)pl->out_n_elem);
   }
 
+  // ---- output metadata on the host (the drop-in tensor keeps it there) ----
+  pl->h_out_begin.resize((size_t)np * ro);
+  pl->h_out_end.resize((size_t)np * ro);
+  pl->h_c_off.resize(np + 1);
+  if (np > 0 && ro > 0) {
+    TQ_HIP(h, hipMemcpyAsync(pl->h_out_begin.data(), d_out_begin,
+                             sizeof(i64) * np * ro, hipMemcpyDeviceToHost, h->stream));
+    TQ_HIP(h, hipMemcpyAsync(pl->h_out_end.data(), d_out_end, sizeof(i64) * np * ro,
+                             hipMemcpyDeviceToHost, h->stream));
+  }
+  TQ_HIP(h, hipMemcpyAsync(pl->h_c_off.data(), pl->c_off.p, sizeof(i64) * (np + 1),
+                           hipMemcpyDeviceToHost, h->stream));
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  // a merged plan streams its groups when every pair is short and no group
+  // holds more than kGroupMaxMembers pairs (not for the split-BF16 tier,
+  // which streams from split panels)
+  bool member_stream = false;
+  if (merged && pl->prec != TQ_PREC_BF16X3) {
+    member_stream = true;
+    for (int64_t p = 0; p < np && member_stream; ++p) member_stream = hd[p].k <= kStreamMaxK;
+    if (member_stream) {
+      std::map<std::vector<i64>, int> count;
+      std::vector<i64> key(2 * ro);
+      for (int64_t p = 0; p < np && member_stream; ++p) {
+        std::copy_n(pl->h_out_begin.begin() + p * ro, ro, key.begin());
+        std::copy_n(pl->h_out_end.begin() + p * ro, ro, key.begin() + ro);
+        member_stream = ++count[key] <= kGroupMaxMembers;
+      }
+    }
+  }
+
   // ---- 4. operand layouts and pack jobs ----
-  if (np > 0 && !merged) {
+  if (np > 0 && (!merged || member_stream)) {
     // The split-BF16 tier reads split panels, or, when both operands are
     // plain matrices, may read them in place and split FP64 values in
     // registers (no pack pass); it starts in place and plan_set_split_form
@@ -1501,21 +1582,8 @@ static tq_status plan_contract(tq_handle_t h, const tq_bs_desc *a, const tq_bs_d
   }
   set_layouts(pl.get());
 
-  // ---- output metadata on the host (the drop-in tensor keeps it there) ----
-  pl->h_out_begin.resize((size_t)np * ro);
-  pl->h_out_end.resize((size_t)np * ro);
-  pl->h_c_off.resize(np + 1);
-  if (np > 0 && ro > 0) {
-    TQ_HIP(h, hipMemcpyAsync(pl->h_out_begin.data(), d_out_begin,
-                             sizeof(i64) * np * ro, hipMemcpyDeviceToHost, h->stream));
-    TQ_HIP(h, hipMemcpyAsync(pl->h_out_end.data(), d_out_end, sizeof(i64) * np * ro,
-                             hipMemcpyDeviceToHost, h->stream));
-  }
-  TQ_HIP(h, hipMemcpyAsync(pl->h_c_off.data(), pl->c_off.p, sizeof(i64) * (np + 1),
-                           hipMemcpyDeviceToHost, h->stream));
-  TQ_HIP(h, hipStreamSynchronize(h->stream));
   if (merged) {
-    TQ_CHECK(merge_groups(pl.get(), hd, operand_elems));
+    TQ_CHECK(merge_groups(pl.get(), hd, operand_elems, member_stream));
     set_layouts(pl.get());
   }
 

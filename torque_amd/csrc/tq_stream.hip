@@ -75,6 +75,120 @@ __global__ __launch_bounds__(kThreads) void small_k_kernel(
   if (kRound3 && bad && range_flag) atomicOr(range_flag, 1);
 }
 
+// Streamed groups of a merged-output plan: a slice's output column sums the
+// products of all its members, member by member in group order and each
+// member's contracted values in order (the sum over the concatenated range),
+// and is written once. Each lane holds its row of every member's A; the
+// slice's B values (at most kStreamCols columns) are loaded once per segment,
+// lane j taking column j of every member, into the wavefront's LDS slot and
+// read back by every lane as broadcasts.
+constexpr int kGroupRow = kGroupMaxMembers * kStreamMaxK + 1;  // odd: no bank conflicts
+
+template <class TIn, class TC, class TOut, int NM>
+__device__ inline void group_slice(const GroupSlice &g, const GemmDesc *__restrict__ members,
+                                   int row0, const TIn *__restrict__ a,
+                                   const TIn *__restrict__ a_packed, const TIn *__restrict__ b,
+                                   const TIn *__restrict__ b_packed, TOut *__restrict__ c,
+                                   TC (*brow)[kGroupRow]) {
+  const int lane = threadIdx.x & 63;
+  const int m = row0 + lane;
+  const bool live = m < g.m;
+  TC av[NM][kStreamMaxK];
+  int kq[NM];
+  TC bl[NM][kStreamMaxK];
+#pragma unroll
+  for (int q = 0; q < NM; ++q) {
+    const GemmDesc d = members[g.mem0 + q];
+    const TIn *pa = ((d.flags & kAPacked) ? a_packed : a) + d.a_off;
+    const int64_t sam = (d.flags & kAKFast) ? d.lda : 1;
+    const int64_t sak = (d.flags & kAKFast) ? 1 : d.lda;
+    const int64_t sbn = (d.flags & kBKFast) ? d.ldb : 1;
+    const int64_t sbk = (d.flags & kBKFast) ? 1 : d.ldb;
+    const TIn *pb = ((d.flags & kBPacked) ? b_packed : b) + d.b_off +
+                    (int64_t)(g.n0 + min(lane, g.n - 1)) * sbn;
+    kq[q] = d.k;
+#pragma unroll
+    for (int k = 0; k < kStreamMaxK; ++k) {
+      av[q][k] = (live && k < d.k) ? (TC)pa[m * sam + k * sak] : (TC)0;
+      bl[q][k] = k < d.k ? (TC)pb[k * sbk] : (TC)0;
+    }
+  }
+  // the slot's previous rows were read by this wavefront before this point
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int q = 0; q < NM; ++q)
+#pragma unroll
+    for (int k = 0; k < kStreamMaxK; ++k) brow[lane][q * kStreamMaxK + k] = bl[q][k];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  TOut *pc = c + g.c_off + m;
+  for (int j = 0; j < g.n; ++j) {
+    // member 0 holds at least one contracted value
+    TC acc = av[0][0] * brow[j][0];
+#pragma unroll
+    for (int k = 1; k < kStreamMaxK; ++k)
+      if (k < kq[0]) acc = fma(av[0][k], brow[j][k], acc);
+#pragma unroll
+    for (int q = 1; q < NM; ++q)
+#pragma unroll
+      for (int k = 0; k < kStreamMaxK; ++k)
+        if (k < kq[q]) acc = fma(av[q][k], brow[j][q * kStreamMaxK + k], acc);
+    if (live) pc[(int64_t)g.m * j] = (TOut)acc;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <class TIn, class TC, class TOut>
+__global__ __launch_bounds__(kThreads) void small_k_group_kernel(
+    const GroupSlice *__restrict__ slices, const GemmDesc *__restrict__ members,
+    const int32_t *__restrict__ seg_slice, const int32_t *__restrict__ slice_first,
+    int64_t n_segs, const TIn *__restrict__ a, const TIn *__restrict__ a_packed,
+    const TIn *__restrict__ b, const TIn *__restrict__ b_packed, TOut *__restrict__ c) {
+  static_assert(kGroupMaxMembers == 2, "one instance per member count");
+  static_assert(kStreamCols <= 64, "a slice's columns fit the wavefront");
+  __shared__ TC brow[kWaves][64][kGroupRow];
+  const int wave = threadIdx.x >> 6;
+  const int wave_id = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWaves + wave));
+  const int n_waves = (int)(gridDim.x * kWaves);
+  for (int sg = wave_id; sg < n_segs; sg += n_waves) {
+    const int si = __builtin_amdgcn_readfirstlane(seg_slice[sg]);
+    const GroupSlice g = slices[si];
+    const int row0 = (int)(sg - slice_first[si]) * 64;
+    if (g.nmem == 1)
+      group_slice<TIn, TC, TOut, 1>(g, members, row0, a, a_packed, b, b_packed, c, brow[wave]);
+    else
+      group_slice<TIn, TC, TOut, 2>(g, members, row0, a, a_packed, b, b_packed, c, brow[wave]);
+  }
+}
+
+template <class TIn, class TC, class TOut>
+tq_status launch_groups(tq_handle_t h, const GroupSlice *slices, const GemmDesc *members,
+                        const int32_t *seg_slice, const int32_t *slice_first, int64_t n_segs,
+                        const void *a, const void *ap, const void *b, const void *bp,
+                        void *c) {
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &occ, small_k_group_kernel<TIn, TC, TOut>, kThreads, 0) != hipSuccess ||
+        occ < 1)
+      occ = 1;
+    per_cu = occ;
+  }
+  const int64_t grid = std::max<int64_t>(
+      1, std::min<int64_t>(ceil_div(n_segs, kWaves), (int64_t)h->n_cu * per_cu));
+  hipLaunchKernelGGL((small_k_group_kernel<TIn, TC, TOut>), dim3((unsigned)grid),
+                     dim3(kThreads), 0, h->stream, slices, members, seg_slice, slice_first,
+                     n_segs, reinterpret_cast<const TIn *>(a),
+                     reinterpret_cast<const TIn *>(ap), reinterpret_cast<const TIn *>(b),
+                     reinterpret_cast<const TIn *>(bp), reinterpret_cast<TOut *>(c));
+  TQ_HIP(h, hipGetLastError());
+  return TQ_SUCCESS;
+}
+
 __device__ inline double bf16_bits_value(uint32_t h) {
   return (double)__uint_as_float(h << 16);
 }
@@ -213,4 +327,26 @@ tq_status tq_launch_small_k(tq_handle_t h, tq_dtype dt, tq_precision prec,
                                          a_packed, b, b_packed, c);
   return launch<double, double, double>(h, descs, chunk_pair, chunk_first, n_chunks, a,
                                         a_packed, b, b_packed, c);
+}
+
+tq_status tq_launch_small_k_groups(tq_handle_t h, tq_dtype dt, tq_precision prec,
+                                   const GroupSlice *slices, const GemmDesc *members,
+                                   const int32_t *seg_slice, const int32_t *slice_first,
+                                   int64_t n_segs, const void *a, const void *a_packed,
+                                   const void *b, const void *b_packed, void *c) {
+  if (n_segs <= 0) return TQ_SUCCESS;
+  if (prec == TQ_PREC_BF16X3)
+    return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: split-BF16 groups stream from panels");
+  if (dt == TQ_F32)
+    return launch_groups<float, float, float>(h, slices, members, seg_slice, slice_first,
+                                              n_segs, a, a_packed, b, b_packed, c);
+  if (dt == TQ_F16)
+    return launch_groups<_Float16, float, _Float16>(h, slices, members, seg_slice,
+                                                    slice_first, n_segs, a, a_packed, b,
+                                                    b_packed, c);
+  if (prec == TQ_PREC_F32)
+    return launch_groups<double, float, double>(h, slices, members, seg_slice, slice_first,
+                                                n_segs, a, a_packed, b, b_packed, c);
+  return launch_groups<double, double, double>(h, slices, members, seg_slice, slice_first,
+                                               n_segs, a, a_packed, b, b_packed, c);
 }
