@@ -190,7 +190,11 @@ tq_status tq_bs_contract_plan(tq_handle_t handle, const tq_bs_desc *a,
  * src/gpu/block_sparse.cuh:972-975). Blocks are listed in the order of their
  * first pair, column-major over [A free legs, B free legs]; the dense image
  * equals the unmerged output's. Each output block is one GEMM over the
- * concatenated contracted ranges of its pairs (operands packed per block).
+ * concatenated contracted ranges of its pairs (operands packed per block);
+ * when every pair has K <= 8 and no output block has more than two pairs
+ * (not at TQ_PREC_BF16X3) the blocks are instead streamed from the unmerged
+ * plan's operand panels, summing the pairs' products in the same order (the
+ * same bits); such a plan cannot join a batch (tq_batch_create refuses it).
  * tq_plan_get_pairs still lists the matched pairs (tq_plan_info.n_matched);
  * tq_plan_get_output and the worklist list the merged blocks. */
 tq_status tq_bs_contract_plan_merged(tq_handle_t handle, const tq_bs_desc *a,
@@ -314,9 +318,10 @@ tq_status tq_plan_split_range_check(tq_plan_t plan, int32_t *out_of_range);
  * streaming launch and one grouped-GEMM launch per tile class. Plan i reads A at element
  * offset a_offset[i] of `a`, B at b_offset[i] of `b`, and writes its output
  * blocks at c_offset[i] of `c`. The plans must share dtype, precision and
- * small-K threshold and have tensor (rank > 0) results; they may be destroyed
- * after the call. The batch supports execute, autotune, set_config,
- * capture and destroy. */
+ * small-K threshold and have tensor (rank > 0) results, and none may be a
+ * merged plan that streams its output blocks (tq_bs_contract_plan_merged);
+ * they may be destroyed after the call. The batch supports execute,
+ * autotune, set_config, capture and destroy. */
 tq_status tq_batch_create(tq_handle_t handle, int32_t n, const tq_plan_t *plans,
                           const int64_t *a_offset, const int64_t *b_offset,
                           const int64_t *c_offset, tq_plan_t *batch);
