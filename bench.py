@@ -91,7 +91,7 @@ def parse():
     p.add_argument("--no-large", action="store_true",
                    help="skip the D=10, chi=256 enlarged-corner product")
     p.add_argument("--workload", default="peps",
-                   choices=["peps", "batched", "permute", "large"],
+                   choices=["peps", "batched", "permute", "large", "shapes"],
                    help="peps: the D, chi absorption step (default); batched: --n-batch "
                         "independent rank-4 x rank-4 contractions in one captured launch")
     p.add_argument("--n-batch", type=int, default=512)
@@ -573,6 +573,70 @@ def bench_ctmrg(dev, D, chi, steps, warmup):
     return res
 
 
+SHAPES = ((128, 128, 128), (64, 64, 256), (64, 64, 64), (40, 36, 35), (32, 32, 32),
+          (16, 16, 64))
+
+
+def bench_shapes(dev, gflop=20.0):
+    """FP64 grouped GEMM on uniform block-diagonal products (nb identical
+    M x K . K x N pairs, A stored row index fastest, B k fastest, as the batched
+    workload's operands), autotuned, against the vendor library's FP64 batched
+    GEMM on the same shapes (torch.bmm: rocBLAS / hipBLASLt): the rates the
+    batched workload's pair shapes allow."""
+    import torch
+
+    from torque_amd import BlockSparseTensor
+
+    def timed(f, n=5):
+        f()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(n):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    out = []
+    for (M, N, K) in SHAPES:
+        nb = int(min(gflop * 1e9 / (2 * M * N * K), 3e8 / max(M * K, N * K, M * N)))
+        ia = np.arange(nb, dtype=np.int64)
+        ba = np.stack([ia * M, ia * K], 1)
+        ea = np.stack([ia * M + M - 1, ia * K + K - 1], 1)
+        bb = np.stack([ia * K, ia * N], 1)
+        eb = np.stack([ia * K + K - 1, ia * N + N - 1], 1)
+        da = torch.randn(nb * M * K, dtype=torch.float64, device=dev)
+        db = torch.randn(nb * K * N, dtype=torch.float64, device=dev)
+        A = BlockSparseTensor([nb * M, nb * K], da, ba, ea, device=dev)
+        B = BlockSparseTensor([nb * K, nb * N], db, bb, eb, device=dev)
+        p = A.plan_contract(B, [[1, 0]], "native")
+        c = torch.empty(p.out_n_elem, dtype=torch.float64, device=dev)
+        best, _ = p.autotune(A.data, B.data, c, reps=2)
+        ms = timed(lambda: p.execute(A.data, B.data, c))
+        # the same products as one strided batch of contiguous A_i (M x K)
+        # and B_i (K x N)
+        ta = da.view(nb, K, M).transpose(1, 2).contiguous()
+        tb = db.view(nb, N, K).transpose(1, 2).contiguous()
+        ms_lib = timed(lambda: torch.bmm(ta, tb))
+        ref = torch.bmm(ta, tb)
+        got = c.view(nb, N, M).transpose(1, 2)
+        err = float((got - ref).norm() / ref.norm())
+        del ta, tb, ref, got
+        r = {"m": M, "n": N, "k": K, "pairs": nb, "gflop": round(p.flops / 1e9, 2),
+             "tflops": round(p.flops / ms / 1e9, 1),
+             "library_tflops": round(p.flops / ms_lib / 1e9, 1), "config": best,
+             "rel_err_vs_library": err}
+        out.append(r)
+        print("# %d x (%d, %d, %d): grouped GEMM %.1f TFLOP/s (configs %s), library batched "
+              "GEMM %.1f TFLOP/s, rel diff %.1e" % (nb, M, N, K, r["tflops"], best,
+                                                    r["library_tflops"], err), flush=True)
+        del A, B, da, db, c, p
+        torch.cuda.empty_cache()
+    return {"workload": "uniform_block_diagonal_fp64", "shapes": out}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -581,6 +645,12 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         print(json.dumps(bench_batched(args)), flush=True)
+        return
+    if args.workload == "shapes":
+        import torch
+        dev = torch.device("cuda:0")
+        torch.cuda.set_device(dev)
+        print(json.dumps(bench_shapes(dev)), flush=True)
         return
     if args.workload in ("permute", "large"):
         import torch
