@@ -492,8 +492,19 @@ tq_status tq_plan_choose_config(tq_plan_t pl, tq_costmodel_t md, int32_t *best) 
     double arg_t = 0;
     // split-K enters through the shapes it produces: a split class is the
     // grouped GEMM of its (pair, slice) GEMMs, predicted by the same model,
-    // plus the reduction's traffic (its slices read, the output written)
+    // plus the reduction's traffic (its slices read, the output written).
+    // Splitting adds parallelism, so it is weighed only for a class whose
+    // whole-pair tiles under the best unsplit configuration leave the GPU
+    // short of two rounds of workgroups (the model, trained on whole
+    // pairs, misjudges slices of classes that already fill it).
     for (int sp : {1, 2, 4, 8, 16}) {
+      if (sp > 1 && arg >= 0) {
+        const GemmConfig g = tq_gemm_config(pl->prec, arg);
+        double tiles = 0;
+        for (size_t p = 0; p < m.size(); ++p)
+          tiles += (double)ceil_div(m[p], (int64_t)g.bm) * (double)ceil_div(n[p], (int64_t)g.bn);
+        if (tiles >= 2.0 * n_cu) break;
+      }
       std::vector<int64_t> vm, vn, vk;
       double out_elems = 0, slices = 0;
       for (size_t p = 0; p < m.size(); ++p) {
