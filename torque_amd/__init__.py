@@ -10,8 +10,9 @@ from .tensor import (BatchPlan, BlockSparseTensor, ContractionPlan, DenseTensor,
                      SparseTensor,
                      default_handle, dot_f64, gemm_configs, index_table,
                      nest_sum, normalize_f64, normalize_f64_noalloc, nrm2_f64, scale_f64)
+from .u1 import U1Leg, U1Tensor  # noqa: F401
 
 __all__ = ["BatchPlan", "BlockSparseTensor", "ContractionPlan", "DenseTensor", "Handle",
-           "SparseTensor",
+           "SparseTensor", "U1Leg", "U1Tensor",
            "TorqueError", "default_handle", "dot_f64", "gemm_configs",
            "index_table", "nest_sum", "normalize_f64", "normalize_f64_noalloc", "nrm2_f64", "scale_f64", "lib"]
