@@ -168,6 +168,43 @@ tq_status tq_bs_slice(tq_handle_t handle, tq_dtype dtype, const tq_bs_desc *in_d
                       const int64_t *divisor, const void *in, void *out,
                       int64_t *out_begin, int64_t *out_end);
 
+/* ---- box copies (leg fusion) ------------------------------------------ */
+
+/* One box of a block copy: the box of extents dims[0..rank) is read at
+ * in[src_off + sum_l i_l * src_strides[l]] and written to
+ * out[dst_off + sum_l i_l * dst_strides[l]] (element units, strides >= 0,
+ * rank 0 = one element). */
+typedef struct tq_box {
+  int64_t src_off;
+  int64_t dst_off;
+  int32_t rank;
+  int32_t reserved;
+  int64_t dims[TQ_MAX_RANK];
+  int64_t src_strides[TQ_MAX_RANK];
+  int64_t dst_strides[TQ_MAX_RANK];
+} tq_box;
+
+typedef struct tq_copy_plan_s *tq_copy_plan_t;
+
+/* Plans the copy of n boxes from a buffer of in_elems elements into one of
+ * out_elems elements: each box is checked to lie inside both buffers (an
+ * error otherwise), and legs contiguous in both buffers are merged. Boxes
+ * must not overlap in out and in must not overlap out (neither is checked).
+ * The charge-aware front end fuses groups of U(1) legs into one leg, and
+ * splits them again, as such copies: each block of the tensor is one box of
+ * a block of the fused tensor. No counterpart in the reference, whose
+ * tensors have no leg fusion; the nearest operation is its per-block
+ * gather / scatter between strided blocks and a stacked layout
+ * (reshape_kernel, src/gpu/block_sparse.cu:9-64). */
+tq_status tq_copy_plan_create(tq_handle_t handle, tq_dtype dtype, int64_t n,
+                              const tq_box *boxes, int64_t in_elems, int64_t out_elems,
+                              tq_copy_plan_t *plan);
+/* Runs the planned copies on the handle's stream: no allocation and no
+ * synchronisation, so a capture may record it. Elements of out that no box
+ * covers are left as they are. */
+tq_status tq_copy_execute(tq_copy_plan_t plan, const void *in, void *out);
+tq_status tq_copy_plan_destroy(tq_copy_plan_t plan);
+
 /* ---- block-sparse contraction -------------------------------------- */
 
 /* Plan C = contract(A, B, legs): legs is n_legs x 2, row k = (leg of A,

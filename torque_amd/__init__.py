@@ -6,13 +6,13 @@ the reference's tensor classes. C++ users include
 torque_amd/include/torque/gpu/*.h instead.
 """
 from ._lib import TorqueError, lib, LIB_PATH  # noqa: F401
-from .tensor import (BatchPlan, BlockSparseTensor, ContractionPlan, DenseTensor, Handle,  # noqa: F401
-                     SparseTensor,
+from .tensor import (BatchPlan, BlockSparseTensor, ContractionPlan, CopyPlan,  # noqa: F401
+                     DenseTensor, Handle, SparseTensor,
                      default_handle, dot_f64, gemm_configs, index_table,
                      nest_sum, normalize_f64, normalize_f64_noalloc, nrm2_f64, scale_f64)
-from .u1 import U1Leg, U1Tensor  # noqa: F401
+from .u1 import FusionPlan, U1Leg, U1Tensor, fuse_legs  # noqa: F401
 
-__all__ = ["BatchPlan", "BlockSparseTensor", "ContractionPlan", "DenseTensor", "Handle",
-           "SparseTensor", "U1Leg", "U1Tensor",
+__all__ = ["BatchPlan", "BlockSparseTensor", "ContractionPlan", "CopyPlan", "DenseTensor",
+           "FusionPlan", "Handle", "SparseTensor", "U1Leg", "U1Tensor", "fuse_legs",
            "TorqueError", "default_handle", "dot_f64", "gemm_configs",
            "index_table", "nest_sum", "normalize_f64", "normalize_f64_noalloc", "nrm2_f64", "scale_f64", "lib"]

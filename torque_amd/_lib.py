@@ -54,6 +54,18 @@ class PlanInfo(ctypes.Structure):
                 ("n_matched", ctypes.c_int64)]
 
 
+TQ_MAX_RANK = 12
+
+
+class Box(ctypes.Structure):
+    """tq_box: one box of a block copy (tq_copy_plan_create)."""
+    _fields_ = [("src_off", ctypes.c_int64), ("dst_off", ctypes.c_int64),
+                ("rank", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("dims", ctypes.c_int64 * TQ_MAX_RANK),
+                ("src_strides", ctypes.c_int64 * TQ_MAX_RANK),
+                ("dst_strides", ctypes.c_int64 * TQ_MAX_RANK)]
+
+
 class TorqueError(RuntimeError):
     """Raised for a non-zero tq_status; the message keeps the reference's
     "Error: " prefix (src/error.h:8-14)."""
@@ -84,6 +96,10 @@ def _proto(lib):
         "tq_permute": (S, [H, S, ctypes.c_int32, c_i64_p, c_i64_p, c_i32_p, V, V]),
         "tq_bs_permute": (S, [H, S, ctypes.POINTER(BsDesc), c_i32_p, V, V, c_i64_p]),
         "tq_bs_slice": (S, [H, S, ctypes.POINTER(BsDesc), c_i64_p, V, V, c_i64_p, c_i64_p]),
+        "tq_copy_plan_create": (S, [H, S, ctypes.c_int64, ctypes.POINTER(Box), ctypes.c_int64,
+                                    ctypes.c_int64, ctypes.POINTER(V)]),
+        "tq_copy_execute": (S, [V, V, V]),
+        "tq_copy_plan_destroy": (S, [V]),
         "tq_bs_contract_plan": (S, [H, ctypes.POINTER(BsDesc), ctypes.POINTER(BsDesc),
                                     ctypes.c_int32, c_i32_p, S, S, ctypes.POINTER(V)]),
         "tq_bs_contract_plan_merged": (S, [H, ctypes.POINTER(BsDesc), ctypes.POINTER(BsDesc),

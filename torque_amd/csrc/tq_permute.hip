@@ -11,10 +11,14 @@
 //    its "input group" the legs with the smallest input strides; offset
 //    tables for both groups are built per tile in LDS, so reads are
 //    coalesced along the input and writes along the output.
+// and a box copy (tq_copy_plan_create / tq_copy_execute) that moves boxes
+// between strided blocks of two buffers: the charge-aware front end's leg
+// fusion and its inverse.
 // Semantics: out dims[k] = in dims[perm[k]]; out[n] = in[sum_k n_k *
 // in_stride[perm[k]]] (src/tensor/dense.h:325-366,
 // src/tensor/block_sparse.h:608-651).
 #include <algorithm>
+#include <memory>
 #include <numeric>
 
 #include "tq_common.h"
@@ -108,6 +112,69 @@ __global__ __launch_bounds__(kGatherThreads) void gather_jobs_kernel(
     const CopyJob &job = jobs[lo];
     const int64_t e0 = (int64_t)(c - job_wg_start[lo]) * kGatherPerWG;
     gather_range(job, e0, 64, lane, src, dst);
+  }
+}
+
+// ---- box copies ----------------------------------------------------------
+
+// One box of a block copy after the host merged its contiguous legs: element
+// e of the box (decoded leg 0 fastest over dims) moves from
+// src_off + sum_l i_l * sstr[l] to dst_off + sum_l i_l * dstr[l].
+struct BoxJob {
+  i64 src_off, dst_off, n_elem;
+  int32_t rank, pad;
+  uint32_t dims[TQ_MAX_RANK];
+  i64 sstr[TQ_MAX_RANK];
+  i64 dstr[TQ_MAX_RANK];
+};
+
+// Persistent copy over a box list, one kGatherPerWG-element work item per
+// wavefront (work item c belongs to box j with job_wg_start[j] <= c <
+// job_wg_start[j+1]): both offsets of each element are decoded, then all
+// loads are issued, then the stores. The first merged leg is the one that is
+// contiguous in the destination (or the source), so a wavefront moves runs
+// of consecutive elements.
+template <class W>
+__global__ __launch_bounds__(kGatherThreads) void box_jobs_kernel(
+    const BoxJob *__restrict__ jobs, const int32_t *__restrict__ job_wg_start,
+    int64_t n_jobs, const W *__restrict__ src, W *__restrict__ dst) {
+  constexpr int kWaves = kGatherThreads / 64;
+  const int32_t total = job_wg_start[n_jobs];
+  const int lane = threadIdx.x & 63;
+  const int32_t wave = __builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
+  for (int32_t c = (int32_t)blockIdx.x * kWaves + wave; c < total;
+       c += (int32_t)gridDim.x * kWaves) {
+    const int64_t j = find_job(job_wg_start, n_jobs, c);
+    const BoxJob &job = jobs[j];
+    const int64_t e0 = (int64_t)(c - job_wg_start[j]) * kGatherPerWG;
+    const int r = job.rank;
+    i64 so[kPerThread], dof[kPerThread];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+      const int64_t e = e0 + (int64_t)q * 64 + lane;
+      so[q] = -1;
+      dof[q] = 0;
+      if (e < job.n_elem) {
+        uint32_t rem = (uint32_t)e;
+        i64 s = job.src_off, d = job.dst_off;
+        for (int l = 0; l < r; ++l) {
+          const uint32_t n = job.dims[l];
+          const uint32_t qd = rem / n;
+          const uint32_t i = rem - qd * n;
+          s += (i64)i * job.sstr[l];
+          d += (i64)i * job.dstr[l];
+          rem = qd;
+        }
+        so[q] = s;
+        dof[q] = d;
+      }
+    }
+    W v[kPerThread];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) v[q] = so[q] >= 0 ? src[so[q]] : W(0);
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q)
+      if (so[q] >= 0) dst[dof[q]] = v[q];
   }
 }
 
@@ -890,6 +957,148 @@ tq_status tq_bs_slice(tq_handle_t h, tq_dtype dtype, const tq_bs_desc *d,
                                  reinterpret_cast<const int32_t *>(dmeta.as<char>() + wg_off),
                                  n_jobs, wg_total, in, out));
   TQ_HIP(h, hipStreamSynchronize(h->stream));
+  return TQ_SUCCESS;
+}
+
+}  // extern "C"
+
+// ---- box copy plans ------------------------------------------------------
+
+struct tq_copy_plan_s {
+  tq_handle_t h = nullptr;
+  tq_dtype dtype = TQ_F64;
+  int64_t n_jobs = 0;
+  int64_t total_wg = 0;
+  size_t wg_off = 0;  // byte offset of the int32 work-item starts in meta
+  DevBuf meta;        // BoxJob[n_jobs], then job_wg_start[n_jobs + 1]
+};
+
+namespace {
+
+// Largest offset a box reaches from `off` (its extents all >= 1), or -1
+// when it does not fit in int64.
+i64 box_last(i64 off, int r, const int64_t *dims, const int64_t *str) {
+  __int128 last = off;
+  for (int l = 0; l < r; ++l) last += (__int128)(dims[l] - 1) * str[l];
+  return last > (__int128)INT64_MAX ? -1 : (i64)last;
+}
+
+}  // namespace
+
+extern "C" {
+
+tq_status tq_copy_plan_create(tq_handle_t h, tq_dtype dtype, int64_t n, const tq_box *boxes,
+                              int64_t in_elems, int64_t out_elems, tq_copy_plan_t *plan) {
+  if (!h) return TQ_ERROR_NOT_INITIALIZED;
+  if (!plan || n < 0 || (n > 0 && !boxes) || in_elems < 0 || out_elems < 0)
+    return TQ_ERROR_INVALID_VALUE;
+  if (!tq_valid_dtype(dtype)) return TQ_ERROR_UNSUPPORTED;
+  *plan = nullptr;
+  // every box is checked and its legs merged before anything is uploaded
+  std::vector<BoxJob> jobs;
+  jobs.reserve((size_t)n);
+  std::vector<int32_t> wg;
+  wg.reserve((size_t)n + 1);
+  i64 wg_total = 0;
+  for (int64_t b = 0; b < n; ++b) {
+    const tq_box &x = boxes[b];
+    if (x.rank < 0 || x.rank > TQ_MAX_RANK)
+      return tq_fail(h, TQ_ERROR_UNSUPPORTED, "copy: box rank above TQ_MAX_RANK");
+    if (x.src_off < 0 || x.dst_off < 0)
+      return tq_fail(h, TQ_ERROR_INVALID_VALUE, "copy: negative box offset");
+    bool empty = false;
+    i64 n_elem = 1;
+    for (int l = 0; l < x.rank; ++l) {
+      if (x.dims[l] < 0 || x.src_strides[l] < 0 || x.dst_strides[l] < 0)
+        return tq_fail(h, TQ_ERROR_INVALID_VALUE, "copy: negative box extent or stride");
+      if (x.dims[l] == 0) empty = true;
+      else if (n_elem < (i64(1) << 40)) n_elem *= x.dims[l];
+    }
+    if (empty) continue;
+    if (n_elem >= (i64(1) << 32))
+      return tq_fail(h, TQ_ERROR_UNSUPPORTED, "copy: box above 2^32 elements");
+    const i64 ls = box_last(x.src_off, x.rank, x.dims, x.src_strides);
+    const i64 ld = box_last(x.dst_off, x.rank, x.dims, x.dst_strides);
+    if (ls < 0 || ls >= in_elems || ld < 0 || ld >= out_elems)
+      return tq_fail(h, TQ_ERROR_OUT_OF_RANGE, "copy: box outside its buffer");
+    // merge leg l into the previous kept leg when it continues it in both
+    // buffers; unit legs drop out
+    BoxJob j{};
+    j.src_off = x.src_off;
+    j.dst_off = x.dst_off;
+    j.n_elem = n_elem;
+    int r = 0;
+    for (int l = 0; l < x.rank; ++l) {
+      if (x.dims[l] == 1) continue;
+      if (r > 0 && x.src_strides[l] == j.sstr[r - 1] * (i64)j.dims[r - 1] &&
+          x.dst_strides[l] == j.dstr[r - 1] * (i64)j.dims[r - 1]) {
+        j.dims[r - 1] *= (uint32_t)x.dims[l];
+        continue;
+      }
+      j.dims[r] = (uint32_t)x.dims[l];
+      j.sstr[r] = x.src_strides[l];
+      j.dstr[r] = x.dst_strides[l];
+      ++r;
+    }
+    j.rank = r;
+    jobs.push_back(j);
+    wg.push_back((int32_t)wg_total);
+    wg_total += ceil_div(n_elem, kGatherPerWG);
+    if (wg_total > 0x7fffffff)
+      return tq_fail(h, TQ_ERROR_UNSUPPORTED, "copy: too many elements");
+  }
+  wg.push_back((int32_t)wg_total);
+  std::unique_ptr<tq_copy_plan_s> p(new tq_copy_plan_s);
+  p->h = h;
+  p->dtype = dtype;
+  p->n_jobs = (int64_t)jobs.size();
+  p->total_wg = wg_total;
+  if (p->n_jobs > 0) {
+    const size_t job_bytes = sizeof(BoxJob) * jobs.size();
+    p->wg_off = (job_bytes + 15) / 16 * 16;
+    const size_t bytes = p->wg_off + sizeof(int32_t) * wg.size();
+    void *host = nullptr;
+    TQ_CHECK(tq_stage(h, bytes, &host));
+    std::copy(jobs.begin(), jobs.end(), reinterpret_cast<BoxJob *>(host));
+    std::copy(wg.begin(), wg.end(),
+              reinterpret_cast<int32_t *>(reinterpret_cast<char *>(host) + p->wg_off));
+    TQ_HIP(h, p->meta.alloc(bytes));
+    TQ_HIP(h, hipMemcpyAsync(p->meta.p, host, bytes, hipMemcpyHostToDevice, h->stream));
+    // the staging buffer is reused by the next upload
+    TQ_HIP(h, hipStreamSynchronize(h->stream));
+  }
+  *plan = p.release();
+  return TQ_SUCCESS;
+}
+
+tq_status tq_copy_execute(tq_copy_plan_t p, const void *in, void *out) {
+  if (!p) return TQ_ERROR_NOT_INITIALIZED;
+  if (p->n_jobs == 0) return TQ_SUCCESS;
+  tq_handle_t h = p->h;
+  if (!in || !out) return TQ_ERROR_INVALID_VALUE;
+  const BoxJob *jobs = p->meta.as<BoxJob>();
+  const int32_t *wg = reinterpret_cast<const int32_t *>(p->meta.as<char>() + p->wg_off);
+  // one work item per wavefront, four wavefronts per workgroup
+  const i64 grid = std::max<i64>(
+      1, std::min<i64>(ceil_div(p->total_wg, kGatherThreads / 64), (i64)h->n_cu * 8));
+  if (p->dtype == TQ_F64)
+    hipLaunchKernelGGL(box_jobs_kernel<uint64_t>, dim3((unsigned)grid), dim3(kGatherThreads), 0,
+                       h->stream, jobs, wg, p->n_jobs, static_cast<const uint64_t *>(in),
+                       static_cast<uint64_t *>(out));
+  else if (p->dtype == TQ_F16)
+    hipLaunchKernelGGL(box_jobs_kernel<uint16_t>, dim3((unsigned)grid), dim3(kGatherThreads), 0,
+                       h->stream, jobs, wg, p->n_jobs, static_cast<const uint16_t *>(in),
+                       static_cast<uint16_t *>(out));
+  else
+    hipLaunchKernelGGL(box_jobs_kernel<uint32_t>, dim3((unsigned)grid), dim3(kGatherThreads), 0,
+                       h->stream, jobs, wg, p->n_jobs, static_cast<const uint32_t *>(in),
+                       static_cast<uint32_t *>(out));
+  TQ_HIP(h, hipGetLastError());
+  return TQ_SUCCESS;
+}
+
+tq_status tq_copy_plan_destroy(tq_copy_plan_t p) {
+  delete p;
   return TQ_SUCCESS;
 }
 

@@ -255,6 +255,24 @@ class BlockSparseTensor:
                    device=device)
 
     # -- derived metadata (src/gpu/block_sparse.cuh:199-212) --
+    @classmethod
+    def _adopt(cls, dimension, begin, end, data, dtype):
+        """A tensor over `data` as is (no copy), compact column-major blocks
+        in the order given."""
+        res = cls.__new__(cls)
+        res.dimension = np.asarray(dimension, dtype=np.int64).reshape(-1).copy()
+        res.rank = int(res.dimension.size)
+        res.dtype, res.device = dtype, data.device
+        res.begin_points = np.asarray(begin, np.int64).reshape(-1, res.rank).copy()
+        res.end_points = np.asarray(end, np.int64).reshape(-1, res.rank).copy()
+        bd = res.end_points - res.begin_points + 1
+        res.index_tables = (np.concatenate([np.ones((len(bd), 1), np.int64),
+                                            np.cumprod(bd[:, :-1], axis=1)], axis=1)
+                            if len(bd) and res.rank else np.zeros((len(bd), res.rank), np.int64))
+        res.data = data
+        res._meta_changed()
+        return res
+
     def _meta_changed(self):
         self.blocks_dimension = self.end_points - self.begin_points + 1
         self.block_n_elem = (np.prod(self.blocks_dimension, axis=1).astype(np.int64)
@@ -447,6 +465,56 @@ class BlockSparseTensor:
                           device=self.data.device)
         plan.execute(self.data, tensor.data, out)
         return plan.output_tensor(out)
+
+
+BOX_DTYPE = np.dtype([("src_off", "<i8"), ("dst_off", "<i8"), ("rank", "<i4"),
+                      ("reserved", "<i4"), ("dims", "<i8", _lib.TQ_MAX_RANK),
+                      ("src_strides", "<i8", _lib.TQ_MAX_RANK),
+                      ("dst_strides", "<i8", _lib.TQ_MAX_RANK)])
+assert BOX_DTYPE.itemsize == ctypes.sizeof(_lib.Box)
+
+
+class CopyPlan:
+    """Owner of a tq_copy_plan_t: boxes moved from a buffer of in_elems
+    elements into one of out_elems (tq_copy_plan_create). `boxes` is a
+    structured array of BOX_DTYPE, one row per tq_box."""
+
+    def __init__(self, handle, dtype, boxes, in_elems, out_elems):
+        boxes = np.ascontiguousarray(boxes, dtype=BOX_DTYPE)
+        self.h = handle
+        self.dtype = dtype
+        self.in_elems, self.out_elems = int(in_elems), int(out_elems)
+        self.n_boxes = int(boxes.shape[0])
+        p = ctypes.c_void_p()
+        handle.check(lib().tq_copy_plan_create(
+            handle.ptr, _DT[dtype], self.n_boxes,
+            boxes.ctypes.data_as(ctypes.POINTER(_lib.Box)), self.in_elems, self.out_elems,
+            ctypes.byref(p)))
+        self.ptr = p
+
+    def execute(self, src, dst):
+        """Copies the boxes from src to dst on torch's current stream."""
+        if src.dtype != self.dtype or dst.dtype != self.dtype:
+            raise TorqueError(_lib.TQ_ERROR_UNSUPPORTED, "buffers must have the plan's dtype")
+        if src.numel() < self.in_elems or dst.numel() < self.out_elems:
+            raise TorqueError(_lib.TQ_ERROR_INVALID_VALUE,
+                              "buffer shorter than the planned copy")
+        _require_gpu(src)
+        _require_gpu(dst)
+        self.h.bind_stream()
+        self.h.check(lib().tq_copy_execute(self.ptr, ctypes.c_void_p(src.data_ptr()),
+                                           ctypes.c_void_p(dst.data_ptr())))
+
+    def destroy(self):
+        if getattr(self, "ptr", None):
+            lib().tq_copy_plan_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 class ContractionPlan:

@@ -10,6 +10,13 @@ charge tuple: the sum over the contracted charges of the matching products.
 The blocks are a plain BlockSparseTensor (column-major blocks, leg 0
 fastest), which every other operation of the package takes as is.
 
+Groups of legs fuse into one leg (fuse_legs, U1Tensor.fuse) and fused legs
+split back into their parts (U1Tensor.split): each block of the unfused
+tensor is one box of a block of the fused tensor, and the device moves the
+boxes in one launch (tq_copy_plan_create / tq_copy_execute), planned once per
+block structure. A fused block-diagonal product is then a few large GEMMs
+where the unfused contraction is many small ones.
+
 The reference's tensors take their blocks as given begin and end points
 (src/gpu/block_sparse.cuh:190-283) and keep no charge information; this
 module derives the blocks from the charges and addresses them by charge.
@@ -21,14 +28,14 @@ import torch
 
 from . import _lib
 from ._lib import TorqueError
-from .tensor import BlockSparseTensor
+from .tensor import BOX_DTYPE, BlockSparseTensor, CopyPlan, default_handle
 
 
 class U1Leg:
     """Charge sectors (charge, size) of one leg, in the order they occupy
     the leg's index range, and the leg's direction (+1 or -1)."""
 
-    def __init__(self, charges, sizes, sign=1):
+    def __init__(self, charges, sizes, sign=1, parts=None):
         charges = [int(q) for q in charges]
         sizes = [int(n) for n in sizes]
         if len(charges) != len(sizes) or not charges:
@@ -46,42 +53,96 @@ class U1Leg:
         self.begins = tuple(int(b) for b in np.concatenate([[0], np.cumsum(sizes)[:-1]]))
         self.dim = int(sum(sizes))
         self._at = {b: i for i, b in enumerate(self.begins)}
+        self._index = {q: i for i, q in enumerate(self.charges)}
+        # the legs a fused leg was made of (fuse_legs), None for a plain leg
+        self.parts = tuple(parts) if parts is not None else None
 
     def dual(self):
-        """The same sectors with the opposite direction (the leg it contracts with)."""
-        return U1Leg(self.charges, self.sizes, -self.sign)
+        """The same sectors with the opposite direction (the leg it contracts
+        with); a fused leg's parts are dualised with it."""
+        parts = None if self.parts is None else [p.dual() for p in self.parts]
+        return U1Leg(self.charges, self.sizes, -self.sign, parts)
 
     def same_sectors(self, other):
-        return self.charges == other.charges and self.sizes == other.sizes
+        """Same sectors, and for fused legs the same parts in the same order
+        (so the sector tuples sit at the same places inside each sector)."""
+        if self.charges != other.charges or self.sizes != other.sizes:
+            return False
+        if (self.parts is None) != (other.parts is None):
+            return False
+        return self.parts is None or (
+            len(self.parts) == len(other.parts) and
+            all(p.same_sectors(q) for p, q in zip(self.parts, other.parts)))
+
+    def sector(self, charge):
+        """Index of the sector of `charge`."""
+        return self._index[int(charge)]
 
     def charge_at(self, begin):
         """Charge of the sector starting at index `begin`."""
         return self.charges[self._at[int(begin)]]
 
     def __eq__(self, other):
-        return isinstance(other, U1Leg) and self.same_sectors(other) and self.sign == other.sign
+        return (isinstance(other, U1Leg) and self.same_sectors(other) and
+                self.sign == other.sign and
+                (self.parts is None or all(p == q for p, q in zip(self.parts, other.parts))))
 
     def __repr__(self):
+        if self.parts is not None:
+            return "U1Leg(fused %s, sign=%+d)" % (list(self.parts), self.sign)
         return "U1Leg(%s, %s, sign=%+d)" % (list(self.charges), list(self.sizes), self.sign)
+
+
+def fuse_legs(legs):
+    """The leg fusing `legs` (legs[0] fastest) and the place of each sector
+    tuple in it. The fused leg points like legs[0] (sign s) and has one
+    sector per distinct charge Q = s * sum_l s_l q_l, in ascending order;
+    Q's sector holds the sector tuples of charge Q one after another (leg 0's
+    sector fastest), each as a run of prod(sizes) indices ordered leg 0
+    fastest. Conservation is kept: s * Q is the tuple's signed charge, and
+    the dual legs fuse into the dual of the fused leg.
+
+    Returns (fused leg, {sector-index tuple: (Q, offset in Q's sector)})."""
+    legs = list(legs)
+    if not legs:
+        raise TorqueError(_lib.TQ_ERROR_INVALID_VALUE, "nothing to fuse")
+    s = legs[0].sign
+    combos = {}
+    for rev in itertools.product(*[range(len(l.charges)) for l in legs[::-1]]):
+        idx = rev[::-1]
+        q = s * sum(l.sign * l.charges[i] for l, i in zip(legs, idx))
+        combos.setdefault(q, []).append(idx)
+    place, sizes = {}, []
+    charges = sorted(combos)
+    for q in charges:
+        off = 0
+        for idx in combos[q]:
+            place[idx] = (q, off)
+            off += int(np.prod([l.sizes[i] for l, i in zip(legs, idx)]))
+        sizes.append(off)
+    return U1Leg(charges, sizes, s, parts=legs), place
 
 
 def structure(legs, flux=0):
     """Charge tuples and begin/end points of every charge-conserving block
     (sum over legs of sign * charge == flux), leg 0's sector fastest."""
-    keys, begin, end = [], [], []
-    ranges = [range(len(l.charges)) for l in legs]
-    # itertools.product varies the last factor fastest: reverse for leg 0
-    for rev in itertools.product(*ranges[::-1]):
-        idx = rev[::-1]
-        q = sum(l.sign * l.charges[i] for l, i in zip(legs, idx))
-        if q != flux:
-            continue
-        keys.append(tuple(l.charges[i] for l, i in zip(legs, idx)))
-        begin.append([l.begins[i] for l, i in zip(legs, idx)])
-        end.append([l.begins[i] + l.sizes[i] - 1 for l, i in zip(legs, idx)])
     r = len(legs)
-    return (keys, np.asarray(begin, np.int64).reshape(-1, r),
-            np.asarray(end, np.int64).reshape(-1, r))
+    counts = [len(l.charges) for l in legs]
+    # every sector tuple, leg 0 fastest: row-major indices over the reversed legs
+    idx = np.indices(counts[::-1]).reshape(r, -1)[::-1].T if r else np.zeros((1, 0), np.int64)
+    q = np.zeros(len(idx), np.int64)
+    for l, leg in enumerate(legs):
+        q += leg.sign * np.asarray(leg.charges, np.int64)[idx[:, l]]
+    idx = idx[q == flux]
+    ch = np.stack([np.asarray(l.charges, np.int64)[idx[:, k]] for k, l in enumerate(legs)],
+                  axis=1) if r else np.zeros((len(idx), 0), np.int64)
+    begin = np.stack([np.asarray(l.begins, np.int64)[idx[:, k]] for k, l in enumerate(legs)],
+                     axis=1) if r else np.zeros((len(idx), 0), np.int64)
+    size = np.stack([np.asarray(l.sizes, np.int64)[idx[:, k]] for k, l in enumerate(legs)],
+                    axis=1) if r else np.zeros((len(idx), 0), np.int64)
+    keys = [tuple(int(x) for x in row) for row in ch]
+    n = len(idx)
+    return keys, begin.reshape(n, r), (begin + size - 1).reshape(n, r)
 
 
 class U1Tensor:
@@ -161,7 +222,7 @@ class U1Tensor:
             if not la.same_sectors(lb):
                 raise TorqueError(_lib.TQ_ERROR_INVALID_VALUE,
                                   "contracted legs %d and %d carry different sectors" % (a, b))
-            if la.sign != -lb.sign:
+            if not _opposite(la, lb):
                 raise TorqueError(_lib.TQ_ERROR_INVALID_VALUE,
                                   "contracted legs %d and %d must point in opposite "
                                   "directions" % (a, b))
@@ -171,6 +232,103 @@ class U1Tensor:
         legs = ([l for i, l in enumerate(self.legs) if i not in ca] +
                 [l for j, l in enumerate(other.legs) if j not in cb])
         return U1Tensor._wrap(out, legs, self.flux + other.flux)
+
+    # -- leg fusion --
+    @classmethod
+    def _from_data(cls, legs, flux, data, layout=None):
+        """The tensor holding every conserving block of legs / flux, over
+        `data` as is (no copy); layout = structure(legs, flux) if known."""
+        keys, begin, end = layout if layout is not None else structure(legs, flux)
+        t = cls.__new__(cls)
+        t.legs = list(legs)
+        t.flux = int(flux)
+        t.bs = BlockSparseTensor._adopt([l.dim for l in t.legs], begin, end, data, data.dtype)
+        t._set_keys(keys)
+        return t
+
+    def fusion_plan(self, groups, handle=None):
+        """The plan of fuse(groups) for tensors of this block structure
+        (cached per handle)."""
+        groups = [[int(l) for l in g] for g in groups]
+        if any(not g for g in groups) or \
+                sorted(l for g in groups for l in g) != list(range(self.rank)):
+            raise TorqueError(_lib.TQ_ERROR_INVALID_VALUE,
+                              "the groups must name every leg exactly once")
+        h = handle or default_handle(self.bs.data.device.index)
+        key = ("fuse", self.bs.structure_key(), _legs_sig(self.legs), self.flux,
+               str(self.bs.dtype), tuple(tuple(g) for g in groups))
+        plan = h.plans.get(key)
+        if plan is None:
+            fused, places = [], []
+            for g in groups:
+                if len(g) == 1:
+                    leg = self.legs[g[0]]
+                    fused.append(leg)
+                    places.append({(i,): (q, 0) for i, q in enumerate(leg.charges)})
+                else:
+                    leg, place = fuse_legs([self.legs[l] for l in g])
+                    fused.append(leg)
+                    places.append(place)
+            layout = structure(fused, self.flux)
+            fus = U1Tensor._from_data(fused, self.flux, self.bs.data[:1], layout)
+            boxes, covered, _ = _fusion_boxes(self, fus, groups, places)
+            n_out = int(fus.bs.block_n_elem.sum())
+            copy = CopyPlan(h, self.bs.dtype, boxes, int(self.bs.block_n_elem.sum()), n_out)
+            # a fused block the blocks of this tensor do not fill is zeroed first
+            zero = bool(np.any(covered != fus.bs.block_n_elem))
+            plan = FusionPlan(copy, fused, self.flux, layout, n_out, zero)
+            h.plans[key] = plan
+        return plan
+
+    def fuse(self, groups):
+        """Fuses each group of legs into one leg (fuse_legs): leg g of the
+        result fuses legs groups[g] of this tensor, groups[g][0] fastest; a
+        group of one leg keeps its leg. Every block of this tensor becomes a
+        box of one block of the result (one device copy launch)."""
+        return self.fusion_plan(groups).run(self)
+
+    def split_plan(self, handle=None):
+        """The plan of split() for tensors of this block structure (cached
+        per handle)."""
+        h = handle or default_handle(self.bs.data.device.index)
+        key = ("split", self.bs.structure_key(), _legs_sig(self.legs), self.flux,
+               str(self.bs.dtype))
+        plan = h.plans.get(key)
+        if plan is None:
+            groups, legs, places = [], [], []
+            for leg in self.legs:
+                parts = list(leg.parts) if leg.parts is not None else [leg]
+                groups.append(list(range(len(legs), len(legs) + len(parts))))
+                legs += parts
+                if leg.parts is None:
+                    places.append({(i,): (q, 0) for i, q in enumerate(leg.charges)})
+                else:
+                    again, place = fuse_legs(parts)
+                    if not again.same_sectors(leg) or again.sign != leg.sign:
+                        raise TorqueError(_lib.TQ_ERROR_INVALID_VALUE,
+                                          "a fused leg does not match the fusion of its parts")
+                    places.append(place)
+            layout = structure(legs, self.flux)
+            unf = U1Tensor._from_data(legs, self.flux, self.bs.data[:1], layout)
+            boxes, _, kept = _fusion_boxes(unf, self, groups, places)
+            # the boxes move fused -> unfused
+            boxes["src_off"], boxes["dst_off"] = boxes["dst_off"].copy(), boxes["src_off"].copy()
+            boxes["src_strides"], boxes["dst_strides"] = (boxes["dst_strides"].copy(),
+                                                          boxes["src_strides"].copy())
+            n_out = int(unf.bs.block_n_elem.sum())
+            copy = CopyPlan(h, self.bs.dtype, boxes, int(self.bs.block_n_elem.sum()), n_out)
+            # blocks whose fused block this tensor does not hold are zero
+            plan = FusionPlan(copy, legs, self.flux, layout, n_out, not bool(kept.all()))
+            h.plans[key] = plan
+        return plan
+
+    def split(self):
+        """Splits every fused leg back into its parts (the inverse of fuse):
+        the result's legs are the parts of each fused leg in order, plain
+        legs kept, one block per conserving sector tuple."""
+        if all(l.parts is None for l in self.legs):
+            return self
+        return self.split_plan().run(self)
 
     def transpose(self, permutation):
         """Leg i of the result is leg permutation[i] (a hard transpose of
@@ -194,3 +352,98 @@ class U1Tensor:
 
     def __repr__(self):
         return "U1Tensor(legs=%s, flux=%d, blocks=%d)" % (self.legs, self.flux, self.n_blocks)
+
+
+def _opposite(la, lb):
+    """Whether la and lb point in opposite directions, and so do the parts
+    of fused legs pairwise (the sector tuples then sit at the same places)."""
+    if la.sign != -lb.sign:
+        return False
+    return la.parts is None or all(_opposite(p, q) for p, q in zip(la.parts, lb.parts))
+
+
+def _legs_sig(legs):
+    """Hashable description of legs (sectors, directions and parts)."""
+    return tuple((l.charges, l.sizes, l.sign,
+                  None if l.parts is None else _legs_sig(l.parts)) for l in legs)
+
+
+def _fusion_boxes(unf, fus, groups, places):
+    """Box copies between the blocks of `unf` and those of `fus`, whose leg g
+    fuses legs groups[g] of `unf` (sector tuples placed by places[g]).
+
+    Returns (boxes moving unf -> fus, elements covered per fus block, which
+    unf blocks have a fus block). A box iterates the unfused block's legs
+    group by group (leg groups[0][0] fastest), so the fused side is written
+    (or read) in runs along its first leg."""
+    nb = unf.n_blocks
+    order = [l for g in groups for l in g]
+    r = len(order)
+    if r > _lib.TQ_MAX_RANK:
+        raise TorqueError(_lib.TQ_ERROR_UNSUPPORTED, "rank above TQ_MAX_RANK")
+    ubd, ut, uoff = unf.bs.blocks_dimension, unf.bs.index_tables, unf.bs.block_offsets
+    ft, foff = fus.bs.index_tables, fus.bs.block_offsets
+    src_off = np.zeros(nb, np.int64)
+    dst_off = np.zeros(nb, np.int64)
+    dims = np.ones((nb, _lib.TQ_MAX_RANK), np.int64)
+    sstr = np.zeros((nb, _lib.TQ_MAX_RANK), np.int64)
+    dstr = np.zeros((nb, _lib.TQ_MAX_RANK), np.int64)
+    kept = np.zeros(nb, bool)
+    covered = np.zeros(fus.n_blocks, np.int64)
+    for b, key in enumerate(unf.keys):
+        fkey, offs = [], []
+        for g, place in zip(groups, places):
+            q, o = place[tuple(unf.legs[l].sector(key[l]) for l in g)]
+            fkey.append(q)
+            offs.append(o)
+        fb = fus._index.get(tuple(fkey))
+        if fb is None:
+            continue
+        kept[b] = True
+        src_off[b] = uoff[b]
+        dst_off[b] = foff[fb] + sum(o * int(ft[fb, g]) for g, o in enumerate(offs))
+        k = 0
+        for g, ls in enumerate(groups):
+            c = 1
+            for l in ls:
+                dims[b, k] = ubd[b, l]
+                sstr[b, k] = ut[b, l]
+                dstr[b, k] = c * int(ft[fb, g])
+                c *= int(ubd[b, l])
+                k += 1
+        covered[fb] += int(unf.bs.block_n_elem[b])
+    boxes = np.zeros(int(kept.sum()), BOX_DTYPE)
+    boxes["src_off"], boxes["dst_off"] = src_off[kept], dst_off[kept]
+    boxes["rank"] = r
+    boxes["dims"], boxes["src_strides"], boxes["dst_strides"] = dims[kept], sstr[kept], dstr[kept]
+    return boxes, covered, kept
+
+
+class FusionPlan:
+    """The box copies that move a tensor of one block structure into the
+    fused layout of a leg grouping, or back (U1Tensor.fusion_plan,
+    U1Tensor.split_plan): planned once, run on any data of that structure."""
+
+    def __init__(self, copy, legs, flux, layout, n_out, zero):
+        self.copy = copy
+        self.legs, self.flux, self.layout = list(legs), int(flux), layout
+        self.n_out = int(n_out)
+        self.zero = bool(zero)
+
+    def new_output(self, dtype, device):
+        alloc = torch.zeros if self.zero else torch.empty
+        return alloc(max(self.n_out, 1), dtype=dtype, device=device)
+
+    def execute(self, src, out):
+        """Moves the blocks of data `src` into `out` (from new_output: the
+        elements no block fills must already be zero)."""
+        self.copy.execute(src, out)
+
+    def run(self, t, out=None):
+        if out is None:
+            out = self.new_output(t.bs.dtype, t.bs.data.device)
+        self.execute(t.bs.data, out)
+        return U1Tensor._from_data(self.legs, self.flux, out, self.layout)
+
+    def destroy(self):
+        self.copy.destroy()
