@@ -91,7 +91,7 @@ def parse():
     p.add_argument("--no-large", action="store_true",
                    help="skip the D=10, chi=256 enlarged-corner product")
     p.add_argument("--workload", default="peps",
-                   choices=["peps", "batched", "permute", "large", "shapes"],
+                   choices=["peps", "batched", "permute", "large", "shapes", "corner6"],
                    help="peps: the D, chi absorption step (default); batched: --n-batch "
                         "independent rank-4 x rank-4 contractions in one captured launch")
     p.add_argument("--n-batch", type=int, default=512)
@@ -464,6 +464,174 @@ def bench_large(dev, steps=5, warmup=2, tiers=("fp32", "bf16x3"), oracle_max_blo
                                      % (oracle_max_block, sample_flops / 1e9, cpu_s)}}
 
 
+def bench_corner_rank6(dev, steps=5, warmup=2, D=10, chi=256, seed=0, oracle_max_block=1368):
+    """The corner product on enlarged corners held as rank-6 U(1) tensors
+    C(chi, D ket, D bra | chi, D ket, D bra) (charges -3..3 at chi > 128),
+    contracted over the three legs of the shared side, two ways:
+      * fused: both corners fused into block-diagonal matrices over the fused
+        (chi, D, D) legs (U1Tensor.fuse, one box-copy launch each), one GEMM
+        per fused charge (19 GEMMs of 4..3600 at D=10, chi=256), and the
+        product split back into rank 6; FP64 and split-BF16;
+      * unfused: the merged-output contraction of the rank-6 tensors (one
+        K-concatenated GEMM per output block), FP64.
+    Times each step with HIP events (whole chains, and fuse / product /
+    split alone), compares the two paths' FP64 results, and checks the fused
+    product's blocks up to oracle_max_block against the oracle."""
+    import torch
+
+    from oracle import oracle as O
+    from torque_amd import CopyPlan, U1Leg, U1Tensor, default_handle
+    from torque_amd import workloads as W
+    from torque_amd.tensor import BOX_DTYPE
+
+    lchi, lD, _ = W.peps_legs(D, chi)
+    lc = U1Leg([q for q, _, _ in lchi], [n for _, _, n in lchi])
+    ld = U1Leg([q for q, _, _ in lD], [n for _, _, n in lD])
+    legs = [lc, ld, ld.dual(), lc.dual(), ld.dual(), ld]
+    g = torch.Generator(device=dev).manual_seed(seed)
+    C1 = U1Tensor.random(legs, 0, g, device=dev)
+    C2 = U1Tensor.random(legs, 0, g, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    groups = [[0, 1, 2], [3, 4, 5]]
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    t0 = time.perf_counter()
+    fplan = C1.fusion_plan(groups)  # C2 has the same structure: the same plan
+    F1, F2 = fplan.run(C1), fplan.run(C2)
+    torch.cuda.synchronize()
+    fuse_plan_s = time.perf_counter() - t0
+    n_elem = int(C1.bs.block_n_elem.sum())
+    n_blocks6 = C1.n_blocks
+
+    def fuse_both():
+        fplan.execute(C1.bs.data, F1.bs.data)
+        fplan.execute(C2.bs.data, F2.bs.data)
+
+    fuse_ms = timed(fuse_both)
+    sizes = [int(n) for n in F1.legs[0].sizes]
+
+    # oracle on the fused blocks up to oracle_max_block (block-diagonal: key (q, q))
+    sel = [i for i, k in enumerate(F1.keys) if F1.legs[0].sizes[F1.legs[0].sector(k[0])]
+           <= oracle_max_block]
+
+    def sub(F):
+        host = F.bs.data.cpu().numpy()
+        offs, nel = F.bs.block_offsets, F.bs.block_n_elem
+        data = np.concatenate([host[offs[i]:offs[i] + nel[i]] for i in sel])
+        return O.BS(F.bs.dimension, F.bs.begin_points[sel], F.bs.end_points[sel]), data
+
+    TA, dA = sub(F1)
+    TB, dB = sub(F2)
+    threads = min(16, os.cpu_count() or 1)
+    ref, ref_plan = O.contract(TA, dA, TB, dB, [[1, 0]], threads=threads)
+    ref_off = {tuple(b): o for b, o in zip(ref_plan["out_begin"].tolist(),
+                                            ref_plan["out_offsets"].tolist())}
+
+    res = {"fuse_both_ms": round(fuse_ms, 4),
+           "fuse_gbs": round(2 * 2 * n_elem * 8 / fuse_ms / 1e6, 1),
+           "fuse_plan_s": round(fuse_plan_s, 3)}
+    split_out = None
+    flops = None
+    for prec in ("native", "bf16x3"):
+        p = F1.bs.plan_contract(F2.bs, [[1, 0]], precision=prec, merged=True)
+        flops = p.flops
+        out = torch.empty(max(p.out_n_elem, 1), dtype=torch.float64, device=dev)
+        best, _ = p.autotune(F1.bs.data, F2.bs.data, out, reps=2)
+        Fc = U1Tensor._wrap(p.output_tensor(out), [F1.legs[0], F2.legs[1]], C1.flux + C2.flux)
+        t0 = time.perf_counter()
+        splan = Fc.split_plan()
+        split_plan_s = time.perf_counter() - t0
+        cout = splan.new_output(torch.float64, dev)
+        prod_ms = timed(lambda: p.execute(F1.bs.data, F2.bs.data, out))
+        split_ms = timed(lambda: splan.execute(out, cout))
+
+        def chain():
+            fuse_both()
+            p.execute(F1.bs.data, F2.bs.data, out)
+            splan.execute(out, cout)
+
+        chain_ms = timed(chain)
+        host = out.cpu().numpy()
+        got = np.concatenate([host[int(o):int(o) + int(n)] for b, o, n in zip(
+            Fc.bs.begin_points.tolist(), Fc.bs.block_offsets, Fc.bs.block_n_elem)
+            if tuple(b) in ref_off])
+        want = np.concatenate([ref[ref_off[tuple(b)]:ref_off[tuple(b)] + int(n)] for b, n in zip(
+            Fc.bs.begin_points.tolist(), Fc.bs.block_n_elem) if tuple(b) in ref_off])
+        err = float(np.linalg.norm(got - want) / np.linalg.norm(want))
+        name = "fp64" if prec == "native" else prec
+        res[name] = {"chain_ms": round(chain_ms, 4), "chain_gflops": round(flops / chain_ms / 1e6, 1),
+                     "product_ms": round(prod_ms, 4),
+                     "product_gflops": round(flops / prod_ms / 1e6, 1),
+                     "split_ms": round(split_ms, 4), "split_plan_s": round(split_plan_s, 3),
+                     "gemm_config": best, "rel_err_vs_oracle": err}
+        print("# rank-6 corner product D=%d chi=%d %s, fused: chain %.3f ms (%.1f GFLOP/s) = fuse "
+              "both %.3f + product %.3f + split %.3f ms; rel err vs oracle %.2e (fused blocks "
+              "up to %d)" % (D, chi, name, chain_ms, flops / chain_ms / 1e6, fuse_ms, prod_ms,
+                              split_ms, err, oracle_max_block), flush=True)
+        if prec == "native":
+            split_out = splan.run(Fc)
+        p.destroy()
+        del out, cout
+
+    # the unfused path: merged-output contraction of the rank-6 tensors
+    t0 = time.perf_counter()
+    pd = C1.bs.plan_contract(C2.bs, [[3, 0], [4, 1], [5, 2]], merged=True)
+    outd = torch.empty(max(pd.out_n_elem, 1), dtype=torch.float64, device=dev)
+    pd.autotune(C1.bs.data, C2.bs.data, outd, reps=2)
+    torch.cuda.synchronize()
+    direct_plan_s = time.perf_counter() - t0
+    direct_ms = timed(lambda: pd.execute(C1.bs.data, C2.bs.data, outd))
+    Cd = U1Tensor._wrap(pd.output_tensor(outd), split_out.legs, 0)
+    # both results block by block: the direct blocks moved into the split
+    # result's block order by one copy launch
+    boxes = np.zeros(split_out.n_blocks, BOX_DTYPE)
+    for i, k in enumerate(split_out.keys):
+        j = Cd._index[k]
+        boxes[i]["src_off"] = Cd.bs.block_offsets[j]
+        boxes[i]["dst_off"] = split_out.bs.block_offsets[i]
+        boxes[i]["rank"] = 1
+        boxes[i]["dims"][0] = split_out.bs.block_n_elem[i]
+        boxes[i]["src_strides"][0] = 1
+        boxes[i]["dst_strides"][0] = 1
+    h = default_handle(dev.index)
+    reorder = CopyPlan(h, torch.float64, boxes, int(Cd.bs.block_n_elem.sum()), n_elem)
+    moved = torch.zeros_like(split_out.bs.data)
+    reorder.execute(Cd.bs.data, moved)
+    diff = float(torch.linalg.vector_norm(moved[:n_elem] - split_out.bs.data[:n_elem]) /
+                 torch.linalg.vector_norm(moved[:n_elem]))
+    reorder.destroy()
+    res["unfused_fp64"] = {"ms": round(direct_ms, 4), "gflops": round(pd.flops / direct_ms / 1e6, 1),
+                           "pairs": int(pd.n_matched), "out_blocks": int(pd.n_pairs),
+                           "plan_s": round(direct_plan_s, 3),
+                           "rel_diff_vs_fused": diff}
+    print("# rank-6 corner product D=%d chi=%d fp64, unfused merged contraction: %.3f ms "
+          "(%.1f GFLOP/s), %d pairs into %d blocks; rel diff vs the fused chain %.2e"
+          % (D, chi, direct_ms, pd.flops / direct_ms / 1e6, pd.n_matched, pd.n_pairs, diff),
+          flush=True)
+    res["fp64"]["speedup_vs_unfused"] = round(direct_ms / res["fp64"]["chain_ms"], 3)
+    res["bf16x3"]["chain_speedup_vs_fp64"] = round(res["fp64"]["chain_ms"] /
+                                                   res["bf16x3"]["chain_ms"], 3)
+    pd.destroy()
+    del outd, Cd, split_out, moved, F1, F2, C1, C2
+    torch.cuda.empty_cache()
+    return dict({"workload": "enlarged_corner_product_D%d_chi%d_rank6" % (D, chi),
+                 "blocks_per_corner": n_blocks6,
+                 "fused_blocks": sizes, "gflop": round(flops / 1e9, 3),
+                 "elements_per_corner": n_elem}, **res)
+
+
 def bench_split_k(dev, reps=20):
     """Split-K on a contraction with few output tiles and long contracted
     ranges (block-diagonal, 8 blocks of 64 x 16384 times 16384 x 64, FP64):
@@ -656,11 +824,12 @@ def main():
         torch.cuda.set_device(dev)
         print(json.dumps(bench_shapes(dev)), flush=True)
         return
-    if args.workload in ("permute", "large"):
+    if args.workload in ("permute", "large", "corner6"):
         import torch
         dev = torch.device("cuda:0")
         torch.cuda.set_device(dev)
-        fn = bench_permute if args.workload == "permute" else bench_large
+        fn = {"permute": bench_permute, "large": bench_large,
+              "corner6": bench_corner_rank6}[args.workload]
         print(json.dumps(fn(dev)), flush=True)
         return
     import torch
@@ -909,6 +1078,7 @@ def main():
     permute = None if args.no_permute else bench_permute(dev)
     large = None if args.no_large else bench_large(dev)
     corner128 = None if args.no_large else bench_large(dev, D=8, chi=128, oracle_max_block=4096)
+    corner6 = None if args.no_large else bench_corner_rank6(dev)
     batched = None
     if not args.no_batched:
         bl = bench_batched(args)
@@ -940,6 +1110,7 @@ def main():
         "enlarged_corner_D8_chi128": corner128,
         "permute_rank6_200M": permute,
         "enlarged_corner_D10_chi256": large,
+        "enlarged_corner_D10_chi256_rank6": corner6,
         "batched_512_u1_rank4x4": batched,
     }
     print(json.dumps(line), flush=True)

@@ -232,6 +232,9 @@ tq_status tq_bs_contract_plan(tq_handle_t handle, const tq_bs_desc *a,
  * (not at TQ_PREC_BF16X3) the blocks are instead streamed from the unmerged
  * plan's operand panels, summing the pairs' products in the same order (the
  * same bits); such a plan cannot join a batch (tq_batch_create refuses it).
+ * When every pair writes an output block of its own (a block-diagonal
+ * product, for one) the merged output is the unmerged one and the plan is
+ * the unmerged plan, operand layouts included.
  * tq_plan_get_pairs still lists the matched pairs (tq_plan_info.n_matched);
  * tq_plan_get_output and the worklist list the merged blocks. */
 tq_status tq_bs_contract_plan_merged(tq_handle_t handle, const tq_bs_desc *a,

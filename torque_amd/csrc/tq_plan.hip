@@ -1530,6 +1530,21 @@ static tq_status plan_contract(tq_handle_t h, const tq_bs_desc *a, const tq_bs_d
   TQ_HIP(h, hipMemcpyAsync(pl->h_c_off.data(), pl->c_off.p, sizeof(i64) * (np + 1),
                            hipMemcpyDeviceToHost, h->stream));
   TQ_HIP(h, hipStreamSynchronize(h->stream));
+  // a merged plan whose pairs all write distinct output blocks is the
+  // unmerged plan (one pair per block, in the same order): it keeps the
+  // unmerged operand layouts (block-diagonal products read their blocks in
+  // place) instead of copying every pair into a group panel of its own
+  if (merged) {
+    std::map<std::vector<i64>, int> seen;
+    std::vector<i64> key(2 * ro);
+    bool distinct = true;
+    for (int64_t p = 0; p < np && distinct; ++p) {
+      std::copy_n(pl->h_out_begin.begin() + p * ro, ro, key.begin());
+      std::copy_n(pl->h_out_end.begin() + p * ro, ro, key.begin() + ro);
+      distinct = seen.emplace(key, 0).second;
+    }
+    if (distinct) merged = false;
+  }
   // a merged plan streams its groups when every pair is short and no group
   // holds more than kGroupMaxMembers pairs (not for the split-BF16 tier,
   // which streams from split panels)
