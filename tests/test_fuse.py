@@ -280,3 +280,51 @@ def test_fused_legs_must_match_part_for_part(cuda):
                         axes=([2, 3], [0, 1]))
     got = ok.split().to_dense().cpu().numpy()
     assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+
+
+@pytest.mark.gpu
+def test_fused_corner_product_full_size(cuda):
+    # the D=10, chi=256 enlarged corners as rank-6 U(1) tensors (9,331 blocks,
+    # 66.35 M values each): the fused chain (fuse, 19 block-diagonal GEMMs,
+    # split) against the unfused merged contraction over three legs (287,503
+    # pairs), and the split-BF16 fused product against FP64 block by block
+    lchi, lD, _ = W.peps_legs(10, 256)
+    chi, D = u1leg(lchi), u1leg(lD)
+    legs = [chi, D, D.dual(), chi.dual(), D.dual(), D]
+    g = torch.Generator(device=cuda).manual_seed(21)
+    C1 = U1Tensor.random(legs, 0, g, device=cuda)
+    C2 = U1Tensor.random(legs, 0, g, device=cuda)
+    assert C1.n_blocks == 9331 and int(C1.bs.block_n_elem.sum()) == 66349632
+    F1 = C1.fuse([[0, 1, 2], [3, 4, 5]])
+    F2 = C2.fuse([[0, 1, 2], [3, 4, 5]])
+    assert [int(n) for n in F1.legs[0].sizes] == [n for _, _, n in
+                                                 W.fused_leg([lchi, lD, lD], [1, 1, -1])]
+    F = F1.contract(F2, [(1, 0)])
+    C = F.split()
+    direct = C1.contract(C2, [(3, 0), (4, 1), (5, 2)])
+    assert sorted(direct.keys) == sorted(C.keys)
+    # the direct blocks moved into the chain's block order by one copy launch
+    boxes = np.zeros(C.n_blocks, BOX_DTYPE)
+    for i, k in enumerate(C.keys):
+        j = direct._index[k]
+        assert np.array_equal(direct.bs.blocks_dimension[j], C.bs.blocks_dimension[i])
+        boxes[i]["src_off"] = direct.bs.block_offsets[j]
+        boxes[i]["dst_off"] = C.bs.block_offsets[i]
+        boxes[i]["rank"] = 1
+        boxes[i]["dims"][0] = C.bs.block_n_elem[i]
+        boxes[i]["src_strides"][0] = 1
+        boxes[i]["dst_strides"][0] = 1
+    n = int(C.bs.block_n_elem.sum())
+    moved = torch.empty(n, dtype=torch.float64, device=cuda)
+    plan = CopyPlan(default_handle(cuda.index), torch.float64, boxes,
+                    int(direct.bs.block_n_elem.sum()), n)
+    plan.execute(direct.bs.data, moved)
+    plan.destroy()
+    ref = moved.norm()
+    assert float((moved - C.bs.data[:n]).norm() / ref) <= 1e-12
+    # split-BF16 on the fused product: every block to 1e-6 of FP64
+    Fb = F1.contract(F2, [(1, 0)], precision="bf16x3")
+    assert Fb.keys == F.keys
+    for k in F.keys:
+        x, y = F.block(k), Fb.block(k)
+        assert float((x - y).norm() / x.norm()) <= 1e-6, k
