@@ -122,7 +122,60 @@ struct CopyJob {
   int32_t nk;
   int32_t dims[TQ_MAX_RANK];
   i64 strides[TQ_MAX_RANK];
+  // ceil(2^32 / dims[l]) (0 for a unit leg): the kernels split an index by
+  // a leg with a multiply-high and one correction (job_div) instead of an
+  // integer division
+  uint32_t magic[TQ_MAX_RANK];
+  // plain gathers only: dst_strided = 1 writes element e to dst_off +
+  // sum_l i_l * dst_strides[l] instead of dst_off + e (a pair's slice of a
+  // k-fast group panel, whose rows are longer than the pair's K)
+  int32_t dst_strided;
+  int32_t pad;
+  i64 dst_strides[TQ_MAX_RANK];
 };
+
+// Last step of building any CopyJob. A plain gather (kdim == 0) drops its
+// unit legs and merges each leg into the previous one when it continues it
+// in the source and in the destination (a compact destination always
+// continues): a block moved into a panel with its legs regrouped decodes
+// few legs. Then the reciprocal of every leg.
+__host__ __device__ inline void finalize_job(CopyJob &j) {
+  if (j.kdim <= 0) {
+    int r = 0;
+    for (int l = 0; l < j.rank; ++l) {
+      const int32_t d = j.dims[l];
+      if (d == 1) continue;
+      if (r > 0 && j.strides[r - 1] * (i64)j.dims[r - 1] == j.strides[l] &&
+          (!j.dst_strided || j.dst_strides[r - 1] * (i64)j.dims[r - 1] == j.dst_strides[l]) &&
+          (i64)j.dims[r - 1] * d < (i64(1) << 31)) {
+        j.dims[r - 1] *= d;
+        continue;
+      }
+      j.dims[r] = d;
+      j.strides[r] = j.strides[l];
+      j.dst_strides[r] = j.dst_strides[l];
+      ++r;
+    }
+    for (int l = r; l < j.rank; ++l) {
+      j.dims[l] = 1;
+      j.strides[l] = 0;
+      j.dst_strides[l] = 0;
+    }
+    j.rank = r;
+  }
+  for (int l = 0; l < TQ_MAX_RANK; ++l) {
+    const uint64_t d = l < j.rank && j.dims[l] > 1 ? (uint64_t)j.dims[l] : 0;
+    j.magic[l] = d ? (uint32_t)((((uint64_t)1 << 32) + d - 1) / d) : 0u;
+  }
+}
+
+// e / d for any 32-bit e from m = ceil(2^32 / d) (m = 0: d = 1): the
+// multiply-high exceeds the quotient by at most one.
+__host__ __device__ inline uint32_t job_div(uint32_t e, uint32_t d, uint32_t m) {
+  if (m == 0) return e;
+  const uint32_t q = (uint32_t)(((uint64_t)e * m) >> 32);
+  return (uint64_t)q * d > e ? q - 1 : q;
+}
 
 // Work items of the pack kernels per job, one wavefront each: chunks of
 // kGatherPerWG elements for a plain gather. A split-BF16 job is cut into its

@@ -496,7 +496,10 @@ tq_status tq_plan_choose_config(tq_plan_t pl, tq_costmodel_t md, int32_t *best) 
     // Splitting adds parallelism, so it is weighed only for a class whose
     // whole-pair tiles under the best unsplit configuration leave the GPU
     // short of two rounds of workgroups (the model, trained on whole
-    // pairs, misjudges slices of classes that already fill it).
+    // pairs, misjudges slices of classes that already fill it), and a split
+    // is taken only when it is predicted at least 20% faster than the best
+    // whole-pair configuration (a margin for that extrapolation).
+    double whole_t = 0;
     for (int sp : {1, 2, 4, 8, 16}) {
       if (sp > 1 && arg >= 0) {
         const GemmConfig g = tq_gemm_config(pl->prec, arg);
@@ -528,12 +531,14 @@ tq_status tq_plan_choose_config(tq_plan_t pl, tq_costmodel_t md, int32_t *best) 
       for (int q = 0; q < nc; ++q) {
         if (!plan_config_ok(*pl, q)) continue;
         const double t = std::exp((double)pred[q]) + reduce_ms;
+        if (sp > 1 && t > 0.8 * whole_t) continue;
         if (arg < 0 || t < arg_t) {
           arg = q;
           arg_t = t;
           arg_split = sp;
         }
       }
+      if (sp == 1) whole_t = arg_t;
     }
     pl->drop_graph();
     pl->cls[c].cfg = arg < 0 ? -1 : arg;

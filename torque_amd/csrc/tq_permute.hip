@@ -46,7 +46,7 @@ __device__ inline void gather_range(const CopyJob &job, int64_t e0, int stride, 
     i64 o = job.src_off;
     for (int l = 0; l < r; ++l) {
       const uint32_t d = (uint32_t)job.dims[l];
-      const uint32_t qd = rem / d;
+      const uint32_t qd = job_div(rem, d, job.magic[l]);
       o += (i64)(rem - qd * d) * job.strides[l];
       rem = qd;
     }
@@ -72,6 +72,42 @@ __device__ inline void gather_range(const CopyJob &job, int64_t e0, int stride, 
     const int64_t e = e0 + (int64_t)q * stride + t;
     if (off[q] >= 0) dst[job.dst_off + e] = v[q];
   }
+}
+
+// gather_range for a job with a strided destination: both offsets of each
+// element come from one decode of its index.
+template <class W>
+__device__ inline void gather_range_strided(const CopyJob &job, int64_t e0, int stride, int t,
+                                            const W *__restrict__ src,
+                                            W *__restrict__ dst) {
+  const int r = job.rank;
+  i64 so[kPerThread], dof[kPerThread];
+#pragma unroll
+  for (int q = 0; q < kPerThread; ++q) {
+    const int64_t e = e0 + (int64_t)q * stride + t;
+    so[q] = -1;
+    dof[q] = 0;
+    if (e < job.n_elem) {
+      uint32_t rem = (uint32_t)e;
+      i64 s = job.src_off, d = job.dst_off;
+      for (int l = 0; l < r; ++l) {
+        const uint32_t n = (uint32_t)job.dims[l];
+        const uint32_t qd = job_div(rem, n, job.magic[l]);
+        const uint32_t i = rem - qd * n;
+        s += (i64)i * job.strides[l];
+        d += (i64)i * job.dst_strides[l];
+        rem = qd;
+      }
+      so[q] = s;
+      dof[q] = d;
+    }
+  }
+  W v[kPerThread];
+#pragma unroll
+  for (int q = 0; q < kPerThread; ++q) v[q] = so[q] >= 0 ? src[so[q]] : W(0);
+#pragma unroll
+  for (int q = 0; q < kPerThread; ++q)
+    if (so[q] >= 0) dst[dof[q]] = v[q];
 }
 
 // Job of workgroup chunk c: the last j < n with start[j] <= c (start[0] = 0).
@@ -111,7 +147,10 @@ __global__ __launch_bounds__(kGatherThreads) void gather_jobs_kernel(
     const int64_t lo = find_job(job_wg_start, n_jobs, c);
     const CopyJob &job = jobs[lo];
     const int64_t e0 = (int64_t)(c - job_wg_start[lo]) * kGatherPerWG;
-    gather_range(job, e0, 64, lane, src, dst);
+    if (job.dst_strided)
+      gather_range_strided(job, e0, 64, lane, src, dst);
+    else
+      gather_range(job, e0, 64, lane, src, dst);
   }
 }
 
@@ -374,6 +413,7 @@ tq_status dense_permute_t(tq_handle_t h, const Legs &L, i64 n_total,
       job.dims[k] = (int32_t)L.d[k];
       job.strides[k] = L.s[k];
     }
+    finalize_job(job);
     return launch_gather_single<W>(h, job, in, out);
   }
   TileDesc td{};
@@ -475,6 +515,7 @@ __global__ void bs_permute_jobs_kernel(int32_t rank, int64_t n_blocks,
     n *= d;
   }
   j.n_elem = n;
+  finalize_job(j);
   jobs[b] = j;
   wg_count[b] = ceil_div(n, kGatherPerWG);
 }
@@ -522,7 +563,7 @@ __device__ inline i64 split_row_offset(const CopyJob &job, uint32_t row) {
   for (int l = 0; l < TQ_MAX_RANK; ++l) {
     if (l < job.nk || l >= job.rank) continue;
     const uint32_t d = (uint32_t)job.dims[l];
-    const uint32_t q = row / d;
+    const uint32_t q = job_div(row, d, job.magic[l]);
     o += (i64)(row - q * d) * job.strides[l];
     row = q;
   }
@@ -554,7 +595,7 @@ __device__ inline void split_kfast_tile(const CopyJob &job, int64_t rows, int64_
       for (int l = 0; l < TQ_MAX_RANK; ++l) {
         if (l >= nk) continue;
         const uint32_t d = (uint32_t)job.dims[l];
-        const uint32_t q = rem / d;
+        const uint32_t q = job_div(rem, d, job.magic[l]);
         o += (i64)(rem - q * d) * job.strides[l];
         rem = q;
       }
@@ -574,7 +615,7 @@ __device__ inline void split_kfast_tile(const CopyJob &job, int64_t rows, int64_
     for (int l = 0; l < TQ_MAX_RANK; ++l) {
       if (l < nk || l >= job.rank) continue;
       const uint32_t d = (uint32_t)job.dims[l];
-      const uint32_t q = rem / d;
+      const uint32_t q = job_div(rem, d, job.magic[l]);
       o += (i64)(rem - q * d) * job.strides[l];
       rem = q;
     }
@@ -684,7 +725,7 @@ __global__ __launch_bounds__(kGatherThreads) void split_jobs_kernel(
       dig[l] = 0;
       if (l >= nk) continue;
       const uint32_t d = (uint32_t)job.dims[l];
-      const uint32_t q = rem / d;
+      const uint32_t q = job_div(rem, d, job.magic[l]);
       dig[l] = (int32_t)(rem - q * d);
       off += (i64)dig[l] * job.strides[l];
       rem = q;
@@ -943,6 +984,7 @@ tq_status tq_bs_slice(tq_handle_t h, tq_dtype dtype, const tq_bs_desc *d,
       jb.src_off = src;
       jb.dst_off = dst;
       jb.n_elem = n;
+      finalize_job(jb);
       wg[b * n_sub + j] = (int32_t)wg_total;
       wg_total += ceil_div(n, kGatherPerWG);
       dst += n;

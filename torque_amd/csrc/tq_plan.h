@@ -35,6 +35,7 @@ struct BoxArgs {
   int64_t n_pairs;
   int32_t is_b;
   int32_t split;
+  int32_t kfast;  // plain panel stored k-fast ([contracted..., free...])
   int32_t self_rank, other_rank, nl, n_free;
   const i64 *self_begin, *self_end, *other_begin, *other_end;
   int32_t free_legs[TQ_MAX_RANK];
@@ -125,6 +126,9 @@ struct tq_plan_s {
   int64_t na = 0, nb = 0;
   int32_t legs_a[TQ_MAX_RANK] = {0}, legs_b[TQ_MAX_RANK] = {0};
   int32_t a_free[TQ_MAX_RANK] = {0}, b_free[TQ_MAX_RANK] = {0};
+  // plain (not split) panels of an operand whose fastest leg in memory is a
+  // contracted one are stored k-fast, so the pack reads along that leg
+  bool kfast_panel_a = false, kfast_panel_b = false;
   std::vector<i64> out_dims;
   int64_t n_pairs = 0;
   int64_t out_n_elem = 1;

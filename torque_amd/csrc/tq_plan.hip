@@ -303,6 +303,7 @@ __global__ void block_panel_jobs_kernel(PanelArgs g,
     j.nk = g.nk;
   }
   j.n_elem = used[b] ? n : 0;
+  finalize_job(j);
   jobs[b] = j;
   wg[b] = pack_job_wgs(j);
 }
@@ -329,15 +330,17 @@ __global__ void block_panel_desc_kernel(BoxArgs g,
   // plain panel: column stride of contracted index starts at rows;
   // split panel: the pair reads the block's whole contracted extent (pairs
   // that do not get pair panels), so its offset stays at the panel start
-  i64 off = panel_off[i], cstr = g.split ? 1 : rows;
+  // (k-fast plain panel: the contracted index has stride 1, rows kfull)
+  i64 off = panel_off[i], cstr = (g.split || g.kfast) ? 1 : rows;
   for (int k = 0; k < g.nl; ++k) {
     const int ls = g.self_legs[k], lo_ = g.other_legs[k];
     const i64 lo = max(sb[ls], ob[lo_]);
     off += (lo - sb[ls]) * cstr;
     cstr *= se[ls] - sb[ls] + 1;
   }
-  const i64 ld = g.split ? round8(kfull) : (rows > 0 ? rows : 1);
-  const int32_t kf = g.split ? (g.is_b ? kBKFast : kAKFast) : 0;
+  const i64 ld = g.split ? round8(kfull) : g.kfast ? (kfull > 0 ? kfull : 1)
+                                                    : (rows > 0 ? rows : 1);
+  const int32_t kf = (g.split || g.kfast) ? (g.is_b ? kBKFast : kAKFast) : 0;
   GemmDesc d = desc[p];
   if (g.is_b) {
     d.b_off = off;
@@ -370,6 +373,7 @@ __global__ void pair_panel_jobs_kernel(BoxArgs g,
                                        const i64 *__restrict__ self_str,
                                        const i64 *__restrict__ self_offsets,
                                        const i64 *__restrict__ panel_off,
+                                       const i64 *__restrict__ panel_ld,
                                        GemmDesc *__restrict__ desc,
                                        CopyJob *__restrict__ jobs,
                                        i64 *__restrict__ wg) {
@@ -383,8 +387,9 @@ __global__ void pair_panel_jobs_kernel(BoxArgs g,
             *oe = g.other_end + j * g.other_rank;
   CopyJob job{};
   job.rank = g.n_free + g.nl;
-  const int f0 = g.split ? g.nl : 0;   // position of the first free leg
-  const int c0 = g.split ? 0 : g.n_free;  // position of the first contracted leg
+  const bool kfirst = g.split || g.kfast;  // contracted legs first
+  const int f0 = kfirst ? g.nl : 0;          // position of the first free leg
+  const int c0 = kfirst ? 0 : g.n_free;      // position of the first contracted leg
   i64 src = self_offsets[i], n = 1, rows = 1, kk = 1;
   for (int q = 0; q < g.n_free; ++q) {
     const int l = g.free_legs[q];
@@ -413,18 +418,36 @@ __global__ void pair_panel_jobs_kernel(BoxArgs g,
   job.src_off = src;
   job.dst_off = panel_off[p];
   job.n_elem = n;
+  i64 ld = g.split ? round8(kk) : g.kfast ? (kk > 0 ? kk : 1) : (rows > 0 ? rows : 1);
+  if (g.kfast && !g.split && panel_ld && panel_ld[p] > kk) {
+    // the pair's slice of a k-fast panel with longer rows (a group panel):
+    // contracted legs compact, each row panel_ld[p] apart
+    job.dst_strided = 1;
+    i64 st = 1;
+    for (int k = 0; k < g.nl; ++k) {
+      job.dst_strides[c0 + k] = st;
+      st *= job.dims[c0 + k];
+    }
+    st = panel_ld[p];
+    for (int q = 0; q < g.n_free; ++q) {
+      job.dst_strides[f0 + q] = st;
+      st *= job.dims[f0 + q];
+    }
+    ld = panel_ld[p];
+  }
+  finalize_job(job);
   jobs[p] = job;
   wg[p] = pack_job_wgs(job);
   GemmDesc d = desc[p];
-  const i64 ld = g.split ? round8(kk) : (rows > 0 ? rows : 1);
+  const int32_t kf = kfirst ? (g.is_b ? kBKFast : kAKFast) : 0;
   if (g.is_b) {
     d.b_off = panel_off[p];
     d.ldb = (int32_t)ld;
-    d.flags = (d.flags & ~kBKFast) | kBPacked | (g.split ? kBKFast : 0);
+    d.flags = (d.flags & ~kBKFast) | kBPacked | kf;
   } else {
     d.a_off = panel_off[p];
     d.lda = (int32_t)ld;
-    d.flags = (d.flags & ~kAKFast) | kAPacked | (g.split ? kAKFast : 0);
+    d.flags = (d.flags & ~kAKFast) | kAPacked | kf;
   }
   desc[p] = d;
 }
@@ -916,7 +939,8 @@ tq_status plan_build_tiles(tq_plan_t pl) {
 // merged-output plan.
 static tq_status build_pack(tq_plan_t pl, bool is_b, int mode, bool split,
                             const PlanUpload &up,
-                            const std::vector<i64> *pair_dst = nullptr) {
+                            const std::vector<i64> *pair_dst = nullptr,
+                            const std::vector<i64> *pair_ld = nullptr) {
   tq_handle_t h = pl->h;
   const int64_t np = pl->n_pairs;
   PackSide &side = is_b ? pl->pack_b : pl->pack_a;
@@ -931,6 +955,7 @@ static tq_status build_pack(tq_plan_t pl, bool is_b, int mode, bool split,
   box.n_pairs = np;
   box.is_b = is_b;
   box.split = split;
+  box.kfast = !split && (is_b ? pl->kfast_panel_b : pl->kfast_panel_a);
   box.self_rank = self_rank;
   box.other_rank = other_rank;
   box.nl = nl;
@@ -970,9 +995,10 @@ static tq_status build_pack(tq_plan_t pl, bool is_b, int mode, bool split,
     pa.end = box.self_end;
     pa.strides = self_str;
     pa.offsets = self_off;
-    if (split) {
+    if (split || box.kfast) {
       for (int k = 0; k < nl; ++k) pa.order[k] = box.self_legs[k];
       for (int q = 0; q < box.n_free; ++q) pa.order[nl + q] = box.free_legs[q];
+      if (!split) pa.nk = 0;  // a plain gather in that leg order
     } else {
       for (int q = 0; q < box.n_free; ++q) pa.order[q] = box.free_legs[q];
       for (int k = 0; k < nl; ++k) pa.order[box.n_free + k] = box.self_legs[k];
@@ -1006,10 +1032,19 @@ static tq_status build_pack(tq_plan_t pl, bool is_b, int mode, bool split,
       TQ_CHECK(tq_scan_i64(h, sizes.as<i64>(), side.panel_off.as<i64>(), np,
                            side.panel_off.as<i64>() + np, tmp.p));
     }
+    DevBuf ld_dev;
+    if (pair_ld) {
+      TQ_HIP(h, ld_dev.alloc(sizeof(i64) * std::max<int64_t>(np, 1)));
+      TQ_HIP(h, hipMemcpyAsync(ld_dev.p, pair_ld->data(), sizeof(i64) * np,
+                               hipMemcpyHostToDevice, h->stream));
+    }
     hipLaunchKernelGGL(pair_panel_jobs_kernel, dim3(grid_for(np)), dim3(256), 0,
                        h->stream, box, pair_self, pair_other, self_str, self_off,
-                       side.panel_off.as<i64>(), pl->desc.as<GemmDesc>(),
-                       side.jobs.as<CopyJob>(), side.wg.as<i64>());
+                       side.panel_off.as<i64>(), pair_ld ? ld_dev.as<i64>() : nullptr,
+                       pl->desc.as<GemmDesc>(), side.jobs.as<CopyJob>(), side.wg.as<i64>());
+    // (the host vectors and ld_dev outlive the kernel: the sizes read below
+    // synchronise the stream before they go)
+    TQ_HIP(h, hipStreamSynchronize(h->stream));
   }
   TQ_HIP(h, hipGetLastError());
   TQ_CHECK(tq_scan_i64(h, side.wg.as<i64>(), side.wg.as<i64>(), n_jobs,
@@ -1176,16 +1211,21 @@ static tq_status merge_groups(tq_plan_t pl, const std::vector<GemmDesc> &hd,
     a_tot += d.m * ktot[g];
     b_tot += d.n * ktot[g];
   }
-  std::vector<i64> dst_a(np + 1), dst_b(np + 1);
+  // a pair's slice of its group panel: split and m-fast panels hold it as
+  // kofs_p whole columns (chunks); a k-fast plain panel as a column band of
+  // rows ktot long
+  const bool kfa = !split && pl->kfast_panel_a, kfb = !split && pl->kfast_panel_b;
+  std::vector<i64> dst_a(np + 1), dst_b(np + 1), ld_a(np), ld_b(np);
   for (int64_t p = 0; p < np; ++p) {
-    dst_a[p] = a_base[grp[p]] + kofs[p] * hd[p].m;
-    dst_b[p] = b_base[grp[p]] + kofs[p] * hd[p].n;
+    dst_a[p] = a_base[grp[p]] + (kfa ? kofs[p] : kofs[p] * hd[p].m);
+    dst_b[p] = b_base[grp[p]] + (kfb ? kofs[p] : kofs[p] * hd[p].n);
+    ld_a[p] = ld_b[p] = ktot[grp[p]];
   }
   dst_a[np] = a_tot;
   dst_b[np] = b_tot;
   if (!member_stream) {
-    TQ_CHECK(build_pack(pl, false, kModePairPanel, split, pl->up, &dst_a));
-    TQ_CHECK(build_pack(pl, true, kModePairPanel, split, pl->up, &dst_b));
+    TQ_CHECK(build_pack(pl, false, kModePairPanel, split, pl->up, &dst_a, &ld_a));
+    TQ_CHECK(build_pack(pl, true, kModePairPanel, split, pl->up, &dst_b, &ld_b));
   }
 
   // one GEMM unit per group; the matched pairs keep their own maps
@@ -1203,13 +1243,14 @@ static tq_status merge_groups(tq_plan_t pl, const std::vector<GemmDesc> &hd,
   // the GEMM operands)
   if (!member_stream) pl->max_ld = 0;
   i64 c = 0;
-  const int32_t kf = split ? (kAKFast | kBKFast) : 0;
+  const bool ka = split || kfa, kb = split || kfb;
+  const int32_t kf = (ka ? kAKFast : 0) | (kb ? kBKFast : 0);
   for (int64_t g = 0; g < ng; ++g) {
     const int64_t p0 = first[g];
     const GemmDesc &d = hd[p0];
     const int32_t kt = (int32_t)ktot[g];
-    gd[g] = GemmDesc{a_base[g], b_base[g], c, d.m, d.n, kt, split ? kt : d.m,
-                     split ? kt : d.n, kAPacked | kBPacked | kf};
+    gd[g] = GemmDesc{a_base[g], b_base[g], c, d.m, d.n, kt, ka ? kt : d.m,
+                     kb ? kt : d.n, kAPacked | kBPacked | kf};
     std::copy_n(pl->h_out_begin.begin() + p0 * ro, ro, ob.begin() + g * ro);
     std::copy_n(pl->h_out_end.begin() + p0 * ro, ro, oe.begin() + g * ro);
     coff[g] = c;
@@ -1314,6 +1355,27 @@ static tq_status plan_contract(tq_handle_t h, const tq_bs_desc *a, const tq_bs_d
     }
   }
   pl->ro = (int)pl->out_dims.size();
+  // panel orientation: k-fast when the operand's fastest leg (smallest
+  // stride of a non-unit leg of its first block) is contracted
+  {
+    auto fastest_contracted = [&](const tq_bs_desc *t, int side) {
+      if (t->n_blocks <= 0 || t->rank <= 0) return false;
+      int best = -1;
+      i64 bs = 0;
+      for (int l = 0; l < t->rank; ++l) {
+        if (t->end[l] - t->begin[l] < 1) continue;
+        if (best < 0 || t->strides[l] < bs) {
+          best = l;
+          bs = t->strides[l];
+        }
+      }
+      for (int k = 0; k < n_legs; ++k)
+        if (legs[2 * k + side] == best) return true;
+      return false;
+    };
+    pl->kfast_panel_a = fastest_contracted(a, 0);
+    pl->kfast_panel_b = fastest_contracted(b, 1);
+  }
   // a rank-0 result (the network scalar) is always reduced in FP64
   if (pl->ro == 0) pl->prec = TQ_PREC_NATIVE;
   if (pl->prec == TQ_PREC_BF16X3) {
