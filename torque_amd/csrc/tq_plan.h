@@ -220,6 +220,7 @@ struct tq_plan_s {
   bool member_stream = false;
   std::vector<GemmDesc> h_members;
   std::vector<int64_t> h_member_first;
+  DevBuf member_first;  // h_member_first on the device
   DevBuf member_desc;
   DevBuf group_slices;  // GroupSlice per column slice of a streamed group
 

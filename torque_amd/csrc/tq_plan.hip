@@ -556,6 +556,65 @@ __global__ void expand_tiles_kernel(int64_t n, const i64 *__restrict__ order,
   for (i64 t = t0; t < t1; ++t) tile_pair[t] = (int32_t)p;
 }
 
+// Streaming small-K work (pairs with K <= max_k, or every unit of a plan
+// whose groups stream): column slices of kStreamCols per pair at work-list
+// position q, each cut into segments of kStreamRows rows.
+__global__ void stream_slices_kernel(int64_t n, const i64 *__restrict__ order,
+                                     const GemmDesc *__restrict__ desc, int32_t max_k,
+                                     i64 *__restrict__ slices) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const GemmDesc d = desc[order[q]];
+  slices[q] = d.k <= max_k ? ceil_div(d.n, kStreamCols) : 0;
+}
+
+// The slices of the pair at position q (B and C offsets moved to the slice's
+// first column; for streamed groups the group's member range instead) and
+// each slice's segment count.
+__global__ void stream_slice_desc_kernel(int64_t n, const i64 *__restrict__ order,
+                                         const GemmDesc *__restrict__ desc,
+                                         const i64 *__restrict__ slice_start, int32_t split,
+                                         const i64 *__restrict__ member_first,
+                                         GemmDesc *__restrict__ slices,
+                                         GroupSlice *__restrict__ gslices,
+                                         i64 *__restrict__ segs) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int64_t p = order[q];
+  const GemmDesc d = desc[p];
+  const i64 s0 = slice_start[q], s1 = slice_start[q + 1];
+  const i64 row_segs = ceil_div(d.m, kStreamRows);
+  // element distance between consecutive B rows (output columns): a split
+  // panel with one chunk holds 8 values per row
+  const i64 b_row = split ? 8 : (d.flags & kBKFast) ? d.ldb : 1;
+  for (i64 s = s0; s < s1; ++s) {
+    const int32_t n0 = (int32_t)(s - s0) * kStreamCols;
+    const int32_t nc = min(kStreamCols, d.n - n0);
+    if (gslices) {
+      gslices[s] = GroupSlice{d.c_off + (i64)d.m * n0, d.m, nc, n0, (int32_t)member_first[p],
+                              (int32_t)(member_first[p + 1] - member_first[p]), 0};
+    } else {
+      GemmDesc sd = d;
+      sd.b_off += n0 * b_row;
+      sd.c_off += (i64)d.m * n0;
+      sd.n = nc;
+      slices[s] = sd;
+    }
+    segs[s] = row_segs;
+  }
+}
+
+// Segment t of slice s for seg_start[s] <= t < seg_start[s + 1].
+__global__ void stream_segments_kernel(int64_t n_slices, const i64 *__restrict__ seg_start,
+                                       int32_t *__restrict__ seg_slice,
+                                       int32_t *__restrict__ slice_first) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_slices) return;
+  const i64 t0 = seg_start[s], t1 = seg_start[s + 1];
+  slice_first[s] = (int32_t)t0;
+  for (i64 t = t0; t < t1; ++t) seg_slice[t] = (int32_t)s;
+}
+
 unsigned grid_for(int64_t n, int threads = 256) {
   return (unsigned)std::max<int64_t>(1, ceil_div(n, threads));
 }
@@ -725,75 +784,58 @@ static tq_status build_stream_segments(tq_plan_t pl) {
   pl->n_chunks = 0;
   pl->chunks_ready = true;
   if (np == 0) return TQ_SUCCESS;
-  if ((int64_t)pl->h_desc.size() != np)
-    return tq_fail(h, TQ_ERROR_INVALID_VALUE, "contract: host descriptors missing");
-  std::vector<i64> order(np);
-  TQ_HIP(h, hipMemcpyAsync(order.data(), pl->order.p, sizeof(i64) * np, hipMemcpyDeviceToHost,
-                           h->stream));
-  TQ_HIP(h, hipStreamSynchronize(h->stream));
   const bool split = pl->prec == TQ_PREC_BF16X3 && !pl->split_direct;
-  std::vector<GemmDesc> slices;
-  std::vector<int32_t> seg_slice, slice_first;
-  if (pl->member_stream) {
-    // streamed groups: slices of a group carry its member range and first
-    // column (each member's B is offset by the column in the kernel)
-    std::vector<GroupSlice> gs;
-    for (int64_t q = 0; q < np; ++q) {
-      const int64_t g = order[q];
-      const GemmDesc &d = pl->h_desc[g];
-      const int64_t row_segs = ceil_div(d.m, kStreamRows);
-      for (int32_t n0 = 0; n0 < d.n; n0 += kStreamCols) {
-        if ((int64_t)seg_slice.size() + row_segs > 0x7fffffff)
-          return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many output segments");
-        slice_first.push_back((int32_t)seg_slice.size());
-        seg_slice.insert(seg_slice.end(), (size_t)row_segs, (int32_t)gs.size());
-        gs.push_back(GroupSlice{d.c_off + (i64)d.m * n0, d.m,
-                                std::min<int32_t>(kStreamCols, d.n - n0), n0,
-                                (int32_t)pl->h_member_first[g],
-                                (int32_t)(pl->h_member_first[g + 1] - pl->h_member_first[g]),
-                                0});
-      }
-    }
-    if (seg_slice.empty()) return TQ_SUCCESS;
-    TQ_HIP(h, pl->group_slices.alloc(sizeof(GroupSlice) * gs.size()));
-    TQ_HIP(h, hipMemcpyAsync(pl->group_slices.p, gs.data(), sizeof(GroupSlice) * gs.size(),
-                             hipMemcpyHostToDevice, h->stream));
-  }
-  for (int64_t q = 0; q < np && !pl->member_stream; ++q) {
-    const GemmDesc &d = pl->h_desc[order[q]];
-    if (pair_class(*pl, d.m, d.n, d.k) >= 0) continue;
-    // element distance between consecutive B rows (output columns): a split
-    // panel with one chunk holds 8 values per row
-    const i64 b_row = split ? 8 : (d.flags & kBKFast) ? d.ldb : 1;
-    const int64_t row_segs = ceil_div(d.m, kStreamRows);
-    for (int32_t n0 = 0; n0 < d.n; n0 += kStreamCols) {
-      GemmDesc s = d;
-      s.b_off += n0 * b_row;
-      s.c_off += (i64)d.m * n0;
-      s.n = std::min<int32_t>(kStreamCols, d.n - n0);
-      if ((int64_t)seg_slice.size() + row_segs > 0x7fffffff)
-        return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many output segments");
-      slice_first.push_back((int32_t)seg_slice.size());
-      seg_slice.insert(seg_slice.end(), (size_t)row_segs, (int32_t)slices.size());
-      slices.push_back(s);
-    }
-  }
-  if (seg_slice.empty()) return TQ_SUCCESS;
-  TQ_HIP(h, pl->chunk_pair.alloc(sizeof(int32_t) * seg_slice.size()));
-  TQ_HIP(h, pl->chunk_first.alloc(sizeof(int32_t) * slice_first.size()));
-  if (!pl->member_stream) {
-    TQ_HIP(h, pl->stream_desc.alloc(sizeof(GemmDesc) * slices.size()));
-    TQ_HIP(h, hipMemcpyAsync(pl->stream_desc.p, slices.data(),
-                             sizeof(GemmDesc) * slices.size(), hipMemcpyHostToDevice,
-                             h->stream));
-  }
-  TQ_HIP(h, hipMemcpyAsync(pl->chunk_pair.p, seg_slice.data(),
-                           sizeof(int32_t) * seg_slice.size(), hipMemcpyHostToDevice, h->stream));
-  TQ_HIP(h, hipMemcpyAsync(pl->chunk_first.p, slice_first.data(),
-                           sizeof(int32_t) * slice_first.size(), hipMemcpyHostToDevice,
-                           h->stream));
+  // slices per work-list position, their scan, then the slices and their
+  // segment counts, that scan, then the segment list: all on the device,
+  // two 8-byte totals read back
+  DevBuf slice_start, seg_start, tmp;
+  TQ_HIP(h, slice_start.alloc(sizeof(i64) * (np + 1)));
+  TQ_HIP(h, tmp.alloc(tq_scan_tmp_bytes(np + 1)));
+  hipLaunchKernelGGL(stream_slices_kernel, dim3(grid_for(np)), dim3(256), 0, h->stream, np,
+                     pl->order.as<i64>(), pl->desc.as<GemmDesc>(),
+                     (int32_t)(pl->member_stream ? 0x7fffffff : pl->stream_max_k),
+                     slice_start.as<i64>());
+  TQ_CHECK(tq_scan_i64(h, slice_start.as<i64>(), slice_start.as<i64>(), np,
+                       slice_start.as<i64>() + np, tmp.p));
+  i64 n_slices = 0;
+  TQ_HIP(h, hipMemcpyAsync(&n_slices, slice_start.as<i64>() + np, sizeof(i64),
+                           hipMemcpyDeviceToHost, h->stream));
   TQ_HIP(h, hipStreamSynchronize(h->stream));
-  pl->n_chunks = (int64_t)seg_slice.size();
+  if (n_slices == 0) return TQ_SUCCESS;
+  if (n_slices > 0x7fffffff)
+    return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many output segments");
+  GemmDesc *slices = nullptr;
+  GroupSlice *gslices = nullptr;
+  if (pl->member_stream) {
+    TQ_HIP(h, pl->group_slices.alloc(sizeof(GroupSlice) * n_slices));
+    gslices = pl->group_slices.as<GroupSlice>();
+  } else {
+    TQ_HIP(h, pl->stream_desc.alloc(sizeof(GemmDesc) * n_slices));
+    slices = pl->stream_desc.as<GemmDesc>();
+  }
+  TQ_HIP(h, seg_start.alloc(sizeof(i64) * (n_slices + 1)));
+  TQ_HIP(h, tmp.alloc(tq_scan_tmp_bytes(n_slices + 1)));
+  hipLaunchKernelGGL(stream_slice_desc_kernel, dim3(grid_for(np)), dim3(256), 0, h->stream,
+                     np, pl->order.as<i64>(), pl->desc.as<GemmDesc>(), slice_start.as<i64>(),
+                     (int32_t)split, pl->member_stream ? pl->member_first.as<i64>() : nullptr,
+                     slices, gslices, seg_start.as<i64>());
+  TQ_CHECK(tq_scan_i64(h, seg_start.as<i64>(), seg_start.as<i64>(), n_slices,
+                       seg_start.as<i64>() + n_slices, tmp.p));
+  i64 n_segs = 0;
+  TQ_HIP(h, hipMemcpyAsync(&n_segs, seg_start.as<i64>() + n_slices, sizeof(i64),
+                           hipMemcpyDeviceToHost, h->stream));
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  if (n_segs > 0x7fffffff)
+    return tq_fail(h, TQ_ERROR_UNSUPPORTED, "contract: too many output segments");
+  TQ_HIP(h, pl->chunk_pair.alloc(sizeof(int32_t) * std::max<i64>(n_segs, 1)));
+  TQ_HIP(h, pl->chunk_first.alloc(sizeof(int32_t) * n_slices));
+  hipLaunchKernelGGL(stream_segments_kernel, dim3(grid_for(n_slices)), dim3(256), 0,
+                     h->stream, n_slices, seg_start.as<i64>(), pl->chunk_pair.as<int32_t>(),
+                     pl->chunk_first.as<int32_t>());
+  TQ_HIP(h, hipGetLastError());
+  // the temporaries are freed on return
+  TQ_HIP(h, hipStreamSynchronize(h->stream));
+  pl->n_chunks = n_segs;
   return TQ_SUCCESS;
 }
 
@@ -1200,6 +1242,9 @@ static tq_status merge_groups(tq_plan_t pl, const std::vector<GemmDesc> &hd,
     TQ_HIP(h, pl->member_desc.alloc(sizeof(GemmDesc) * std::max<int64_t>(np, 1)));
     TQ_HIP(h, hipMemcpyAsync(pl->member_desc.p, pl->h_members.data(), sizeof(GemmDesc) * np,
                              hipMemcpyHostToDevice, h->stream));
+    TQ_HIP(h, pl->member_first.alloc(sizeof(i64) * (ng + 1)));
+    TQ_HIP(h, hipMemcpyAsync(pl->member_first.p, pl->h_member_first.data(),
+                             sizeof(i64) * (ng + 1), hipMemcpyHostToDevice, h->stream));
     pl->member_stream = true;
   }
   for (int64_t g = 0; g < ng && !member_stream; ++g) {
