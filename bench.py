@@ -141,7 +141,9 @@ def bench_batched(args):
     plans, c_off, batch = build("native")
     c = torch.empty(max(batch.out_n_elem, 1), dtype=torch.float64, device=dev)
     plan_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
     best, times = batch.autotune(a_all, b_all, c, reps=2)
+    sweep_s = time.perf_counter() - t0
     batch.capture(a_all, b_all, c)
     n_tiles, n_segs = batch.work()
     print("# batched: %d contractions, %d pairs, %.1f GFLOP, out %.1f M elem; gen %.1f s, "
@@ -191,6 +193,35 @@ def bench_batched(args):
         return float(np.sqrt(num / den)), float(worst)
 
     err64, worst64 = errors(c)
+
+    # the committed cost model's choice against the sweep: regret per class
+    # (from the sweep's own per-configuration times) and on the whole replay,
+    # and its time to schedule against the sweep's
+    from torque_amd import costmodel
+    cost_model = None
+    model = costmodel.load_default()
+    if model is not None:
+        t0 = time.perf_counter()
+        chosen = model.choose(batch)
+        choose_s = time.perf_counter() - t0
+        split = batch.split()
+        regret = {}
+        for cls in range(2):
+            t = np.array(times[cls])
+            if (t > 0).any() and split[cls] == 1:
+                regret["class%d" % cls] = round(float(t[chosen[cls]] / t[t > 0].min() - 1), 4)
+        batch.capture(a_all, b_all, c)
+        mdt = time_graph(batch)
+        regret["replay"] = round(mdt / dt - 1, 4)
+        cost_model = {"chosen": chosen, "split": split, "regret_vs_sweep": regret,
+                      "replay_ms": round(mdt * 1e3, 4),
+                      "time_to_schedule_s": round(choose_s, 4),
+                      "sweep_time_to_schedule_s": round(sweep_s, 3)}
+        print("# cost model: chose %s split %s, replay %.3f ms, regret %s, %.4f s to schedule "
+              "(sweep %.2f s)" % (chosen, split, mdt * 1e3, regret, choose_s, sweep_s), flush=True)
+        for cls in range(2):
+            batch.set_class_split(cls, 1)
+            batch.set_class_config(cls, best[cls])
     tiers = {"fp64": {"gflops": round(value, 1), "rel_err": err64, "max_contraction_rel_err": worst64,
                       "frac_fp64_peak": round(value / (FP64_PEAK_TFLOPS * 1e3), 4)}}
     for tier in [t for t in args.tiers.split(",") if t]:
@@ -248,6 +279,7 @@ def bench_batched(args):
                    "pairs": int(batch.n_pairs), "gflop_per_step": round(batch.flops / 1e9, 3),
                    "out_elements": int(batch.out_n_elem), "gemm_config": best,
                    "mfma_tiles": int(n_tiles), "stream_segments": int(n_segs),
+                   "autotune_sweep_s": round(sweep_s, 3), "cost_model": cost_model,
                    "launch": "one hipGraph replay (pack A, pack B, small-K, grouped GEMM)",
                    "seed": args.seed},
         "roofline": {"bound": "mfma", "achieved": round(value / 1e3, 3),
