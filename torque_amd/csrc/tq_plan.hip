@@ -493,7 +493,7 @@ __global__ __launch_bounds__(256) void sort_scatter_kernel(
     const i64 *__restrict__ base_off, i64 *__restrict__ order) {
   __shared__ i64 running[kBuckets];
   __shared__ int wave_cnt[4][kBuckets];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wave = threadIdx.x >> 6;
   if (threadIdx.x < kBuckets)
     running[threadIdx.x] = base_off[(int64_t)threadIdx.x * n_chunks + blockIdx.x];
   const int64_t base = (int64_t)blockIdx.x * kSortChunk;

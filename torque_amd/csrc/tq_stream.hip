@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kThreads) void small_k_kernel(
     const int64_t sak = (d.flags & kAKFast) ? 1 : d.lda;
     const int64_t sbn = (d.flags & kBKFast) ? d.ldb : 1;
     const int64_t sbk = (d.flags & kBKFast) ? 1 : d.ldb;
-    auto in = [&bad](TIn x) -> TC {
+    auto in = [&](TIn x) -> TC {
       if constexpr (kRound3) {
         bad = bad || tqs::split_out_of_range((double)x);
         return (TC)tqs::round3((double)x);
