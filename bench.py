@@ -91,7 +91,8 @@ def parse():
     p.add_argument("--no-large", action="store_true",
                    help="skip the D=10, chi=256 enlarged-corner product")
     p.add_argument("--workload", default="peps",
-                   choices=["peps", "batched", "permute", "large", "shapes", "corner6"],
+                   choices=["peps", "batched", "permute", "large", "shapes", "corner6",
+                            "scaling"],
                    help="peps: the D, chi absorption step (default); batched: --n-batch "
                         "independent rank-4 x rank-4 contractions in one captured launch")
     p.add_argument("--n-batch", type=int, default=512)
@@ -699,6 +700,8 @@ def bench_split_k(dev, reps=20):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
 
+    default_split = p.split()  # the analytic choice, before any tuning
+    ms_default = timed()
     best, _ = p.autotune(A.data, B.data, out, reps=3)
     split = p.split()
     ms_split = timed()
@@ -708,9 +711,12 @@ def bench_split_k(dev, reps=20):
     res = {"workload": "block_diagonal_8x_64x16384x64", "gflop": round(p.flops / 1e9, 3),
            "whole_pairs_ms": round(ms_whole, 4), "split_ms": round(ms_split, 4),
            "split": split, "gemm_config": best, "speedup": round(ms_whole / ms_split, 2),
+           "default_split": default_split, "default_ms": round(ms_default, 4),
            "split_gflops": round(p.flops / ms_split / 1e6, 1)}
-    print("# split-K (8 blocks 64 x 16384 x 64): whole pairs %.3f ms, split %s %.3f ms (%.2fx)"
-          % (ms_whole, split, ms_split, ms_whole / ms_split), flush=True)
+    print("# split-K (8 blocks 64 x 16384 x 64): whole pairs %.3f ms, split %s %.3f ms (%.2fx); "
+          "analytic default split %s %.3f ms"
+          % (ms_whole, split, ms_split, ms_whole / ms_split, default_split, ms_default),
+          flush=True)
     p.destroy()
     return res
 
@@ -775,6 +781,90 @@ def bench_ctmrg(dev, D, chi, steps, warmup):
     del bufs
     torch.cuda.empty_cache()
     return res
+
+
+SCALING_LENGTHS = (2, 4, 8, 16, 32, 64)
+SCALING_SLICES = (0, 2, 3, 4, 4, 4)
+
+
+def bench_scaling(dev, reps=5):
+    """The reference's large-tensor benchmark (test/gpu/large_tensor_benchmark.cu:28-760)
+    through the drop-in classes: per-call contract time of a rank-2, -3 and -4
+    tensor with a matrix (legs {1,0}, or {1,1},{2,0}) at leg lengths 2..64,
+    block-sparse with the blocks sliced j times by 2 on every leg (4^j, 8^j,
+    16^j blocks), dense, and element-sparse with every index stored. Block
+    sparse calls are timed cold (a new plan, as the reference plans every
+    call) and warm (the handle's cached plan); each time is the median of
+    `reps` calls, host to host around a device synchronisation."""
+    import torch
+
+    from torque_amd import BlockSparseTensor, DenseTensor, SparseTensor, default_handle
+    from torque_amd.tensor import ContractionPlan
+
+    rng = np.random.default_rng(0)
+    h = default_handle(dev.index)
+
+    def median_call(fn):
+        ts = []
+        for _ in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts[1:])) * 1e3
+
+    def legs_for(rank):
+        return np.array([[1, 0]] if rank == 2 else [[1, 1], [2, 0]], np.int32)
+
+    bs, dense, sparse = [], [], []
+    for rank in (2, 3, 4):
+        legs = legs_for(rank)
+        for length, jmax in zip(SCALING_LENGTHS, SCALING_SLICES):
+            dims_t, dims_m = [length] * rank, [length] * 2
+            xt = rng.random(length ** rank)
+            xm = rng.random(length ** 2)
+            t = BlockSparseTensor(dims_t, xt, np.zeros((1, rank), np.int64),
+                                  np.full((1, rank), length - 1, np.int64), device=dev)
+            m = BlockSparseTensor(dims_m, xm, np.zeros((1, 2), np.int64),
+                                  np.full((1, 2), length - 1, np.int64), device=dev)
+            for j in range(jmax + 1):
+                if j:
+                    t = t.slice([2] * rank)
+                    m = m.slice([2, 2])
+
+                def cold():
+                    p = ContractionPlan(h, t, m, legs, "native", False)
+                    out = torch.empty(max(p.out_n_elem, 1), dtype=torch.float64, device=dev)
+                    p.execute(t.data, m.data, out)
+                    p.output_tensor(out)
+                    p.destroy()
+
+                cold_ms = median_call(cold)
+                t.contract(m, legs)
+                warm_ms = median_call(lambda: t.contract(m, legs))
+                p = t.plan_contract(m, legs)
+                bs.append({"rank": rank, "length": length, "slices": j,
+                           "blocks": int(t.n_blocks), "pairs": int(p.n_pairs),
+                           "cold_ms": round(cold_ms, 3), "warm_ms": round(warm_ms, 3)})
+            dt_ = DenseTensor(dims_t, xt, device=dev)
+            dm = DenseTensor(dims_m, xm, device=dev)
+            d_ms = median_call(lambda: dt_.contract(dm, legs))
+            flop = 2.0 * length ** (rank + (1 if rank == 2 else 0))
+            dense.append({"rank": rank, "length": length, "ms": round(d_ms, 3),
+                          "gflops": round(flop / d_ms / 1e6, 3)})
+            st = SparseTensor(dims_t, xt, np.arange(length ** rank), device=dev)
+            sm = SparseTensor(dims_m, xm, np.arange(length ** 2), device=dev)
+            s_ms = median_call(lambda: st.contract(sm, legs))
+            sparse.append({"rank": rank, "length": length, "nnz": length ** rank,
+                           "ms": round(s_ms, 3)})
+        big = [e for e in bs if e["rank"] == rank][-1]
+        print("# scaling rank %d: block-sparse %d blocks (%d pairs) cold %.2f ms warm %.3f ms; "
+              "dense length 64 %.3f ms; sparse %d nnz %.3f ms"
+              % (rank, big["blocks"], big["pairs"], big["cold_ms"], big["warm_ms"],
+                 dense[-1]["ms"], sparse[-1]["nnz"], sparse[-1]["ms"]), flush=True)
+    return {"source": "test/gpu/large_tensor_benchmark.cu", "lengths": list(SCALING_LENGTHS),
+            "block_sparse": bs, "dense": dense, "sparse": sparse}
 
 
 SHAPES = ((128, 128, 128), (64, 64, 256), (64, 64, 64), (40, 36, 35), (32, 32, 32),
@@ -850,11 +940,12 @@ def main():
     if args.workload == "batched":
         print(json.dumps(bench_batched(args)), flush=True)
         return
-    if args.workload == "shapes":
+    if args.workload in ("shapes", "scaling"):
         import torch
         dev = torch.device("cuda:0")
         torch.cuda.set_device(dev)
-        print(json.dumps(bench_shapes(dev)), flush=True)
+        fn = bench_shapes if args.workload == "shapes" else bench_scaling
+        print(json.dumps(fn(dev)), flush=True)
         return
     if args.workload in ("permute", "large", "corner6"):
         import torch
@@ -1111,6 +1202,7 @@ def main():
     large = None if args.no_large else bench_large(dev)
     corner128 = None if args.no_large else bench_large(dev, D=8, chi=128, oracle_max_block=4096)
     corner6 = None if args.no_large else bench_corner_rank6(dev)
+    scaling = bench_scaling(dev)
     batched = None
     if not args.no_batched:
         bl = bench_batched(args)
@@ -1118,6 +1210,8 @@ def main():
                    "gflop": bl["config"]["gflop_per_step"], "ms": bl["ms_per_step"],
                    "gflops": bl["value"], "frac_fp64_mfma_peak": bl["roofline"]["frac"],
                    "gemm_config": bl["config"]["gemm_config"], "tiers": bl["tiers"],
+                   "cost_model": bl["config"]["cost_model"],
+                   "autotune_sweep_s": bl["config"]["autotune_sweep_s"],
                    "cpu_baseline": bl["cpu_baseline"]}
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "GFLOP/s", "n_gpus": 1,
@@ -1144,6 +1238,7 @@ def main():
         "enlarged_corner_D10_chi256": large,
         "enlarged_corner_D10_chi256_rank6": corner6,
         "batched_512_u1_rank4x4": batched,
+        "reference_scaling": scaling,
     }
     print(json.dumps(line), flush=True)
     if args.copy_calibration:

@@ -297,7 +297,10 @@ tq_status tq_plan_get_config(tq_plan_t plan, int32_t *config /* [2] */);
  * pairs with K <= 32 stay whole) that run as GEMMs of their own into a
  * plan-owned workspace; a reduction adds each pair's slices into C in slice
  * order, so results repeat bit for bit (and differ from split = 1 in the
- * last bits). tq_plan_autotune times splits 1, 2, 4, 8 and 16 on each
+ * last bits). A new plan starts with an analytic split: 1, unless a class's
+ * tiles would fill less than half of the GPU's tile slots, then the fewest
+ * power-of-two slices (at most 16, each at least 256 k values) that fill
+ * them. tq_plan_autotune times splits 1, 2, 4, 8 and 16 on each
  * class's chosen configuration and keeps the fastest; tq_plan_choose_config
  * predicts them. get_split writes both classes' splits. No reference
  * counterpart (src/gpu/block_sparse.cuh:894-930 runs K whole). */
@@ -408,7 +411,9 @@ tq_status tq_gemm_config_describe(tq_precision precision, int32_t config,
 
 /* Dense C = contract(A, B, legs) with compact column-major operands;
  * C has dims [A free legs, B free legs]. Replaces the permute + cublasXgemm
- * path of DenseTensor::contract (src/gpu/dense.cuh:439-555). */
+ * path of DenseTensor::contract (src/gpu/dense.cuh:439-555). The one-block
+ * plan comes from the handle's plan cache (tq_bs_contract_plan_cached), so
+ * repeated shapes plan once; returns after C is written. */
 tq_status tq_dense_contract(tq_handle_t handle, tq_dtype dtype, int32_t rank_a,
                             const int64_t *dims_a, int32_t rank_b,
                             const int64_t *dims_b, int32_t n_legs,

@@ -509,6 +509,10 @@ def test_split_operand_forms(cuda):
     ref_c, _ = O.contract(A, da, B, db, legs)
     ga, gb = torch_tensor(A, da, cuda), torch_tensor(B, db, cuda)
     plan = ga.plan_contract(gb, legs, "bf16x3")
+    # whole pairs (the autotuner below keeps them on this small contraction),
+    # so the forms and the autotuned result compare bit for bit
+    plan.set_class_split(0, 1)
+    plan.set_class_split(1, 1)
     out = torch.empty(plan.out_n_elem, dtype=torch.float64, device=cuda)
     outs = {}
     for form in ("in_place", "panels", "in_place", "panels"):

@@ -167,7 +167,7 @@ tq_status tq_batch_create(tq_handle_t h, int32_t n, const tq_plan_t *plans,
   TQ_CHECK(tq_sort_by_cost(h, dcost.as<i64>(), pl->n_pairs, pl->order.as<i64>()));
   pl->a_layout = pl->pack_a.mode == kModeDirect ? 0 : 2;
   pl->b_layout = pl->pack_b.mode == kModeDirect ? 0 : 2;
-  for (int c = 0; c < 2; ++c) pl->cls[c].auto_cfg = tq_plan_pick_config(pl.get(), c);
+  plan_pick_defaults(pl.get());
   if (pl->n_pairs > 0) TQ_CHECK(plan_build_tiles(pl.get()));
   TQ_HIP(h, hipStreamSynchronize(h->stream));
   *batch = pl.release();

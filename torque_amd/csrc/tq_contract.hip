@@ -118,6 +118,39 @@ int tq_plan_pick_config(tq_plan_t pl, int cls) {
   return best;
 }
 
+// Analytic split-K for one tile class under configuration cfg: a class whose
+// tiles would occupy less than half the GPU's tile slots (one wavefront tile
+// per SIMD for wave-tile configurations, two workgroup tiles per CU
+// otherwise) leaves most of it idle while each tile walks its whole K; its
+// contracted range is then cut into the fewest power-of-two slices that fill
+// the slots, keeping at least 256 k values per slice (the partial products
+// and their reduction cost M N per slice; at 256 k a slice does 64 flops per
+// byte of partials). The autotuner measures splits instead.
+int tq_plan_pick_split(tq_plan_t pl, int cls, int cfg) {
+  const GemmConfig g = tq_gemm_config(pl->prec, cfg);
+  const double n_cu = pl->h ? pl->h->n_cu : 256;
+  const double slots = n_cu * (g.wave_tile ? 4.0 * g.min_waves : 2.0);
+  double tiles = 0;
+  int64_t max_k = 0;
+  for (int64_t p = 0; p < pl->n_pairs; ++p) {
+    if (pair_class(*pl, pl->h_m[p], pl->h_n[p], pl->h_k[p]) != cls) continue;
+    tiles += std::ceil((double)pl->h_m[p] / g.bm) * std::ceil((double)pl->h_n[p] / g.bn);
+    max_k = std::max<int64_t>(max_k, pl->h_k[p]);
+  }
+  if (tiles <= 0 || 2.0 * tiles >= slots) return 1;
+  int s = 1;
+  while (s < 16 && tiles * s < slots && max_k / (2 * s) >= 256) s *= 2;
+  return s;
+}
+
+void plan_pick_defaults(tq_plan_t pl) {
+  for (int c = 0; c < 2; ++c) {
+    TileClass &tc = pl->cls[c];
+    tc.auto_cfg = tq_plan_pick_config(pl, c);
+    if (!tc.split_user) tc.split = tq_plan_pick_split(pl, c, tc.current());
+  }
+}
+
 namespace {
 
 // One grouped-GEMM launch over the tiles of one MFMA tile class (for a
@@ -204,6 +237,7 @@ tq_status tq_plan_set_class_split(tq_plan_t pl, int32_t cls, int32_t split) {
     return tq_fail(pl->h, TQ_ERROR_INVALID_VALUE, "split-K: class 0 or 1, split 1..64");
   pl->drop_graph();
   pl->cls[cls].split = split;
+  pl->cls[cls].split_user = true;
   return TQ_SUCCESS;
 }
 
@@ -321,10 +355,8 @@ tq_status tq_plan_set_small_k(tq_plan_t pl, int32_t max_k) {
   pl->stream_max_k = max_k;
   pl->chunks_ready = false;
   pl->runs_built = false;  // runs hold pairs of one class
-  for (int c = 0; c < 2; ++c) {
-    pl->cls[c].ready = false;
-    pl->cls[c].auto_cfg = tq_plan_pick_config(pl, c);
-  }
+  for (int c = 0; c < 2; ++c) pl->cls[c].ready = false;
+  plan_pick_defaults(pl);
   return TQ_SUCCESS;
 }
 
@@ -389,6 +421,7 @@ tq_status sweep_configs(tq_plan_t pl, const void *a, const void *b, void *c, int
       }
     }
     tc.split = best_split;
+    tc.split_user = true;
   }
   return plan_build_tiles(pl);
 }
@@ -497,9 +530,12 @@ tq_status tq_dense_contract(tq_handle_t h, tq_dtype dtype, int32_t rank_a,
   }
   tq_bs_desc da{rank_a, 1, dims_a, ba, ea, ta, &off0};
   tq_bs_desc db{rank_b, 1, dims_b, bb, eb, tb, &off0};
+  // the handle's plan cache: a repeated contraction of the same shapes plans
+  // once (the reference re-plans nothing either: cuBLAS and cuTT take the
+  // shapes per call)
   tq_plan_t pl = nullptr;
-  TQ_CHECK(tq_bs_contract_plan(h, &da, &db, n_legs, legs, dtype, TQ_PREC_NATIVE, &pl));
-  std::unique_ptr<tq_plan_s> guard(pl);
+  TQ_CHECK(tq_bs_contract_plan_cached(h, &da, &db, n_legs, legs, dtype, TQ_PREC_NATIVE, 0,
+                                      &pl));
   TQ_CHECK(tq_bs_contract_execute(pl, a, b, c));
   TQ_HIP(h, hipStreamSynchronize(h->stream));
   return TQ_SUCCESS;

@@ -543,6 +543,7 @@ tq_status tq_plan_choose_config(tq_plan_t pl, tq_costmodel_t md, int32_t *best) 
     pl->drop_graph();
     pl->cls[c].cfg = arg < 0 ? -1 : arg;
     pl->cls[c].split = arg_split;
+    pl->cls[c].split_user = true;
     if (best) best[c] = pl->cls[c].current();
   }
   return TQ_SUCCESS;

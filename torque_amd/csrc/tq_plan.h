@@ -92,6 +92,9 @@ struct TileClass {
   // the slices' partial products go to ws (one output block per slice) and a
   // reduction adds them into C in slice order (deterministic)
   int split = 1;
+  // the split was chosen by the caller, the autotuner or the cost model (the
+  // analytic default no longer applies)
+  bool split_user = false;
   int tiles_split = 1;  // the split the tile list was built for
   DevBuf vdesc;         // descriptors of the (pair, slice) GEMMs
   DevBuf ws;
@@ -238,6 +241,10 @@ tq_status plan_build_tiles(tq_plan_t pl);
 tq_status plan_set_split_form(tq_plan_t pl, bool in_place);
 // Analytic configuration choice for one tile class.
 int tq_plan_pick_config(tq_plan_t pl, int cls);
+int tq_plan_pick_split(tq_plan_t pl, int cls, int cfg);
+// The analytic defaults of both tile classes: configuration, and split-K
+// where no caller, autotuner or cost model chose one.
+void plan_pick_defaults(tq_plan_t pl);
 // Whether configuration cfg can run this plan: wave tiles address operands
 // with 32-bit offsets; FP16 data runs the workgroup-tile kernel with
 // 16-value k steps only.

@@ -1184,10 +1184,8 @@ tq_status plan_set_split_form(tq_plan_t pl, bool in_place) {
   // runs, segments and tiles address the operands through the descriptors
   pl->runs_built = false;
   pl->chunks_ready = false;
-  for (int c = 0; c < 2; ++c) {
-    pl->cls[c].ready = false;
-    pl->cls[c].auto_cfg = tq_plan_pick_config(pl, c);
-  }
+  for (int c = 0; c < 2; ++c) pl->cls[c].ready = false;
+  plan_pick_defaults(pl);
   set_layouts(pl);
   return TQ_SUCCESS;
 }
@@ -1712,7 +1710,7 @@ static tq_status plan_contract(tq_handle_t h, const tq_bs_desc *a, const tq_bs_d
   // short contractions are written as streams (the split-BF16 tier streams
   // them from its split panels)
   pl->stream_max_k = kStreamMaxK;
-  for (int c = 0; c < 2; ++c) pl->cls[c].auto_cfg = tq_plan_pick_config(pl.get(), c);
+  plan_pick_defaults(pl.get());
   if (np > 0 && ro > 0) TQ_CHECK(plan_build_tiles(pl.get()));
   *plan_out = pl.release();
   return TQ_SUCCESS;
