@@ -596,6 +596,36 @@ def test_split_k(cuda, precision):
         plan.set_class_split(0, 65)
 
 
+def test_default_split_for_few_tiles(cuda):
+    # a matrix times a vector (M = 4096, N = 1, K = 4096) gives one column of
+    # tiles: the new plan splits K by default (slots filled, 256 k per slice
+    # at least), matches the FP64 product and repeats bit for bit; a plan with
+    # many tiles keeps whole pairs; an explicit split survives re-picking
+    rng = np.random.default_rng(11)
+    m = k = 4096
+    xa, xb = rng.standard_normal(m * k), rng.standard_normal(k)
+    A = BlockSparseTensor([m, k], xa, np.zeros((1, 2), np.int64), np.array([[m - 1, k - 1]]),
+                          device=cuda)
+    B = BlockSparseTensor([k, 1], xb, np.zeros((1, 2), np.int64), np.array([[k - 1, 0]]),
+                          device=cuda)
+    plan = A.plan_contract(B, [[1, 0]])
+    split = plan.split()
+    assert split[0] > 1 and split[1] == 1, split
+    out = torch.empty(plan.out_n_elem, dtype=torch.float64, device=cuda)
+    plan.execute(A.data, B.data, out)
+    first = out.cpu().numpy()
+    want = xa.reshape(k, m).T @ xb
+    assert rel_fro(first, want) < FP64_TOL
+    plan.execute(A.data, B.data, out)
+    assert np.array_equal(out.cpu().numpy(), first)
+    plan.set_class_split(0, 2)
+    plan.set_small_k(8)  # re-picks the defaults
+    assert plan.split()[0] == 2
+    a, b, legs = W.c1(0)
+    assert BlockSparseTensor(a.dims, a.data, a.begin, a.end, device=cuda).plan_contract(
+        BlockSparseTensor(b.dims, b.data, b.begin, b.end, device=cuda), legs).split() == [1, 1]
+
+
 def test_split_k_cost_model_choice(cuda):
     # the committed cost model predicts split classes from the shapes of
     # their slices; whatever it picks must compute the same contraction
