@@ -328,12 +328,17 @@ GemmConfig tq_gemm_config(tq_precision prec, int cfg);
 // Launch the grouped GEMM over `tile_pair` (pair index per tile, in work-list
 // order) with tile_start[p] = first tile of pair p; or, with runs != null (a
 // wave-tile configuration only), over runs: tile_pair then holds run indices.
+// tile_ctr (may be null): kTileCounters device words the 64 x 64 wave-tile
+// configurations use to hand out tiles per XCD (zeroed by the launch; a
+// concurrent launch must not share them).
+constexpr int kTileCounters = 8;
 tq_status tq_launch_grouped_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec,
                                  int cfg, const GemmDesc *descs, const RunDesc *runs,
                                  const int32_t *tile_pair,
                                  const int32_t *tile_first, int64_t n_tiles,
                                  const void *a, const void *a_packed,
-                                 const void *b, const void *b_packed, void *c);
+                                 const void *b, const void *b_packed, void *c,
+                                 uint32_t *tile_ctr);
 
 // ---- scans (tq_scan.hip) ----------------------------------------------
 

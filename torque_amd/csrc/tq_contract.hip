@@ -174,7 +174,7 @@ tq_status launch_class(tq_plan_t pl, int cls, const void *a, const void *b, void
     TQ_CHECK(tq_launch_grouped_gemm(h, pl->dtype, pl->prec, tc.tiles_cfg, descs, runs,
                                     tc.tile_pair.as<int32_t>(), tc.tile_first.as<int32_t>(),
                                     tc.n_tiles, a, pl->pack_a.panel.p, b, pl->pack_b.panel.p,
-                                    out));
+                                    out, tc.tile_ctr.as<uint32_t>()));
   if (!split) return TQ_SUCCESS;
   return tq_launch_split_reduce(h, pl->dtype, tc.red_jobs.as<SplitReduceJob>(), tc.n_red_jobs,
                                 tc.ws.p, c);

@@ -339,7 +339,8 @@ tq_status tq_launch_wave_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec, int
                               const GemmDesc *descs, const RunDesc *runs,
                               const int32_t *tile_pair, const int32_t *tile_first,
                               int64_t n_tiles, const void *a, const void *a_packed,
-                              const void *b, const void *b_packed, void *c);
+                              const void *b, const void *b_packed, void *c,
+                              uint32_t *tile_ctr);
 
 // FP64 / FP32 configurations: the workgroup-tile kernel's (0 .. kNCfg-1),
 // then the wave-tile kernel's (tq_wave_gemm.hip).
@@ -364,7 +365,8 @@ tq_status tq_launch_grouped_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec,
                                  const int32_t *tile_pair,
                                  const int32_t *tile_first, int64_t n_tiles,
                                  const void *a, const void *a_packed,
-                                 const void *b, const void *b_packed, void *c) {
+                                 const void *b, const void *b_packed, void *c,
+                                 uint32_t *tile_ctr) {
   if (n_tiles <= 0) return TQ_SUCCESS;
   if (prec == TQ_PREC_BF16X3)
     return tq_fail(h, TQ_ERROR_INVALID_VALUE, "split-BF16 tier runs on split panels");
@@ -372,7 +374,7 @@ tq_status tq_launch_grouped_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec,
     if (dt == TQ_F16)
       return tq_fail(h, TQ_ERROR_UNSUPPORTED, "gemm: FP16 has no wave-tile configuration");
     return tq_launch_wave_gemm(h, dt, prec, cfg - kNCfg, descs, runs, tile_pair, tile_first,
-                               n_tiles, a, a_packed, b, b_packed, c);
+                               n_tiles, a, a_packed, b, b_packed, c, tile_ctr);
   }
   if (runs)
     return tq_fail(h, TQ_ERROR_INVALID_VALUE, "gemm: runs need a wave-tile configuration");

@@ -81,6 +81,7 @@ constexpr int kLargeMin = 128;
 // One MFMA tile class of a plan: its configuration and tile list.
 struct TileClass {
   DevBuf tile_pair, tile_first;
+  DevBuf tile_ctr;  // the XCDs' tile counters of the wave-tile GEMM launch
   int64_t n_tiles = 0;
   int tiles_cfg = -1;
   bool ready = false;

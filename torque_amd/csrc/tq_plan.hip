@@ -925,6 +925,7 @@ tq_status plan_build_tiles(tq_plan_t pl) {
   for (int c = 0; c < 2; ++c) {
     TileClass &tc = pl->cls[c];
     const int cfg = tc.current();
+    TQ_HIP(h, tc.tile_ctr.alloc(sizeof(uint32_t) * kTileCounters));
     if (!(tc.ready && tc.tiles_cfg == cfg && tc.tiles_split == tc.split)) {
       const GemmConfig g = tq_gemm_config(pl->prec, cfg);
       if (tc.split > 1) {

@@ -258,22 +258,54 @@ __device__ inline void row_pair(TC x, TC y, int lane, int h, TOut &v0, TOut &v1,
   }
 }
 
+// Tile order. With counters (the 64 x 64 configurations), the tile list is
+// dealt to the 8 XCDs in stripes of kStripe consecutive tiles (stripe s to
+// XCD s % 8, a workgroup's XCD taken as blockIdx.x % 8 as the dispatcher
+// deals them), and every wavefront takes its next tile from its XCD's
+// counter: the wavefronts of an XCD then work through neighbouring tiles,
+// which share their operand blocks, at the same time, and find them in that
+// XCD's L2. A fixed assignment (tile t + k * stride to one wavefront) drifts
+// apart as tiles of different K finish, and loses that sharing. Without
+// counters each XCD still walks a contiguous stretch of the list per round.
+// Either way every tile is computed once, the same way: the order changes
+// no bits.
+constexpr int kStripe = 64;
+
+__global__ void zero_counters_kernel(uint32_t *__restrict__ ctr) {
+  if (threadIdx.x < kTileCounters) ctr[threadIdx.x] = 0;
+}
+
+// The next tile of XCD `part` (wave-uniform): counter value c is tile
+// (c / kStripe) * 8 kStripe + part * kStripe + c % kStripe, increasing in
+// c, so a wavefront whose tile lies past the list is done.
+__device__ inline int64_t grab_tile(uint32_t *ctr, int part) {
+  uint32_t c = 0;
+  if ((threadIdx.x & 63) == 0) c = atomicAdd(ctr + part, 1u);
+  c = __builtin_amdgcn_readfirstlane(c);
+  return (int64_t)(c / kStripe) * (kStripe * kTileCounters) + (int64_t)part * kStripe +
+         (int64_t)(c % kStripe);
+}
+
 // MINW: wavefronts per SIMD the register allocation must allow; 2 keeps a
 // second wavefront resident to cover the first one's operand waits. RUNS:
-// the tile list indexes runs (RunDesc) rather than pairs.
+// the tile list indexes runs (RunDesc) rather than pairs. tile_ctr: the
+// XCDs' tile counters (zeroed), or null for the fixed assignment.
 template <class TIn, class TC, class TOut, int TM, int TN, int KPL, int MINW, bool RUNS>
 __global__ __launch_bounds__(256, MINW) void wave_gemm_kernel(
     const GemmDesc *__restrict__ descs, const RunDesc *__restrict__ runs,
     const int32_t *__restrict__ tile_pair, const int32_t *__restrict__ tile_first,
     int64_t n_tiles, const TIn *__restrict__ a, const TIn *__restrict__ a_packed,
-    const TIn *__restrict__ b, const TIn *__restrict__ b_packed, TOut *__restrict__ c) {
+    const TIn *__restrict__ b, const TIn *__restrict__ b_packed, TOut *__restrict__ c,
+    uint32_t *__restrict__ tile_ctr) {
   typedef typename Acc<TC>::type AccT;
   constexpr int BM = 16 * TM, BN = 16 * TN, BK = 4 * KPL;
   constexpr bool kF64 = sizeof(TC) == 8;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t stride = (int64_t)gridDim.x * 4;
-  int64_t t = uniform64(tqg::xcd_virtual_id(blockIdx.x, gridDim.x) * 4 + wave);
+  const int part = (int)(blockIdx.x % kTileCounters);
+  int64_t t = tile_ctr ? uniform64(grab_tile(tile_ctr, part))
+                       : uniform64(tqg::xcd_virtual_id(blockIdx.x, gridDim.x) * 4 + wave);
   if (t >= n_tiles) return;
 
   RunDesc run_cur, run_nxt;
@@ -299,7 +331,7 @@ __global__ __launch_bounds__(256, MINW) void wave_gemm_kernel(
   while (true) {
     // the next wave tile is known from the start, so the prefetch of its first
     // k tile needs no wait (the last tile prefetches itself again)
-    const int64_t t_next = t + stride;
+    const int64_t t_next = tile_ctr ? uniform64(grab_tile(tile_ctr, part)) : t + stride;
     const bool more = t_next < n_tiles;
     WaveTile<TIn> nxt = cur;
     if constexpr (RUNS) run_nxt = run_cur;
@@ -403,7 +435,8 @@ constexpr int kNWave = sizeof(kWaveCfgs) / sizeof(kWaveCfgs[0]);
 template <class TIn, class TC, class TOut, int TM, int TN, int KPL, int MINW, bool RUNS>
 tq_status launch_one(tq_handle_t h, const GemmDesc *descs, const RunDesc *runs,
                      const int32_t *tile_pair, const int32_t *tile_first, int64_t n_tiles,
-                     const void *a, const void *ap, const void *b, const void *bp, void *c) {
+                     const void *a, const void *ap, const void *b, const void *bp, void *c,
+                     uint32_t *tile_ctr) {
   // persistent grid: the workgroups that fit at once, a multiple of the 8 XCDs
   static int per_cu = 0;
   if (per_cu == 0) {
@@ -418,11 +451,15 @@ tq_status launch_one(tq_handle_t h, const GemmDesc *descs, const RunDesc *runs,
   int64_t grid = std::min<int64_t>(ceil_div(n_tiles, (int64_t)4), (int64_t)h->n_cu * per_cu);
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) grid = 1;
+  // counters for the 64 x 64 tiles (one atomic per smaller tile costs more
+  // than it saves), with every XCD holding workgroups
+  uint32_t *ctr = (TM * TN >= 16 && grid % kTileCounters == 0) ? tile_ctr : nullptr;
+  if (ctr) hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, h->stream, ctr);
   hipLaunchKernelGGL((wave_gemm_kernel<TIn, TC, TOut, TM, TN, KPL, MINW, RUNS>),
                      dim3((unsigned)grid), dim3(256), 0, h->stream, descs, runs, tile_pair,
                      tile_first, n_tiles, reinterpret_cast<const TIn *>(a),
                      reinterpret_cast<const TIn *>(ap), reinterpret_cast<const TIn *>(b),
-                     reinterpret_cast<const TIn *>(bp), reinterpret_cast<TOut *>(c));
+                     reinterpret_cast<const TIn *>(bp), reinterpret_cast<TOut *>(c), ctr);
   TQ_HIP(h, hipGetLastError());
   return TQ_SUCCESS;
 }
@@ -430,25 +467,27 @@ tq_status launch_one(tq_handle_t h, const GemmDesc *descs, const RunDesc *runs,
 template <class TIn, class TC, class TOut, int TM, int TN, int KPL, int MINW>
 tq_status launch_cfg(tq_handle_t h, const GemmDesc *descs, const RunDesc *runs,
                      const int32_t *tile_pair, const int32_t *tile_first, int64_t n_tiles,
-                     const void *a, const void *ap, const void *b, const void *bp, void *c) {
+                     const void *a, const void *ap, const void *b, const void *bp, void *c,
+                     uint32_t *tile_ctr) {
   if (runs)
     return launch_one<TIn, TC, TOut, TM, TN, KPL, MINW, true>(h, descs, runs, tile_pair,
                                                               tile_first, n_tiles, a, ap, b,
-                                                              bp, c);
+                                                              bp, c, tile_ctr);
   return launch_one<TIn, TC, TOut, TM, TN, KPL, MINW, false>(h, descs, runs, tile_pair,
                                                              tile_first, n_tiles, a, ap, b, bp,
-                                                             c);
+                                                             c, tile_ctr);
 }
 
 template <class TIn, class TC, class TOut>
 tq_status dispatch(tq_handle_t h, int cfg, const GemmDesc *descs, const RunDesc *runs,
                    const int32_t *tile_pair, const int32_t *tile_first, int64_t n_tiles,
-                   const void *a, const void *ap, const void *b, const void *bp, void *c) {
+                   const void *a, const void *ap, const void *b, const void *bp, void *c,
+                   uint32_t *tile_ctr) {
 #define TQ_WCFG(I, TM, TN, KPL, MINW)                                                     \
   case I:                                                                                 \
     return launch_cfg<TIn, TC, TOut, TM, TN, KPL, MINW>(h, descs, runs, tile_pair,        \
                                                         tile_first, n_tiles, a, ap, b, bp, \
-                                                        c);
+                                                        c, tile_ctr);
   switch (cfg) {
     TQ_WCFG(0, 1, 1, 4, 1)
     TQ_WCFG(1, 2, 1, 4, 1)
@@ -478,14 +517,15 @@ tq_status tq_launch_wave_gemm(tq_handle_t h, tq_dtype dt, tq_precision prec, int
                               const GemmDesc *descs, const RunDesc *runs,
                               const int32_t *tile_pair, const int32_t *tile_first,
                               int64_t n_tiles, const void *a, const void *a_packed,
-                              const void *b, const void *b_packed, void *c) {
+                              const void *b, const void *b_packed, void *c,
+                              uint32_t *tile_ctr) {
   if (n_tiles <= 0) return TQ_SUCCESS;
   if (dt == TQ_F32)
     return dispatch<float, float, float>(h, cfg, descs, runs, tile_pair, tile_first, n_tiles,
-                                         a, a_packed, b, b_packed, c);
+                                         a, a_packed, b, b_packed, c, tile_ctr);
   if (prec == TQ_PREC_F32)
     return dispatch<double, float, double>(h, cfg, descs, runs, tile_pair, tile_first, n_tiles,
-                                           a, a_packed, b, b_packed, c);
+                                           a, a_packed, b, b_packed, c, tile_ctr);
   return dispatch<double, double, double>(h, cfg, descs, runs, tile_pair, tile_first, n_tiles,
-                                          a, a_packed, b, b_packed, c);
+                                          a, a_packed, b, b_packed, c, tile_ctr);
 }
