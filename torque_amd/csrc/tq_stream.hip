@@ -4,8 +4,12 @@
 // output element: its cost is writing C, not multiplying, so it skips the
 // MFMA tiles. One wavefront takes a 64-row segment of the pair's column-major
 // output: each lane holds its row of A (K values) in registers and walks the
-// N columns; B[n, :] is the same for the whole wavefront (scalar loads), and
-// each column is one contiguous 64-element store. The planner hands wide
+// N columns, and each column is one contiguous 64-element store. B[n, :] is
+// the same for the whole wavefront: the lanes load the B rows of 64 columns
+// at once (lane j the K values of column j) into the wavefront's LDS slot and
+// read them back as broadcasts, and the column walk is compiled once per K,
+// so a column costs exactly K LDS reads and K products with no per-k branch
+// or scalar-load wait. The planner hands wide
 // pairs over as slices of at most kStreamCols columns, each a pair of its own
 // here, so that the segments stay small enough to balance.
 //
@@ -23,6 +27,41 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 
+// The columns [0, n_cols) of one segment for a pair of K contracted values:
+// av holds this lane's row of A, pb points at B row 0 (element (n, k) at
+// n * sbn + k * sbk), pc at this lane's row of column 0 of C (column stride
+// ldc), bs is the wavefront's LDS slot. Products are summed k = 0, 1, ... as
+// fma(a_k, b_k, acc) from a_0 * b_0.
+template <int K, class TIn, class TC, class TOut, class In>
+__device__ inline void walk_columns(const TC (&av)[kStreamMaxK], const TIn *pb, int64_t sbn,
+                                    int64_t sbk, int n_cols, int64_t ldc, TOut *pc, bool live,
+                                    In in, TC (*bs)[kStreamMaxK + 1], int lane) {
+  for (int n0 = 0; n0 < n_cols; n0 += 64) {
+    // lane j loads column n0 + j (columns past the segment repeat its last)
+    const TIn *rb = pb + (int64_t)min(n0 + lane, n_cols - 1) * sbn;
+    TC bv[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) bv[k] = in(rb[k * sbk]);
+    // the slot's previous rows were read by this wavefront before this point
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int k = 0; k < K; ++k) bs[lane][k] = bv[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int nn = min(64, n_cols - n0);
+    for (int j = 0; j < nn; ++j) {
+      TC acc = av[0] * bs[j][0];
+#pragma unroll
+      for (int k = 1; k < K; ++k) acc = fma(av[k], bs[j][k], acc);
+      if (live) pc[ldc * (n0 + j)] = (TOut)acc;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 // kRound3: each operand value enters as its split-BF16 representation
 // hi + mid + lo (the split-BF16 tier reading plain operands in place).
 template <class TIn, class TC, class TOut, bool kRound3 = false>
@@ -32,12 +71,16 @@ __global__ __launch_bounds__(kThreads) void small_k_kernel(
     const TIn *__restrict__ a, const TIn *__restrict__ a_packed,
     const TIn *__restrict__ b, const TIn *__restrict__ b_packed,
     TOut *__restrict__ c, int32_t *__restrict__ range_flag) {
+  static_assert(kStreamMaxK == 8, "one column walk per K = 1 .. 8");
+  // B rows of 64 columns per wavefront (K + 1 values per row: rows start on
+  // different banks)
+  __shared__ TC brows[kWaves][64][kStreamMaxK + 1];
   const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
   bool bad = false;  // kRound3: a value outside the split's domain
   // the segment index is the same on every lane of the wavefront: say so, so
-  // that the descriptor and B reads become scalar loads
-  const int wave_id = __builtin_amdgcn_readfirstlane(
-      (int)(blockIdx.x * kWaves + (threadIdx.x >> 6)));
+  // that the descriptor reads become scalar loads
+  const int wave_id = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWaves + wave));
   const int n_waves = (int)(gridDim.x * kWaves);
   for (int sg = wave_id; sg < n_segs; sg += n_waves) {
     const int p = __builtin_amdgcn_readfirstlane(seg_pair[sg]);
@@ -63,13 +106,23 @@ __global__ __launch_bounds__(kThreads) void small_k_kernel(
     for (int k = 0; k < kStreamMaxK; ++k)
       av[k] = (live && k < d.k) ? in(pa[m * sam + k * sak]) : (TC)0;
     TOut *pc = c + d.c_off + m;
-    for (int n = 0; n < d.n; ++n) {
-      const TIn *rb = pb + n * sbn;
-      TC acc = av[0] * in(rb[0]);
-#pragma unroll
-      for (int k = 1; k < kStreamMaxK; ++k)
-        if (k < d.k) acc = fma(av[k], in(rb[k * sbk]), acc);
-      if (live) pc[(int64_t)d.m * n] = (TOut)acc;
+    // 1 <= K <= kStreamMaxK for every streamed pair
+    switch (d.k) {
+#define TQ_WALK(K)                                                                       \
+  case K:                                                                                \
+    walk_columns<K, TIn, TC, TOut>(av, pb, sbn, sbk, d.n, d.m, pc, live, in, brows[wave], \
+                                   lane);                                                \
+    break;
+      TQ_WALK(1)
+      TQ_WALK(2)
+      TQ_WALK(3)
+      TQ_WALK(4)
+      TQ_WALK(5)
+      TQ_WALK(6)
+      TQ_WALK(7)
+      default:
+      TQ_WALK(8)
+#undef TQ_WALK
     }
   }
   if (kRound3 && bad && range_flag) atomicOr(range_flag, 1);
