@@ -262,9 +262,23 @@ __device__ inline void split_row(const uint16_t *p, int64_t plane, double (&v)[8
   }
 }
 
+// nn columns of a segment from the rebuilt B rows in LDS (row j: column j),
+// K products per column summed k = 0, 1, ... as in walk_columns.
+template <int K>
+__device__ inline void split_walk(const double (&av)[8], const double (*bs)[9], int nn,
+                                  int64_t ldc, double *pc, bool live) {
+  for (int j = 0; j < nn; ++j) {
+    double acc = av[0] * bs[j][0];
+#pragma unroll
+    for (int k = 1; k < K; ++k) acc = fma(av[k], bs[j][k], acc);
+    if (live) pc[ldc * j] = acc;
+  }
+}
+
 // B rows are rebuilt 64 at a time, one per lane, into the wavefront's LDS
 // slot, then read back by every lane as broadcasts: the rebuild costs one
-// pass per 64 output columns instead of one per column and lane.
+// pass per 64 output columns instead of one per column and lane. The column
+// walk is compiled once per K.
 __global__ __launch_bounds__(kThreads) void small_k_split_kernel(
     const GemmDesc *__restrict__ descs, const int32_t *__restrict__ seg_pair,
     const int32_t *__restrict__ seg_first, int64_t n_segs, const uint16_t *__restrict__ a,
@@ -298,12 +312,21 @@ __global__ __launch_bounds__(kThreads) void small_k_split_kernel(
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       const int nn = min(64, d.n - n0);
-      for (int j = 0; j < nn; ++j) {
-        double acc = av[0] * brow[wave][j][0];
-#pragma unroll
-        for (int k = 1; k < 8; ++k)
-          if (k < d.k) acc = fma(av[k], brow[wave][j][k], acc);
-        if (live) pc[(int64_t)d.m * (n0 + j)] = acc;
+      switch (d.k) {
+#define TQ_SPLIT_WALK(K)                                                         \
+  case K:                                                                        \
+    split_walk<K>(av, brow[wave], nn, (int64_t)d.m, pc + (int64_t)d.m * n0, live); \
+    break;
+        TQ_SPLIT_WALK(1)
+        TQ_SPLIT_WALK(2)
+        TQ_SPLIT_WALK(3)
+        TQ_SPLIT_WALK(4)
+        TQ_SPLIT_WALK(5)
+        TQ_SPLIT_WALK(6)
+        TQ_SPLIT_WALK(7)
+        default:
+        TQ_SPLIT_WALK(8)
+#undef TQ_SPLIT_WALK
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
