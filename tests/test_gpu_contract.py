@@ -179,11 +179,13 @@ def test_small_k_stream_and_mfma_agree(cuda, small_k):
 def test_small_k_wide_blocks_are_sliced(cuda, layout):
     # streamed pairs (K <= 8) wider than 64 columns are cut into 64-column
     # slices, each with its own B and C offsets; every tier must match the
-    # oracle whatever the B layout (k fastest, n fastest, or per block)
+    # oracle whatever the B layout (k fastest, n fastest, or per block). The
+    # streaming kernels walk columns with one loop per K: every K from 1 to 8
+    # occurs.
     rng = np.random.default_rng(17)
     nb = 12
     r = rng.integers(40, 151, nb)
-    k = rng.integers(1, 9, nb)
+    k = rng.permutation(np.concatenate([np.arange(1, 9), rng.integers(1, 9, nb - 8)]))
     n = rng.integers(65, 201, nb)
 
     def diag(rows, cols):
