@@ -271,7 +271,10 @@ tq_status tq_plan_get_pairs(tq_plan_t plan, int64_t *a_block, int64_t *b_block,
  * list): a permutation of 0..n_pairs-1, largest estimated cost first. */
 tq_status tq_plan_get_worklist(tq_plan_t plan, int64_t *order);
 /* Run the plan: C receives out_n_elem elements. A rank-0 output (full
- * contraction) is the network scalar, reduced on the GPU in FP64. */
+ * contraction) is the network scalar, reduced on the GPU in FP64. The plan's
+ * device state (operand panels, split-K workspace, the tile counters of its
+ * GEMM launches) serves one execute at a time: executes of one plan on
+ * different streams must not overlap. */
 tq_status tq_bs_contract_execute(tq_plan_t plan, const void *a, const void *b,
                                  void *c);
 /* The two stages of tq_bs_contract_execute, for callers that time or
