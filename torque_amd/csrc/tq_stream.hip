@@ -137,6 +137,49 @@ __global__ __launch_bounds__(kThreads) void small_k_kernel(
 // read back by every lane as broadcasts.
 constexpr int kGroupRow = kGroupMaxMembers * kStreamMaxK + 1;  // odd: no bank conflicts
 
+// Column walk of a slice for members of K0 and K1 contracted values (K1 = 0:
+// one member): member 0's products, then member 1's, each summed k = 0, 1,
+// ... from a_0 * b_0 of member 0 -- the order of the concatenated range.
+template <int K0, int K1, class TC, class TOut>
+__device__ inline void group_walk(const TC (*av)[kStreamMaxK], const TC (*brow)[kGroupRow],
+                                  int n, int64_t ldc, TOut *pc, bool live) {
+  for (int j = 0; j < n; ++j) {
+    TC acc = av[0][0] * brow[j][0];
+#pragma unroll
+    for (int k = 1; k < K0; ++k) acc = fma(av[0][k], brow[j][k], acc);
+#pragma unroll
+    for (int k = 0; k < K1; ++k) acc = fma(av[1][k], brow[j][kStreamMaxK + k], acc);
+    if (live) pc[ldc * j] = (TOut)acc;
+  }
+}
+
+// group_walk for member 0 of K0 values and member 1 (if any) of k1 values.
+template <int K0, int NM, class TC, class TOut>
+__device__ inline void group_walk_k1(int k1, const TC (*av)[kStreamMaxK],
+                                     const TC (*brow)[kGroupRow], int n, int64_t ldc,
+                                     TOut *pc, bool live) {
+  if constexpr (NM == 1) {
+    group_walk<K0, 0, TC, TOut>(av, brow, n, ldc, pc, live);
+  } else {
+    switch (k1) {
+#define TQ_GWALK(K1)                                                 \
+  case K1:                                                           \
+    group_walk<K0, K1, TC, TOut>(av, brow, n, ldc, pc, live);        \
+    break;
+      TQ_GWALK(1)
+      TQ_GWALK(2)
+      TQ_GWALK(3)
+      TQ_GWALK(4)
+      TQ_GWALK(5)
+      TQ_GWALK(6)
+      TQ_GWALK(7)
+      default:
+      TQ_GWALK(8)
+#undef TQ_GWALK
+    }
+  }
+}
+
 template <class TIn, class TC, class TOut, int NM>
 __device__ inline void group_slice(const GroupSlice &g, const GemmDesc *__restrict__ members,
                                    int row0, const TIn *__restrict__ a,
@@ -177,18 +220,24 @@ __device__ inline void group_slice(const GroupSlice &g, const GemmDesc *__restri
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   TOut *pc = c + g.c_off + m;
-  for (int j = 0; j < g.n; ++j) {
-    // member 0 holds at least one contracted value
-    TC acc = av[0][0] * brow[j][0];
-#pragma unroll
-    for (int k = 1; k < kStreamMaxK; ++k)
-      if (k < kq[0]) acc = fma(av[0][k], brow[j][k], acc);
-#pragma unroll
-    for (int q = 1; q < NM; ++q)
-#pragma unroll
-      for (int k = 0; k < kStreamMaxK; ++k)
-        if (k < kq[q]) acc = fma(av[q][k], brow[j][q * kStreamMaxK + k], acc);
-    if (live) pc[(int64_t)g.m * j] = (TOut)acc;
+  // every member holds 1 <= K <= kStreamMaxK contracted values; one walk per
+  // (K0, K1)
+  const int k1 = NM > 1 ? kq[NM - 1] : 0;
+  switch (kq[0]) {
+#define TQ_GWALK0(K0)                                                                   \
+  case K0:                                                                              \
+    group_walk_k1<K0, NM, TC, TOut>(k1, av, brow, g.n, (int64_t)g.m, pc, live);         \
+    break;
+    TQ_GWALK0(1)
+    TQ_GWALK0(2)
+    TQ_GWALK0(3)
+    TQ_GWALK0(4)
+    TQ_GWALK0(5)
+    TQ_GWALK0(6)
+    TQ_GWALK0(7)
+    default:
+    TQ_GWALK0(8)
+#undef TQ_GWALK0
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
