@@ -430,6 +430,50 @@ def test_bf16x3_many_tiles_per_workgroup(cuda):
         assert np.array_equal(got, first), cfg
 
 
+def test_tile_counters_repeat_and_replay(cuda):
+    # the 64 x 64 wave-tile configurations hand out tiles from per-XCD
+    # counters that a small launch clears before every GEMM launch: repeated
+    # executes, and graph replays with eager executes in between, must give
+    # the same bits as each other and as a configuration without counters
+    rng = np.random.default_rng(29)
+    a, b, legs = block_diagonal_pair(rng, 64, 130, 300)
+    (A, da), (B, db) = from_workload(a), from_workload(b)
+    ref_c, _ = O.contract(A, da, B, db, legs, threads=8)
+    ga, gb = torch_tensor(A, da, cuda), torch_tensor(B, db, cuda)
+    configs = gemm_configs(with_kind=True)
+    counted = [q for q, c in enumerate(configs) if c[3] == 1 and c[0] == 64 and c[1] == 64]
+    plain = [q for q, c in enumerate(configs) if c[3] == 1 and c[0] * c[1] < 64 * 64]
+    assert counted and plain
+    plan = ga.plan_contract(gb, legs, "native")
+    out = torch.empty(plan.out_n_elem, dtype=torch.float64, device=cuda)
+    plan.set_class_split(0, 1)  # whole pairs in every configuration
+    plan.set_class_split(1, 1)
+    plan.set_config(plain[0])
+    plan.execute(ga.data, gb.data, out)
+    torch.cuda.synchronize()
+    base = out.clone()
+    assert rel_fro(base.cpu().numpy(), ref_c) < FP64_TOL
+    for cfg in counted:
+        plan.set_config(cfg)
+        for r in range(3):
+            out.fill_(float("nan"))
+            plan.execute(ga.data, gb.data, out)
+            torch.cuda.synchronize()
+            assert torch.equal(out, base), (cfg, r)
+        assert plan.work()[0] >= 32 * 8  # a grid of whole rounds over the 8 XCDs
+        got = torch.zeros_like(out)
+        plan.capture(ga.data, gb.data, got)
+        for r in range(3):
+            got.fill_(float("nan"))
+            torch.cuda.synchronize()
+            plan.graph_launch()
+            torch.cuda.synchronize()
+            assert torch.equal(got, base), (cfg, r)
+            plan.execute(ga.data, gb.data, out)  # eager work between replays
+            torch.cuda.synchronize()
+            assert torch.equal(out, base), (cfg, r)
+
+
 @pytest.mark.parametrize("permuted", [False, True])
 def test_runs_match_pair_tiles(cuda, permuted):
     # U(1) pairs that share an A block and write adjacent output blocks are
