@@ -460,7 +460,7 @@ def test_tile_counters_repeat_and_replay(cuda):
             plan.execute(ga.data, gb.data, out)
             torch.cuda.synchronize()
             assert torch.equal(out, base), (cfg, r)
-        assert plan.work()[0] >= 32 * 8  # a grid of whole rounds over the 8 XCDs
+        assert plan.work()[0] >= 32 * 8  # enough tiles for a grid that is a multiple of 8
         got = torch.zeros_like(out)
         plan.capture(ga.data, gb.data, got)
         for r in range(3):

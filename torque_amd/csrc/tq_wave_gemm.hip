@@ -266,7 +266,7 @@ __device__ inline void row_pair(TC x, TC y, int lane, int h, TOut &v0, TOut &v1,
 // which share their operand blocks, at the same time, and find them in that
 // XCD's L2. A fixed assignment (tile t + k * stride to one wavefront) drifts
 // apart as tiles of different K finish, and loses that sharing. Without
-// counters each XCD still walks a contiguous stretch of the list per round.
+// counters each XCD still walks a contiguous stretch of the list per pass of the grid.
 // Either way every tile is computed once, the same way: the order changes
 // no bits.
 constexpr int kStripe = 64;
