@@ -203,12 +203,14 @@ tq_status tq_launch_split_jobs(tq_handle_t h, const CopyJob *jobs,
 // Split-BF16 grouped GEMM. direct = false: a and b are the split panels
 // (planes a_plane / b_plane elements apart); direct = true: a and b are the
 // FP64 operands read in place (the descriptors' offsets, leading dimensions
-// and k-fast flags address them) and split on the way to LDS.
+// and k-fast flags address them) and split on the way to LDS. tile_ctr (may
+// be null): kTileCounters device words the single-buffered configurations use
+// to hand out tiles per XCD, as in tq_launch_grouped_gemm.
 tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, bool direct, const GemmDesc *descs,
                                 const RunDesc *runs, const int32_t *tile_pair,
                                 const int32_t *tile_first, int64_t n_tiles, const void *a,
                                 const void *b, int64_t a_plane, int64_t b_plane, double *c,
-                                int32_t *range_flag);
+                                int32_t *range_flag, uint32_t *tile_ctr);
 
 // Runs a list of gather jobs that already sit in device memory. job_wg_start
 // is the exclusive prefix sum of workgroups per job (n_jobs + 1 entries).

@@ -169,7 +169,8 @@ tq_status launch_class(tq_plan_t pl, int cls, const void *a, const void *b, void
                                    tc.n_tiles, pl->split_direct ? a : pl->pack_a.panel.p,
                                    pl->split_direct ? b : pl->pack_b.panel.p,
                                    pl->pack_a.panel_elems, pl->pack_b.panel_elems,
-                                   reinterpret_cast<double *>(out), pl->range_flag.as<int32_t>()));
+                                   reinterpret_cast<double *>(out), pl->range_flag.as<int32_t>(),
+                                   tc.tile_ctr.as<uint32_t>()));
   else
     TQ_CHECK(tq_launch_grouped_gemm(h, pl->dtype, pl->prec, tc.tiles_cfg, descs, runs,
                                     tc.tile_pair.as<int32_t>(), tc.tile_first.as<int32_t>(),

@@ -300,7 +300,7 @@ __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
     const int32_t *__restrict__ tile_pair, const int32_t *__restrict__ tile_first,
     int64_t n_tiles, const void *__restrict__ a, const void *__restrict__ b,
     int64_t a_plane, int64_t b_plane, double *__restrict__ c,
-    int32_t *__restrict__ range_flag) {
+    int32_t *__restrict__ range_flag, uint32_t *__restrict__ tile_ctr) {
   constexpr int WAVES_N = 4 / WAVES_M;
   constexpr int kT = kThreads;
   constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
@@ -315,7 +315,20 @@ __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
   const int wm0 = (wave % WAVES_M) * WM, wn0 = (wave / WAVES_M) * WN;
   const int lrow = lane & 15, lchunk = lane >> 4;
 
-  int64_t t = tqg::xcd_virtual_id(blockIdx.x, gridDim.x);
+  // tiles from the XCD's counter (tqg::stripe_tile) when tile_ctr is given:
+  // thread 0 draws the workgroup's next tile at the start of each tile into
+  // one of two LDS slots, read by every thread after step 0's first barrier
+  // (the slot is next written two tiles later, long after every read)
+  __shared__ int64_t next_tile[2];
+  const int part = (int)(blockIdx.x % kTileCounters);
+  int64_t t;
+  if (tile_ctr) {
+    if (threadIdx.x == 0) next_tile[0] = tqg::stripe_tile(atomicAdd(tile_ctr + part, 1u), part);
+    __syncthreads();
+    t = next_tile[0];
+  } else {
+    t = tqg::xcd_virtual_id(blockIdx.x, gridDim.x);
+  }
   if (t >= n_tiles) return;
   TileRef cur = fetch<BM, BN, kT, DIRECT>(t, descs, runs, tile_pair, tile_first, a, b);
   StepLoader<BM, kT, DIRECT> la;
@@ -323,7 +336,9 @@ __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
   la.flag = lb.flag = range_flag;
   la.load(cur.a, a_plane, cur.d.k, 0);
   lb.load(cur.b, b_plane, cur.d.k, 0);
-  while (true) {
+  for (int it = 1;; ++it) {
+    if (tile_ctr && threadIdx.x == 0)
+      next_tile[it & 1] = tqg::stripe_tile(atomicAdd(tile_ctr + part, 1u), part);
     const GemmDesc &d = cur.d;
     const int steps = tile_steps(d);
     // every MFMA block runs (blocks past M or N compute on clamped rows and
@@ -334,10 +349,11 @@ __global__ __launch_bounds__(kThreads, MINW) void bf16x3_gemm_kernel(
 #pragma unroll
       for (int j = 0; j < TN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-    const int64_t t_next = t + gridDim.x;
+    int64_t t_next = t + gridDim.x;
     TileRef nxt = cur;
     for (int s = 0; s < steps; ++s) {
       __syncthreads();  // the previous step (or epilogue staging) is read
+      if (tile_ctr && s == 0) t_next = next_tile[it & 1];
       la.store(As);
       lb.store(Bs);
       __syncthreads();
@@ -374,7 +390,7 @@ __global__ __launch_bounds__(NW * 64, 1) void bf16x3_db_kernel(
     const int32_t *__restrict__ tile_pair, const int32_t *__restrict__ tile_first,
     int64_t n_tiles, const void *__restrict__ a, const void *__restrict__ b,
     int64_t a_plane, int64_t b_plane, double *__restrict__ c,
-    int32_t *__restrict__ range_flag) {
+    int32_t *__restrict__ range_flag, uint32_t *__restrict__ tile_ctr) {
   constexpr int WAVES_N = NW / WAVES_M;
   constexpr int kT = NW * 64;
   constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
@@ -466,7 +482,7 @@ __global__ __launch_bounds__(NW * 64, 1) void bf16x3_db2_kernel(
     const int32_t *__restrict__ tile_pair, const int32_t *__restrict__ tile_first,
     int64_t n_tiles, const void *__restrict__ a, const void *__restrict__ b,
     int64_t a_plane, int64_t b_plane, double *__restrict__ c,
-    int32_t *__restrict__ range_flag) {
+    int32_t *__restrict__ range_flag, uint32_t *__restrict__ tile_ctr) {
   constexpr int WAVES_N = NW / WAVES_M;
   constexpr int kT = NW * 64;
   constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
@@ -569,13 +585,13 @@ constexpr int kNCfg = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
 typedef void (*KernelFn)(const GemmDesc *, const RunDesc *, const int32_t *, const int32_t *,
                          int64_t, const void *, const void *, int64_t, int64_t, double *,
-                         int32_t *);
+                         int32_t *, uint32_t *);
 
 template <int BM, int BN, int WAVES_M, int MINW, int DB, bool DIRECT>
 tq_status launch(tq_handle_t h, const GemmDesc *descs, const RunDesc *runs,
                  const int32_t *tile_pair, const int32_t *tile_first, int64_t n_tiles,
                  const void *a, const void *b, int64_t ap, int64_t bp, double *c,
-                 int32_t *range_flag) {
+                 int32_t *range_flag, uint32_t *tile_ctr) {
   KernelFn kernel;
   const int threads = DB == 2 ? 512 : kThreads;
   if constexpr (DB == 2 && DIRECT)
@@ -599,8 +615,13 @@ tq_status launch(tq_handle_t h, const GemmDesc *descs, const RunDesc *runs,
   int64_t grid = std::min<int64_t>(n_tiles, (int64_t)h->n_cu * per_cu);
   if (grid >= 8) grid -= grid % 8;
   if (grid < 1) grid = 1;
+  // per-XCD counters for the single-buffered kernels (the double-buffered
+  // ones walk a fixed sequence of tiles), with every XCD holding workgroups
+  uint32_t *ctr = (DB == 0 && grid % kTileCounters == 0) ? tile_ctr : nullptr;
+  if (ctr)
+    hipLaunchKernelGGL(tqg::zero_tile_counters_kernel, dim3(1), dim3(64), 0, h->stream, ctr);
   hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(threads), 0, h->stream, descs, runs,
-                     tile_pair, tile_first, n_tiles, a, b, ap, bp, c, range_flag);
+                     tile_pair, tile_first, n_tiles, a, b, ap, bp, c, range_flag, ctr);
   TQ_HIP(h, hipGetLastError());
   return TQ_SUCCESS;
 }
@@ -609,12 +630,13 @@ template <bool DIRECT>
 tq_status dispatch(tq_handle_t h, int cfg, const GemmDesc *descs, const RunDesc *runs,
                    const int32_t *tile_pair, const int32_t *tile_first, int64_t n_tiles,
                    const void *a, const void *b, int64_t a_plane, int64_t b_plane, double *c,
-                   int32_t *range_flag) {
+                   int32_t *range_flag, uint32_t *tile_ctr) {
   switch (cfg) {
 #define TQ_CFG(I, BM, BN, WM, MINW, DB)                                                       \
   case I:                                                                                     \
     return launch<BM, BN, WM, MINW, DB, DIRECT>(h, descs, runs, tile_pair, tile_first, n_tiles, \
-                                                a, b, a_plane, b_plane, c, range_flag);
+                                                a, b, a_plane, b_plane, c, range_flag,          \
+                                                tile_ctr);
     TQ_CFG(0, 64, 64, 2, 1, 0)
     TQ_CFG(1, 128, 64, 2, 1, 0)
     TQ_CFG(2, 64, 128, 2, 1, 0)
@@ -646,11 +668,11 @@ tq_status tq_launch_bf16x3_gemm(tq_handle_t h, int cfg, bool direct, const GemmD
                                 const RunDesc *runs, const int32_t *tile_pair,
                                 const int32_t *tile_first, int64_t n_tiles, const void *a,
                                 const void *b, int64_t a_plane, int64_t b_plane, double *c,
-                                int32_t *range_flag) {
+                                int32_t *range_flag, uint32_t *tile_ctr) {
   if (n_tiles <= 0) return TQ_SUCCESS;
   if (direct)
     return dispatch<true>(h, cfg, descs, runs, tile_pair, tile_first, n_tiles, a, b, a_plane,
-                          b_plane, c, range_flag);
+                          b_plane, c, range_flag, tile_ctr);
   return dispatch<false>(h, cfg, descs, runs, tile_pair, tile_first, n_tiles, a, b, a_plane,
-                         b_plane, c, range_flag);
+                         b_plane, c, range_flag, tile_ctr);
 }

@@ -11,6 +11,25 @@ constexpr int kThreads = 256;
 // Workgroups are dealt to the 8 XCDs round-robin; this virtual index makes
 // each XCD walk a contiguous stretch of the tile list (tiles of one pair share
 // operand panels, so they should share an L2).
+// Tile handed out by per-XCD counters: the tile list is dealt to the 8 XCDs
+// in stripes of kTileStripe consecutive tiles (stripe s to XCD s % 8, a
+// workgroup's XCD taken as blockIdx.x % 8 as the dispatcher deals them), and
+// value c of XCD part's counter is tile (c / kTileStripe) * 8 kTileStripe +
+// part * kTileStripe + c % kTileStripe -- increasing in c, so a draw past the
+// list means the XCD's tiles are all taken.
+constexpr int kTileStripe = 64;
+__device__ inline int64_t stripe_tile(uint32_t c, int part) {
+  return (int64_t)(c / kTileStripe) * (kTileStripe * kTileCounters) +
+         (int64_t)part * kTileStripe + (int64_t)(c % kTileStripe);
+}
+
+// Clears the kTileCounters counters before a launch that draws from them (a
+// kernel, so that a captured graph clears them on every replay; one copy per
+// translation unit).
+static __global__ void zero_tile_counters_kernel(uint32_t *__restrict__ ctr) {
+  if (threadIdx.x < kTileCounters) ctr[threadIdx.x] = 0;
+}
+
 __device__ inline int64_t xcd_virtual_id(int64_t bid, int64_t grid) {
   const int64_t per = grid / 8;
   if (per == 0 || grid % 8 != 0) return bid;

@@ -258,32 +258,19 @@ __device__ inline void row_pair(TC x, TC y, int lane, int h, TOut &v0, TOut &v1,
   }
 }
 
-// Tile order. With counters (the 64 x 64 configurations), the tile list is
-// dealt to the 8 XCDs in stripes of kStripe consecutive tiles (stripe s to
-// XCD s % 8, a workgroup's XCD taken as blockIdx.x % 8 as the dispatcher
-// deals them), and every wavefront takes its next tile from its XCD's
-// counter: the wavefronts of an XCD then work through neighbouring tiles,
-// which share their operand blocks, at the same time, and find them in that
-// XCD's L2. A fixed assignment (tile t + k * stride to one wavefront) drifts
-// apart as tiles of different K finish, and loses that sharing. Without
-// counters each XCD still walks a contiguous stretch of the list per pass of the grid.
-// Either way every tile is computed once, the same way: the order changes
-// no bits.
-constexpr int kStripe = 64;
-
-__global__ void zero_counters_kernel(uint32_t *__restrict__ ctr) {
-  if (threadIdx.x < kTileCounters) ctr[threadIdx.x] = 0;
-}
-
-// The next tile of XCD `part` (wave-uniform): counter value c is tile
-// (c / kStripe) * 8 kStripe + part * kStripe + c % kStripe, increasing in
-// c, so a wavefront whose tile lies past the list is done.
+// Tile order. With counters (the 64 x 64 configurations), every wavefront
+// takes its next tile from its XCD's counter (tqg::stripe_tile): the
+// wavefronts of an XCD then work through neighbouring tiles, which share
+// their operand blocks, at the same time, and find them in that XCD's L2. A
+// fixed assignment (tile t + k * stride to one wavefront) drifts apart as
+// tiles of different K finish, and loses that sharing. Without counters each
+// XCD still walks a contiguous stretch of the list per pass of the grid.
+// Either way every tile is computed once, the same way: the order changes no
+// bits.
 __device__ inline int64_t grab_tile(uint32_t *ctr, int part) {
   uint32_t c = 0;
   if ((threadIdx.x & 63) == 0) c = atomicAdd(ctr + part, 1u);
-  c = __builtin_amdgcn_readfirstlane(c);
-  return (int64_t)(c / kStripe) * (kStripe * kTileCounters) + (int64_t)part * kStripe +
-         (int64_t)(c % kStripe);
+  return tqg::stripe_tile(__builtin_amdgcn_readfirstlane(c), part);
 }
 
 // MINW: wavefronts per SIMD the register allocation must allow; 2 keeps a
@@ -454,7 +441,8 @@ tq_status launch_one(tq_handle_t h, const GemmDesc *descs, const RunDesc *runs,
   // counters for the 64 x 64 tiles (one atomic per smaller tile costs more
   // than it saves), with every XCD holding workgroups
   uint32_t *ctr = (TM * TN >= 16 && grid % kTileCounters == 0) ? tile_ctr : nullptr;
-  if (ctr) hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, h->stream, ctr);
+  if (ctr)
+    hipLaunchKernelGGL(tqg::zero_tile_counters_kernel, dim3(1), dim3(64), 0, h->stream, ctr);
   hipLaunchKernelGGL((wave_gemm_kernel<TIn, TC, TOut, TM, TN, KPL, MINW, RUNS>),
                      dim3((unsigned)grid), dim3(256), 0, h->stream, descs, runs, tile_pair,
                      tile_first, n_tiles, reinterpret_cast<const TIn *>(a),
