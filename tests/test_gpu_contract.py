@@ -474,6 +474,46 @@ def test_tile_counters_repeat_and_replay(cuda):
             assert torch.equal(out, base), (cfg, r)
 
 
+def test_bf16x3_tile_counters_repeat_and_replay(cuda):
+    # the single-buffered split-BF16 configurations draw tiles from per-XCD
+    # counters (thread 0 posts the workgroup's next tile in LDS): repeated
+    # executes and graph replays give the bits of the double-buffered
+    # configurations, which walk a fixed sequence of tiles
+    rng = np.random.default_rng(43)
+    a, b, legs = block_diagonal_pair(rng, 64, 130, 300)
+    (A, da), (B, db) = from_workload(a), from_workload(b)
+    ref_c, _ = O.contract(A, da, B, db, legs, threads=8)
+    ga, gb = torch_tensor(A, da, cuda), torch_tensor(B, db, cuda)
+    plan = ga.plan_contract(gb, legs, "bf16x3")
+    plan.set_class_split(0, 1)
+    plan.set_class_split(1, 1)
+    out = torch.empty(plan.out_n_elem, dtype=torch.float64, device=cuda)
+    n_cfg = len(gemm_configs("bf16x3"))
+    plan.set_config(n_cfg - 1)  # a double-buffered configuration
+    plan.execute(ga.data, gb.data, out)
+    torch.cuda.synchronize()
+    base = out.clone()
+    assert rel_fro(base.cpu().numpy(), ref_c) < 1e-6
+    for cfg in (0, 3, 8):  # single-buffered: 64 x 64, 128 x 128, 128 x 128 two per SIMD
+        plan.set_config(cfg)
+        for r in range(2):
+            out.fill_(float("nan"))
+            plan.execute(ga.data, gb.data, out)
+            torch.cuda.synchronize()
+            assert torch.equal(out, base), (cfg, r)
+        got = torch.zeros_like(out)
+        plan.capture(ga.data, gb.data, got)
+        for r in range(3):
+            got.fill_(float("nan"))
+            torch.cuda.synchronize()
+            plan.graph_launch()
+            torch.cuda.synchronize()
+            assert torch.equal(got, base), (cfg, r)
+            plan.execute(ga.data, gb.data, out)
+            torch.cuda.synchronize()
+            assert torch.equal(out, base), (cfg, r)
+
+
 @pytest.mark.parametrize("permuted", [False, True])
 def test_runs_match_pair_tiles(cuda, permuted):
     # U(1) pairs that share an A block and write adjacent output blocks are
